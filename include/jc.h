@@ -1,0 +1,174 @@
+/*
+ * jc.h -- C ABI of libjc_hip.so, the MI355X (gfx950) layout / graph-compute
+ * engines.  Every entry point is plain C: pointers and sizes, no C++ or torch
+ * types, no exception crosses it.  Reference paths are relative to the root of
+ * ocasazza/jump-cannon.
+ *
+ * Conventions
+ *   - Return 0 (JC_OK) on success, a negative JC_E_* code otherwise; the
+ *     message of the last failure on this thread is jc_last_error().
+ *   - Callers own every input and output buffer; inputs are copied at init.
+ *   - One engine handle is driven by one thread at a time; separate handles are
+ *     independent.  Each engine owns one HIP stream and its device buffers.
+ *   - Positions are interleaved x,y,z f32, length 3*n, as at the reference
+ *     LayoutEngine boundary.
+ */
+#ifndef JC_H
+#define JC_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    JC_OK = 0,
+    JC_E_NODEVICE = -1, /* no HIP device: the reference's "requires a wgpu device" Err */
+    JC_E_INVALID = -2,  /* malformed input (CSR, positions, settings JSON) */
+    JC_E_OOM = -3,      /* device allocation failed */
+    JC_E_HIP = -4,      /* a HIP runtime call failed */
+    JC_E_STATE = -5,    /* call order violated (e.g. step before init) */
+    JC_E_DANGLING = -6  /* PageRank input has a degree-0 node (pagerank.rs:92-106) */
+};
+
+typedef struct jc_engine jc_engine;
+
+/* Field-for-field Fa2BhSettings (crates/graph-compute/src/engines/fa2_bh.rs:85-125);
+ * bools as u32.  prevent_overlap is accepted and unused, as in the reference. */
+typedef struct {
+    float gravity;
+    uint32_t strong_gravity;
+    float scaling_ratio;
+    float edge_weight_influence;
+    float jitter_tolerance;
+    uint32_t lin_log_mode;
+    uint32_t prevent_overlap;
+    float time_step;
+    float theta;
+    float max_displacement;
+} jc_fa2bh_settings;
+
+/* Field-for-field SgdStressSettings (crates/graph-compute/src/engines/sgd_stress.rs:59-98). */
+typedef struct {
+    uint32_t n_pivots;
+    uint32_t sweeps_per_step;
+    float eta_max;
+    float eta_min;
+    uint32_t n_anneal_steps;
+    uint32_t reserved;
+    uint64_t seed;
+} jc_sgd_settings;
+
+/* LayoutDescriptor (crates/graph-layouts/src/layout/layout_trait.rs:35-42). */
+typedef struct {
+    const char *id;
+    const char *kind;
+    const char *display_name;
+    const char *description;
+    uint32_t needs_edges;
+    uint32_t needs_cpu_positions;
+    uint32_t needs_gpu_positions_buffer;
+} jc_descriptor;
+
+/* Thread-local message of the last failing call (empty if none). */
+const char *jc_last_error(void);
+/* Number of visible HIP devices (0 when there is none). */
+int jc_device_count(void);
+
+/* ---- engines: the LayoutEngine trait (crates/graph-compute/src/engines/mod.rs:383-444) ---- */
+
+/* Fa2BhSettings::default() (fa2_bh.rs:109-125). */
+void jc_fa2bh_default_settings(jc_fa2bh_settings *out);
+/* Fa2BhEngine::new (fa2_bh.rs:179-186), registry id "fa2-bh-hip". */
+int jc_fa2bh_create(const jc_fa2bh_settings *settings, jc_engine **out);
+/* SgdStressSettings::default() (sgd_stress.rs:84-98). */
+void jc_sgd_default_settings(jc_sgd_settings *out);
+/* SgdStressGpuEngine::new (sgd_stress_gpu.rs:96-120), registry id "sgd-stress-hip". */
+int jc_sgd_create(const jc_sgd_settings *settings, jc_engine **out);
+
+/* LayoutEngine::descriptor (mod.rs:393). */
+int jc_engine_descriptor(const jc_engine *e, jc_descriptor *out);
+/* LayoutEngine::set_params (mod.rs:400-402; fa2_bh.rs:212-220): serde-compatible JSON
+ * object; "null" keeps the current settings; a missing key takes its default; an
+ * unknown key is ignored; a wrongly typed value is JC_E_INVALID. */
+int jc_engine_set_params_json(jc_engine *e, const char *json, size_t len);
+/* LayoutEngine::init (mod.rs:407-412; fa2_bh.rs:222-544): CSR offsets[n+1],
+ * neighbors[nnz], xyz[3n].  JC_E_NODEVICE when no GPU is visible. */
+int jc_engine_init(jc_engine *e, uint32_t n, const uint32_t *offsets, const uint32_t *neighbors,
+                   uint64_t nnz, const float *xyz);
+/* LayoutEngine::reinit (mod.rs:422-431): same contract as init. */
+int jc_engine_reinit(jc_engine *e, uint32_t n, const uint32_t *offsets,
+                     const uint32_t *neighbors, uint64_t nnz, const float *xyz);
+/* LayoutEngine::step (mod.rs:435; fa2_bh.rs:546-692) run n_iters times on the device.
+ * out_xyz (3n) receives the final positions, or NULL for no read-back (the
+ * positions stay resident in HBM).  The call returns once the work is queued
+ * when out_xyz is NULL; jc_engine_sync waits for it. */
+int jc_engine_step(jc_engine *e, uint32_t n_iters, float *out_xyz);
+int jc_engine_sync(jc_engine *e);
+/* Checkpoint / resume: positions, F_{t-1} (3n each), {speed, speed_efficiency}
+ * (the AdaptiveSpeed state, fa2_speed.rs:25-39) and the iteration count. */
+int jc_engine_get_state(jc_engine *e, float *xyz, float *old_force, float *speed_eff2,
+                        uint64_t *iteration);
+int jc_engine_set_state(jc_engine *e, const float *xyz, const float *old_force,
+                        const float *speed_eff2, uint64_t iteration);
+/* LayoutEngine::is_halted (mod.rs:442-444): always 0 for these engines. */
+int jc_engine_is_halted(const jc_engine *e);
+void jc_engine_destroy(jc_engine *e);
+
+/* ---- FA2+BH parity and measurement hooks (no counterpart in the reference) ---- */
+
+/* One force pass at positions xyz (tree build + walk + attraction + gravity)
+ * against the engine's current F_{t-1}; writes F (3n) and the exact
+ * {sum swing, sum traction}.  Engine state is left unchanged. */
+int jc_fa2bh_forces_at(jc_engine *e, const float *xyz, float *out_force, double *out_sums2);
+/* Copies the octree built by the last tree build: sorted Morton keys and
+ * permutation (n each), node arrays (cap each): node_a = com.xyz,mass (4 f32);
+ * node_info = internal: bits of (2*half)^2, leaf: 0x80000000|body;
+ * node_skip = pre-order rope (0xFFFFFFFF = end); node_depth = depth, 0x80 set
+ * on leaves.  root4 = centre xyz, half.  Any output pointer may be NULL. */
+int jc_fa2bh_tree(jc_engine *e, uint32_t cap, uint32_t *n_tree, uint32_t *n_valid, uint64_t *keys,
+                  uint32_t *perm, float *node_a, uint32_t *node_info, uint32_t *node_skip,
+                  uint8_t *node_depth, float *root4);
+/* Runs n_iters iterations (the octree build replayed from a captured hipGraph)
+ * with HIP events around the force kernel of every iteration, on the engine
+ * stream.  Returns the summed force-kernel time and the summed whole-iteration
+ * time (ms). */
+int jc_fa2bh_step_timed(jc_engine *e, uint32_t n_iters, double *force_ms_sum,
+                        double *iter_ms_sum);
+/* Octree size, number of interactions summed by the walk (accepted cells plus
+ * leaf bodies) and the exact {sum swing, sum traction} of the last iteration. */
+int jc_fa2bh_last_stats(jc_engine *e, uint64_t *n_tree, uint64_t *interactions, double *sums2);
+/* Eager (uncaptured) iterations with events around every phase; phase_ms[k]
+ * receives the mean ms of phase k over n_iters.  Phase names: jc_fa2bh_phase_name. */
+int jc_fa2bh_profile(jc_engine *e, uint32_t n_iters, double *phase_ms, uint32_t n_phases);
+const char *jc_fa2bh_phase_name(uint32_t k);
+uint32_t jc_fa2bh_phase_count(void);
+
+/* ---- PageRank (crates/graph-compute/src/analytics/pagerank.rs:67-287) ----
+ * Pull power iteration: rank_{t+1}[v] = (1-d)/n + d * sum_{u in N(v)} rank_t[u]/deg(u),
+ * rank_0 = 1/n, stopping when the f64 L1 norm of rank_{t+1}-rank_t < tol_l1 or
+ * after max_iters.  JC_E_DANGLING when a node has no neighbours. */
+int jc_pagerank(uint32_t n, const uint32_t *offsets, const uint32_t *neighbors, uint64_t nnz,
+                float damping, uint32_t max_iters, double tol_l1, float *out_ranks,
+                uint32_t *out_iters, double *out_residual_l1);
+
+/* ---- fixtures (crates/graph-compute/src/stability.rs:9-110) ---- */
+/* synthetic_scale_free(n, m, seed) as undirected pairs; returns the pair count
+ * (pairs written only while count <= cap). */
+uint64_t jc_gen_ba_pairs(uint32_t n, uint32_t m, uint64_t seed, uint32_t *pairs, uint64_t cap);
+uint64_t jc_gen_sbm_pairs(uint32_t n, uint32_t blocks, double p_in, double p_out, uint64_t seed,
+                          uint32_t *pairs, uint64_t cap);
+uint64_t jc_gen_rgg_pairs(uint32_t n, double radius, uint64_t seed, float *xy, uint32_t *pairs,
+                          uint64_t cap);
+/* csr_from_pairs: symmetric, sorted, deduplicated rows; returns nnz
+ * (nbrs needs room for 2*npairs). */
+uint64_t jc_csr_from_pairs(uint32_t n, const uint32_t *pairs, uint64_t npairs, uint32_t *offsets,
+                           uint32_t *nbrs);
+/* Positions uniform in [-half, half]^3 ((2u-1)*half per coordinate). */
+void jc_gen_uniform_cube(uint32_t n, float half, uint64_t seed, float *xyz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

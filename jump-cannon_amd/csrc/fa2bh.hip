@@ -1,0 +1,1164 @@
+// ForceAtlas2 + Barnes-Hut on gfx950: the whole iteration stays on the device.
+//
+// Reference: crates/graph-compute/src/engines/fa2_bh.rs (host octree, two GPU
+// passes and two read-backs per step) and shaders/fa2_barnes_hut.wgsl.  Here one
+// iteration is a fixed sequence of kernels on the engine's stream, captured once
+// into a hipGraph:
+//
+//   bbox        min/max over finite coordinates (order-preserving u32 atomics)
+//   keys        per body, the octant path through the reference's own f32 cell
+//               chain (fa2_bh.rs:768-799, 841-861) for 21 levels -> 63-bit key
+//   sort        stable LSD radix sort of (key, body)
+//   count/scan  cells opened by each body from the common-prefix lengths of
+//               neighbouring keys; exclusive scan -> pre-order positions
+//   emit        leaves and internal cells written straight into pre-order, with
+//               the skip rope (first node after a subtree) found by a galloping
+//               search on the sorted keys
+//   com         centre of mass level by level from the deepest cells, children
+//               in octant order (fa2_bh.rs:925-967): no float atomics
+//   force       rope walk (fa2_barnes_hut.wgsl:108-156), attraction over the
+//               reference's edge-scan order (:158-181), gravity, stats; bodies
+//               processed in Morton order so a wave walks one region of the tree
+//   speed       exact swing/traction sums -> AdaptiveSpeed (fa2_speed.rs:48-100)
+//               on one device thread
+//   apply       fa2_barnes_hut.wgsl:203-226, ping-pong positions
+//
+// A point-region octree whose cells split only while they hold two bodies is
+// unique for a point set, so this build equals the reference insertion tree
+// (up to the 21-level key depth); with FMA contraction off and the reference's
+// operation order every f32 value matches the restated host path bit for bit.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace jc {
+namespace {
+
+constexpr int kLevels = 21;
+constexpr uint32_t kLeaf = 0x80000000u;
+constexpr uint32_t kEnd = 0xFFFFFFFFu;
+constexpr uint32_t kTB = 256;             // threads per block for per-node kernels
+constexpr uint32_t kScanItems = 16;       // count/scan tile = 4096 bodies
+constexpr uint32_t kScanTile = kTB * kScanItems;
+constexpr uint32_t kLimbs = 10;           // exact-sum limbs per quantity
+constexpr uint32_t kTimedIters = 10;      // timed iterations per host read-back of events
+
+struct Frame {
+    uint32_t bbmin[3];
+    uint32_t bbmax[3];
+    uint32_t n_invalid;
+    uint32_t n_tree;
+    float root[4];
+    float speed;
+    float speed_eff;
+    double sum_swing;
+    double sum_traction;
+    uint32_t max_depth;
+    uint32_t walk_cap_hits;
+    unsigned long long interactions;  // accepted cells + leaf bodies summed by the walk
+};
+
+struct Params {
+    uint32_t n;
+    float gravity;
+    uint32_t strong_gravity;
+    float scaling_ratio;
+    float jitter_tolerance;
+    uint32_t lin_log_mode;
+    float time_step;
+    float theta2;
+    float max_displacement;
+    float ewi_weight;  // pow(max(1, 1e-6), edge_weight_influence): every fa2-bh edge weighs 1
+    double est_jt;     // 0.05 * sqrt(n)           (fa2_speed.rs:72)
+    double min_jt;     // sqrt(est_jt)             (fa2_speed.rs:73)
+    uint32_t cap;      // node capacity of the octree arrays
+};
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ uint32_t f2ord(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t o) {
+    uint32_t u = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
+    return __uint_as_float(u);
+}
+__device__ __forceinline__ bool finitef(float x) { return __builtin_isfinite(x); }
+__device__ __forceinline__ float len3(float x, float y, float z) {
+    return sqrtf(x * x + y * y + z * z);
+}
+
+// Root cube of fa2_bh.rs:768-799 from the reduced box.
+__device__ __forceinline__ void compute_root(const Frame *fr, float *r4) {
+    float mn[3], mx[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        mn[k] = ord2f(fr->bbmin[k]);
+        mx[k] = ord2f(fr->bbmax[k]);
+    }
+    if (!finitef(mn[0])) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            mn[k] = -1.0f;
+            mx[k] = 1.0f;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) r4[k] = 0.5f * (mn[k] + mx[k]);
+    float half = fmaxf(fmaxf(mx[0] - mn[0], mx[1] - mn[1]), mx[2] - mn[2]) * 0.5f;
+    if (!finitef(half) || half <= 0.0f) half = 1.0f;
+    r4[3] = half * 1.05f;
+}
+
+__device__ __forceinline__ int lcp_digits(uint64_t a, uint64_t b) {
+    uint64_t x = a ^ b;
+    if (!x) return kLevels;
+    return (__clzll((long long)x) - 1) / 3;
+}
+
+// ---------------------------------------------------------------- kernels
+__global__ void k_frame_reset(Frame *fr) {
+    if (threadIdx.x == 0) {
+        const uint32_t pinf = 0xFF800000u;  // f2ord(+inf)
+        for (int k = 0; k < 3; k++) {
+            fr->bbmin[k] = pinf;
+            fr->bbmax[k] = ~pinf;  // f2ord(-inf)
+        }
+        fr->n_invalid = 0;
+        fr->max_depth = 0;
+        fr->interactions = 0ull;
+    }
+}
+
+__global__ __launch_bounds__(kTB) void k_bbox(const float4 *__restrict__ pos, uint32_t n,
+                                              Frame *fr) {
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float4 p = pos[i];
+        float v[3] = {p.x + 0.0f, p.y + 0.0f, p.z + 0.0f};  // +0: canonical zero sign
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            if (finitef(v[k])) {
+                mn[k] = fminf(mn[k], v[k]);
+                mx[k] = fmaxf(mx[k], v[k]);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            mn[k] = fminf(mn[k], __shfl_xor(mn[k], o, 64));
+            mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], o, 64));
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            if (finitef(mn[k])) atomicMin(&fr->bbmin[k], f2ord(mn[k]));
+            if (finitef(mx[k])) atomicMax(&fr->bbmax[k], f2ord(mx[k]));
+        }
+    }
+}
+
+// Octant path of every body through the reference cell chain: digit
+// (x>=cx) | (y>=cy)<<1 | (z>=cz)<<2, child half = half*0.5, child centre =
+// centre +- child half (fa2_bh.rs:841-861).  Non-finite bodies get an all-ones
+// key and sort last (the reference skips them at insertion, fa2_bh.rs:810-812).
+__global__ __launch_bounds__(kTB) void k_keys(const float4 *__restrict__ pos, uint32_t n,
+                                              Frame *fr, uint64_t *__restrict__ keys,
+                                              uint32_t *__restrict__ vals) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    float r4[4];
+    compute_root(fr, r4);
+    if (i == 0) {
+        for (int k = 0; k < 4; k++) fr->root[k] = r4[k];
+    }
+    if (i >= n) return;
+    float4 p = pos[i];
+    uint64_t key;
+    if (finitef(p.x) && finitef(p.y) && finitef(p.z)) {
+        float cx = r4[0], cy = r4[1], cz = r4[2], h = r4[3];
+        key = 0;
+#pragma unroll
+        for (int l = 0; l < kLevels; l++) {
+            uint32_t o = (p.x >= cx ? 1u : 0u) | (p.y >= cy ? 2u : 0u) | (p.z >= cz ? 4u : 0u);
+            key = (key << 3) | o;
+            h = h * 0.5f;
+            cx = cx + ((o & 1u) ? h : -h);
+            cy = cy + ((o & 2u) ? h : -h);
+            cz = cz + ((o & 4u) ? h : -h);
+        }
+    } else {
+        key = ~0ull;
+        atomicAdd(&fr->n_invalid, 1u);
+    }
+    keys[i] = key;
+    vals[i] = i;
+}
+
+// Cells opened by sorted body i: c_i = max(0, l_i - l_{i-1}) with l = common
+// leading octal digits of neighbouring keys (-1 past either end); a lone body
+// opens the root.  Block-local exclusive scan of c, plus the block total.
+__device__ __forceinline__ uint32_t cells_opened(const uint64_t *keys, uint32_t i, uint32_t nv) {
+    if (i >= nv) return 0;
+    if (nv == 1) return 1;
+    const uint64_t k = keys[i];
+    const int lp = i > 0 ? lcp_digits(keys[i - 1], k) : -1;
+    const int li = i + 1 < nv ? lcp_digits(k, keys[i + 1]) : -1;
+    return li > lp ? (uint32_t)(li - lp) : 0u;
+}
+
+__global__ __launch_bounds__(kTB) void k_count_scan(const uint64_t *__restrict__ keys, uint32_t n,
+                                                    const Frame *fr, uint32_t *__restrict__ loc,
+                                                    uint32_t *__restrict__ blk_tot) {
+    __shared__ uint32_t ws[kTB / 64];
+    const uint32_t nv = n - fr->n_invalid;
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t c[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kScanItems; j++) {
+        c[j] = cells_opened(keys, base + j, nv);
+        s += c[j];
+    }
+    // block exclusive scan of s
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t incl = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(incl, o, 64);
+        if ((int)lane >= o) incl += t;
+    }
+    if (lane == 63) ws[w] = incl;
+    __syncthreads();
+    uint32_t wp = 0, tot = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kTB / 64; q++) {
+        if (q < w) wp += ws[q];
+        tot += ws[q];
+    }
+    uint32_t run = incl - s + wp;
+#pragma unroll
+    for (uint32_t j = 0; j < kScanItems; j++) {
+        if (base + j < n) loc[base + j] = run;
+        run += c[j];
+    }
+    if (threadIdx.x == 0) blk_tot[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(uint32_t *__restrict__ blk, uint32_t nb,
+                                                   uint32_t n, Frame *fr) {
+    __shared__ uint32_t ws[16];
+    const uint32_t per = (nb + 1023u) / 1024u;
+    const uint32_t b0 = min(threadIdx.x * per, nb), b1 = min(b0 + per, nb);
+    uint32_t s = 0;
+    for (uint32_t i = b0; i < b1; i++) s += blk[i];
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t incl = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(incl, o, 64);
+        if ((int)lane >= o) incl += t;
+    }
+    if (lane == 63) ws[w] = incl;
+    __syncthreads();
+    uint32_t wp = 0, tot = 0;
+    for (uint32_t q = 0; q < 16; q++) {
+        if (q < w) wp += ws[q];
+        tot += ws[q];
+    }
+    uint32_t run = incl - s + wp;
+    for (uint32_t i = b0; i < b1; i++) {
+        uint32_t v = blk[i];
+        blk[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t nv = n - fr->n_invalid;
+        fr->n_tree = nv ? tot + nv : 0u;
+    }
+}
+
+__device__ __forceinline__ uint32_t first_node_of(const uint32_t *loc, const uint32_t *blk,
+                                                  uint32_t i) {
+    return loc[i] + blk[i / kScanTile] + i;
+}
+
+// Last sorted index j >= i whose first d octal digits equal those of key k.
+__device__ __forceinline__ uint32_t cell_end(const uint64_t *keys, uint32_t i, uint32_t nv,
+                                             uint64_t k, int d) {
+    if (d == 0) return nv - 1;
+    const int sh = 3 * (kLevels - d);
+    const uint64_t pre = k >> sh;
+    uint32_t lo = i, step = 1, hi;
+    for (;;) {
+        uint32_t probe = lo + step;
+        if (probe >= nv || (keys[probe] >> sh) != pre) {
+            hi = probe < nv ? probe : nv;
+            break;
+        }
+        lo = probe;
+        step <<= 1;
+    }
+    while (hi - lo > 1) {
+        uint32_t mid = lo + ((hi - lo) >> 1);
+        if ((keys[mid] >> sh) == pre) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
+                                              const uint32_t *__restrict__ perm,
+                                              const float4 *__restrict__ pos,
+                                              const uint32_t *__restrict__ deg, uint32_t n,
+                                              Frame *fr, const uint32_t *__restrict__ loc,
+                                              const uint32_t *__restrict__ blk,
+                                              float4 *__restrict__ node_a,
+                                              uint2 *__restrict__ node_b,
+                                              uint8_t *__restrict__ node_depth, uint64_t cap) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nv = n - fr->n_invalid;
+    if (i >= nv) return;
+    const uint64_t k = keys[i];
+    int lp, li;
+    if (nv == 1) {
+        lp = -1;
+        li = 0;  // the lone body hangs below an internal root
+    } else {
+        lp = i > 0 ? lcp_digits(keys[i - 1], k) : -1;
+        li = i + 1 < nv ? lcp_digits(k, keys[i + 1]) : -1;
+    }
+    const uint32_t c = li > lp ? (uint32_t)(li - lp) : 0u;
+    const int a0 = lp + 1;
+    const uint32_t pos0 = first_node_of(loc, blk, i);
+    if ((uint64_t)pos0 + c >= cap) return;  // cannot happen for a consistent scan
+    if (c) {
+        float h = fr->root[3];
+        for (int d = 0; d < a0; d++) h = h * 0.5f;
+        for (uint32_t t = 0; t < c; t++) {
+            const int d = a0 + (int)t;
+            const uint32_t j = cell_end(keys, i, nv, k, d);
+            const uint32_t skip = (j + 1 < nv) ? first_node_of(loc, blk, j + 1) : kEnd;
+            const float s = h * 2.0f;
+            const float s2 = s * s;
+            node_b[pos0 + t] = make_uint2(__float_as_uint(s2), skip);
+            node_depth[pos0 + t] = (uint8_t)d;
+            h = h * 0.5f;
+        }
+        atomicMax(&fr->max_depth, (uint32_t)(a0 + (int)c - 1));
+    }
+    const uint32_t leaf = pos0 + c;
+    const uint32_t b = perm[i];
+    if (b >= n) return;
+    const float4 p = pos[b];
+    const float m = fmaxf((float)deg[b] + 1.0f, 1e-3f);
+    node_a[leaf] = make_float4(p.x, p.y, p.z, m);
+    node_b[leaf] = make_uint2(kLeaf | b, (i + 1 < nv) ? leaf + 1 : kEnd);
+    const int ld = (lp > li ? lp : li) + 1;
+    node_depth[leaf] = (uint8_t)(0x80 | ld);
+}
+
+// Centre of mass of the internal cells at depth d from their children, in
+// octant (pre-order) order: m = sum m_c; com = sum(com_c * m_c) / m (fa2_bh.rs:944-965).
+__global__ __launch_bounds__(kTB) void k_com_level(const Frame *fr, uint32_t d, uint32_t cap,
+                                                   float4 *__restrict__ node_a,
+                                                   const uint2 *__restrict__ node_b,
+                                                   const uint8_t *__restrict__ node_depth) {
+    if (d > fr->max_depth) return;
+    const uint32_t N = min(fr->n_tree, cap);
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
+        if (node_depth[p] != d) continue;
+        const uint32_t end = node_b[p].y;
+        float tm = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+        for (uint32_t ch = p + 1; ch < N && ch != end; ch = node_b[ch].y) {
+            const float4 a = node_a[ch];
+            tm = tm + a.w;
+            cx = cx + a.x * a.w;
+            cy = cy + a.y * a.w;
+            cz = cz + a.z * a.w;
+        }
+        if (tm > 0.0f) {
+            cx = cx / tm;
+            cy = cy / tm;
+            cz = cz / tm;
+        }
+        node_a[p] = make_float4(cx, cy, cz, tm);
+    }
+}
+
+// Exact accumulation of a non-negative finite f32 into 32-bit limbs held in u64
+// (LSB = 2^-149): the mantissa lands in at most two limbs.
+__device__ __forceinline__ void acc_add(unsigned long long *acc, float v) {
+    uint32_t u = __float_as_uint(v) & 0x7FFFFFFFu;
+    if (!u) return;
+    uint32_t e = u >> 23, m = u & 0x7FFFFFu;
+    if (e) m |= 0x800000u;
+    uint32_t sh = (e ? e : 1u) - 1u;
+    uint64_t t = (uint64_t)m << (sh & 31u);
+    atomicAdd(&acc[sh >> 5], (unsigned long long)(t & 0xFFFFFFFFull));
+    if (t >> 32) atomicAdd(&acc[(sh >> 5) + 1], (unsigned long long)(t >> 32));
+}
+
+__global__ __launch_bounds__(kTB) void k_force(const Params P, Frame *fr,
+                                               const float4 *__restrict__ pos,
+                                               const float4 *__restrict__ old_force,
+                                               float4 *__restrict__ force,
+                                               const uint32_t *__restrict__ deg,
+                                               const uint32_t *__restrict__ att_off,
+                                               const uint32_t *__restrict__ att_adj,
+                                               const uint32_t *__restrict__ perm,
+                                               const float4 *__restrict__ node_a,
+                                               const uint2 *__restrict__ node_b,
+                                               unsigned long long *__restrict__ partial) {
+    __shared__ unsigned long long acc[2 * kLimbs];
+    __shared__ unsigned long long inter;
+    if (threadIdx.x < 2 * kLimbs) acc[threadIdx.x] = 0ull;
+    if (threadIdx.x == 0) inter = 0ull;
+    __syncthreads();
+    uint32_t taken = 0;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t b = t < P.n ? perm[t] : P.n;
+    if (b < P.n) {
+        const float4 p = pos[b];
+        const float deg_i = (float)deg[b] + 1.0f;
+        const float krd = P.scaling_ratio * deg_i;
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        const uint32_t N = min(fr->n_tree, P.cap);
+        if (N > 0) {
+            const uint64_t cap = max((uint64_t)N * 4ull, 16ull);
+            uint64_t walk = 0;
+            uint32_t idx = 0;
+            while (idx < N && walk < cap) {
+                walk++;
+                const float4 A = node_a[idx];
+                const uint2 B = node_b[idx];
+                const float dx = p.x - A.x, dy = p.y - A.y, dz = p.z - A.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                const bool leaf = (B.x & kLeaf) != 0u;
+                bool take;
+                if (leaf) take = ((B.x & ~kLeaf) != b) && A.w > 0.0f;
+                else take = A.w > 0.0f && r2 > 0.0f && __uint_as_float(B.x) < P.theta2 * r2;
+                if (take) {
+                    taken++;
+                    const float r2c = fmaxf(r2, 1.0e-4f);
+                    const float cf = krd * A.w / r2c;
+                    fx = fx + dx * cf;
+                    fy = fy + dy * cf;
+                    fz = fz + dz * cf;
+                }
+                idx = (leaf || take) ? B.y : idx + 1u;
+            }
+        }
+        // attraction: the other endpoint of every edge touching b, in the
+        // reference's edge-list order (fa2_barnes_hut.wgsl:158-181)
+        const uint32_t e0 = att_off[b], e1 = att_off[b + 1];
+        for (uint32_t e = e0; e < e1; e++) {
+            const float4 q = pos[att_adj[e]];
+            const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+            float att = P.ewi_weight;
+            if (P.lin_log_mode) {
+                const float r = fmaxf(len3(dx, dy, dz), 1.0e-3f);
+                att = P.ewi_weight * logf(1.0f + r) / r;
+            }
+            fx = fx + dx * att;
+            fy = fy + dy * att;
+            fz = fz + dz * att;
+        }
+        // gravity (fa2_barnes_hut.wgsl:183-189)
+        const float gd = P.gravity * deg_i;
+        if (P.strong_gravity) {
+            fx = fx - p.x * gd;
+            fy = fy - p.y * gd;
+            fz = fz - p.z * gd;
+        } else {
+            const float r0 = fmaxf(len3(p.x, p.y, p.z), 1.0e-3f);
+            fx = fx - (p.x / r0) * gd;
+            fy = fy - (p.y / r0) * gd;
+            fz = fz - (p.z / r0) * gd;
+        }
+        const float4 o = old_force[b];
+        force[b] = make_float4(fx, fy, fz, 0.0f);
+        const float swing = deg_i * len3(o.x - fx, o.y - fy, o.z - fz);
+        const float tra = deg_i * 0.5f * len3(o.x + fx, o.y + fy, o.z + fz);
+        if (finitef(swing) && finitef(tra)) {
+            acc_add(acc, swing);
+            acc_add(acc + kLimbs, tra);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) taken += __shfl_xor(taken, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&inter, (unsigned long long)taken);
+    __syncthreads();
+    if (threadIdx.x < 2 * kLimbs) partial[blockIdx.x * 2 * kLimbs + threadIdx.x] = acc[threadIdx.x];
+    if (threadIdx.x == 0) atomicAdd(&fr->interactions, inter);
+}
+
+// Exact limb sum -> nearest f64, ties to even (same algorithm as the oracle).
+__device__ double limbs_to_double(const unsigned long long *in) {
+    unsigned long long l[kLimbs];
+    for (uint32_t k = 0; k < kLimbs; k++) l[k] = in[k];
+    for (uint32_t k = 0; k + 1 < kLimbs; k++) {
+        l[k + 1] += l[k] >> 32;
+        l[k] &= 0xFFFFFFFFull;
+    }
+    int top = -1;
+    for (int k = (int)kLimbs - 1; k >= 0; k--)
+        if (l[k]) {
+            top = 32 * k + 63 - __clzll((long long)l[k]);
+            break;
+        }
+    if (top < 0) return 0.0;
+    auto bit = [&](int q) -> uint32_t {
+        return q < 0 ? 0u : (uint32_t)((l[q >> 5] >> (q & 31)) & 1ull);
+    };
+    unsigned long long mant = 0;
+    const int lo = top - 52;
+    for (int q = top; q >= (lo > 0 ? lo : 0); q--) mant = (mant << 1) | bit(q);
+    if (lo <= 0) return ldexp((double)mant, -149);
+    const uint32_t rb = bit(lo - 1);
+    uint32_t sticky = 0;
+    for (int q = lo - 2; q >= 0 && !sticky; q--) sticky |= bit(q);
+    if (rb && (sticky || (mant & 1ull))) mant++;
+    return ldexp((double)mant, lo - 149);
+}
+
+// Sums the per-block limbs, then runs AdaptiveSpeed::update (fa2_speed.rs:48-100)
+// on one thread.  update_speed = 0 only records the sums (parity hook).
+__global__ __launch_bounds__(256) void k_speed(const Params P, Frame *fr,
+                                               const unsigned long long *__restrict__ partial,
+                                               uint32_t nblocks, uint32_t update_speed) {
+    constexpr uint32_t G = 12;
+    __shared__ unsigned long long red[G][2 * kLimbs];
+    const uint32_t t = threadIdx.x;
+    if (t < G * 2 * kLimbs) {
+        const uint32_t k = t % (2 * kLimbs), g = t / (2 * kLimbs);
+        unsigned long long s = 0;
+        for (uint32_t b = g; b < nblocks; b += G) s += partial[b * 2 * kLimbs + k];
+        red[g][k] = s;
+    }
+    __syncthreads();
+    if (t != 0) return;
+    unsigned long long tot[2 * kLimbs];
+    for (uint32_t k = 0; k < 2 * kLimbs; k++) {
+        unsigned long long s = 0;
+        for (uint32_t g = 0; g < G; g++) s += red[g][k];
+        tot[k] = s;
+    }
+    const double ts = limbs_to_double(tot), tt = limbs_to_double(tot + kLimbs);
+    fr->sum_swing = ts;
+    fr->sum_traction = tt;
+    if (!update_speed) return;
+    float speed = fr->speed, eff = fr->speed_eff;
+    if (!(ts > 0.0) || !(tt > 0.0) || P.n == 0) return;
+    const double nn = (double)P.n, jt0 = (double)P.jitter_tolerance;
+    double jt = jt0 * fmax(P.min_jt, fmin(10.0, P.est_jt * tt / (nn * nn)));
+    const float min_eff = 0.05f;
+    if (ts / tt > 2.0) {
+        if (eff > min_eff) eff = eff * 0.5f;
+        jt = fmax(jt, jt0);
+    }
+    const double target = jt * (double)eff * tt / ts;
+    if (ts > jt * tt) {
+        if (eff > min_eff) eff = eff * 0.7f;
+    } else if (speed < 1000.0f) {
+        eff = eff * 1.3f;
+    }
+    const double sp = (double)speed;
+    speed = (float)(sp + fmin(target - sp, 0.5 * sp));
+    fr->speed = speed;
+    fr->speed_eff = eff;
+}
+
+__global__ __launch_bounds__(kTB) void k_apply(const Params P, const Frame *fr,
+                                               const float4 *__restrict__ pos_in,
+                                               float4 *__restrict__ pos_out,
+                                               const float4 *__restrict__ force,
+                                               float4 *__restrict__ old_force,
+                                               const uint32_t *__restrict__ deg) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const float speed = fr->speed;
+    const float4 f = force[i];
+    const float4 o = old_force[i];
+    const float deg_i = (float)deg[i] + 1.0f;
+    const float swinging = deg_i * len3(o.x - f.x, o.y - f.y, o.z - f.z);
+    float factor = speed / (1.0f + sqrtf(speed * swinging));
+    const float df = len3(f.x, f.y, f.z);
+    if (P.max_displacement > 0.0f && df > 1.0e-9f) factor = fminf(factor, P.max_displacement / df);
+    const float k = factor * P.time_step;
+    const float4 p = pos_in[i];
+    pos_out[i] = make_float4(p.x + f.x * k, p.y + f.y * k, p.z + f.z * k, 0.0f);
+    old_force[i] = make_float4(f.x, f.y, f.z, 0.0f);
+}
+
+__global__ void k_set_speed(Frame *fr, float speed, float eff) {
+    if (threadIdx.x == 0) {
+        fr->speed = speed;
+        fr->speed_eff = eff;
+    }
+}
+
+inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------- engine
+enum Phase { PH_BBOX, PH_KEYS, PH_SORT, PH_SCAN, PH_EMIT, PH_COM, PH_FORCE, PH_SPEED, PH_APPLY, PH_N };
+const char *kPhaseNames[PH_N] = {"bbox", "keys", "sort", "count_scan", "emit",
+                                 "com", "force", "speed", "apply"};
+
+class Fa2BhEngine final : public LayoutEngine {
+  public:
+    explicit Fa2BhEngine(const jc_fa2bh_settings &s) : settings_(s) {
+        desc_.id = "fa2-bh-hip";
+        desc_.kind = "physics";
+        desc_.display_name = "ForceAtlas2 (Barnes-Hut, MI355X)";
+        desc_.description =
+            "O(n log n) ForceAtlas2 with the octree built, walked and integrated on the GPU: "
+            "same attraction, gravity, repulsion law and adaptive speed as fa2-bh.";
+        desc_.needs_edges = 1;
+        desc_.needs_cpu_positions = 1;
+        desc_.needs_gpu_positions_buffer = 0;
+    }
+    ~Fa2BhEngine() override { teardown(); }
+
+    const jc_descriptor &descriptor() const override { return desc_; }
+
+    Status set_params(const char *json, size_t len) override {
+        jc_fa2bh_settings s;
+        jc_fa2bh_default_settings(&s);
+        bool strong = false, linlog = false, overlap = false;
+        std::vector<JsonField> f = {
+            {JsonField::F32, "gravity", &s.gravity},
+            {JsonField::BOOL, "strong_gravity", &strong},
+            {JsonField::F32, "scaling_ratio", &s.scaling_ratio},
+            {JsonField::F32, "edge_weight_influence", &s.edge_weight_influence},
+            {JsonField::F32, "jitter_tolerance", &s.jitter_tolerance},
+            {JsonField::BOOL, "lin_log_mode", &linlog},
+            {JsonField::BOOL, "prevent_overlap", &overlap},
+            {JsonField::F32, "time_step", &s.time_step},
+            {JsonField::F32, "theta", &s.theta},
+            {JsonField::F32, "max_displacement", &s.max_displacement},
+        };
+        int rc = decode_json_object(json, len, f, "fa2-bh settings");
+        if (rc < 0) return Status{JC_E_INVALID};
+        if (rc == 1) return Status::ok();
+        s.strong_gravity = strong;
+        s.lin_log_mode = linlog;
+        s.prevent_overlap = overlap;
+        settings_ = s;
+        params_dirty_ = true;
+        return Status::ok();
+    }
+
+    Status init(const CsrView &g, const float *xyz) override {
+        int dev = 0;
+        JC_TRY(require_device(&dev));
+        JC_TRY(validate_csr(g));
+        if (g.n && !xyz) return Status::err(JC_E_INVALID, "positions pointer is NULL");
+        if (g.n >= 0x7FFFFFFFu) return Status::err(JC_E_INVALID, "too many nodes");
+        teardown();
+        n_ = g.n;
+        // fa2_bh.rs:264-294: degrees, undirected edge list (v<u), attraction order
+        std::vector<uint32_t> degs(n_ ? n_ : 1, 0), aoff(n_ + 1, 0), adj;
+        {
+            std::vector<uint64_t> cnt((size_t)n_ + 1, 0);
+            for (uint32_t v = 0; v < n_; v++)
+                for (uint64_t e = g.offsets[v]; e < g.offsets[v + 1]; e++) {
+                    uint32_t u = g.neighbors[e];
+                    if (u == v) continue;
+                    degs[v]++;
+                    if (v < u) {
+                        cnt[v + 1]++;
+                        cnt[u + 1]++;
+                    }
+                }
+            for (uint32_t v = 0; v < n_; v++) cnt[v + 1] += cnt[v];
+            if (cnt[n_] > 0xFFFFFFFFull) return Status::err(JC_E_INVALID, "too many edges");
+            for (uint32_t v = 0; v <= n_; v++) aoff[v] = (uint32_t)cnt[v];
+            adj.resize(cnt[n_] ? cnt[n_] : 1);
+            for (uint32_t v = 0; v < n_; v++)
+                for (uint64_t e = g.offsets[v]; e < g.offsets[v + 1]; e++) {
+                    uint32_t u = g.neighbors[e];
+                    if (u == v || !(v < u)) continue;
+                    adj[cnt[v]++] = u;
+                    adj[cnt[u]++] = v;
+                }
+            n_att_ = aoff[n_];
+        }
+        JC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        const uint32_t nn = n_ ? n_ : 1;
+        JC_TRY(pos_[0].alloc(nn));
+        JC_TRY(pos_[1].alloc(nn));
+        JC_TRY(old_force_.alloc(nn));
+        JC_TRY(force_.alloc(nn));
+        JC_TRY(deg_.alloc(nn));
+        JC_TRY(att_off_.alloc((size_t)n_ + 1));
+        JC_TRY(att_adj_.alloc(adj.size()));
+        JC_TRY(keys_.alloc(nn));
+        JC_TRY(vals_.alloc(nn));
+        JC_TRY(sort_.alloc(n_));
+        nb_scan_ = cdiv(nn, kScanTile);
+        JC_TRY(loc_.alloc(nn));
+        JC_TRY(blk_.alloc(nb_scan_ + 1));
+        // Internal cells at one depth are disjoint and hold >= 2 bodies each:
+        // at most n/2 per depth over 22 depths, plus the n leaves.
+        cap_ = nn + (uint64_t)(kLevels + 1) * (nn / 2 + 1) + 8;
+        JC_TRY(node_a_.alloc(cap_));
+        JC_TRY(node_b_.alloc(cap_));
+        JC_TRY(node_depth_.alloc(cap_));
+        nb_force_ = cdiv(nn, kTB);
+        JC_TRY(partial_.alloc((size_t)nb_force_ * 2 * kLimbs));
+        JC_TRY(frame_.alloc(1));
+        std::vector<float4> p4(nn, make_float4(0, 0, 0, 0));
+        for (uint32_t i = 0; i < n_; i++)
+            p4[i] = make_float4(xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2],
+                                0.0f);
+        JC_HIP(hipMemcpyAsync(pos_[0].p, p4.data(), nn * sizeof(float4), hipMemcpyHostToDevice,
+                              stream_));
+        JC_HIP(hipMemcpyAsync(pos_[1].p, p4.data(), nn * sizeof(float4), hipMemcpyHostToDevice,
+                              stream_));
+        JC_HIP(hipMemsetAsync(old_force_.p, 0, nn * sizeof(float4), stream_));
+        JC_HIP(hipMemsetAsync(force_.p, 0, nn * sizeof(float4), stream_));
+        JC_HIP(hipMemcpyAsync(deg_.p, degs.data(), nn * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              stream_));
+        JC_HIP(hipMemcpyAsync(att_off_.p, aoff.data(), ((size_t)n_ + 1) * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, stream_));
+        JC_HIP(hipMemcpyAsync(att_adj_.p, adj.data(), adj.size() * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, stream_));
+        JC_HIP(hipMemsetAsync(frame_.p, 0, sizeof(Frame), stream_));
+        k_set_speed<<<1, 64, 0, stream_>>>(frame_.p, 1.0f, 1.0f);  // fa2_speed.rs:32-39
+        JC_HIP(hipGetLastError());
+        JC_HIP(hipStreamSynchronize(stream_));
+        cur_ = 0;
+        iteration_ = 0;
+        params_dirty_ = true;
+        inited_ = true;
+        return Status::ok();
+    }
+
+    Status step(uint32_t n_iters, float *out_xyz) override {
+        if (!inited_) return Status::err(JC_E_STATE, "fa2-bh-hip step called before successful init");
+        if (n_ == 0) return Status::ok();
+        refresh_params();
+        uint32_t left = n_iters;
+        if (left >= 2) {
+            JC_TRY(ensure_graph());
+            if (cur_ == 1) {  // realign so the 2-iteration graph starts from buffer 0
+                JC_TRY(enqueue_iteration(cur_, true, true, nullptr));
+                cur_ ^= 1;
+                left--;
+            }
+            while (left >= 2) {
+                JC_HIP(hipGraphLaunch(graph_exec_, stream_));
+                left -= 2;
+            }
+        }
+        while (left > 0) {
+            JC_TRY(enqueue_iteration(cur_, true, true, nullptr));
+            cur_ ^= 1;
+            left--;
+        }
+        iteration_ += n_iters;
+        if (out_xyz) JC_TRY(read_positions(out_xyz));
+        return Status::ok();
+    }
+
+    Status sync() override {
+        if (stream_) JC_HIP(hipStreamSynchronize(stream_));
+        return Status::ok();
+    }
+
+    Status get_state(float *xyz, float *old_force, float *se2, uint64_t *it) override {
+        if (!inited_) return Status::err(JC_E_STATE, "get_state before init");
+        if (xyz) JC_TRY(read_positions(xyz));
+        if (old_force) JC_TRY(read_vec4(old_force_.p, old_force));
+        Frame fr;
+        JC_HIP(hipMemcpyAsync(&fr, frame_.p, sizeof(Frame), hipMemcpyDeviceToHost, stream_));
+        JC_HIP(hipStreamSynchronize(stream_));
+        if (se2) {
+            se2[0] = fr.speed;
+            se2[1] = fr.speed_eff;
+        }
+        if (it) *it = iteration_;
+        return Status::ok();
+    }
+
+    Status set_state(const float *xyz, const float *old_force, const float *se2,
+                     uint64_t it) override {
+        if (!inited_) return Status::err(JC_E_STATE, "set_state before init");
+        if (xyz) JC_TRY(write_vec4(pos_[cur_].p, xyz));
+        if (old_force) JC_TRY(write_vec4(old_force_.p, old_force));
+        if (se2) {
+            k_set_speed<<<1, 64, 0, stream_>>>(frame_.p, se2[0], se2[1]);
+            JC_HIP(hipGetLastError());
+        }
+        iteration_ = it;
+        JC_HIP(hipStreamSynchronize(stream_));
+        return Status::ok();
+    }
+
+    // ---- parity / measurement hooks ----
+    Status forces_at(const float *xyz, float *out_force, double *sums2) {
+        if (!inited_) return Status::err(JC_E_STATE, "forces_at before init");
+        if (n_ == 0) return Status::ok();
+        refresh_params();
+        DevBuf<float4> tmp;
+        JC_TRY(tmp.alloc(n_));
+        JC_TRY(write_vec4(tmp.p, xyz));
+        JC_TRY(enqueue_tree(tmp.p, nullptr));
+        JC_TRY(enqueue_force(tmp.p, nullptr));
+        k_speed<<<1, 256, 0, stream_>>>(params_, frame_.p, partial_.p, nb_force_, 0u);
+        JC_HIP(hipGetLastError());
+        if (out_force) JC_TRY(read_vec4(force_.p, out_force));
+        Frame fr;
+        JC_HIP(hipMemcpyAsync(&fr, frame_.p, sizeof(Frame), hipMemcpyDeviceToHost, stream_));
+        JC_HIP(hipStreamSynchronize(stream_));
+        if (sums2) {
+            sums2[0] = fr.sum_swing;
+            sums2[1] = fr.sum_traction;
+        }
+        return Status::ok();
+    }
+
+    Status tree(uint32_t cap, uint32_t *n_tree, uint32_t *n_valid, uint64_t *keys, uint32_t *perm,
+                float *node_a, uint32_t *info, uint32_t *skip, uint8_t *depth, float *root4) {
+        if (!inited_) return Status::err(JC_E_STATE, "tree before init");
+        Frame fr;
+        JC_HIP(hipMemcpyAsync(&fr, frame_.p, sizeof(Frame), hipMemcpyDeviceToHost, stream_));
+        JC_HIP(hipStreamSynchronize(stream_));
+        if (n_tree) *n_tree = fr.n_tree;
+        if (n_valid) *n_valid = n_ - fr.n_invalid;
+        if (root4) std::memcpy(root4, fr.root, sizeof fr.root);
+        if (n_ == 0) return Status::ok();
+        if (keys) JC_HIP(hipMemcpy(keys, keys_.p, n_ * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        if (perm) JC_HIP(hipMemcpy(perm, vals_.p, n_ * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        const uint32_t m = std::min<uint32_t>(cap, fr.n_tree);
+        if (m) {
+            if (node_a)
+                JC_HIP(hipMemcpy(node_a, node_a_.p, m * sizeof(float4), hipMemcpyDeviceToHost));
+            if (info || skip) {
+                std::vector<uint2> b(m);
+                JC_HIP(hipMemcpy(b.data(), node_b_.p, m * sizeof(uint2), hipMemcpyDeviceToHost));
+                for (uint32_t q = 0; q < m; q++) {
+                    if (info) info[q] = b[q].x;
+                    if (skip) skip[q] = b[q].y;
+                }
+            }
+            if (depth) JC_HIP(hipMemcpy(depth, node_depth_.p, m, hipMemcpyDeviceToHost));
+        }
+        return Status::ok();
+    }
+
+    // Timed iterations: the tree build of each iteration is replayed from a
+    // captured graph, and the force kernel is launched between two events on
+    // the engine stream, so the events time exactly that kernel; the host reads
+    // them back once per kTimedIters iterations.
+    Status step_timed(uint32_t n_iters, double *force_ms, double *iter_ms) {
+        if (!inited_) return Status::err(JC_E_STATE, "step_timed before init");
+        refresh_params();
+        double fsum = 0, isum = 0;
+        if (n_ && n_iters) {
+            JC_TRY(ensure_tree_graphs());
+            for (auto &e : tev_)
+                if (!e) JC_HIP(hipEventCreate(&e));
+            uint32_t done = 0;
+            while (done < n_iters) {
+                const uint32_t chunk = std::min<uint32_t>(kTimedIters, n_iters - done);
+                JC_HIP(hipEventRecord(tev_[2 * kTimedIters], stream_));
+                for (uint32_t k = 0; k < chunk; k++) {
+                    const uint32_t c = cur_;
+                    JC_HIP(hipGraphLaunch(tree_exec_[c], stream_));
+                    JC_HIP(hipEventRecord(tev_[2 * k], stream_));
+                    JC_TRY(enqueue_force(pos_[c].p, nullptr));
+                    JC_HIP(hipEventRecord(tev_[2 * k + 1], stream_));
+                    k_speed<<<1, 256, 0, stream_>>>(params_, frame_.p, partial_.p, nb_force_, 1u);
+                    k_apply<<<cdiv(n_, kTB), kTB, 0, stream_>>>(params_, frame_.p, pos_[c].p,
+                                                                 pos_[c ^ 1].p, force_.p,
+                                                                 old_force_.p, deg_.p);
+                    JC_HIP(hipGetLastError());
+                    cur_ ^= 1;
+                }
+                JC_HIP(hipEventRecord(tev_[2 * kTimedIters + 1], stream_));
+                JC_HIP(hipEventSynchronize(tev_[2 * kTimedIters + 1]));
+                for (uint32_t k = 0; k < chunk; k++) {
+                    float a;
+                    JC_HIP(hipEventElapsedTime(&a, tev_[2 * k], tev_[2 * k + 1]));
+                    fsum += (double)a;
+                }
+                float c;
+                JC_HIP(hipEventElapsedTime(&c, tev_[2 * kTimedIters], tev_[2 * kTimedIters + 1]));
+                isum += (double)c;
+                done += chunk;
+            }
+            iteration_ += n_iters;
+        }
+        if (force_ms) *force_ms = fsum;
+        if (iter_ms) *iter_ms = isum;
+        return Status::ok();
+    }
+
+    Status last_stats(uint64_t *n_tree, uint64_t *interactions, double *sums2) {
+        if (!inited_) return Status::err(JC_E_STATE, "last_stats before init");
+        Frame fr;
+        JC_HIP(hipMemcpyAsync(&fr, frame_.p, sizeof(Frame), hipMemcpyDeviceToHost, stream_));
+        JC_HIP(hipStreamSynchronize(stream_));
+        if (n_tree) *n_tree = fr.n_tree;
+        if (interactions) *interactions = fr.interactions;
+        if (sums2) {
+            sums2[0] = fr.sum_swing;
+            sums2[1] = fr.sum_traction;
+        }
+        return Status::ok();
+    }
+
+    Status profile(uint32_t n_iters, double *phase_ms, uint32_t n_phases) {
+        if (!inited_) return Status::err(JC_E_STATE, "profile before init");
+        refresh_params();
+        std::vector<hipEvent_t> ev(PH_N + 1);
+        for (auto &e : ev) JC_HIP(hipEventCreate(&e));
+        std::vector<double> acc(PH_N, 0.0);
+        for (uint32_t it = 0; it < n_iters; it++) {
+            JC_TRY(enqueue_iteration(cur_, true, true, ev.data()));
+            cur_ ^= 1;
+            iteration_++;
+            JC_HIP(hipEventSynchronize(ev[PH_N]));
+            for (int k = 0; k < PH_N; k++) {
+                float ms;
+                JC_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+                acc[k] += ms;
+            }
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        for (uint32_t k = 0; k < n_phases && k < PH_N; k++)
+            phase_ms[k] = n_iters ? acc[k] / n_iters : 0.0;
+        return Status::ok();
+    }
+
+  private:
+    void refresh_params() {
+        if (!params_dirty_) return;
+        const jc_fa2bh_settings &s = settings_;
+        params_.n = n_;
+        params_.gravity = s.gravity;
+        params_.strong_gravity = s.strong_gravity ? 1u : 0u;
+        params_.scaling_ratio = s.scaling_ratio;
+        params_.jitter_tolerance = s.jitter_tolerance;
+        params_.lin_log_mode = s.lin_log_mode ? 1u : 0u;
+        params_.time_step = s.time_step;
+        const float th = std::fmax(s.theta, 0.0f);  // fa2_bh.rs:716
+        params_.theta2 = th * th;
+        params_.max_displacement = s.max_displacement;
+        params_.ewi_weight = std::pow(std::fmax(1.0f, 1.0e-6f), s.edge_weight_influence);
+        params_.est_jt = 0.05 * std::sqrt((double)n_);
+        params_.min_jt = std::sqrt(params_.est_jt);
+        params_.cap = (uint32_t)cap_;
+        params_dirty_ = false;
+        drop_graphs();  // kernel arguments changed
+    }
+
+    Status read_vec4(const float4 *src, float *dst) {
+        std::vector<float4> h(n_);
+        JC_HIP(hipMemcpyAsync(h.data(), src, n_ * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+        JC_HIP(hipStreamSynchronize(stream_));
+        for (uint32_t i = 0; i < n_; i++) {
+            dst[3 * (size_t)i] = h[i].x;
+            dst[3 * (size_t)i + 1] = h[i].y;
+            dst[3 * (size_t)i + 2] = h[i].z;
+        }
+        return Status::ok();
+    }
+    Status write_vec4(float4 *dst, const float *src) {
+        std::vector<float4> h(n_);
+        for (uint32_t i = 0; i < n_; i++)
+            h[i] = make_float4(src[3 * (size_t)i], src[3 * (size_t)i + 1], src[3 * (size_t)i + 2], 0.0f);
+        JC_HIP(hipMemcpyAsync(dst, h.data(), n_ * sizeof(float4), hipMemcpyHostToDevice, stream_));
+        JC_HIP(hipStreamSynchronize(stream_));
+        return Status::ok();
+    }
+    Status read_positions(float *out) { return read_vec4(pos_[cur_].p, out); }
+
+    Status enqueue_tree(const float4 *pos, hipEvent_t *ev) {
+        const uint32_t n = n_;
+        if (ev) JC_HIP(hipEventRecord(ev[PH_BBOX], stream_));
+        k_frame_reset<<<1, 64, 0, stream_>>>(frame_.p);
+        k_bbox<<<std::min<uint32_t>(cdiv(n, kTB), 1024), kTB, 0, stream_>>>(pos, n, frame_.p);
+        if (ev) JC_HIP(hipEventRecord(ev[PH_KEYS], stream_));
+        k_keys<<<cdiv(n, kTB), kTB, 0, stream_>>>(pos, n, frame_.p, keys_.p, vals_.p);
+        if (ev) JC_HIP(hipEventRecord(ev[PH_SORT], stream_));
+        JC_TRY(radix_sort_pairs_u64(keys_.p, vals_.p, sort_, stream_));
+        if (ev) JC_HIP(hipEventRecord(ev[PH_SCAN], stream_));
+        k_count_scan<<<nb_scan_, kTB, 0, stream_>>>(keys_.p, n, frame_.p, loc_.p, blk_.p);
+        k_scan_top<<<1, 1024, 0, stream_>>>(blk_.p, nb_scan_, n, frame_.p);
+        if (ev) JC_HIP(hipEventRecord(ev[PH_EMIT], stream_));
+        k_emit<<<cdiv(n, kTB), kTB, 0, stream_>>>(keys_.p, vals_.p, pos, deg_.p, n, frame_.p,
+                                                  loc_.p, blk_.p, node_a_.p, node_b_.p,
+                                                  node_depth_.p, cap_);
+        if (ev) JC_HIP(hipEventRecord(ev[PH_COM], stream_));
+        const uint32_t gcom = std::min<uint32_t>(cdiv(cap_, kTB), 2048);
+        for (int d = kLevels; d >= 0; d--)
+            k_com_level<<<gcom, kTB, 0, stream_>>>(frame_.p, (uint32_t)d, (uint32_t)cap_,
+                                                   node_a_.p, node_b_.p, node_depth_.p);
+        JC_HIP(hipGetLastError());
+        return Status::ok();
+    }
+
+    Status enqueue_force(const float4 *pos, hipEvent_t *ev) {
+        if (ev) JC_HIP(hipEventRecord(ev[PH_FORCE], stream_));
+        k_force<<<nb_force_, kTB, 0, stream_>>>(params_, frame_.p, pos, old_force_.p, force_.p,
+                                                deg_.p, att_off_.p, att_adj_.p, vals_.p,
+                                                node_a_.p, node_b_.p, partial_.p);
+        JC_HIP(hipGetLastError());
+        return Status::ok();
+    }
+
+    // One full iteration reading pos_[c], writing pos_[c^1].
+    Status enqueue_iteration(uint32_t c, bool speed, bool apply, hipEvent_t *ev) {
+        JC_TRY(enqueue_tree(pos_[c].p, ev));
+        JC_TRY(enqueue_force(pos_[c].p, ev));
+        if (ev) JC_HIP(hipEventRecord(ev[PH_SPEED], stream_));
+        if (speed)
+            k_speed<<<1, 256, 0, stream_>>>(params_, frame_.p, partial_.p, nb_force_, 1u);
+        if (ev) JC_HIP(hipEventRecord(ev[PH_APPLY], stream_));
+        if (apply)
+            k_apply<<<cdiv(n_, kTB), kTB, 0, stream_>>>(params_, frame_.p, pos_[c].p,
+                                                         pos_[c ^ 1].p, force_.p, old_force_.p,
+                                                         deg_.p);
+        if (ev) JC_HIP(hipEventRecord(ev[PH_N], stream_));
+        JC_HIP(hipGetLastError());
+        return Status::ok();
+    }
+
+    Status ensure_graph() {
+        if (graph_exec_) return Status::ok();
+        JC_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        Status s1 = enqueue_iteration(0, true, true, nullptr);
+        Status s2 = s1.failed() ? s1 : enqueue_iteration(1, true, true, nullptr);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(stream_, &g);
+        if (s2.failed()) return s2;
+        if (e != hipSuccess) return Status::err(JC_E_HIP, std::string("capture: ") + hipGetErrorString(e));
+        graph_ = g;
+        JC_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+        return Status::ok();
+    }
+
+    // The tree build alone, captured once per ping-pong buffer.
+    Status ensure_tree_graphs() {
+        for (uint32_t c = 0; c < 2; c++) {
+            if (tree_exec_[c]) continue;
+            JC_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            Status s = enqueue_tree(pos_[c].p, nullptr);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(stream_, &g);
+            if (s.failed()) return s;
+            if (e != hipSuccess)
+                return Status::err(JC_E_HIP, std::string("capture: ") + hipGetErrorString(e));
+            tree_graph_[c] = g;
+            JC_HIP(hipGraphInstantiate(&tree_exec_[c], g, nullptr, nullptr, 0));
+        }
+        return Status::ok();
+    }
+
+    void drop_graphs() {
+        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+        graph_exec_ = nullptr;
+        graph_ = nullptr;
+        for (uint32_t c = 0; c < 2; c++) {
+            if (tree_exec_[c]) (void)hipGraphExecDestroy(tree_exec_[c]);
+            if (tree_graph_[c]) (void)hipGraphDestroy(tree_graph_[c]);
+            tree_exec_[c] = nullptr;
+            tree_graph_[c] = nullptr;
+        }
+    }
+
+    void teardown() {
+        if (stream_) (void)hipStreamSynchronize(stream_);
+        drop_graphs();
+        for (auto &e : tev_)
+            if (e) {
+                (void)hipEventDestroy(e);
+                e = nullptr;
+            }
+        pos_[0].release();
+        pos_[1].release();
+        old_force_.release();
+        force_.release();
+        deg_.release();
+        att_off_.release();
+        att_adj_.release();
+        keys_.release();
+        vals_.release();
+        loc_.release();
+        blk_.release();
+        node_a_.release();
+        node_b_.release();
+        node_depth_.release();
+        partial_.release();
+        frame_.release();
+        if (stream_) (void)hipStreamDestroy(stream_);
+        stream_ = nullptr;
+        inited_ = false;
+    }
+
+    jc_descriptor desc_{};
+    jc_fa2bh_settings settings_;
+    Params params_{};
+    bool params_dirty_ = true;
+    bool inited_ = false;
+    uint32_t n_ = 0;
+    uint64_t n_att_ = 0;
+    uint32_t cur_ = 0;
+    uint64_t iteration_ = 0;
+    uint64_t cap_ = 0;
+    uint32_t nb_scan_ = 0, nb_force_ = 0;
+    hipStream_t stream_ = nullptr;
+    DevBuf<float4> pos_[2], old_force_, force_;
+    DevBuf<uint32_t> deg_, att_off_, att_adj_;
+    DevBuf<uint64_t> keys_;
+    DevBuf<uint32_t> vals_, loc_, blk_;
+    RadixScratch sort_;
+    DevBuf<float4> node_a_;
+    DevBuf<uint2> node_b_;
+    DevBuf<uint8_t> node_depth_;
+    DevBuf<unsigned long long> partial_;
+    DevBuf<Frame> frame_;
+    hipGraph_t graph_ = nullptr, tree_graph_[2] = {};
+    hipGraphExec_t graph_exec_ = nullptr, tree_exec_[2] = {};
+    hipEvent_t tev_[2 * kTimedIters + 2] = {};
+};
+
+}  // namespace
+
+LayoutEngine *make_fa2bh(const jc_fa2bh_settings &s) { return new Fa2BhEngine(s); }
+bool is_fa2bh(const LayoutEngine *e) { return dynamic_cast<const Fa2BhEngine *>(e) != nullptr; }
+
+Status fa2bh_forces_at(LayoutEngine *e, const float *xyz, float *out_force, double *sums2) {
+    return static_cast<Fa2BhEngine *>(e)->forces_at(xyz, out_force, sums2);
+}
+Status fa2bh_tree(LayoutEngine *e, uint32_t cap, uint32_t *n_tree, uint32_t *n_valid,
+                  uint64_t *keys, uint32_t *perm, float *node_a, uint32_t *node_info,
+                  uint32_t *node_skip, uint8_t *node_depth, float *root4) {
+    return static_cast<Fa2BhEngine *>(e)->tree(cap, n_tree, n_valid, keys, perm, node_a, node_info,
+                                               node_skip, node_depth, root4);
+}
+Status fa2bh_step_timed(LayoutEngine *e, uint32_t n_iters, double *force_ms, double *iter_ms) {
+    return static_cast<Fa2BhEngine *>(e)->step_timed(n_iters, force_ms, iter_ms);
+}
+Status fa2bh_last_stats(LayoutEngine *e, uint64_t *n_tree, uint64_t *interactions, double *sums2) {
+    return static_cast<Fa2BhEngine *>(e)->last_stats(n_tree, interactions, sums2);
+}
+Status fa2bh_profile(LayoutEngine *e, uint32_t n_iters, double *phase_ms, uint32_t n_phases) {
+    return static_cast<Fa2BhEngine *>(e)->profile(n_iters, phase_ms, n_phases);
+}
+
+}  // namespace jc
+
+extern "C" const char *jc_fa2bh_phase_name(uint32_t k) {
+    return k < jc::PH_N ? jc::kPhaseNames[k] : "";
+}
+extern "C" uint32_t jc_fa2bh_phase_count(void) { return jc::PH_N; }

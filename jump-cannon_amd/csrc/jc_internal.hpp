@@ -1,0 +1,91 @@
+// Internal helpers shared by the HIP translation units of libjc_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/jc.h"
+
+namespace jc {
+
+// Thread-local last error message returned by jc_last_error().
+void set_error(const std::string &msg);
+const char *last_error();
+
+struct Status {
+    int code = JC_OK;
+    static Status ok() { return {}; }
+    static Status err(int c, const std::string &m) {
+        set_error(m);
+        Status s;
+        s.code = c;
+        return s;
+    }
+    bool failed() const { return code != JC_OK; }
+};
+
+#define JC_HIP(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return ::jc::Status::err(JC_E_HIP, std::string(#expr " failed: ") +             \
+                                                   hipGetErrorString(e_));                  \
+    } while (0)
+
+#define JC_TRY(expr)                                                                        \
+    do {                                                                                    \
+        ::jc::Status s_ = (expr);                                                           \
+        if (s_.failed()) return s_;                                                         \
+    } while (0)
+
+// Returns JC_E_NODEVICE (the reference's "requires a GPU" error) when no HIP
+// device is visible; selects device `dev` otherwise.
+Status require_device(int *dev_out);
+
+// Owning device allocation.
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    Status alloc(size_t count) {
+        release();
+        size_t bytes = (count ? count : 1) * sizeof(T);
+        hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), bytes);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return Status::err(JC_E_OOM, std::string("hipMalloc of ") + std::to_string(bytes) +
+                                             " bytes failed: " + hipGetErrorString(e));
+        }
+        n = count;
+        return Status::ok();
+    }
+};
+
+// ---- device-wide primitives (prims.hip) ----
+// Stable LSD radix sort of (key,val) pairs by all 64 key bits.  `n` is fixed for
+// a given scratch set so the launches can be captured into a hipGraph.
+struct RadixScratch {
+    DevBuf<uint64_t> keys_alt;
+    DevBuf<uint32_t> vals_alt;
+    DevBuf<uint32_t> hist;   // 256 * n_blocks
+    uint32_t n = 0, n_blocks = 0;
+    Status alloc(uint32_t n);
+};
+Status radix_sort_pairs_u64(uint64_t *keys, uint32_t *vals, RadixScratch &s, hipStream_t st);
+
+constexpr uint32_t kSortThreads = 256;
+constexpr uint32_t kSortItems = 8;
+constexpr uint32_t kSortTile = kSortThreads * kSortItems;
+
+}  // namespace jc

@@ -1,0 +1,154 @@
+// Device-wide primitives for gfx950: stable LSD radix sort of 64-bit keys with
+// 32-bit payloads.  Each 8-bit pass is histogram -> one-block scan -> stable
+// scatter.  Ranking inside a tile is wave-level (64 lanes): the lanes holding
+// the same digit are found with eight ballots, the lowest of them reserves a
+// run with one LDS atomic and broadcasts it, so equal digits keep their input
+// order (stability) without any block barrier between rounds.
+#include "jc_internal.hpp"
+
+namespace jc {
+
+namespace {
+
+__global__ __launch_bounds__(kSortThreads) void k_rs_hist(const uint64_t *__restrict__ keys,
+                                                          uint32_t n, uint32_t shift,
+                                                          uint32_t *__restrict__ hist,
+                                                          uint32_t nb) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kSortTile;
+#pragma unroll
+    for (uint32_t j = 0; j < kSortItems; j++) {
+        uint32_t i = base + j * kSortThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+__device__ inline uint32_t wave_incl_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan of m entries in place, one block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_scan_inplace(uint32_t *__restrict__ a, uint32_t m) {
+    __shared__ uint32_t ws[16];
+    const uint32_t per = (m + 1023u) / 1024u;
+    const uint32_t b0 = min(threadIdx.x * per, m), b1 = min(b0 + per, m);
+    uint32_t s = 0;
+    for (uint32_t i = b0; i < b1; i++) s += a[i];
+    uint32_t incl = wave_incl_scan(s);
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 63) ws[w] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int k = 0; k < 16; k++) {
+            uint32_t t = ws[k];
+            ws[k] = run;
+            run += t;
+        }
+    }
+    __syncthreads();
+    uint32_t run = incl - s + ws[w];
+    for (uint32_t i = b0; i < b1; i++) {
+        uint32_t v = a[i];
+        a[i] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_rs_scatter(
+    const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+    uint64_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t n, uint32_t shift,
+    const uint32_t *__restrict__ hist, uint32_t nb) {
+    constexpr uint32_t W = kSortThreads / 64;
+    __shared__ uint32_t wcnt[W][256];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+#pragma unroll
+    for (uint32_t q = 0; q < W; q++) wcnt[q][tid] = 0;
+    __syncthreads();
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const uint32_t base = blockIdx.x * kSortTile + w * (64 * kSortItems);
+    uint64_t k[kSortItems];
+    uint32_t v[kSortItems], rank[kSortItems];
+#pragma unroll
+    for (uint32_t r = 0; r < kSortItems; r++) {
+        const uint32_t i = base + r * 64 + lane;
+        const bool valid = i < n;
+        k[r] = valid ? kin[i] : 0ull;
+        v[r] = valid ? vin[i] : 0u;
+        const uint32_t d = (uint32_t)(k[r] >> shift) & 255u;
+        uint64_t mask = __ballot(valid);
+#pragma unroll
+        for (uint32_t b = 0; b < 8; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bb = __ballot(bit);
+            mask &= bit ? bb : ~bb;
+        }
+        const uint32_t before = (uint32_t)__popcll(mask & lt);
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+        uint32_t old = 0;
+        if (valid && before == 0) old = atomicAdd(&wcnt[w][d], (uint32_t)__popcll(mask));
+        old = __shfl(old, (int)leader, 64);
+        rank[r] = old + before;
+    }
+    __syncthreads();
+    {
+        uint32_t run = hist[tid * nb + blockIdx.x];
+#pragma unroll
+        for (uint32_t q = 0; q < W; q++) {
+            uint32_t c = wcnt[q][tid];
+            wcnt[q][tid] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < kSortItems; r++) {
+        const uint32_t i = base + r * 64 + lane;
+        if (i < n) {
+            const uint32_t d = (uint32_t)(k[r] >> shift) & 255u;
+            const uint32_t dst = wcnt[w][d] + rank[r];
+            kout[dst] = k[r];
+            vout[dst] = v[r];
+        }
+    }
+}
+
+}  // namespace
+
+Status RadixScratch::alloc(uint32_t count) {
+    n = count;
+    n_blocks = (count + kSortTile - 1) / kSortTile;
+    if (n_blocks == 0) n_blocks = 1;
+    JC_TRY(keys_alt.alloc(count));
+    JC_TRY(vals_alt.alloc(count));
+    JC_TRY(hist.alloc(256ull * n_blocks));
+    return Status::ok();
+}
+
+Status radix_sort_pairs_u64(uint64_t *keys, uint32_t *vals, RadixScratch &s, hipStream_t st) {
+    if (s.n == 0) return Status::ok();
+    uint64_t *ka = keys, *kb = s.keys_alt.p;
+    uint32_t *va = vals, *vb = s.vals_alt.p;
+    for (uint32_t shift = 0; shift < 64; shift += 8) {
+        k_rs_hist<<<s.n_blocks, kSortThreads, 0, st>>>(ka, s.n, shift, s.hist.p, s.n_blocks);
+        k_scan_inplace<<<1, 1024, 0, st>>>(s.hist.p, 256u * s.n_blocks);
+        k_rs_scatter<<<s.n_blocks, kSortThreads, 0, st>>>(ka, va, kb, vb, s.n, shift, s.hist.p,
+                                                          s.n_blocks);
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    JC_HIP(hipGetLastError());
+    return Status::ok();
+}
+
+}  // namespace jc

@@ -1,0 +1,374 @@
+"""Python front-end of libjc_hip.so (the MI355X layout / graph-compute engines).
+
+Mirrors the reference's engine API (crates/graph-compute/src/engines/mod.rs:383-444):
+``set_params`` -> ``init`` -> ``step`` per tick, plus the analytics free
+functions (``pagerank``).  Everything runs through the C ABI declared in
+include/jc.h; there is no CPU fallback -- a missing library or device raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(PKG_DIR), "libjc_hip.so")
+
+JC_OK = 0
+JC_E_NODEVICE = -1
+JC_E_INVALID = -2
+JC_E_OOM = -3
+JC_E_HIP = -4
+JC_E_STATE = -5
+JC_E_DANGLING = -6
+
+_ERRNAMES = {JC_E_NODEVICE: "NODEVICE", JC_E_INVALID: "INVALID", JC_E_OOM: "OOM",
+             JC_E_HIP: "HIP", JC_E_STATE: "STATE", JC_E_DANGLING: "DANGLING"}
+
+
+class JcError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_ERRNAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class NoDeviceError(JcError):
+    pass
+
+
+class Fa2BhSettings(C.Structure):
+    """Field-for-field Fa2BhSettings (crates/graph-compute/src/engines/fa2_bh.rs:85-125)."""
+
+    _fields_ = [("gravity", C.c_float), ("strong_gravity", C.c_uint32),
+                ("scaling_ratio", C.c_float), ("edge_weight_influence", C.c_float),
+                ("jitter_tolerance", C.c_float), ("lin_log_mode", C.c_uint32),
+                ("prevent_overlap", C.c_uint32), ("time_step", C.c_float),
+                ("theta", C.c_float), ("max_displacement", C.c_float)]
+
+
+class SgdSettings(C.Structure):
+    """Field-for-field SgdStressSettings (crates/graph-compute/src/engines/sgd_stress.rs:59-98)."""
+
+    _fields_ = [("n_pivots", C.c_uint32), ("sweeps_per_step", C.c_uint32),
+                ("eta_max", C.c_float), ("eta_min", C.c_float),
+                ("n_anneal_steps", C.c_uint32), ("reserved", C.c_uint32), ("seed", C.c_uint64)]
+
+
+class Descriptor(C.Structure):
+    _fields_ = [("id", C.c_char_p), ("kind", C.c_char_p), ("display_name", C.c_char_p),
+                ("description", C.c_char_p), ("needs_edges", C.c_uint32),
+                ("needs_cpu_positions", C.c_uint32), ("needs_gpu_positions_buffer", C.c_uint32)]
+
+
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+# Symbol -> (restype, argtypes).  Pointer arguments are c_void_p so that None
+# (NULL) and numpy arrays (via .ctypes.data) are both accepted.
+SIGNATURES = {
+    "jc_last_error": (C.c_char_p, []),
+    "jc_device_count": (C.c_int, []),
+    "jc_fa2bh_default_settings": (None, [_vp]),
+    "jc_fa2bh_create": (C.c_int, [_vp, _vp]),
+    "jc_sgd_default_settings": (None, [_vp]),
+    "jc_sgd_create": (C.c_int, [_vp, _vp]),
+    "jc_engine_descriptor": (C.c_int, [_vp, _vp]),
+    "jc_engine_set_params_json": (C.c_int, [_vp, C.c_char_p, C.c_size_t]),
+    "jc_engine_init": (C.c_int, [_vp, C.c_uint32, _vp, _vp, C.c_uint64, _vp]),
+    "jc_engine_reinit": (C.c_int, [_vp, C.c_uint32, _vp, _vp, C.c_uint64, _vp]),
+    "jc_engine_step": (C.c_int, [_vp, C.c_uint32, _vp]),
+    "jc_engine_sync": (C.c_int, [_vp]),
+    "jc_engine_get_state": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "jc_engine_set_state": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint64]),
+    "jc_engine_is_halted": (C.c_int, [_vp]),
+    "jc_engine_destroy": (None, [_vp]),
+    "jc_fa2bh_forces_at": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "jc_fa2bh_tree": (C.c_int, [_vp, C.c_uint32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "jc_fa2bh_step_timed": (C.c_int, [_vp, C.c_uint32, _vp, _vp]),
+    "jc_fa2bh_profile": (C.c_int, [_vp, C.c_uint32, _vp, C.c_uint32]),
+    "jc_fa2bh_last_stats": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "jc_fa2bh_phase_name": (C.c_char_p, [C.c_uint32]),
+    "jc_fa2bh_phase_count": (C.c_uint32, []),
+    "jc_pagerank": (C.c_int, [C.c_uint32, _vp, _vp, C.c_uint64, C.c_float, C.c_uint32,
+                              C.c_double, _vp, _vp, _vp]),
+    "jc_gen_ba_pairs": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint64, _vp, C.c_uint64]),
+    "jc_gen_sbm_pairs": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_double, C.c_double,
+                                      C.c_uint64, _vp, C.c_uint64]),
+    "jc_gen_rgg_pairs": (C.c_uint64, [C.c_uint32, C.c_double, C.c_uint64, _vp, _vp, C.c_uint64]),
+    "jc_csr_from_pairs": (C.c_uint64, [C.c_uint32, _vp, C.c_uint64, _vp, _vp]),
+    "jc_gen_uniform_cube": (None, [C.c_uint32, C.c_float, C.c_uint64, _vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libjc_hip.so from the package tree; raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(
+            f"{LIB_PATH} is missing: build it with `python jump-cannon_amd/build.py`")
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def _check(rc: int):
+    if rc != JC_OK:
+        msg = lib().jc_last_error().decode(errors="replace")
+        if rc == JC_E_NODEVICE:
+            raise NoDeviceError(rc, msg)
+        raise JcError(rc, msg)
+
+
+def device_count() -> int:
+    return int(lib().jc_device_count())
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class LayoutEngine:
+    """One engine handle (the reference LayoutEngine trait object)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        self.n = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                lib().jc_engine_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def close(self):
+        self.__del__()
+
+    @property
+    def descriptor(self) -> dict:
+        d = Descriptor()
+        _check(lib().jc_engine_descriptor(self._h, C.addressof(d)))
+        return {"id": d.id.decode(), "kind": d.kind.decode(),
+                "display_name": d.display_name.decode(), "description": d.description.decode(),
+                "needs_edges": bool(d.needs_edges),
+                "needs_cpu_positions": bool(d.needs_cpu_positions),
+                "needs_gpu_positions_buffer": bool(d.needs_gpu_positions_buffer)}
+
+    def set_params(self, params) -> None:
+        """LayoutEngine::set_params: a dict (or JSON text); None keeps the settings."""
+        text = params if isinstance(params, str) else json.dumps(params)
+        b = text.encode()
+        _check(lib().jc_engine_set_params_json(self._h, b, len(b)))
+
+    def init(self, offsets, neighbors, xyz) -> None:
+        offsets = _u32(offsets)
+        neighbors = _u32(neighbors)
+        xyz = _f32(xyz)
+        n = len(offsets) - 1
+        if len(xyz) != 3 * n:
+            raise JcError(JC_E_INVALID,
+                          f"initial positions length {len(xyz)} != 3 * n_nodes {3 * n}")
+        self._keep = (offsets, neighbors)
+        _check(lib().jc_engine_init(self._h, n, _ptr(offsets), _ptr(neighbors) if len(neighbors) else None,
+                                    len(neighbors), _ptr(xyz) if n else None))
+        self.n = n
+
+    def reinit(self, offsets, neighbors, xyz) -> None:
+        offsets = _u32(offsets)
+        neighbors = _u32(neighbors)
+        xyz = _f32(xyz)
+        n = len(offsets) - 1
+        _check(lib().jc_engine_reinit(self._h, n, _ptr(offsets), _ptr(neighbors) if len(neighbors) else None,
+                                      len(neighbors), _ptr(xyz) if n else None))
+        self.n = n
+
+    def step(self, n_iters: int = 1, readback: bool = True):
+        """LayoutEngine::step, n_iters times.  Returns positions (3n) or None."""
+        out = np.empty(3 * self.n, np.float32) if readback else None
+        _check(lib().jc_engine_step(self._h, n_iters, _ptr(out) if readback and self.n else None))
+        return out
+
+    def sync(self):
+        _check(lib().jc_engine_sync(self._h))
+
+    def get_state(self):
+        xyz = np.empty(3 * self.n, np.float32)
+        old = np.empty(3 * self.n, np.float32)
+        se = np.empty(2, np.float32)
+        it = np.zeros(1, np.uint64)
+        _check(lib().jc_engine_get_state(self._h, _ptr(xyz), _ptr(old), _ptr(se), _ptr(it)))
+        return {"positions": xyz, "old_force": old, "speed": float(se[0]),
+                "speed_efficiency": float(se[1]), "iteration": int(it[0]), "speed_eff2": se}
+
+    def set_state(self, positions=None, old_force=None, speed_eff2=None, iteration=0):
+        p = None if positions is None else _f32(positions)
+        o = None if old_force is None else _f32(old_force)
+        s = None if speed_eff2 is None else _f32(speed_eff2)
+        _check(lib().jc_engine_set_state(self._h, _ptr(p), _ptr(o), _ptr(s), iteration))
+
+    def is_halted(self) -> bool:
+        return bool(lib().jc_engine_is_halted(self._h))
+
+
+def default_fa2bh_settings(**overrides) -> Fa2BhSettings:
+    s = Fa2BhSettings()
+    lib().jc_fa2bh_default_settings(C.addressof(s))
+    for k, v in overrides.items():
+        setattr(s, k, v)
+    return s
+
+
+class Fa2BhEngine(LayoutEngine):
+    """ForceAtlas2 + Barnes-Hut ("fa2-bh-hip"), the drop-in for fa2_bh.rs."""
+
+    ID = "fa2-bh-hip"
+
+    def __init__(self, settings: Fa2BhSettings | None = None, **overrides):
+        s = settings if settings is not None else default_fa2bh_settings()
+        for k, v in overrides.items():
+            setattr(s, k, v)
+        h = C.c_void_p()
+        _check(lib().jc_fa2bh_create(C.addressof(s), C.addressof(h)))
+        super().__init__(h.value)
+
+    def forces_at(self, xyz):
+        """One force pass at xyz: returns (F (3n) f32, [sum swing, sum traction] f64)."""
+        xyz = _f32(xyz)
+        f = np.empty(3 * self.n, np.float32)
+        sums = np.zeros(2, np.float64)
+        _check(lib().jc_fa2bh_forces_at(self._h, _ptr(xyz), _ptr(f), _ptr(sums)))
+        return f, sums
+
+    def tree(self, cap: int | None = None):
+        """The octree of the last tree build (pre-order arrays)."""
+        n = self.n
+        cap = cap if cap is not None else 23 * max(n, 1) + 8
+        nt = np.zeros(1, np.uint32)
+        nv = np.zeros(1, np.uint32)
+        keys = np.zeros(max(n, 1), np.uint64)
+        perm = np.zeros(max(n, 1), np.uint32)
+        na = np.zeros(4 * cap, np.float32)
+        info = np.zeros(cap, np.uint32)
+        skip = np.zeros(cap, np.uint32)
+        depth = np.zeros(cap, np.uint8)
+        root4 = np.zeros(4, np.float32)
+        _check(lib().jc_fa2bh_tree(self._h, cap, _ptr(nt), _ptr(nv), _ptr(keys), _ptr(perm),
+                                   _ptr(na), _ptr(info), _ptr(skip), _ptr(depth), _ptr(root4)))
+        N = int(nt[0])
+        return {"n_tree": N, "n_valid": int(nv[0]), "keys": keys[:n], "perm": perm[:n],
+                "node_a": na[: 4 * N].reshape(-1, 4), "info": info[:N], "skip": skip[:N],
+                "depth": depth[:N], "root4": root4}
+
+    def step_timed(self, n_iters: int):
+        """Graph-replayed iterations with HIP events around each force kernel.
+        Returns (sum of force-kernel ms, sum of whole-iteration ms)."""
+        f = np.zeros(1, np.float64)
+        it = np.zeros(1, np.float64)
+        _check(lib().jc_fa2bh_step_timed(self._h, n_iters, _ptr(f), _ptr(it)))
+        return float(f[0]), float(it[0])
+
+    def last_stats(self) -> dict:
+        nt = np.zeros(1, np.uint64)
+        it = np.zeros(1, np.uint64)
+        sums = np.zeros(2, np.float64)
+        _check(lib().jc_fa2bh_last_stats(self._h, _ptr(nt), _ptr(it), _ptr(sums)))
+        return {"n_tree": int(nt[0]), "interactions": int(it[0]), "sum_swing": float(sums[0]),
+                "sum_traction": float(sums[1])}
+
+    def profile(self, n_iters: int) -> dict:
+        k = int(lib().jc_fa2bh_phase_count())
+        out = np.zeros(k, np.float64)
+        _check(lib().jc_fa2bh_profile(self._h, n_iters, _ptr(out), k))
+        return {lib().jc_fa2bh_phase_name(i).decode(): float(out[i]) for i in range(k)}
+
+
+# ------------------------------------------------------------------ analytics
+def pagerank(offsets, neighbors, damping: float = 0.85, max_iters: int = 100, tol_l1: float = 1e-6):
+    """gpu_pagerank (crates/graph-compute/src/analytics/pagerank.rs:67-287) with an
+    L1 stopping rule.  Returns (ranks, iterations, final L1 residual)."""
+    offsets = _u32(offsets)
+    neighbors = _u32(neighbors)
+    n = len(offsets) - 1
+    ranks = np.empty(max(n, 1), np.float32)
+    it = np.zeros(1, np.uint32)
+    res = np.zeros(1, np.float64)
+    _check(lib().jc_pagerank(n, _ptr(offsets), _ptr(neighbors) if len(neighbors) else None,
+                             len(neighbors), damping, max_iters, tol_l1, _ptr(ranks), _ptr(it),
+                             _ptr(res)))
+    return ranks[:n], int(it[0]), float(res[0])
+
+
+# ------------------------------------------------------------------ fixtures
+def csr_from_pairs(n: int, pairs):
+    pairs = _u32(pairs).reshape(-1)
+    offsets = np.empty(n + 1, np.uint32)
+    nbrs = np.empty(max(len(pairs), 1), np.uint32)
+    nnz = lib().jc_csr_from_pairs(n, _ptr(pairs), len(pairs) // 2, _ptr(offsets), _ptr(nbrs))
+    return offsets, nbrs[:nnz].copy()
+
+
+def ba_pairs(n: int, m: int, seed: int):
+    cap = 1 + (max(n, 2) - 2) * max(m, 1)
+    pairs = np.empty(2 * cap, np.uint32)
+    cnt = lib().jc_gen_ba_pairs(n, m, seed, _ptr(pairs), cap)
+    return pairs[: 2 * cnt]
+
+
+def ba_graph(n: int, m: int, seed: int):
+    """synthetic_scale_free (crates/graph-compute/src/stability.rs:57-82) as CSR."""
+    return csr_from_pairs(max(n, 2), ba_pairs(n, m, seed))
+
+
+def sbm_pairs(n, blocks, p_in, p_out, seed):
+    cap = 1 << 16
+    while True:
+        pairs = np.empty(2 * cap, np.uint32)
+        cnt = lib().jc_gen_sbm_pairs(n, blocks, p_in, p_out, seed, _ptr(pairs), cap)
+        if cnt <= cap:
+            return pairs[: 2 * cnt]
+        cap = int(cnt)
+
+
+def sbm_graph(n, blocks, p_in, p_out, seed):
+    return csr_from_pairs(n, sbm_pairs(n, blocks, p_in, p_out, seed))
+
+
+def rgg_pairs(n, radius, seed):
+    xy = np.empty(2 * max(n, 1), np.float32)
+    cap = max(16, n * 6)
+    while True:
+        pairs = np.empty(2 * cap, np.uint32)
+        cnt = lib().jc_gen_rgg_pairs(n, radius, seed, _ptr(xy), _ptr(pairs), cap)
+        if cnt <= cap:
+            return xy[: 2 * n], pairs[: 2 * cnt]
+        cap = int(cnt)
+
+
+def uniform_cube(n: int, half: float, seed: int):
+    out = np.empty(3 * max(n, 1), np.float32)
+    lib().jc_gen_uniform_cube(n, half, seed, _ptr(out))
+    return out[: 3 * n]

@@ -1,0 +1,388 @@
+"""ctypes front-end of the CPU oracle (libjc_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py``, always as the checker or the timed
+CPU baseline, never as the product path.  The C sources cite the reference
+(ocasazza/jump-cannon) file:line each function restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libjc_oracle.so")
+
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+u16p = np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS")
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+INTERNAL = 0xFFFFFFFF
+END = 0xFFFFFFFF
+LEVELS = 21
+
+
+class Fa2Settings(C.Structure):
+    """Field-for-field Fa2BhSettings (crates/graph-compute/src/engines/fa2_bh.rs:85-125)."""
+
+    _fields_ = [
+        ("gravity", C.c_float),
+        ("strong_gravity", C.c_uint32),
+        ("scaling_ratio", C.c_float),
+        ("edge_weight_influence", C.c_float),
+        ("jitter_tolerance", C.c_float),
+        ("lin_log_mode", C.c_uint32),
+        ("prevent_overlap", C.c_uint32),
+        ("time_step", C.c_float),
+        ("theta", C.c_float),
+        ("max_displacement", C.c_float),
+    ]
+
+    @classmethod
+    def make(cls, **kw):
+        d = dict(gravity=1.0, strong_gravity=0, scaling_ratio=2.0, edge_weight_influence=1.0,
+                 jitter_tolerance=1.0, lin_log_mode=0, prevent_overlap=0, time_step=1.0,
+                 theta=0.7, max_displacement=10.0)
+        d.update(kw)
+        return cls(**d)
+
+
+def build() -> str:
+    subprocess.check_call(["make", "-s", "-C", HERE])
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = C.CDLL(LIB_PATH)
+    S = C.POINTER(Fa2Settings)
+    sig = {
+        "jco_next_u64": (C.c_uint64, [C.POINTER(C.c_uint64)]),
+        "jco_next_f32": (C.c_float, [C.POINTER(C.c_uint64)]),
+        "jco_uniform_cube": (None, [C.c_uint32, C.c_float, C.c_uint64, f32p]),
+        "jco_ball_seed": (None, [C.c_uint32, C.c_uint64, f32p]),
+        "jco_ba_pairs": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint64, u32p, C.c_uint64]),
+        "jco_sbm_pairs": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_uint64,
+                                       u32p, C.c_uint64]),
+        "jco_rgg_pairs": (C.c_uint64, [C.c_uint32, C.c_double, C.c_uint64, f32p, u32p, C.c_uint64]),
+        "jco_csr_from_pairs": (C.c_uint64, [C.c_uint32, u32p, C.c_uint64, u32p, u32p]),
+        "jco_largest_component": (C.c_uint32, [C.c_uint32, u32p, u32p, u32p]),
+        "jco_fa2_prepare": (C.c_uint64, [C.c_uint32, u32p, u32p, u32p, u32p, u32p]),
+        "jco_octree_ref": (C.c_uint32, [C.c_uint32, f32p, f32p, C.c_uint32, f32p, f32p, u32p, u32p,
+                                        u32p, f32p]),
+        "jco_octree_morton": (C.c_uint32, [C.c_uint32, f32p, f32p, f32p, f32p, u32p, u32p, u32p,
+                                           u64p, u32p, u32p, f32p]),
+        "jco_morton_keys": (None, [C.c_uint32, f32p, u64p, f32p]),
+        "jco_fa2_forces": (None, [C.c_uint32, f32p, f32p, u32p, u32p, u32p, C.c_uint32, f32p, f32p,
+                                  u32p, u32p, S, f32p, f32p, C.c_int]),
+        "jco_fa2_forces_f64": (None, [C.c_uint32, f32p, u32p, u32p, u32p, C.c_uint32, f32p, f32p,
+                                      u32p, u32p, S, f64p, u64p, C.c_int]),
+        "jco_fa2_brute_f64": (None, [C.c_uint32, f32p, u32p, u32p, u32p, S, f64p]),
+        "jco_exact_sum2": (None, [C.c_uint32, f32p, f64p]),
+        "jco_speed_update": (C.c_float, [f32p, C.c_uint32, C.c_float, f32p, C.c_int, f64p]),
+        "jco_fa2_apply": (None, [C.c_uint32, f32p, f32p, f32p, f32p, u32p, S, C.c_float, C.c_int]),
+        "jco_fa2_run": (C.c_int, [C.c_uint32, u32p, u32p, f32p, f32p, f32p, C.c_uint32, S, C.c_int,
+                                  C.c_int, C.c_int, f64p]),
+        "jco_pagerank_pull": (C.c_int, [C.c_uint32, u32p, u32p, C.c_float, C.c_uint32, C.c_double,
+                                        f32p, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
+        "jco_pagerank_push": (None, [C.c_uint32, u32p, u32p, C.c_float, C.c_uint32, f32p]),
+        "jco_pagerank_f64": (None, [C.c_uint32, u32p, u32p, C.c_double, C.c_uint32, f64p]),
+        "jco_bfs": (None, [C.c_uint32, u32p, u32p, C.c_uint32, u32p]),
+        "jco_select_pivots": (C.c_uint32, [C.c_uint32, u32p, u32p, C.c_uint32, C.c_uint64, u32p,
+                                           u16p]),
+        "jco_anneal_eta": (C.c_float, [C.c_uint64, C.c_float, C.c_float, C.c_uint32]),
+        "jco_sgd_sweep": (None, [C.c_uint32, C.c_uint32, u32p, u16p, f32p, f32p, C.c_float, C.c_int]),
+        "jco_stress_normalized": (C.c_double, [f32p, u32p, u32p, f32p, C.c_uint64]),
+        "jco_edge_length_cv": (C.c_double, [f32p, u32p, u32p, C.c_uint64]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+# ---------------------------------------------------------------- fixtures
+def splitmix_u64(seed: int, count: int) -> np.ndarray:
+    st = C.c_uint64(seed)
+    return np.array([lib().jco_next_u64(C.byref(st)) for _ in range(count)], dtype=np.uint64)
+
+
+def uniform_cube(n: int, half: float, seed: int) -> np.ndarray:
+    out = np.empty(3 * n, dtype=np.float32)
+    lib().jco_uniform_cube(n, half, seed, out)
+    return out
+
+
+def ball_seed(n: int, seed: int) -> np.ndarray:
+    out = np.empty(3 * n, dtype=np.float32)
+    lib().jco_ball_seed(n, seed, out)
+    return out
+
+
+def csr_from_pairs(n: int, pairs: np.ndarray):
+    pairs = _u32(pairs).reshape(-1)
+    offsets = np.empty(n + 1, dtype=np.uint32)
+    nbrs = np.empty(max(len(pairs), 1), dtype=np.uint32)
+    nnz = lib().jco_csr_from_pairs(n, pairs, len(pairs) // 2, offsets, nbrs)
+    return offsets, nbrs[:nnz].copy()
+
+
+def ba_pairs(n: int, m: int, seed: int) -> np.ndarray:
+    n2 = max(n, 2)
+    cap = 1 + (n2 - 2) * max(m, 1)
+    pairs = np.empty(2 * cap, dtype=np.uint32)
+    np_ = lib().jco_ba_pairs(n, m, seed, pairs, cap)
+    return pairs[: 2 * np_]
+
+
+def ba_graph(n: int, m: int, seed: int):
+    """synthetic_scale_free (crates/graph-compute/src/stability.rs:57-82)."""
+    return csr_from_pairs(max(n, 2), ba_pairs(n, m, seed))
+
+
+def sbm_pairs(n: int, blocks: int, p_in: float, p_out: float, seed: int) -> np.ndarray:
+    cap = 1 << 16
+    while True:
+        pairs = np.empty(2 * cap, dtype=np.uint32)
+        cnt = lib().jco_sbm_pairs(n, blocks, p_in, p_out, seed, pairs, cap)
+        if cnt <= cap:
+            return pairs[: 2 * cnt]
+        cap = int(cnt)
+
+
+def sbm_graph(n, blocks, p_in, p_out, seed):
+    return csr_from_pairs(n, sbm_pairs(n, blocks, p_in, p_out, seed))
+
+
+def rgg_pairs(n: int, radius: float, seed: int):
+    xy = np.empty(2 * n, dtype=np.float32)
+    cap = max(16, int(n * 6))
+    while True:
+        pairs = np.empty(2 * cap, dtype=np.uint32)
+        cnt = lib().jco_rgg_pairs(n, radius, seed, xy, pairs, cap)
+        if cnt <= cap:
+            return xy, pairs[: 2 * cnt]
+        cap = int(cnt)
+
+
+def largest_component(n, offsets, nbrs):
+    keep = np.empty(n, dtype=np.uint32)
+    size = lib().jco_largest_component(n, _u32(offsets), _u32(nbrs), keep)
+    return size, keep
+
+
+# ---------------------------------------------------------------- FA2
+def fa2_prepare(n, offsets, nbrs):
+    deg = np.empty(max(n, 1), dtype=np.uint32)
+    att_off = np.empty(n + 1, dtype=np.uint32)
+    att_adj = np.empty(max(len(nbrs), 1), dtype=np.uint32)
+    m = lib().jco_fa2_prepare(n, _u32(offsets), _u32(nbrs), deg, att_off, att_adj)
+    return deg[:n], att_off, att_adj[:m] if m else np.zeros(1, np.uint32)
+
+
+class Tree:
+    def __init__(self, N, node_a, half, body, skip, depth, root4, keys=None, perm=None, n_valid=None):
+        self.N = N
+        self.node_a = node_a[: 4 * N].reshape(-1, 4) if N else np.zeros((0, 4), np.float32)
+        self.node_a_flat = node_a
+        self.half = half
+        self.body = body
+        self.skip = skip
+        self.depth = depth
+        self.root4 = root4
+        self.keys = keys
+        self.perm = perm
+        self.n_valid = n_valid
+
+
+def octree_ref(xyz, mass, max_nodes=None):
+    xyz = _f32(xyz)
+    n = len(xyz) // 3
+    cap = max_nodes if max_nodes is not None else 2 * n + 8
+    na = np.zeros(4 * max(cap, 1), np.float32)
+    half = np.zeros(max(cap, 1), np.float32)
+    body = np.zeros(max(cap, 1), np.uint32)
+    skip = np.zeros(max(cap, 1), np.uint32)
+    depth = np.zeros(max(cap, 1), np.uint32)
+    root4 = np.zeros(4, np.float32)
+    N = lib().jco_octree_ref(n, xyz, _f32(mass), cap, na, half, body, skip, depth, root4)
+    return Tree(N, na, half, body, skip, depth, root4)
+
+
+def octree_morton(xyz, mass):
+    xyz = _f32(xyz)
+    n = len(xyz) // 3
+    cap = (LEVELS + 2) * n + 8
+    na = np.zeros(4 * cap, np.float32)
+    half = np.zeros(cap, np.float32)
+    body = np.zeros(cap, np.uint32)
+    skip = np.zeros(cap, np.uint32)
+    depth = np.zeros(cap, np.uint32)
+    keys = np.zeros(max(n, 1), np.uint64)
+    perm = np.zeros(max(n, 1), np.uint32)
+    nv = C.c_uint32(0)
+    root4 = np.zeros(4, np.float32)
+    nvarr = np.zeros(1, np.uint32)
+    N = lib().jco_octree_morton(n, xyz, _f32(mass), na, half, body, skip, depth, keys, perm, nvarr,
+                                root4)
+    return Tree(N, na, half, body, skip, depth, root4, keys[:n], perm[:n], int(nvarr[0]))
+
+
+def morton_keys(xyz):
+    xyz = _f32(xyz)
+    n = len(xyz) // 3
+    keys = np.zeros(max(n, 1), np.uint64)
+    root4 = np.zeros(4, np.float32)
+    lib().jco_morton_keys(n, xyz, keys, root4)
+    return keys[:n], root4
+
+
+def fa2_forces(xyz, old_force, deg, att_off, att_adj, tree: Tree, settings: Fa2Settings, threads=8):
+    xyz = _f32(xyz)
+    n = len(xyz) // 3
+    force = np.zeros(3 * n, np.float32)
+    stats = np.zeros(2 * max(n, 1), np.float32)
+    old = _f32(old_force) if old_force is not None else np.zeros(3 * n, np.float32)
+    lib().jco_fa2_forces(n, xyz, old, _u32(deg), _u32(att_off), _u32(att_adj), tree.N,
+                         tree.node_a_flat, tree.half, tree.body, tree.skip, C.byref(settings),
+                         force, stats, threads)
+    return force, stats[: 2 * n]
+
+
+def fa2_forces_f64(xyz, deg, att_off, att_adj, tree: Tree, settings: Fa2Settings, threads=8):
+    xyz = _f32(xyz)
+    n = len(xyz) // 3
+    force = np.zeros(3 * n, np.float64)
+    flips = np.zeros(1, np.uint64)
+    lib().jco_fa2_forces_f64(n, xyz, _u32(deg), _u32(att_off), _u32(att_adj), tree.N,
+                             tree.node_a_flat, tree.half, tree.body, tree.skip, C.byref(settings),
+                             force, flips, threads)
+    return force, int(flips[0])
+
+
+def fa2_brute_f64(xyz, deg, att_off, att_adj, settings: Fa2Settings):
+    xyz = _f32(xyz)
+    n = len(xyz) // 3
+    force = np.zeros(3 * n, np.float64)
+    lib().jco_fa2_brute_f64(n, xyz, _u32(deg), _u32(att_off), _u32(att_adj), C.byref(settings), force)
+    return force
+
+
+def exact_sum2(stats):
+    stats = _f32(stats)
+    out = np.zeros(2, np.float64)
+    lib().jco_exact_sum2(len(stats) // 2, stats, out)
+    return out
+
+
+def speed_update(se, n, jt, stats, exact=False):
+    se = np.ascontiguousarray(se, dtype=np.float32)
+    sums = np.zeros(2, np.float64)
+    lib().jco_speed_update(se, n, jt, _f32(stats), 1 if exact else 0, sums)
+    return se, sums
+
+
+def fa2_run(offsets, nbrs, xyz, iters, settings: Fa2Settings, tree_kind=1, exact=True, threads=8,
+            old_force=None, se=None):
+    """Whole FA2+BH iterations (fa2_bh.rs:546-692).  Returns (xyz, old_force, se, timings)."""
+    xyz = _f32(xyz).copy()
+    n = len(xyz) // 3
+    old = np.zeros(3 * n, np.float32) if old_force is None else _f32(old_force).copy()
+    se = np.array([1.0, 1.0], np.float32) if se is None else np.array(se, np.float32)
+    t = np.zeros(3, np.float64)
+    lib().jco_fa2_run(n, _u32(offsets), _u32(nbrs), xyz, old, se, iters, C.byref(settings),
+                      tree_kind, 1 if exact else 0, threads, t)
+    return xyz, old, se, t
+
+
+# ---------------------------------------------------------------- PageRank
+def pagerank_pull(offsets, nbrs, damping=0.85, max_iters=100, tol=1e-6):
+    n = len(offsets) - 1
+    ranks = np.zeros(max(n, 1), np.float32)
+    it = C.c_uint32(0)
+    res = C.c_double(0)
+    rc = lib().jco_pagerank_pull(n, _u32(offsets), _u32(nbrs) if len(nbrs) else np.zeros(1, np.uint32),
+                                 damping, max_iters, tol, ranks, C.byref(it), C.byref(res))
+    if rc != 0:
+        raise ValueError("dangling node")
+    return ranks[:n], it.value, res.value
+
+
+def pagerank_push(offsets, nbrs, damping=0.85, iters=50):
+    n = len(offsets) - 1
+    ranks = np.zeros(max(n, 1), np.float32)
+    lib().jco_pagerank_push(n, _u32(offsets), _u32(nbrs) if len(nbrs) else np.zeros(1, np.uint32),
+                            damping, iters, ranks)
+    return ranks[:n]
+
+
+def pagerank_f64(offsets, nbrs, damping=0.85, iters=50):
+    n = len(offsets) - 1
+    ranks = np.zeros(max(n, 1), np.float64)
+    lib().jco_pagerank_f64(n, _u32(offsets), _u32(nbrs) if len(nbrs) else np.zeros(1, np.uint32),
+                           damping, iters, ranks)
+    return ranks[:n]
+
+
+# ---------------------------------------------------------------- SGD
+def bfs(offsets, nbrs, src):
+    n = len(offsets) - 1
+    d = np.zeros(max(n, 1), np.uint32)
+    lib().jco_bfs(n, _u32(offsets), _u32(nbrs) if len(nbrs) else np.zeros(1, np.uint32), src, d)
+    return d[:n]
+
+
+def select_pivots(offsets, nbrs, k, seed=0x5EEDC0DE12345678):
+    n = len(offsets) - 1
+    kk = max(1, min(k, n))
+    piv = np.zeros(kk, np.uint32)
+    dist = np.zeros(max(n * kk, 1), np.uint16)
+    got = lib().jco_select_pivots(n, _u32(offsets), _u32(nbrs) if len(nbrs) else np.zeros(1, np.uint32),
+                                  k, seed, piv, dist)
+    if got == 0xFFFFFFFF:
+        raise ValueError("hop distance exceeds u16")
+    return piv, dist[: n * kk]
+
+
+def anneal_eta(sweep, eta_max, eta_min, n_anneal):
+    return lib().jco_anneal_eta(sweep, eta_max, eta_min, n_anneal)
+
+
+def sgd_sweep(pivots, dist, pos, eta, threads=8):
+    pos = _f32(pos)
+    n = len(pos) // 3
+    out = np.empty_like(pos)
+    lib().jco_sgd_sweep(n, len(pivots), _u32(pivots), np.ascontiguousarray(dist, np.uint16), pos, out,
+                        eta, threads)
+    return out
+
+
+# ---------------------------------------------------------------- metrics
+def stress_normalized(xyz, ti, tj, td):
+    return lib().jco_stress_normalized(_f32(xyz), _u32(ti), _u32(tj), _f32(td), len(ti))
+
+
+def edge_length_cv(xyz, ei, ej):
+    return lib().jco_edge_length_cv(_f32(xyz), _u32(ei), _u32(ej), len(ei))

@@ -1,0 +1,254 @@
+"""FA2 + Barnes-Hut on the GPU against the CPU oracle.
+
+Contract (checked here through the C ABI):
+  * octree: sorted keys, permutation and every node field bit-exact with the
+    oracle's Morton restatement (which equals the reference insertion tree);
+  * per-node forces at fixed positions: bit-exact with the oracle's f32 walk,
+    and within 1e-4 relative of the FP64 oracle (denominator stated below);
+  * exact swing/traction sums bit-exact; whole trajectories bit-exact;
+  * final layouts of the 2,000-node SBM within 1% (normalised stress and
+    edge-length CV) of the reference path (insertion octree, f64 node-order
+    sums) run by the oracle.
+"""
+import numpy as np
+import pytest
+
+import jumpcannon as jc
+from jumpcannon import workloads
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine(off, nb, xyz, **kw):
+    e = jc.Fa2BhEngine(**kw)
+    e.init(off, nb, xyz)
+    return e
+
+
+def _osettings(**kw):
+    return O.Fa2Settings.make(**kw)
+
+
+def _mass(off, nb):
+    deg, _, _ = O.fa2_prepare(len(off) - 1, off, nb)
+    return deg.astype(np.float32) + np.float32(1.0)
+
+
+def _cases():
+    out = []
+    for n in (1, 2, 3, 17, 1000):
+        pairs = O.ba_pairs(max(n, 2), 2, 7 + n) if n >= 2 else np.zeros(0, np.uint32)
+        off, nb = O.csr_from_pairs(n, pairs)
+        out.append((f"ba{n}", off, nb, O.uniform_cube(n, 3.0, 11 + n)))
+    off, nb, xyz = workloads.sbm_2000()
+    out.append(("sbm2000", off, nb, xyz))
+    off, nb = O.ba_graph(20000, 4, 5)
+    out.append(("ba20000", off, nb, O.uniform_cube(20000, 10.0, 6)))
+    return out
+
+
+CASES = _cases()
+
+
+def _tree_equal(g, t):
+    assert g["n_tree"] == t.N
+    assert g["n_valid"] == t.n_valid
+    N = t.N
+    assert np.array_equal(g["keys"], t.keys)
+    assert np.array_equal(g["perm"], t.perm)
+    assert np.array_equal(g["node_a"].view(np.uint32), t.node_a.view(np.uint32))
+    internal = t.body[:N] == O.INTERNAL
+    s2 = ((t.half[:N] * np.float32(2.0)) * (t.half[:N] * np.float32(2.0))).astype(np.float32)
+    want_info = np.where(internal, s2.view(np.uint32), np.uint32(0x80000000) | t.body[:N])
+    assert np.array_equal(g["info"], want_info.astype(np.uint32))
+    assert np.array_equal(g["skip"], t.skip[:N])
+    want_depth = np.where(internal, t.depth[:N], 0x80 | t.depth[:N]).astype(np.uint8)
+    assert np.array_equal(g["depth"], want_depth)
+
+
+@pytest.mark.parametrize("name,off,nb,xyz", CASES, ids=[c[0] for c in CASES])
+def test_tree_and_forces_bit_exact(name, off, nb, xyz):
+    n = len(off) - 1
+    e = _engine(off, nb, xyz, theta=0.8)
+    f_gpu, sums_gpu = e.forces_at(xyz)
+    g = e.tree()
+    mass = _mass(off, nb)
+    t = O.octree_morton(xyz, mass)
+    _tree_equal(g, t)
+    deg, aoff, aadj = O.fa2_prepare(n, off, nb)
+    s = _osettings(theta=0.8)
+    f_cpu, stats = O.fa2_forces(xyz, None, deg, aoff, aadj, t, s)
+    assert np.array_equal(f_gpu.view(np.uint32), f_cpu.view(np.uint32)), \
+        f"max |diff| {np.abs(f_gpu - f_cpu).max()}"
+    want = O.exact_sum2(stats)
+    assert np.array_equal(sums_gpu.view(np.uint64), want.view(np.uint64))
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.8, 1.2])
+def test_forces_within_1e4_of_fp64(theta):
+    """Per-node |F32 - F64| / max(|F64|, floor) <= 1e-4 with floor = 1e-3 * rms|F64|,
+    F64 = same tree walked in f64 with the f32 acceptance decisions (theta > 0)
+    or the brute-force FP64 sum (theta = 0)."""
+    off, nb, xyz = workloads.sbm_2000()
+    n = len(off) - 1
+    e = _engine(off, nb, xyz, theta=theta, scaling_ratio=2.0)
+    f32, _ = e.forces_at(xyz)
+    deg, aoff, aadj = O.fa2_prepare(n, off, nb)
+    s = _osettings(theta=theta)
+    if theta == 0.0:
+        f64 = O.fa2_brute_f64(xyz, deg, aoff, aadj, s)
+    else:
+        t = O.octree_morton(xyz, _mass(off, nb))
+        f64, flips = O.fa2_forces_f64(xyz, deg, aoff, aadj, t, s)
+    a = f32.reshape(-1, 3).astype(np.float64)
+    b = f64.reshape(-1, 3)
+    nb_ = np.linalg.norm(b, axis=1)
+    floor = 1e-3 * np.sqrt(np.mean(nb_ ** 2))
+    rel = np.linalg.norm(a - b, axis=1) / np.maximum(nb_, floor)
+    print(f"theta={theta}: max rel {rel.max():.3e}, p99 {np.quantile(rel, 0.99):.3e}")
+    assert rel.max() <= 1e-4
+
+
+def test_trajectory_bit_exact_sbm2000():
+    """50 iterations: positions, F_{t-1} and the speed state bit-exact with the
+    oracle driving the same (Morton) octree and the exact f64 sums."""
+    off, nb, xyz = workloads.sbm_2000()
+    e = _engine(off, nb, xyz, theta=1.2)
+    e.step(50, readback=False)
+    st = e.get_state()
+    x, old, se, _ = O.fa2_run(off, nb, xyz, 50, _osettings(theta=1.2), tree_kind=1, exact=True)
+    assert np.array_equal(st["positions"].view(np.uint32), x.view(np.uint32))
+    assert np.array_equal(st["old_force"].view(np.uint32), old.view(np.uint32))
+    assert np.array_equal(st["speed_eff2"].view(np.uint32), se.view(np.uint32))
+    assert st["iteration"] == 50
+
+
+def _pivot_terms(off, nb, k=200):
+    piv, dist = O.select_pivots(off, nb, k, seed=0xC0FFEE)
+    n = len(off) - 1
+    kk = len(piv)
+    d = dist.reshape(n, kk)
+    ii, tt = np.nonzero(d > 0)
+    return ii.astype(np.uint32), piv[tt].astype(np.uint32), d[ii, tt].astype(np.float32)
+
+
+def _edges(off, nb):
+    n = len(off) - 1
+    src = np.repeat(np.arange(n, dtype=np.uint32), np.diff(off))
+    m = src < nb
+    return src[m], nb[m]
+
+
+def test_sbm2000_500_iterations_matches_reference_path_within_1pct():
+    off, nb, xyz = workloads.sbm_2000()
+    e = _engine(off, nb, xyz, theta=1.2)
+    pos = e.step(500)
+    ref, _, _, _ = O.fa2_run(off, nb, xyz, 500, _osettings(theta=1.2), tree_kind=0, exact=False)
+    assert np.isfinite(pos).all()
+    ti, tj, td = _pivot_terms(off, nb)
+    ei, ej = _edges(off, nb)
+    s_gpu, s_ref = O.stress_normalized(pos, ti, tj, td), O.stress_normalized(ref, ti, tj, td)
+    c_gpu, c_ref = O.edge_length_cv(pos, ei, ej), O.edge_length_cv(ref, ei, ej)
+    print(f"stress gpu {s_gpu:.5f} ref {s_ref:.5f}; edge CV gpu {c_gpu:.5f} ref {c_ref:.5f}")
+    assert abs(s_gpu - s_ref) <= 0.01 * s_ref
+    assert abs(c_gpu - c_ref) <= 0.01 * c_ref
+
+
+def test_run_to_run_bit_exact_and_resume():
+    off, nb = O.ba_graph(5000, 3, 99)
+    xyz = O.uniform_cube(5000, 5.0, 98)
+    a = _engine(off, nb, xyz, theta=0.7)
+    pa = a.step(40)
+    b = _engine(off, nb, xyz, theta=0.7)
+    b.step(25, readback=False)
+    st = b.get_state()
+    c = _engine(off, nb, xyz, theta=0.7)
+    c.set_state(st["positions"], st["old_force"], st["speed_eff2"], st["iteration"])
+    pc = c.step(15)
+    assert np.array_equal(pa.view(np.uint32), pc.view(np.uint32))
+    assert c.get_state()["iteration"] == 40
+    # odd / even step splits take the graph and eager paths differently
+    d = _engine(off, nb, xyz, theta=0.7)
+    for k in (1, 2, 3, 5, 7, 9, 13):
+        d.step(k, readback=False)
+    assert np.array_equal(d.step(0).view(np.uint32), pa.view(np.uint32))
+
+
+def test_single_edge_finite_400_steps():
+    """gpu_engines.rs:177-195."""
+    off, nb = O.csr_from_pairs(2, np.array([0, 1], np.uint32))
+    e = _engine(off, nb, np.array([0, 0, 0, 5, 0, 0], np.float32))
+    for _ in range(40):
+        p = e.step(10)
+        assert np.isfinite(p).all()
+
+
+def test_triangle_relaxes_equilateral():
+    """gpu_engines.rs:200-224: K3 sides within 15% of their mean after 600 steps."""
+    off, nb = O.csr_from_pairs(3, np.array([0, 1, 1, 2, 2, 0], np.uint32))
+    e = _engine(off, nb, np.array([0, 0, 0, 10, 0, 0, 3, 4, 0], np.float32), theta=0.0)
+    p = e.step(600).reshape(3, 3)
+    d = [np.linalg.norm(p[0] - p[1]), np.linalg.norm(p[1] - p[2]), np.linalg.norm(p[2] - p[0])]
+    mean = np.mean(d)
+    assert mean > 1e-3
+    assert max(abs(x - mean) for x in d) / mean < 0.15
+
+
+def test_degenerate_inputs():
+    # empty graph
+    e = jc.Fa2BhEngine()
+    e.init(np.zeros(1, np.uint32), np.zeros(0, np.uint32), np.zeros(0, np.float32))
+    assert e.step(3).shape == (0,)
+    # coincident bodies, isolated nodes, a self loop, a star
+    pairs = np.array([0, 1, 0, 2, 0, 3, 0, 4, 5, 5], np.uint32)
+    off, nb = O.csr_from_pairs(8, pairs)
+    xyz = np.zeros(24, np.float32)
+    xyz[3:6] = 1.0
+    e = _engine(off, nb, xyz)
+    f, _ = e.forces_at(xyz)
+    t = O.octree_morton(xyz, _mass(off, nb))
+    deg, aoff, aadj = O.fa2_prepare(8, off, nb)
+    fc, _ = O.fa2_forces(xyz, None, deg, aoff, aadj, t, _osettings())
+    assert np.array_equal(f.view(np.uint32), fc.view(np.uint32))
+    p = e.step(100)
+    assert np.isfinite(p).all()
+
+
+def test_non_finite_bodies_are_skipped_like_the_reference():
+    off, nb = O.ba_graph(300, 2, 3)
+    xyz = O.uniform_cube(300, 2.0, 4)
+    xyz[3 * 7] = np.nan
+    xyz[3 * 50 + 2] = np.inf
+    e = _engine(off, nb, xyz)
+    f, sums = e.forces_at(xyz)
+    g = e.tree()
+    t = O.octree_morton(xyz, _mass(off, nb))
+    _tree_equal(g, t)
+    deg, aoff, aadj = O.fa2_prepare(300, off, nb)
+    fc, st = O.fa2_forces(xyz, None, deg, aoff, aadj, t, _osettings())
+    assert np.array_equal(f.view(np.uint32), fc.view(np.uint32))
+    assert np.array_equal(sums.view(np.uint64), O.exact_sum2(st).view(np.uint64))
+
+
+def test_ba_1m_tree_and_force_parity_at_full_size():
+    """The 1M-node BA workload: tree bit-exact and forces bit-exact at iteration 0,
+    plus size-independent tree invariants."""
+    off, nb, xyz = workloads.ba_1m()
+    n = len(off) - 1
+    e = _engine(off, nb, xyz, theta=0.8)
+    f, sums = e.forces_at(xyz)
+    g = e.tree()
+    mass = _mass(off, nb)
+    # invariants: one leaf per valid body, leaf mass sums to sum(deg+1), keys sorted
+    leaf = (g["depth"] & 0x80) != 0
+    assert leaf.sum() == n
+    assert np.all(np.diff(g["keys"].astype(np.uint64)) >= 0)
+    assert np.array_equal(np.sort(g["info"][leaf] & 0x7FFFFFFF), np.arange(n, dtype=np.uint32))
+    assert g["node_a"][leaf, 3].astype(np.float64).sum() == mass.astype(np.float64).sum()
+    t = O.octree_morton(xyz, mass)
+    _tree_equal(g, t)
+    deg, aoff, aadj = O.fa2_prepare(n, off, nb)
+    fc, st = O.fa2_forces(xyz, None, deg, aoff, aadj, t, _osettings(theta=0.8), threads=16)
+    assert np.array_equal(f.view(np.uint32), fc.view(np.uint32))
+    assert np.array_equal(sums.view(np.uint64), O.exact_sum2(st).view(np.uint64))
