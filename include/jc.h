@@ -49,7 +49,8 @@ typedef struct {
     float max_displacement;
 } jc_fa2bh_settings;
 
-/* Field-for-field SgdStressSettings (crates/graph-compute/src/engines/sgd_stress.rs:59-98). */
+/* Field-for-field SgdStressSettings (crates/graph-compute/src/engines/sgd_stress.rs:59-98).
+ * eta_max <= 0 selects the s_gd2 start 1/w_min = (max pivot distance)^2. */
 typedef struct {
     uint32_t n_pivots;
     uint32_t sweeps_per_step;
@@ -145,6 +146,16 @@ int jc_fa2bh_profile(jc_engine *e, uint32_t n_iters, double *phase_ms, uint32_t 
 const char *jc_fa2bh_phase_name(uint32_t k);
 uint32_t jc_fa2bh_phase_count(void);
 
+/* ---- SGD stress hooks ---- */
+/* Pivots chosen at init (sgd_stress.rs:143-175), their hop distances in the
+ * reference's node-major layout dist[i*k + t] (0 = self/unreachable), and the
+ * eta_max in force (settings.eta_max, or max distance^2 when eta_max <= 0).
+ * Call with pivots = dist = NULL first to learn k. */
+int jc_sgd_pivots(jc_engine *e, uint32_t *k, uint32_t *pivots, uint16_t *dist, float *eta_max);
+/* Runs n_steps steps (LayoutEngine::step without read-back) between HIP events
+ * on the engine stream and returns their device time in ms. */
+int jc_sgd_time(jc_engine *e, uint32_t n_steps, double *gpu_ms);
+
 /* ---- PageRank (crates/graph-compute/src/analytics/pagerank.rs:67-287) ----
  * Pull power iteration: rank_{t+1}[v] = (1-d)/n + d * sum_{u in N(v)} rank_t[u]/deg(u),
  * rank_0 = 1/n, stopping when the f64 L1 norm of rank_{t+1}-rank_t < tol_l1 or
@@ -152,6 +163,38 @@ uint32_t jc_fa2bh_phase_count(void);
 int jc_pagerank(uint32_t n, const uint32_t *offsets, const uint32_t *neighbors, uint64_t nnz,
                 float damping, uint32_t max_iters, double tol_l1, float *out_ranks,
                 uint32_t *out_iters, double *out_residual_l1);
+/* The same computation split into set-up (CSR and 1/deg uploaded once) and runs
+ * over the resident graph; out_gpu_ms (nullable) is the device time of the run
+ * measured with HIP events.  plan_time runs n_iters iterations with the
+ * stopping rule disabled and returns the mean device ms per iteration. */
+typedef struct jc_pagerank_plan jc_pagerank_plan;
+int jc_pagerank_plan_create(uint32_t n, const uint32_t *offsets, const uint32_t *neighbors,
+                            uint64_t nnz, float damping, jc_pagerank_plan **out);
+int jc_pagerank_plan_run(jc_pagerank_plan *plan, uint32_t max_iters, double tol_l1,
+                         float *out_ranks, uint32_t *out_iters, double *out_residual_l1,
+                         double *out_gpu_ms);
+int jc_pagerank_plan_time(jc_pagerank_plan *plan, uint32_t n_iters, double *ms_per_iter);
+void jc_pagerank_plan_destroy(jc_pagerank_plan *plan);
+
+/* ---- batched FA2+BH: many independent graphs per launch ----
+ * No reference counterpart (the reference runs one engine per graph,
+ * sim.rs:386-421); each graph follows exactly the fa2-bh-hip semantics with its
+ * own AdaptiveSpeed state.  Inputs are concatenated graph after graph:
+ * offsets = the n_g+1 local offsets of each graph, neighbors = local ids,
+ * xyz = 3*sum(n_g) positions. */
+typedef struct jc_batch jc_batch;
+int jc_fa2bh_batch_create(const jc_fa2bh_settings *settings, jc_batch **out);
+int jc_fa2bh_batch_init(jc_batch *b, uint32_t n_graphs, const uint32_t *n_nodes,
+                        const uint64_t *nnz, const uint32_t *offsets, const uint32_t *neighbors,
+                        const float *xyz);
+int jc_fa2bh_batch_step(jc_batch *b, uint32_t n_iters, float *out_xyz);
+/* {speed, speed_efficiency} per graph (2 * n_graphs floats). */
+int jc_fa2bh_batch_speeds(jc_batch *b, float *speed_eff);
+/* n_iters iterations between HIP events; also reports the force-kernel block
+ * size the autotuner picked. */
+int jc_fa2bh_batch_step_timed(jc_batch *b, uint32_t n_iters, double *gpu_ms,
+                              uint32_t *force_block);
+void jc_fa2bh_batch_destroy(jc_batch *b);
 
 /* ---- fixtures (crates/graph-compute/src/stability.rs:9-110) ---- */
 /* synthetic_scale_free(n, m, seed) as undirected pairs; returns the pair count

@@ -1,0 +1,946 @@
+// Batched FA2 + Barnes-Hut: thousands of independent graphs of uneven size laid
+// out together, one launch per phase for the whole batch ("fa2-bh-batch-hip").
+//
+// The reference lays out one graph per engine instance, one dispatch pair per
+// step (fa2_bh.rs:546-692; one engine per worker, sim.rs:386-421).  Here all
+// graphs form one forest on the device:
+//   * bodies are concatenated graph after graph; gid[i] names the graph;
+//   * every graph gets its own root box (segmented bbox), its own octant keys,
+//     and the sort is graph-major / key-minor (8 key passes + graph-id passes,
+//     stable), so each graph's bodies stay in its own segment;
+//   * each graph's pre-order octree occupies one contiguous node range whose
+//     last rope is the end marker, so a walk never leaves its graph;
+//   * swing/traction are summed exactly per graph and every graph runs its own
+//     AdaptiveSpeed state (one block per graph).
+// Each graph therefore evolves bit-identically to a standalone fa2-bh-hip run
+// (and to the oracle), while the dispatch count per iteration is independent
+// of the number of graphs.  The iteration is captured in a hipGraph; the force
+// kernel's block size is picked at the first step by timing the candidates.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "engine.hpp"
+#include "exact_sum.hpp"
+#include "fa2_device.hpp"
+
+struct jc_batch;
+
+namespace jc {
+namespace {
+
+constexpr uint32_t kBT = 256;
+constexpr uint32_t kBScanItems = 16;
+constexpr uint32_t kBScanTile = kBT * kBScanItems;
+
+struct GFrame {
+    uint32_t bbmin[3];
+    uint32_t bbmax[3];
+    uint32_t n_invalid;
+    uint32_t n_tree;
+    uint32_t tree_start;
+    uint32_t pad;
+    float root[4];
+    float speed;
+    float speed_eff;
+    double sum_swing;
+    double sum_traction;
+};
+
+struct BGlobal {
+    uint32_t n_tree_total;
+    uint32_t max_depth;
+};
+
+struct BParams {
+    uint32_t n;          // total bodies
+    uint32_t n_graphs;
+    float gravity;
+    uint32_t strong_gravity;
+    float scaling_ratio;
+    float jitter_tolerance;
+    uint32_t lin_log_mode;
+    float time_step;
+    float theta2;
+    float max_displacement;
+    float ewi_weight;
+    uint32_t cap;
+};
+
+__global__ void k_b_reset(GFrame *fr, uint32_t G, BGlobal *gl) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g == 0) gl->max_depth = 0;
+    if (g >= G) return;
+    const uint32_t pinf = 0xFF800000u;
+    for (int k = 0; k < 3; k++) {
+        fr[g].bbmin[k] = pinf;
+        fr[g].bbmax[k] = ~pinf;
+    }
+    fr[g].n_invalid = 0;
+}
+
+__global__ __launch_bounds__(kBT) void k_b_bbox(const float4 *__restrict__ pos,
+                                                const uint32_t *__restrict__ gid, uint32_t n,
+                                                GFrame *fr) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < n;
+    const uint32_t g = live ? gid[i] : 0xFFFFFFFFu;
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (live) {
+        const float4 p = pos[i];
+        const float v[3] = {p.x + 0.0f, p.y + 0.0f, p.z + 0.0f};
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            if (finitef(v[k])) mn[k] = mx[k] = v[k];
+    }
+    const uint32_t g0 = __shfl(g, 0, 64);
+    const bool uniform = __all(g == g0);
+    if (uniform) {
+        if (g0 == 0xFFFFFFFFu) return;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                mn[k] = fminf(mn[k], __shfl_xor(mn[k], o, 64));
+                mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], o, 64));
+            }
+        }
+        if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                if (finitef(mn[k])) atomicMin(&fr[g0].bbmin[k], f2ord(mn[k]));
+                if (finitef(mx[k])) atomicMax(&fr[g0].bbmax[k], f2ord(mx[k]));
+            }
+        }
+    } else if (live) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            if (finitef(mn[k])) atomicMin(&fr[g].bbmin[k], f2ord(mn[k]));
+            if (finitef(mx[k])) atomicMax(&fr[g].bbmax[k], f2ord(mx[k]));
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBT) void k_b_keys(const float4 *__restrict__ pos,
+                                                const uint32_t *__restrict__ gid,
+                                                const uint32_t *__restrict__ base, uint32_t n,
+                                                GFrame *fr, uint64_t *__restrict__ keys_raw,
+                                                uint64_t *__restrict__ keys,
+                                                uint32_t *__restrict__ vals) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t g = gid[i];
+    float r4[4];
+    compute_root(fr[g].bbmin, fr[g].bbmax, r4);
+    if (i == base[g])
+        for (int k = 0; k < 4; k++) fr[g].root[k] = r4[k];
+    const float4 p = pos[i];
+    uint64_t key;
+    if (finitef(p.x) && finitef(p.y) && finitef(p.z)) {
+        key = octant_key(r4, p.x, p.y, p.z);
+    } else {
+        key = ~0ull;
+        atomicAdd(&fr[g].n_invalid, 1u);
+    }
+    keys_raw[i] = key;
+    keys[i] = key;
+    vals[i] = i;
+}
+
+// After the key sort: key := graph id of the body (for the graph-major passes).
+__global__ void k_b_gid_keys(const uint32_t *__restrict__ vals, const uint32_t *__restrict__ gid,
+                             uint64_t *__restrict__ keys, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) keys[i] = gid[vals[i]];
+}
+
+__global__ void k_b_gather_keys(const uint32_t *__restrict__ vals,
+                                const uint64_t *__restrict__ keys_raw, uint64_t *__restrict__ keys,
+                                uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) keys[i] = keys_raw[vals[i]];
+}
+
+struct Seg {
+    uint32_t g, lo, ve;  // graph, first sorted index, end of its valid bodies
+};
+__device__ __forceinline__ Seg seg_of(uint32_t i, const uint32_t *gid, const uint32_t *base,
+                                      const uint32_t *count, const GFrame *fr) {
+    Seg s;
+    s.g = gid[i];
+    s.lo = base[s.g];
+    s.ve = s.lo + count[s.g] - fr[s.g].n_invalid;
+    return s;
+}
+
+// Nodes written by sorted body i: its opened cells plus its leaf (0 if invalid).
+__device__ __forceinline__ uint32_t nodes_of(const uint64_t *keys, uint32_t i, const Seg &s) {
+    if (i >= s.ve) return 0;
+    if (s.ve - s.lo == 1) return 2;
+    const uint64_t k = keys[i];
+    const int lp = i > s.lo ? lcp_digits(keys[i - 1], k) : -1;
+    const int li = i + 1 < s.ve ? lcp_digits(k, keys[i + 1]) : -1;
+    return (li > lp ? (uint32_t)(li - lp) : 0u) + 1u;
+}
+
+__global__ __launch_bounds__(kBT) void k_b_count_scan(const uint64_t *__restrict__ keys,
+                                                      const uint32_t *__restrict__ gid,
+                                                      const uint32_t *__restrict__ base,
+                                                      const uint32_t *__restrict__ count,
+                                                      uint32_t n, const GFrame *fr,
+                                                      uint32_t *__restrict__ loc,
+                                                      uint32_t *__restrict__ blk_tot) {
+    __shared__ uint32_t ws[kBT / 64];
+    const uint32_t b0 = blockIdx.x * kBScanTile + threadIdx.x * kBScanItems;
+    uint32_t c[kBScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kBScanItems; j++) {
+        const uint32_t i = b0 + j;
+        c[j] = 0;
+        if (i < n) {
+            const Seg sg = seg_of(i, gid, base, count, fr);
+            c[j] = nodes_of(keys, i, sg);
+        }
+        s += c[j];
+    }
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t incl = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(incl, o, 64);
+        if ((int)lane >= o) incl += t;
+    }
+    if (lane == 63) ws[w] = incl;
+    __syncthreads();
+    uint32_t wp = 0, tot = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kBT / 64; q++) {
+        if (q < w) wp += ws[q];
+        tot += ws[q];
+    }
+    uint32_t run = incl - s + wp;
+#pragma unroll
+    for (uint32_t j = 0; j < kBScanItems; j++) {
+        if (b0 + j < n) loc[b0 + j] = run;
+        run += c[j];
+    }
+    if (threadIdx.x == 0) blk_tot[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_b_scan_top(uint32_t *__restrict__ blk, uint32_t nb,
+                                                     BGlobal *gl) {
+    __shared__ uint32_t ws[16];
+    const uint32_t per = (nb + 1023u) / 1024u;
+    const uint32_t b0 = min(threadIdx.x * per, nb), b1 = min(b0 + per, nb);
+    uint32_t s = 0;
+    for (uint32_t i = b0; i < b1; i++) s += blk[i];
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t incl = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(incl, o, 64);
+        if ((int)lane >= o) incl += t;
+    }
+    if (lane == 63) ws[w] = incl;
+    __syncthreads();
+    uint32_t wp = 0, tot = 0;
+    for (uint32_t q = 0; q < 16; q++) {
+        if (q < w) wp += ws[q];
+        tot += ws[q];
+    }
+    uint32_t run = incl - s + wp;
+    for (uint32_t i = b0; i < b1; i++) {
+        uint32_t v = blk[i];
+        blk[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0) gl->n_tree_total = tot;
+}
+
+// First node written by sorted body i (= total for i == n).
+__device__ __forceinline__ uint32_t node_start(const uint32_t *loc, const uint32_t *blk,
+                                               uint32_t i, uint32_t n, const BGlobal *gl) {
+    return i < n ? loc[i] + blk[i / kBScanTile] : gl->n_tree_total;
+}
+
+__global__ void k_b_graph_tree(const uint32_t *__restrict__ base, const uint32_t *__restrict__ count,
+                               uint32_t G, uint32_t n, const uint32_t *__restrict__ loc,
+                               const uint32_t *__restrict__ blk, const BGlobal *gl, GFrame *fr) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= G) return;
+    const uint32_t s = node_start(loc, blk, base[g], n, gl);
+    const uint32_t e = node_start(loc, blk, base[g] + count[g], n, gl);
+    fr[g].tree_start = s;
+    fr[g].n_tree = e - s;
+}
+
+__global__ __launch_bounds__(kBT) void k_b_emit(
+    const uint64_t *__restrict__ keys, const uint32_t *__restrict__ perm,
+    const float4 *__restrict__ pos, const uint32_t *__restrict__ deg,
+    const uint32_t *__restrict__ gid, const uint32_t *__restrict__ base,
+    const uint32_t *__restrict__ count, uint32_t n, const GFrame *fr, BGlobal *gl,
+    const uint32_t *__restrict__ loc, const uint32_t *__restrict__ blk,
+    float4 *__restrict__ node_a, uint2 *__restrict__ node_b, uint8_t *__restrict__ node_depth,
+    uint64_t cap) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Seg sg = seg_of(i, gid, base, count, fr);
+    if (i >= sg.ve) return;
+    const uint64_t k = keys[i];
+    int lp, li;
+    if (sg.ve - sg.lo == 1) {
+        lp = -1;
+        li = 0;
+    } else {
+        lp = i > sg.lo ? lcp_digits(keys[i - 1], k) : -1;
+        li = i + 1 < sg.ve ? lcp_digits(k, keys[i + 1]) : -1;
+    }
+    const uint32_t c = li > lp ? (uint32_t)(li - lp) : 0u;
+    const int a0 = lp + 1;
+    const uint32_t pos0 = node_start(loc, blk, i, n, gl);
+    if ((uint64_t)pos0 + c >= cap) return;
+    if (c) {
+        float h = fr[sg.g].root[3];
+        for (int d = 0; d < a0; d++) h = h * 0.5f;
+        for (uint32_t t = 0; t < c; t++) {
+            const int d = a0 + (int)t;
+            const uint32_t j = cell_end(keys, i, sg.ve, k, d);
+            const uint32_t skip = (j + 1 < sg.ve) ? node_start(loc, blk, j + 1, n, gl) : kEnd;
+            const float s = h * 2.0f;
+            node_b[pos0 + t] = make_uint2(__float_as_uint(s * s), skip);
+            node_depth[pos0 + t] = (uint8_t)d;
+            h = h * 0.5f;
+        }
+        atomicMax(&gl->max_depth, (uint32_t)(a0 + (int)c - 1));
+    }
+    const uint32_t leaf = pos0 + c;
+    const uint32_t b = perm[i];
+    if (b >= n) return;
+    const float4 p = pos[b];
+    node_a[leaf] = make_float4(p.x, p.y, p.z, fmaxf((float)deg[b] + 1.0f, 1e-3f));
+    node_b[leaf] = make_uint2(kLeaf | b, (i + 1 < sg.ve) ? leaf + 1 : kEnd);
+    node_depth[leaf] = (uint8_t)(0x80 | ((lp > li ? lp : li) + 1));
+}
+
+__global__ __launch_bounds__(kBT) void k_b_com_level(const BGlobal *gl, uint32_t d, uint32_t cap,
+                                                     float4 *__restrict__ node_a,
+                                                     const uint2 *__restrict__ node_b,
+                                                     const uint8_t *__restrict__ node_depth) {
+    if (d > gl->max_depth) return;
+    const uint32_t N = min(gl->n_tree_total, cap);
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
+        if (node_depth[p] != d) continue;
+        const uint32_t end = node_b[p].y;
+        float tm = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+        for (uint32_t ch = p + 1; ch < N && ch != end; ch = node_b[ch].y) {
+            const float4 a = node_a[ch];
+            tm = tm + a.w;
+            cx = cx + a.x * a.w;
+            cy = cy + a.y * a.w;
+            cz = cz + a.z * a.w;
+        }
+        if (tm > 0.0f) {
+            cx = cx / tm;
+            cy = cy / tm;
+            cz = cz / tm;
+        }
+        node_a[p] = make_float4(cx, cy, cz, tm);
+    }
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_b_force(const BParams P, const GFrame *fr,
+                                                const uint32_t *__restrict__ gid,
+                                                const float4 *__restrict__ pos,
+                                                const float4 *__restrict__ old_force,
+                                                float4 *__restrict__ force,
+                                                const uint32_t *__restrict__ deg,
+                                                const uint32_t *__restrict__ att_off,
+                                                const uint32_t *__restrict__ att_adj,
+                                                const uint32_t *__restrict__ perm,
+                                                const float4 *__restrict__ node_a,
+                                                const uint2 *__restrict__ node_b,
+                                                float2 *__restrict__ stats) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.n) return;
+    const uint32_t b = perm[t];
+    if (b >= P.n) return;
+    const uint32_t g = gid[b];
+    const float4 p = pos[b];
+    const float deg_i = (float)deg[b] + 1.0f;
+    const float krd = P.scaling_ratio * deg_i;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    const uint32_t N = fr[g].n_tree;
+    if (N > 0) {
+        const uint32_t start = fr[g].tree_start;
+        const uint64_t wcap = max((uint64_t)N * 4ull, 16ull);
+        uint64_t walk = 0;
+        uint32_t idx = start;
+        while (idx < P.cap && walk < wcap) {
+            walk++;
+            const float4 A = node_a[idx];
+            const uint2 B = node_b[idx];
+            const float dx = p.x - A.x, dy = p.y - A.y, dz = p.z - A.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            const bool leaf = (B.x & kLeaf) != 0u;
+            bool take;
+            if (leaf) take = ((B.x & ~kLeaf) != b) && A.w > 0.0f;
+            else take = A.w > 0.0f && r2 > 0.0f && __uint_as_float(B.x) < P.theta2 * r2;
+            if (take) {
+                const float r2c = fmaxf(r2, 1.0e-4f);
+                const float cf = krd * A.w / r2c;
+                fx = fx + dx * cf;
+                fy = fy + dy * cf;
+                fz = fz + dz * cf;
+            }
+            idx = (leaf || take) ? B.y : idx + 1u;
+        }
+    }
+    const uint32_t e0 = att_off[b], e1 = att_off[b + 1];
+    for (uint32_t e = e0; e < e1; e++) {
+        const float4 q = pos[att_adj[e]];
+        const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+        float att = P.ewi_weight;
+        if (P.lin_log_mode) {
+            const float r = fmaxf(len3(dx, dy, dz), 1.0e-3f);
+            att = P.ewi_weight * logf(1.0f + r) / r;
+        }
+        fx = fx + dx * att;
+        fy = fy + dy * att;
+        fz = fz + dz * att;
+    }
+    const float gd = P.gravity * deg_i;
+    if (P.strong_gravity) {
+        fx = fx - p.x * gd;
+        fy = fy - p.y * gd;
+        fz = fz - p.z * gd;
+    } else {
+        const float r0 = fmaxf(len3(p.x, p.y, p.z), 1.0e-3f);
+        fx = fx - (p.x / r0) * gd;
+        fy = fy - (p.y / r0) * gd;
+        fz = fz - (p.z / r0) * gd;
+    }
+    const float4 o = old_force[b];
+    force[b] = make_float4(fx, fy, fz, 0.0f);
+    stats[b] = make_float2(deg_i * len3(o.x - fx, o.y - fy, o.z - fz),
+                           deg_i * 0.5f * len3(o.x + fx, o.y + fy, o.z + fz));
+}
+
+// One block per graph: exact swing/traction sums over its bodies, then that
+// graph's AdaptiveSpeed::update (fa2_speed.rs:48-100).
+__global__ __launch_bounds__(256) void k_b_speed(const BParams P, GFrame *fr,
+                                                 const uint32_t *__restrict__ base,
+                                                 const uint32_t *__restrict__ count,
+                                                 const double *__restrict__ est_jt,
+                                                 const double *__restrict__ min_jt,
+                                                 const float2 *__restrict__ stats) {
+    __shared__ unsigned long long acc[2 * kLimbs];
+    const uint32_t g = blockIdx.x;
+    if (threadIdx.x < 2 * kLimbs) acc[threadIdx.x] = 0ull;
+    __syncthreads();
+    const uint32_t b0 = base[g], ng = count[g];
+    for (uint32_t i = threadIdx.x; i < ng; i += blockDim.x) {
+        const float2 s = stats[b0 + i];
+        if (finitef(s.x) && finitef(s.y)) {
+            exact_acc_add(acc, s.x);
+            exact_acc_add(acc + kLimbs, s.y);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    unsigned long long tot[2 * kLimbs];
+    for (uint32_t k = 0; k < 2 * kLimbs; k++) tot[k] = acc[k];
+    exact_normalise(tot);
+    exact_normalise(tot + kLimbs);
+    const double ts = exact_to_double(tot), tt = exact_to_double(tot + kLimbs);
+    fr[g].sum_swing = ts;
+    fr[g].sum_traction = tt;
+    float speed = fr[g].speed, eff = fr[g].speed_eff;
+    if (!(ts > 0.0) || !(tt > 0.0) || ng == 0) return;
+    const double nn = (double)ng, jt0 = (double)P.jitter_tolerance;
+    double jt = jt0 * fmax(min_jt[g], fmin(10.0, est_jt[g] * tt / (nn * nn)));
+    const float min_eff = 0.05f;
+    if (ts / tt > 2.0) {
+        if (eff > min_eff) eff = eff * 0.5f;
+        jt = fmax(jt, jt0);
+    }
+    const double target = jt * (double)eff * tt / ts;
+    if (ts > jt * tt) {
+        if (eff > min_eff) eff = eff * 0.7f;
+    } else if (speed < 1000.0f) {
+        eff = eff * 1.3f;
+    }
+    const double sp = (double)speed;
+    fr[g].speed = (float)(sp + fmin(target - sp, 0.5 * sp));
+    fr[g].speed_eff = eff;
+}
+
+__global__ __launch_bounds__(kBT) void k_b_apply(const BParams P, const GFrame *fr,
+                                                 const uint32_t *__restrict__ gid,
+                                                 const float4 *__restrict__ pos_in,
+                                                 float4 *__restrict__ pos_out,
+                                                 const float4 *__restrict__ force,
+                                                 float4 *__restrict__ old_force,
+                                                 const uint32_t *__restrict__ deg) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const float speed = fr[gid[i]].speed;
+    const float4 f = force[i];
+    const float4 o = old_force[i];
+    const float deg_i = (float)deg[i] + 1.0f;
+    const float swinging = deg_i * len3(o.x - f.x, o.y - f.y, o.z - f.z);
+    float factor = speed / (1.0f + sqrtf(speed * swinging));
+    const float df = len3(f.x, f.y, f.z);
+    if (P.max_displacement > 0.0f && df > 1.0e-9f) factor = fminf(factor, P.max_displacement / df);
+    const float k = factor * P.time_step;
+    const float4 p = pos_in[i];
+    pos_out[i] = make_float4(p.x + f.x * k, p.y + f.y * k, p.z + f.z * k, 0.0f);
+    old_force[i] = make_float4(f.x, f.y, f.z, 0.0f);
+}
+
+__global__ void k_b_init_speed(GFrame *fr, uint32_t G) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < G) {
+        fr[g].speed = 1.0f;
+        fr[g].speed_eff = 1.0f;
+    }
+}
+
+inline uint32_t cdivb(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace
+
+class Fa2BhBatch {
+  public:
+    explicit Fa2BhBatch(const jc_fa2bh_settings &s) : settings_(s) {}
+    ~Fa2BhBatch() { teardown(); }
+
+    Status init(uint32_t G, const uint32_t *n_nodes, const uint64_t *nnz, const uint32_t *offsets,
+                const uint32_t *neighbors, const float *xyz) {
+        JC_TRY(require_device(nullptr));
+        if (G == 0) return Status::err(JC_E_INVALID, "empty batch");
+        teardown();
+        G_ = G;
+        std::vector<uint32_t> base(G), count(G), gid;
+        std::vector<double> est(G), mjt(G);
+        uint64_t ntot = 0, off_pos = 0, nb_pos = 0;
+        for (uint32_t g = 0; g < G; g++) {
+            base[g] = (uint32_t)ntot;
+            count[g] = n_nodes[g];
+            ntot += n_nodes[g];
+            if (ntot >= 0x7FFFFFFFull) return Status::err(JC_E_INVALID, "batch too large");
+        }
+        n_ = (uint32_t)ntot;
+        gid.resize(n_ ? n_ : 1);
+        std::vector<uint32_t> degs(n_ ? n_ : 1, 0), aoff((size_t)n_ + 1, 0), adj;
+        // per graph: validate, degrees and attraction lists (fa2_bh.rs:264-287)
+        std::vector<uint64_t> cnt((size_t)n_ + 1, 0);
+        for (uint32_t g = 0; g < G; g++) {
+            const uint32_t ng = n_nodes[g];
+            const uint32_t *off = offsets + off_pos;
+            const uint32_t *nb = neighbors + nb_pos;
+            CsrView v{ng, off, nb, nnz[g]};
+            Status s = validate_csr(v);
+            if (s.failed())
+                return Status::err(JC_E_INVALID, "graph " + std::to_string(g) + ": " + last_error());
+            est[g] = 0.05 * std::sqrt((double)ng);
+            mjt[g] = std::sqrt(est[g]);
+            for (uint32_t v2 = 0; v2 < ng; v2++) {
+                const uint32_t gv = base[g] + v2;
+                gid[gv] = g;
+                for (uint64_t e = off[v2]; e < off[v2 + 1]; e++) {
+                    const uint32_t u = nb[e];
+                    if (u == v2) continue;
+                    degs[gv]++;
+                    if (v2 < u) {
+                        cnt[gv + 1]++;
+                        cnt[base[g] + u + 1]++;
+                    }
+                }
+            }
+            off_pos += (uint64_t)ng + 1;
+            nb_pos += nnz[g];
+        }
+        for (uint32_t v2 = 0; v2 < n_; v2++) cnt[v2 + 1] += cnt[v2];
+        if (cnt[n_] > 0xFFFFFFFFull) return Status::err(JC_E_INVALID, "too many edges");
+        for (uint32_t v2 = 0; v2 <= n_; v2++) aoff[v2] = (uint32_t)cnt[v2];
+        adj.resize(cnt[n_] ? cnt[n_] : 1);
+        off_pos = nb_pos = 0;
+        for (uint32_t g = 0; g < G; g++) {
+            const uint32_t ng = n_nodes[g];
+            const uint32_t *off = offsets + off_pos;
+            const uint32_t *nb = neighbors + nb_pos;
+            for (uint32_t v2 = 0; v2 < ng; v2++)
+                for (uint64_t e = off[v2]; e < off[v2 + 1]; e++) {
+                    const uint32_t u = nb[e];
+                    if (u == v2 || !(v2 < u)) continue;
+                    adj[cnt[base[g] + v2]++] = base[g] + u;
+                    adj[cnt[base[g] + u]++] = base[g] + v2;
+                }
+            off_pos += (uint64_t)ng + 1;
+            nb_pos += nnz[g];
+        }
+        gbits_ = 0;
+        while ((1u << gbits_) < G) gbits_++;
+        gbits_ = (gbits_ + 7) / 8 * 8;
+
+        JC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        const uint32_t nn = n_ ? n_ : 1;
+        JC_TRY(pos_[0].alloc(nn));
+        JC_TRY(pos_[1].alloc(nn));
+        JC_TRY(old_force_.alloc(nn));
+        JC_TRY(force_.alloc(nn));
+        JC_TRY(stats_.alloc(nn));
+        JC_TRY(deg_.alloc(nn));
+        JC_TRY(gid_.alloc(nn));
+        JC_TRY(base_.alloc(G));
+        JC_TRY(count_.alloc(G));
+        JC_TRY(est_.alloc(G));
+        JC_TRY(mjt_.alloc(G));
+        JC_TRY(att_off_.alloc((size_t)n_ + 1));
+        JC_TRY(att_adj_.alloc(adj.size()));
+        JC_TRY(keys_raw_.alloc(nn));
+        JC_TRY(keys_.alloc(nn));
+        JC_TRY(vals_.alloc(nn));
+        JC_TRY(sort_.alloc(n_));
+        nb_scan_ = cdivb(nn, kBScanTile);
+        JC_TRY(loc_.alloc(nn));
+        JC_TRY(blk_.alloc(nb_scan_ + 1));
+        // per graph: n_g leaves + at most n_g/2 cells per depth over 22 depths + 1
+        cap_ = 0;
+        for (uint32_t g = 0; g < G; g++) cap_ += (uint64_t)count[g] + (uint64_t)(kLevels + 1) * (count[g] / 2 + 1) + 1;
+        cap_ += 8;
+        if (cap_ >= 0xFFFFFFF0ull) return Status::err(JC_E_INVALID, "batch too large for u32 node ids");
+        JC_TRY(node_a_.alloc(cap_));
+        JC_TRY(node_b_.alloc(cap_));
+        JC_TRY(node_depth_.alloc(cap_));
+        JC_TRY(frame_.alloc(G));
+        JC_TRY(global_.alloc(1));
+        std::vector<float4> p4(nn, make_float4(0, 0, 0, 0));
+        for (uint32_t i = 0; i < n_; i++)
+            p4[i] = make_float4(xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2], 0.0f);
+        auto up = [&](void *d, const void *h, size_t bytes) -> Status {
+            JC_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream_));
+            return Status::ok();
+        };
+        JC_TRY(up(pos_[0].p, p4.data(), nn * sizeof(float4)));
+        JC_TRY(up(pos_[1].p, p4.data(), nn * sizeof(float4)));
+        JC_HIP(hipMemsetAsync(old_force_.p, 0, nn * sizeof(float4), stream_));
+        JC_TRY(up(deg_.p, degs.data(), nn * 4));
+        JC_TRY(up(gid_.p, gid.data(), nn * 4));
+        JC_TRY(up(base_.p, base.data(), G * 4));
+        JC_TRY(up(count_.p, count.data(), G * 4));
+        JC_TRY(up(est_.p, est.data(), G * 8));
+        JC_TRY(up(mjt_.p, mjt.data(), G * 8));
+        JC_TRY(up(att_off_.p, aoff.data(), ((size_t)n_ + 1) * 4));
+        JC_TRY(up(att_adj_.p, adj.data(), adj.size() * 4));
+        JC_HIP(hipMemsetAsync(frame_.p, 0, G * sizeof(GFrame), stream_));
+        JC_HIP(hipMemsetAsync(global_.p, 0, sizeof(BGlobal), stream_));
+        k_b_init_speed<<<cdivb(G, 256), 256, 0, stream_>>>(frame_.p, G);
+        JC_HIP(hipGetLastError());
+        JC_HIP(hipStreamSynchronize(stream_));
+        base_h_ = base;
+        count_h_ = count;
+        cur_ = 0;
+        iteration_ = 0;
+        inited_ = true;
+        params_dirty_ = true;
+        return Status::ok();
+    }
+
+    Status step(uint32_t n_iters, float *out_xyz) {
+        if (!inited_) return Status::err(JC_E_STATE, "batch step before init");
+        refresh_params();
+        if (n_ && n_iters) {
+            if (!tuned_) JC_TRY(autotune());
+            uint32_t left = n_iters;
+            if (left >= 2) {
+                JC_TRY(ensure_graph());
+                if (cur_ == 1) {
+                    JC_TRY(enqueue_iteration(cur_));
+                    cur_ ^= 1;
+                    left--;
+                }
+                while (left >= 2) {
+                    JC_HIP(hipGraphLaunch(graph_exec_, stream_));
+                    left -= 2;
+                }
+            }
+            while (left) {
+                JC_TRY(enqueue_iteration(cur_));
+                cur_ ^= 1;
+                left--;
+            }
+            iteration_ += n_iters;
+        }
+        if (out_xyz) JC_TRY(read_positions(out_xyz));
+        return Status::ok();
+    }
+
+    Status speeds(float *se) {
+        std::vector<GFrame> h(G_);
+        JC_HIP(hipMemcpyAsync(h.data(), frame_.p, G_ * sizeof(GFrame), hipMemcpyDeviceToHost, stream_));
+        JC_HIP(hipStreamSynchronize(stream_));
+        for (uint32_t g = 0; g < G_; g++) {
+            se[2 * g] = h[g].speed;
+            se[2 * g + 1] = h[g].speed_eff;
+        }
+        return Status::ok();
+    }
+
+    Status timed(uint32_t n_iters, double *ms) {
+        hipEvent_t a, b;
+        JC_HIP(hipEventCreate(&a));
+        JC_HIP(hipEventCreate(&b));
+        if (!tuned_ && n_) JC_TRY(autotune());
+        JC_HIP(hipEventRecord(a, stream_));
+        Status s = step(n_iters, nullptr);
+        JC_HIP(hipEventRecord(b, stream_));
+        JC_HIP(hipEventSynchronize(b));
+        float t = 0;
+        JC_HIP(hipEventElapsedTime(&t, a, b));
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        if (ms) *ms = t;
+        return s;
+    }
+
+    uint32_t force_block() const { return force_bs_; }
+
+  private:
+    void refresh_params() {
+        if (!params_dirty_) return;
+        const jc_fa2bh_settings &s = settings_;
+        P_.n = n_;
+        P_.n_graphs = G_;
+        P_.gravity = s.gravity;
+        P_.strong_gravity = s.strong_gravity ? 1u : 0u;
+        P_.scaling_ratio = s.scaling_ratio;
+        P_.jitter_tolerance = s.jitter_tolerance;
+        P_.lin_log_mode = s.lin_log_mode ? 1u : 0u;
+        P_.time_step = s.time_step;
+        const float th = std::fmax(s.theta, 0.0f);
+        P_.theta2 = th * th;
+        P_.max_displacement = s.max_displacement;
+        P_.ewi_weight = std::pow(std::fmax(1.0f, 1.0e-6f), s.edge_weight_influence);
+        P_.cap = (uint32_t)cap_;
+        params_dirty_ = false;
+        drop_graph();
+    }
+
+    Status enqueue_tree(const float4 *pos) {
+        const uint32_t n = n_;
+        k_b_reset<<<cdivb(G_, 256), 256, 0, stream_>>>(frame_.p, G_, global_.p);
+        k_b_bbox<<<cdivb(n, kBT), kBT, 0, stream_>>>(pos, gid_.p, n, frame_.p);
+        k_b_keys<<<cdivb(n, kBT), kBT, 0, stream_>>>(pos, gid_.p, base_.p, n, frame_.p,
+                                                     keys_raw_.p, keys_.p, vals_.p);
+        JC_TRY(radix_sort_pairs_u64(keys_.p, vals_.p, sort_, stream_, 0, 64));
+        if (G_ > 1) {
+            k_b_gid_keys<<<cdivb(n, 256), 256, 0, stream_>>>(vals_.p, gid_.p, keys_.p, n);
+            JC_TRY(radix_sort_pairs_u64(keys_.p, vals_.p, sort_, stream_, 0, gbits_));
+            k_b_gather_keys<<<cdivb(n, 256), 256, 0, stream_>>>(vals_.p, keys_raw_.p, keys_.p, n);
+        }
+        k_b_count_scan<<<nb_scan_, kBT, 0, stream_>>>(keys_.p, gid_.p, base_.p, count_.p, n,
+                                                      frame_.p, loc_.p, blk_.p);
+        k_b_scan_top<<<1, 1024, 0, stream_>>>(blk_.p, nb_scan_, global_.p);
+        k_b_graph_tree<<<cdivb(G_, 256), 256, 0, stream_>>>(base_.p, count_.p, G_, n, loc_.p,
+                                                            blk_.p, global_.p, frame_.p);
+        k_b_emit<<<cdivb(n, kBT), kBT, 0, stream_>>>(keys_.p, vals_.p, pos, deg_.p, gid_.p,
+                                                     base_.p, count_.p, n, frame_.p, global_.p,
+                                                     loc_.p, blk_.p, node_a_.p, node_b_.p,
+                                                     node_depth_.p, cap_);
+        const uint32_t gcom = std::min<uint32_t>(cdivb(cap_, kBT), 4096);
+        for (int d = kLevels; d >= 0; d--)
+            k_b_com_level<<<gcom, kBT, 0, stream_>>>(global_.p, (uint32_t)d, (uint32_t)cap_,
+                                                     node_a_.p, node_b_.p, node_depth_.p);
+        JC_HIP(hipGetLastError());
+        return Status::ok();
+    }
+
+    Status launch_force(const float4 *pos, uint32_t bs) {
+        const uint32_t grid = cdivb(n_, bs);
+#define JC_BF(BS)                                                                           \
+    k_b_force<BS><<<grid, BS, 0, stream_>>>(P_, frame_.p, gid_.p, pos, old_force_.p, force_.p, \
+                                            deg_.p, att_off_.p, att_adj_.p, vals_.p,          \
+                                            node_a_.p, node_b_.p, stats_.p)
+        switch (bs) {
+            case 64: JC_BF(64); break;
+            case 128: JC_BF(128); break;
+            default: JC_BF(256); break;
+        }
+#undef JC_BF
+        JC_HIP(hipGetLastError());
+        return Status::ok();
+    }
+
+    Status enqueue_iteration(uint32_t c) {
+        JC_TRY(enqueue_tree(pos_[c].p));
+        JC_TRY(launch_force(pos_[c].p, force_bs_));
+        k_b_speed<<<G_, 256, 0, stream_>>>(P_, frame_.p, base_.p, count_.p, est_.p, mjt_.p, stats_.p);
+        k_b_apply<<<cdivb(n_, kBT), kBT, 0, stream_>>>(P_, frame_.p, gid_.p, pos_[c].p,
+                                                       pos_[c ^ 1].p, force_.p, old_force_.p,
+                                                       deg_.p);
+        JC_HIP(hipGetLastError());
+        return Status::ok();
+    }
+
+    // Times the force kernel for each candidate block size on the current
+    // positions (tree built once; no state is advanced) and keeps the fastest.
+    Status autotune() {
+        JC_TRY(enqueue_tree(pos_[cur_].p));
+        hipEvent_t a, b;
+        JC_HIP(hipEventCreate(&a));
+        JC_HIP(hipEventCreate(&b));
+        float best = 1e30f;
+        for (uint32_t bs : {64u, 128u, 256u}) {
+            JC_TRY(launch_force(pos_[cur_].p, bs));  // warm
+            JC_HIP(hipEventRecord(a, stream_));
+            for (int r = 0; r < 3; r++) JC_TRY(launch_force(pos_[cur_].p, bs));
+            JC_HIP(hipEventRecord(b, stream_));
+            JC_HIP(hipEventSynchronize(b));
+            float t = 0;
+            JC_HIP(hipEventElapsedTime(&t, a, b));
+            if (t < best) {
+                best = t;
+                force_bs_ = bs;
+            }
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        tuned_ = true;
+        drop_graph();
+        return Status::ok();
+    }
+
+    Status ensure_graph() {
+        if (graph_exec_) return Status::ok();
+        JC_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        Status s1 = enqueue_iteration(0);
+        Status s2 = s1.failed() ? s1 : enqueue_iteration(1);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(stream_, &g);
+        if (s2.failed()) return s2;
+        if (e != hipSuccess) return Status::err(JC_E_HIP, hipGetErrorString(e));
+        graph_ = g;
+        JC_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+        return Status::ok();
+    }
+
+    void drop_graph() {
+        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+        graph_exec_ = nullptr;
+        graph_ = nullptr;
+    }
+
+    Status read_positions(float *out) {
+        std::vector<float4> h(n_);
+        if (n_) {
+            JC_HIP(hipMemcpyAsync(h.data(), pos_[cur_].p, n_ * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+            JC_HIP(hipStreamSynchronize(stream_));
+        }
+        for (uint32_t i = 0; i < n_; i++) {
+            out[3 * (size_t)i] = h[i].x;
+            out[3 * (size_t)i + 1] = h[i].y;
+            out[3 * (size_t)i + 2] = h[i].z;
+        }
+        return Status::ok();
+    }
+
+    void teardown() {
+        if (stream_) (void)hipStreamSynchronize(stream_);
+        drop_graph();
+        if (stream_) (void)hipStreamDestroy(stream_);
+        stream_ = nullptr;
+        inited_ = false;
+        tuned_ = false;
+    }
+
+    jc_fa2bh_settings settings_;
+    BParams P_{};
+    bool params_dirty_ = true, inited_ = false, tuned_ = false;
+    uint32_t G_ = 0, n_ = 0, cur_ = 0, nb_scan_ = 0, gbits_ = 0, force_bs_ = 256;
+    uint64_t cap_ = 0, iteration_ = 0;
+    std::vector<uint32_t> base_h_, count_h_;
+    hipStream_t stream_ = nullptr;
+    DevBuf<float4> pos_[2], old_force_, force_;
+    DevBuf<float2> stats_;
+    DevBuf<uint32_t> deg_, gid_, base_, count_, att_off_, att_adj_, vals_, loc_, blk_;
+    DevBuf<double> est_, mjt_;
+    DevBuf<uint64_t> keys_raw_, keys_;
+    RadixScratch sort_;
+    DevBuf<float4> node_a_;
+    DevBuf<uint2> node_b_;
+    DevBuf<uint8_t> node_depth_;
+    DevBuf<GFrame> frame_;
+    DevBuf<BGlobal> global_;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+};
+
+}  // namespace jc
+
+struct jc_batch {
+    jc::Fa2BhBatch impl;
+    explicit jc_batch(const jc_fa2bh_settings &s) : impl(s) {}
+};
+
+extern "C" {
+
+int jc_fa2bh_batch_create(const jc_fa2bh_settings *s, jc_batch **out) {
+    if (!out) {
+        jc::set_error("NULL output handle");
+        return JC_E_INVALID;
+    }
+    jc_fa2bh_settings d;
+    jc_fa2bh_default_settings(&d);
+    *out = new jc_batch(s ? *s : d);
+    return JC_OK;
+}
+
+int jc_fa2bh_batch_init(jc_batch *b, uint32_t n_graphs, const uint32_t *n_nodes,
+                        const uint64_t *nnz, const uint32_t *offsets, const uint32_t *neighbors,
+                        const float *xyz) {
+    if (!b) {
+        jc::set_error("NULL batch");
+        return JC_E_INVALID;
+    }
+    jc::Status s = b->impl.init(n_graphs, n_nodes, nnz, offsets, neighbors, xyz);
+    if (!s.failed()) jc::set_error("");
+    return s.code;
+}
+
+int jc_fa2bh_batch_step(jc_batch *b, uint32_t n_iters, float *out_xyz) {
+    if (!b) {
+        jc::set_error("NULL batch");
+        return JC_E_INVALID;
+    }
+    jc::Status s = b->impl.step(n_iters, out_xyz);
+    if (!s.failed()) jc::set_error("");
+    return s.code;
+}
+
+int jc_fa2bh_batch_speeds(jc_batch *b, float *speed_eff) {
+    if (!b) {
+        jc::set_error("NULL batch");
+        return JC_E_INVALID;
+    }
+    return b->impl.speeds(speed_eff).code;
+}
+
+int jc_fa2bh_batch_step_timed(jc_batch *b, uint32_t n_iters, double *gpu_ms, uint32_t *force_block) {
+    if (!b) {
+        jc::set_error("NULL batch");
+        return JC_E_INVALID;
+    }
+    jc::Status s = b->impl.timed(n_iters, gpu_ms);
+    if (force_block) *force_block = b->impl.force_block();
+    if (!s.failed()) jc::set_error("");
+    return s.code;
+}
+
+void jc_fa2bh_batch_destroy(jc_batch *b) { delete b; }
+
+}  // extern "C"

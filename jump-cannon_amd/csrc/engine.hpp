@@ -63,4 +63,8 @@ Status fa2bh_last_stats(LayoutEngine *e, uint64_t *n_tree, uint64_t *interaction
 Status fa2bh_profile(LayoutEngine *e, uint32_t n_iters, double *phase_ms, uint32_t n_phases);
 bool is_fa2bh(const LayoutEngine *e);
 
+// SGD stress hooks (sgd.hip)
+Status sgd_pivots(LayoutEngine *e, uint32_t *k, uint32_t *pivots, uint16_t *dist, float *eta_max);
+Status sgd_time(LayoutEngine *e, uint32_t n, double *ms);
+
 }  // namespace jc

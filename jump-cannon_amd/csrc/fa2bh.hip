@@ -33,17 +33,15 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "exact_sum.hpp"
+#include "fa2_device.hpp"
 
 namespace jc {
 namespace {
 
-constexpr int kLevels = 21;
-constexpr uint32_t kLeaf = 0x80000000u;
-constexpr uint32_t kEnd = 0xFFFFFFFFu;
 constexpr uint32_t kTB = 256;             // threads per block for per-node kernels
 constexpr uint32_t kScanItems = 16;       // count/scan tile = 4096 bodies
 constexpr uint32_t kScanTile = kTB * kScanItems;
-constexpr uint32_t kLimbs = 10;           // exact-sum limbs per quantity
 constexpr uint32_t kTimedIters = 10;      // timed iterations per host read-back of events
 
 struct Frame {
@@ -76,48 +74,6 @@ struct Params {
     double min_jt;     // sqrt(est_jt)             (fa2_speed.rs:73)
     uint32_t cap;      // node capacity of the octree arrays
 };
-
-// ---------------------------------------------------------------- helpers
-__device__ __forceinline__ uint32_t f2ord(float f) {
-    uint32_t u = __float_as_uint(f);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-__device__ __forceinline__ float ord2f(uint32_t o) {
-    uint32_t u = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
-    return __uint_as_float(u);
-}
-__device__ __forceinline__ bool finitef(float x) { return __builtin_isfinite(x); }
-__device__ __forceinline__ float len3(float x, float y, float z) {
-    return sqrtf(x * x + y * y + z * z);
-}
-
-// Root cube of fa2_bh.rs:768-799 from the reduced box.
-__device__ __forceinline__ void compute_root(const Frame *fr, float *r4) {
-    float mn[3], mx[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        mn[k] = ord2f(fr->bbmin[k]);
-        mx[k] = ord2f(fr->bbmax[k]);
-    }
-    if (!finitef(mn[0])) {
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            mn[k] = -1.0f;
-            mx[k] = 1.0f;
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < 3; k++) r4[k] = 0.5f * (mn[k] + mx[k]);
-    float half = fmaxf(fmaxf(mx[0] - mn[0], mx[1] - mn[1]), mx[2] - mn[2]) * 0.5f;
-    if (!finitef(half) || half <= 0.0f) half = 1.0f;
-    r4[3] = half * 1.05f;
-}
-
-__device__ __forceinline__ int lcp_digits(uint64_t a, uint64_t b) {
-    uint64_t x = a ^ b;
-    if (!x) return kLevels;
-    return (__clzll((long long)x) - 1) / 3;
-}
 
 // ---------------------------------------------------------------- kernels
 __global__ void k_frame_reset(Frame *fr) {
@@ -173,7 +129,7 @@ __global__ __launch_bounds__(kTB) void k_keys(const float4 *__restrict__ pos, ui
                                               uint32_t *__restrict__ vals) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     float r4[4];
-    compute_root(fr, r4);
+    compute_root(fr->bbmin, fr->bbmax, r4);
     if (i == 0) {
         for (int k = 0; k < 4; k++) fr->root[k] = r4[k];
     }
@@ -181,17 +137,7 @@ __global__ __launch_bounds__(kTB) void k_keys(const float4 *__restrict__ pos, ui
     float4 p = pos[i];
     uint64_t key;
     if (finitef(p.x) && finitef(p.y) && finitef(p.z)) {
-        float cx = r4[0], cy = r4[1], cz = r4[2], h = r4[3];
-        key = 0;
-#pragma unroll
-        for (int l = 0; l < kLevels; l++) {
-            uint32_t o = (p.x >= cx ? 1u : 0u) | (p.y >= cy ? 2u : 0u) | (p.z >= cz ? 4u : 0u);
-            key = (key << 3) | o;
-            h = h * 0.5f;
-            cx = cx + ((o & 1u) ? h : -h);
-            cy = cy + ((o & 2u) ? h : -h);
-            cz = cz + ((o & 4u) ? h : -h);
-        }
+        key = octant_key(r4, p.x, p.y, p.z);
     } else {
         key = ~0ull;
         atomicAdd(&fr->n_invalid, 1u);
@@ -288,30 +234,6 @@ __device__ __forceinline__ uint32_t first_node_of(const uint32_t *loc, const uin
     return loc[i] + blk[i / kScanTile] + i;
 }
 
-// Last sorted index j >= i whose first d octal digits equal those of key k.
-__device__ __forceinline__ uint32_t cell_end(const uint64_t *keys, uint32_t i, uint32_t nv,
-                                             uint64_t k, int d) {
-    if (d == 0) return nv - 1;
-    const int sh = 3 * (kLevels - d);
-    const uint64_t pre = k >> sh;
-    uint32_t lo = i, step = 1, hi;
-    for (;;) {
-        uint32_t probe = lo + step;
-        if (probe >= nv || (keys[probe] >> sh) != pre) {
-            hi = probe < nv ? probe : nv;
-            break;
-        }
-        lo = probe;
-        step <<= 1;
-    }
-    while (hi - lo > 1) {
-        uint32_t mid = lo + ((hi - lo) >> 1);
-        if ((keys[mid] >> sh) == pre) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
                                               const uint32_t *__restrict__ perm,
                                               const float4 *__restrict__ pos,
@@ -389,19 +311,6 @@ __global__ __launch_bounds__(kTB) void k_com_level(const Frame *fr, uint32_t d, 
         }
         node_a[p] = make_float4(cx, cy, cz, tm);
     }
-}
-
-// Exact accumulation of a non-negative finite f32 into 32-bit limbs held in u64
-// (LSB = 2^-149): the mantissa lands in at most two limbs.
-__device__ __forceinline__ void acc_add(unsigned long long *acc, float v) {
-    uint32_t u = __float_as_uint(v) & 0x7FFFFFFFu;
-    if (!u) return;
-    uint32_t e = u >> 23, m = u & 0x7FFFFFu;
-    if (e) m |= 0x800000u;
-    uint32_t sh = (e ? e : 1u) - 1u;
-    uint64_t t = (uint64_t)m << (sh & 31u);
-    atomicAdd(&acc[sh >> 5], (unsigned long long)(t & 0xFFFFFFFFull));
-    if (t >> 32) atomicAdd(&acc[(sh >> 5) + 1], (unsigned long long)(t >> 32));
 }
 
 __global__ __launch_bounds__(kTB) void k_force(const Params P, Frame *fr,
@@ -486,8 +395,8 @@ __global__ __launch_bounds__(kTB) void k_force(const Params P, Frame *fr,
         const float swing = deg_i * len3(o.x - fx, o.y - fy, o.z - fz);
         const float tra = deg_i * 0.5f * len3(o.x + fx, o.y + fy, o.z + fz);
         if (finitef(swing) && finitef(tra)) {
-            acc_add(acc, swing);
-            acc_add(acc + kLimbs, tra);
+            exact_acc_add(acc, swing);
+            exact_acc_add(acc + kLimbs, tra);
         }
     }
 #pragma unroll
@@ -496,35 +405,6 @@ __global__ __launch_bounds__(kTB) void k_force(const Params P, Frame *fr,
     __syncthreads();
     if (threadIdx.x < 2 * kLimbs) partial[blockIdx.x * 2 * kLimbs + threadIdx.x] = acc[threadIdx.x];
     if (threadIdx.x == 0) atomicAdd(&fr->interactions, inter);
-}
-
-// Exact limb sum -> nearest f64, ties to even (same algorithm as the oracle).
-__device__ double limbs_to_double(const unsigned long long *in) {
-    unsigned long long l[kLimbs];
-    for (uint32_t k = 0; k < kLimbs; k++) l[k] = in[k];
-    for (uint32_t k = 0; k + 1 < kLimbs; k++) {
-        l[k + 1] += l[k] >> 32;
-        l[k] &= 0xFFFFFFFFull;
-    }
-    int top = -1;
-    for (int k = (int)kLimbs - 1; k >= 0; k--)
-        if (l[k]) {
-            top = 32 * k + 63 - __clzll((long long)l[k]);
-            break;
-        }
-    if (top < 0) return 0.0;
-    auto bit = [&](int q) -> uint32_t {
-        return q < 0 ? 0u : (uint32_t)((l[q >> 5] >> (q & 31)) & 1ull);
-    };
-    unsigned long long mant = 0;
-    const int lo = top - 52;
-    for (int q = top; q >= (lo > 0 ? lo : 0); q--) mant = (mant << 1) | bit(q);
-    if (lo <= 0) return ldexp((double)mant, -149);
-    const uint32_t rb = bit(lo - 1);
-    uint32_t sticky = 0;
-    for (int q = lo - 2; q >= 0 && !sticky; q--) sticky |= bit(q);
-    if (rb && (sticky || (mant & 1ull))) mant++;
-    return ldexp((double)mant, lo - 149);
 }
 
 // Sums the per-block limbs, then runs AdaptiveSpeed::update (fa2_speed.rs:48-100)
@@ -549,7 +429,9 @@ __global__ __launch_bounds__(256) void k_speed(const Params P, Frame *fr,
         for (uint32_t g = 0; g < G; g++) s += red[g][k];
         tot[k] = s;
     }
-    const double ts = limbs_to_double(tot), tt = limbs_to_double(tot + kLimbs);
+    exact_normalise(tot);
+    exact_normalise(tot + kLimbs);
+    const double ts = exact_to_double(tot), tt = exact_to_double(tot + kLimbs);
     fr->sum_swing = ts;
     fr->sum_traction = tt;
     if (!update_speed) return;

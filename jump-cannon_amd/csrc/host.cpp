@@ -424,6 +424,16 @@ int jc_fa2bh_profile(jc_engine *e, uint32_t n_iters, double *phase_ms, uint32_t 
     return finish(jc::fa2bh_profile(e->impl, n_iters, phase_ms, n_phases));
 }
 
+int jc_sgd_pivots(jc_engine *e, uint32_t *k, uint32_t *pivots, uint16_t *dist, float *eta_max) {
+    if (!e) return bad_handle();
+    return finish(jc::sgd_pivots(e->impl, k, pivots, dist, eta_max));
+}
+
+int jc_sgd_time(jc_engine *e, uint32_t n_steps, double *gpu_ms) {
+    if (!e) return bad_handle();
+    return finish(jc::sgd_time(e->impl, n_steps, gpu_ms));
+}
+
 // ------------------------------------------------------------------ fixtures
 static inline uint64_t sm_next(uint64_t &s) {
     s += 0x9E3779B97F4A7C15ull;

@@ -73,8 +73,9 @@ struct DevBuf {
 };
 
 // ---- device-wide primitives (prims.hip) ----
-// Stable LSD radix sort of (key,val) pairs by all 64 key bits.  `n` is fixed for
-// a given scratch set so the launches can be captured into a hipGraph.
+// Stable LSD radix sort of (key,val) pairs by key bits [begin_bit, end_bit)
+// (8-bit digits).  `n` is fixed for a given scratch set so the launches can be
+// captured into a hipGraph.
 struct RadixScratch {
     DevBuf<uint64_t> keys_alt;
     DevBuf<uint32_t> vals_alt;
@@ -82,7 +83,8 @@ struct RadixScratch {
     uint32_t n = 0, n_blocks = 0;
     Status alloc(uint32_t n);
 };
-Status radix_sort_pairs_u64(uint64_t *keys, uint32_t *vals, RadixScratch &s, hipStream_t st);
+Status radix_sort_pairs_u64(uint64_t *keys, uint32_t *vals, RadixScratch &s, hipStream_t st,
+                            uint32_t begin_bit = 0, uint32_t end_bit = 64);
 
 constexpr uint32_t kSortThreads = 256;
 constexpr uint32_t kSortItems = 8;

@@ -1,6 +1,398 @@
+// PageRank on gfx950: pull power iteration over the symmetrised CSR.
+//
+// Reference: crates/graph-compute/src/analytics/pagerank.rs:67-287 (host set-up,
+// ping-pong ranks, one submit) and shaders/pagerank.wgsl:30-46 (one thread per
+// row: rank_out[v] = teleport + d * sum_{u in N(v)} rank_in[u] * inv_deg[u]).
+//
+// Layout per iteration: every kernel writes rank_out[v] and x_out[v] =
+// rank_out[v] * inv_deg[v], so the next iteration gathers one f32 per CSR entry
+// (the product is the one the reference forms per edge, rounded identically).
+// Rows are summed in CSR order exactly as the reference thread does:
+//   light rows (<= kHeavy entries): one thread per row;
+//   hub rows: one wave per row gathers the x values into LDS in parallel, then
+//   one lane adds them in order -- the hub's sum stays bit-identical while its
+//   gathers run 64-wide.
+// The L1 residual sum_v |rank_out - rank_in| is accumulated exactly (max into
+// one fixed-point accumulator, min into another) and rounded once to f64, so
+// the stopping decision is deterministic and identical to the oracle's.  The
+// convergence flag lives on the device: iterations after convergence exit at
+// once, so a whole chunk of iterations runs without a host round trip.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 #include "engine.hpp"
-extern "C" int jc_pagerank(uint32_t, const uint32_t *, const uint32_t *, uint64_t, float, uint32_t,
-                           double, float *, uint32_t *, double *) {
-    jc::set_error("not built");
-    return JC_E_STATE;
+#include "exact_sum.hpp"
+
+struct jc_pagerank_plan;
+
+namespace jc {
+namespace {
+
+constexpr uint32_t kPrTB = 256;
+constexpr uint32_t kHeavy = 96;     // rows with more CSR entries go to the hub kernel
+constexpr uint32_t kHubChunk = 1024;
+constexpr uint32_t kChunkIters = 10;
+
+struct PrState {
+    uint32_t converged;
+    uint32_t iters;
+    double resid;
+    double tol;
+};
+
+__device__ __forceinline__ void resid_add(unsigned long long *acc, float r_new, float r_old) {
+    // |r_new - r_old| = max - min, both non-negative ranks: exact as two sums
+    exact_acc_add(acc, fmaxf(r_new, r_old));
+    exact_acc_add(acc + kLimbs, fminf(r_new, r_old));
 }
+
+__global__ __launch_bounds__(kPrTB) void k_pr_init(uint32_t n, float inv_n,
+                                                   const float *__restrict__ inv,
+                                                   float *__restrict__ rank0,
+                                                   float *__restrict__ x0, PrState *st, double tol) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v == 0) {
+        st->converged = 0;
+        st->iters = 0;
+        st->resid = 0.0;
+        st->tol = tol;
+    }
+    if (v >= n) return;
+    rank0[v] = inv_n;
+    x0[v] = inv_n * inv[v];
+}
+
+__global__ __launch_bounds__(kPrTB) void k_pr_light(
+    uint32_t n, const uint32_t *__restrict__ off, const uint32_t *__restrict__ nbr,
+    const float *__restrict__ inv, const float *__restrict__ x_in,
+    const float *__restrict__ rank_in, float *__restrict__ rank_out, float *__restrict__ x_out,
+    float teleport, float damping, const PrState *st, unsigned long long *__restrict__ partial) {
+    if (st->converged) return;
+    __shared__ unsigned long long acc[2 * kLimbs];
+    if (threadIdx.x < 2 * kLimbs) acc[threadIdx.x] = 0ull;
+    __syncthreads();
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) {
+        const uint32_t s = off[v], e = off[v + 1];
+        if (e - s <= kHeavy) {
+            float sum = 0.0f;
+            uint32_t k = s;
+            for (; k + 4 <= e; k += 4) {  // four independent gathers in flight
+                const uint32_t u0 = nbr[k], u1 = nbr[k + 1], u2 = nbr[k + 2], u3 = nbr[k + 3];
+                const float a0 = x_in[u0], a1 = x_in[u1], a2 = x_in[u2], a3 = x_in[u3];
+                sum = sum + a0;
+                sum = sum + a1;
+                sum = sum + a2;
+                sum = sum + a3;
+            }
+            for (; k < e; k++) sum = sum + x_in[nbr[k]];
+            const float r = teleport + damping * sum;
+            rank_out[v] = r;
+            x_out[v] = r * inv[v];
+            resid_add(acc, r, rank_in[v]);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * kLimbs) partial[(size_t)blockIdx.x * 2 * kLimbs + threadIdx.x] = acc[threadIdx.x];
+}
+
+// One 64-lane block per hub row.
+__global__ __launch_bounds__(64) void k_pr_hub(
+    const uint32_t *__restrict__ hubs, const uint32_t *__restrict__ off,
+    const uint32_t *__restrict__ nbr, const float *__restrict__ inv,
+    const float *__restrict__ x_in, const float *__restrict__ rank_in,
+    float *__restrict__ rank_out, float *__restrict__ x_out, float teleport, float damping,
+    const PrState *st, unsigned long long *__restrict__ partial) {
+    if (st->converged) return;
+    __shared__ __align__(16) float buf[kHubChunk];
+    __shared__ unsigned long long acc[2 * kLimbs];
+    const uint32_t v = hubs[blockIdx.x];
+    const uint32_t s = off[v], e = off[v + 1];
+    float sum = 0.0f;
+    for (uint32_t c = s; c < e; c += kHubChunk) {
+        const uint32_t len = min(kHubChunk, e - c);
+        for (uint32_t k = threadIdx.x; k < len; k += 64) buf[k] = x_in[nbr[c + k]];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t k = 0;
+            for (; k + 4 <= len; k += 4) {
+                const float4 q = *reinterpret_cast<const float4 *>(&buf[k]);
+                sum = sum + q.x;
+                sum = sum + q.y;
+                sum = sum + q.z;
+                sum = sum + q.w;
+            }
+            for (; k < len; k++) sum = sum + buf[k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 2 * kLimbs) acc[threadIdx.x] = 0ull;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float r = teleport + damping * sum;
+        rank_out[v] = r;
+        x_out[v] = r * inv[v];
+        resid_add(acc, r, rank_in[v]);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * kLimbs) partial[(size_t)blockIdx.x * 2 * kLimbs + threadIdx.x] = acc[threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void k_pr_reduce(const unsigned long long *__restrict__ partial,
+                                                   uint32_t nparts, PrState *st) {
+    if (st->converged) return;
+    constexpr uint32_t G = 12;
+    __shared__ unsigned long long red[G][2 * kLimbs];
+    const uint32_t t = threadIdx.x;
+    if (t < G * 2 * kLimbs) {
+        const uint32_t k = t % (2 * kLimbs), g = t / (2 * kLimbs);
+        unsigned long long s = 0;
+        for (uint32_t b = g; b < nparts; b += G) s += partial[(size_t)b * 2 * kLimbs + k];
+        red[g][k] = s;
+    }
+    __syncthreads();
+    if (t != 0) return;
+    unsigned long long p[kLimbs], q[kLimbs];
+    for (uint32_t k = 0; k < kLimbs; k++) {
+        unsigned long long a = 0, b = 0;
+        for (uint32_t g = 0; g < G; g++) {
+            a += red[g][k];
+            b += red[g][kLimbs + k];
+        }
+        p[k] = a;
+        q[k] = b;
+    }
+    exact_normalise(p);
+    exact_normalise(q);
+    exact_sub(p, q);
+    const double r = exact_to_double(p);
+    st->resid = r;
+    st->iters += 1;
+    if (r < st->tol) st->converged = 1;
+}
+
+inline uint32_t cdiv32(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace
+
+class PageRankPlan {
+  public:
+    Status create(uint32_t n, const uint32_t *off, const uint32_t *nbr, uint64_t nnz, float damping) {
+        JC_TRY(require_device(nullptr));
+        CsrView g{n, off, nbr, nnz};
+        JC_TRY(validate_csr(g));
+        n_ = n;
+        damping_ = damping;
+        std::vector<float> inv(n ? n : 1, 0.0f);
+        std::vector<uint32_t> hubs;
+        for (uint32_t v = 0; v < n; v++) {
+            const uint32_t d = off[v + 1] - off[v];
+            if (d == 0)
+                return Status::err(JC_E_DANGLING,
+                                   "gpu_pagerank requires no dangling nodes (node " +
+                                       std::to_string(v) + " has degree 0)");
+            inv[v] = 1.0f / (float)d;  // pagerank.rs:105
+            if (d > kHeavy) hubs.push_back(v);
+        }
+        // longest hub rows first: they are the critical path of an iteration
+        std::stable_sort(hubs.begin(), hubs.end(), [&](uint32_t a, uint32_t b) {
+            return off[a + 1] - off[a] > off[b + 1] - off[b];
+        });
+        n_hubs_ = (uint32_t)hubs.size();
+        if (n == 0) return Status::ok();
+        JC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        JC_TRY(off_.alloc((size_t)n + 1));
+        JC_TRY(nbr_.alloc(nnz));
+        JC_TRY(inv_.alloc(n));
+        JC_TRY(hubs_.alloc(hubs.size()));
+        for (int k = 0; k < 2; k++) {
+            JC_TRY(rank_[k].alloc(n));
+            JC_TRY(x_[k].alloc(n));
+        }
+        nb_light_ = cdiv32(n, kPrTB);
+        JC_TRY(partial_.alloc((size_t)(nb_light_ + n_hubs_) * 2 * kLimbs));
+        JC_TRY(st_.alloc(1));
+        JC_HIP(hipMemcpyAsync(off_.p, off, ((size_t)n + 1) * 4, hipMemcpyHostToDevice, stream_));
+        if (nnz) JC_HIP(hipMemcpyAsync(nbr_.p, nbr, nnz * 4, hipMemcpyHostToDevice, stream_));
+        JC_HIP(hipMemcpyAsync(inv_.p, inv.data(), (size_t)n * 4, hipMemcpyHostToDevice, stream_));
+        if (!hubs.empty())
+            JC_HIP(hipMemcpyAsync(hubs_.p, hubs.data(), hubs.size() * 4, hipMemcpyHostToDevice, stream_));
+        JC_HIP(hipStreamSynchronize(stream_));
+        return Status::ok();
+    }
+
+    ~PageRankPlan() {
+        if (stream_) (void)hipStreamSynchronize(stream_);
+        if (chunk_exec_) (void)hipGraphExecDestroy(chunk_exec_);
+        if (chunk_graph_) (void)hipGraphDestroy(chunk_graph_);
+        for (auto &e : ev_)
+            if (e) (void)hipEventDestroy(e);
+        if (stream_) (void)hipStreamDestroy(stream_);
+    }
+
+    Status run(uint32_t max_iters, double tol, float *out, uint32_t *iters, double *resid, double *ms) {
+        if (n_ == 0) {
+            if (iters) *iters = 0;
+            if (resid) *resid = 0.0;
+            if (ms) *ms = 0.0;
+            return Status::ok();
+        }
+        const float inv_n = 1.0f / (float)n_;           // pagerank.rs:108
+        const float teleport = (1.0f - damping_) * inv_n;  // pagerank.rs:109
+        teleport_ = teleport;
+        for (auto &e : ev_)
+            if (!e) JC_HIP(hipEventCreate(&e));
+        JC_HIP(hipEventRecord(ev_[0], stream_));
+        k_pr_init<<<cdiv32(n_, kPrTB), kPrTB, 0, stream_>>>(n_, inv_n, inv_.p, rank_[0].p, x_[0].p,
+                                                            st_.p, tol);
+        JC_HIP(hipGetLastError());
+        PrState hs{};
+        uint32_t launched = 0;
+        while (launched < max_iters) {
+            const uint32_t chunk = std::min(kChunkIters, max_iters - launched);
+            if (chunk == kChunkIters) {
+                JC_TRY(ensure_chunk_graph());
+                JC_HIP(hipGraphLaunch(chunk_exec_, stream_));
+            } else {
+                for (uint32_t k = 0; k < chunk; k++) JC_TRY(enqueue_iteration(k & 1u));
+            }
+            launched += chunk;
+            JC_HIP(hipMemcpyAsync(&hs, st_.p, sizeof hs, hipMemcpyDeviceToHost, stream_));
+            JC_HIP(hipStreamSynchronize(stream_));
+            if (hs.converged) break;
+        }
+        JC_HIP(hipEventRecord(ev_[1], stream_));
+        JC_HIP(hipEventSynchronize(ev_[1]));
+        float t = 0;
+        JC_HIP(hipEventElapsedTime(&t, ev_[0], ev_[1]));
+        if (ms) *ms = t;
+        if (iters) *iters = hs.iters;
+        if (resid) *resid = hs.resid;
+        if (out)
+            JC_HIP(hipMemcpy(out, rank_[hs.iters & 1u].p, (size_t)n_ * 4, hipMemcpyDeviceToHost));
+        return Status::ok();
+    }
+
+    // Kernel-only timing of n_iters iterations with convergence disabled
+    // (tol = -1), events around each iteration's kernels.
+    Status time_iterations(uint32_t n_iters, double *ms_per_iter) {
+        if (n_ == 0) return Status::ok();
+        for (auto &e : ev_)
+            if (!e) JC_HIP(hipEventCreate(&e));
+        const float inv_n = 1.0f / (float)n_;
+        teleport_ = (1.0f - damping_) * inv_n;
+        k_pr_init<<<cdiv32(n_, kPrTB), kPrTB, 0, stream_>>>(n_, inv_n, inv_.p, rank_[0].p, x_[0].p,
+                                                            st_.p, -1.0);
+        JC_HIP(hipGetLastError());
+        JC_HIP(hipEventRecord(ev_[0], stream_));
+        for (uint32_t k = 0; k < n_iters; k++) JC_TRY(enqueue_iteration(k & 1u));
+        JC_HIP(hipEventRecord(ev_[1], stream_));
+        JC_HIP(hipEventSynchronize(ev_[1]));
+        float t = 0;
+        JC_HIP(hipEventElapsedTime(&t, ev_[0], ev_[1]));
+        if (ms_per_iter) *ms_per_iter = n_iters ? t / n_iters : 0.0;
+        return Status::ok();
+    }
+
+  private:
+    Status enqueue_iteration(uint32_t c) {
+        if (n_hubs_)
+            k_pr_hub<<<n_hubs_, 64, 0, stream_>>>(hubs_.p, off_.p, nbr_.p, inv_.p, x_[c].p,
+                                                   rank_[c].p, rank_[c ^ 1].p, x_[c ^ 1].p,
+                                                   teleport_, damping_, st_.p,
+                                                   partial_.p + (size_t)nb_light_ * 2 * kLimbs);
+        k_pr_light<<<nb_light_, kPrTB, 0, stream_>>>(n_, off_.p, nbr_.p, inv_.p, x_[c].p,
+                                                     rank_[c].p, rank_[c ^ 1].p, x_[c ^ 1].p,
+                                                     teleport_, damping_, st_.p, partial_.p);
+        k_pr_reduce<<<1, 256, 0, stream_>>>(partial_.p, nb_light_ + n_hubs_, st_.p);
+        JC_HIP(hipGetLastError());
+        return Status::ok();
+    }
+
+    Status ensure_chunk_graph() {
+        if (chunk_exec_) return Status::ok();
+        JC_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        Status s = Status::ok();
+        for (uint32_t k = 0; k < kChunkIters && !s.failed(); k++) s = enqueue_iteration(k & 1u);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(stream_, &g);
+        if (s.failed()) return s;
+        if (e != hipSuccess) return Status::err(JC_E_HIP, hipGetErrorString(e));
+        chunk_graph_ = g;
+        JC_HIP(hipGraphInstantiate(&chunk_exec_, g, nullptr, nullptr, 0));
+        return Status::ok();
+    }
+
+    uint32_t n_ = 0, n_hubs_ = 0, nb_light_ = 0;
+    float damping_ = 0.85f, teleport_ = 0.0f;
+    hipStream_t stream_ = nullptr;
+    DevBuf<uint32_t> off_, nbr_, hubs_;
+    DevBuf<float> inv_, rank_[2], x_[2];
+    DevBuf<unsigned long long> partial_;
+    DevBuf<PrState> st_;
+    hipGraph_t chunk_graph_ = nullptr;
+    hipGraphExec_t chunk_exec_ = nullptr;
+    hipEvent_t ev_[2] = {};
+};
+
+}  // namespace jc
+
+struct jc_pagerank_plan {
+    jc::PageRankPlan impl;
+};
+
+extern "C" {
+
+int jc_pagerank_plan_create(uint32_t n, const uint32_t *offsets, const uint32_t *neighbors,
+                            uint64_t nnz, float damping, jc_pagerank_plan **out) {
+    if (!out) {
+        jc::set_error("NULL output handle");
+        return JC_E_INVALID;
+    }
+    auto *p = new jc_pagerank_plan();
+    jc::Status s = p->impl.create(n, offsets, neighbors, nnz, damping);
+    if (s.failed()) {
+        delete p;
+        return s.code;
+    }
+    *out = p;
+    jc::set_error("");
+    return JC_OK;
+}
+
+int jc_pagerank_plan_run(jc_pagerank_plan *p, uint32_t max_iters, double tol_l1, float *out_ranks,
+                         uint32_t *out_iters, double *out_residual_l1, double *out_gpu_ms) {
+    if (!p) {
+        jc::set_error("NULL plan");
+        return JC_E_INVALID;
+    }
+    jc::Status s = p->impl.run(max_iters, tol_l1, out_ranks, out_iters, out_residual_l1, out_gpu_ms);
+    if (!s.failed()) jc::set_error("");
+    return s.code;
+}
+
+int jc_pagerank_plan_time(jc_pagerank_plan *p, uint32_t n_iters, double *ms_per_iter) {
+    if (!p) {
+        jc::set_error("NULL plan");
+        return JC_E_INVALID;
+    }
+    jc::Status s = p->impl.time_iterations(n_iters, ms_per_iter);
+    if (!s.failed()) jc::set_error("");
+    return s.code;
+}
+
+void jc_pagerank_plan_destroy(jc_pagerank_plan *p) { delete p; }
+
+int jc_pagerank(uint32_t n, const uint32_t *offsets, const uint32_t *neighbors, uint64_t nnz,
+                float damping, uint32_t max_iters, double tol_l1, float *out_ranks,
+                uint32_t *out_iters, double *out_residual_l1) {
+    jc_pagerank_plan *p = nullptr;
+    int rc = jc_pagerank_plan_create(n, offsets, neighbors, nnz, damping, &p);
+    if (rc) return rc;
+    rc = jc_pagerank_plan_run(p, max_iters, tol_l1, out_ranks, out_iters, out_residual_l1, nullptr);
+    jc_pagerank_plan_destroy(p);
+    return rc;
+}
+
+}  // extern "C"

@@ -135,11 +135,21 @@ Status RadixScratch::alloc(uint32_t count) {
     return Status::ok();
 }
 
-Status radix_sort_pairs_u64(uint64_t *keys, uint32_t *vals, RadixScratch &s, hipStream_t st) {
-    if (s.n == 0) return Status::ok();
+__global__ void k_copy_pairs(const uint64_t *__restrict__ ki, const uint32_t *__restrict__ vi,
+                             uint64_t *__restrict__ ko, uint32_t *__restrict__ vo, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        ko[i] = ki[i];
+        vo[i] = vi[i];
+    }
+}
+
+Status radix_sort_pairs_u64(uint64_t *keys, uint32_t *vals, RadixScratch &s, hipStream_t st,
+                            uint32_t begin_bit, uint32_t end_bit) {
+    if (s.n == 0 || end_bit <= begin_bit) return Status::ok();
     uint64_t *ka = keys, *kb = s.keys_alt.p;
     uint32_t *va = vals, *vb = s.vals_alt.p;
-    for (uint32_t shift = 0; shift < 64; shift += 8) {
+    for (uint32_t shift = begin_bit; shift < end_bit; shift += 8) {
         k_rs_hist<<<s.n_blocks, kSortThreads, 0, st>>>(ka, s.n, shift, s.hist.p, s.n_blocks);
         k_scan_inplace<<<1, 1024, 0, st>>>(s.hist.p, 256u * s.n_blocks);
         k_rs_scatter<<<s.n_blocks, kSortThreads, 0, st>>>(ka, va, kb, vb, s.n, shift, s.hist.p,
@@ -147,6 +157,8 @@ Status radix_sort_pairs_u64(uint64_t *keys, uint32_t *vals, RadixScratch &s, hip
         std::swap(ka, kb);
         std::swap(va, vb);
     }
+    if (ka != keys)  // odd number of passes: result is in the scratch pair
+        k_copy_pairs<<<(s.n + 255) / 256, 256, 0, st>>>(ka, va, keys, vals, s.n);
     JC_HIP(hipGetLastError());
     return Status::ok();
 }

@@ -1,6 +1,354 @@
-// SGD sparse stress engine (placeholder until the HIP kernels land).
+// Sparse-pivot stress by SGD on gfx950 (registry id "sgd-stress-hip").
+//
+// Reference: crates/graph-compute/src/engines/sgd_stress_gpu.rs (host pivot
+// BFS, dense f32 distance matrix, one Jacobi kernel per sweep) and
+// shaders/sgd_stress.wgsl:34-62.  Same arithmetic per term:
+//   w = 1/d^2, mu = min(1, w*eta), r = mu*(|p-q| - d)/|p-q|, p -= r*(p-q)
+// applied in pivot order to the node's own running position (Gauss-Seidel over
+// its pivots, Jacobi across nodes: q comes from the sweep's input positions).
+//
+// MI355X layout: hop distances are u16 (0 = skip) stored pivot-major, so lane i
+// of a wave reads dist[t][i] coalesced; the k pivot positions of the sweep are
+// staged in LDS once per block; w and mu depend only on d and eta, so they are
+// tabulated per sweep in LDS (values identical to the per-term expressions).
+#include <algorithm>
+#include <cmath>
+#include <deque>
+#include <vector>
+
 #include "engine.hpp"
 
 namespace jc {
-LayoutEngine *make_sgd(const jc_sgd_settings &) { return nullptr; }
+namespace {
+
+constexpr uint32_t kSgdTB = 256;
+constexpr uint32_t kMaxTable = 8192;  // distances above this use the inline formula
+
+__device__ __forceinline__ float len3f(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
+
+__global__ __launch_bounds__(kSgdTB) void k_sgd_sweep(uint32_t n, uint32_t k,
+                                                      const uint32_t *__restrict__ pivots,
+                                                      const uint16_t *__restrict__ dist,
+                                                      const float4 *__restrict__ pos_in,
+                                                      float4 *__restrict__ pos_out, float eta,
+                                                      uint32_t dmax, uint32_t lds_pivots) {
+    extern __shared__ float4 smem[];
+    float4 *piv = smem;  // k pivot positions (when lds_pivots)
+    float *mu_tab = reinterpret_cast<float *>(smem + (lds_pivots ? k : 0));  // dmax+1 entries
+    const bool tab = dmax <= kMaxTable;
+    if (lds_pivots)
+        for (uint32_t t = threadIdx.x; t < k; t += blockDim.x) piv[t] = pos_in[pivots[t]];
+    if (tab)
+        for (uint32_t d = threadIdx.x; d <= dmax; d += blockDim.x) {
+            const float df = (float)d;
+            const float w = 1.0f / (df * df);
+            mu_tab[d] = fminf(1.0f, w * eta);
+        }
+    __syncthreads();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p0 = pos_in[i];
+    float px = p0.x, py = p0.y, pz = p0.z;
+    for (uint32_t t = 0; t < k; t++) {
+        const uint32_t du = dist[(size_t)t * n + i];
+        if (du == 0u) continue;                  // self / unreachable sentinel
+        if (pivots[t] == i) continue;
+        const float d = (float)du;
+        const float4 q = lds_pivots ? piv[t] : pos_in[pivots[t]];
+        const float dx = px - q.x, dy = py - q.y, dz = pz - q.z;
+        const float mag = fmaxf(len3f(dx, dy, dz), 1e-6f);
+        float mu;
+        if (tab) mu = mu_tab[du];
+        else {
+            const float w = 1.0f / (d * d);
+            mu = fminf(1.0f, w * eta);
+        }
+        const float r = mu * (mag - d) / mag;
+        px = px - r * dx;
+        py = py - r * dy;
+        pz = pz - r * dz;
+    }
+    pos_out[i] = make_float4(px, py, pz, 0.0f);
+}
+
+inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// sgd_stress.rs:104-125 SplitMix64 (below(n) = next % n)
+struct Rng {
+    uint64_t s;
+    uint64_t next() {
+        s += 0x9E3779B97F4A7C15ull;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+};
+
+// sgd_stress.rs:372-394
+void bfs(const CsrView &g, uint32_t src, std::vector<uint32_t> &dist, std::vector<uint32_t> &q) {
+    std::fill(dist.begin(), dist.end(), 0xFFFFFFFFu);
+    if (g.n == 0) return;
+    size_t h = 0, t = 0;
+    dist[src] = 0;
+    q[t++] = src;
+    while (h < t) {
+        const uint32_t v = q[h++];
+        for (uint64_t e = g.offsets[v]; e < g.offsets[v + 1]; e++) {
+            const uint32_t u = g.neighbors[e];
+            if (dist[u] == 0xFFFFFFFFu) {
+                dist[u] = dist[v] + 1;
+                q[t++] = u;
+            }
+        }
+    }
+}
+
+// sgd_stress.rs:324-334 anneal_eta (host f32 powf, as the reference's host code)
+float anneal_eta(uint64_t sweep, float eta_max, float eta_min, uint32_t n_anneal) {
+    if (n_anneal == 0) return eta_min;
+    const float t = (float)sweep / (float)n_anneal;
+    if (t >= 1.0f) return eta_min;
+    const float ratio = std::fmax(eta_min / eta_max, 1e-9f);
+    return eta_max * powf(ratio, t);
+}
+
+class SgdStressEngine final : public LayoutEngine {
+  public:
+    explicit SgdStressEngine(const jc_sgd_settings &s) : settings_(s) {
+        desc_.id = "sgd-stress-hip";
+        desc_.kind = "physics";
+        desc_.display_name = "SGD stress (pivot, MI355X)";
+        desc_.description =
+            "Sparse-pivot stress majorization by SGD (s_gd2 step, Ortmann max-min pivots): one "
+            "Jacobi pivot update per node per sweep on the GPU, u16 hop distances.";
+        desc_.needs_edges = 1;
+        desc_.needs_cpu_positions = 1;
+        desc_.needs_gpu_positions_buffer = 0;
+    }
+    ~SgdStressEngine() override { teardown(); }
+
+    const jc_descriptor &descriptor() const override { return desc_; }
+
+    Status set_params(const char *json, size_t len) override {
+        jc_sgd_settings s;
+        jc_sgd_default_settings(&s);
+        std::vector<JsonField> f = {
+            {JsonField::U32, "n_pivots", &s.n_pivots},
+            {JsonField::U32, "sweeps_per_step", &s.sweeps_per_step},
+            {JsonField::F32, "eta_max", &s.eta_max},
+            {JsonField::F32, "eta_min", &s.eta_min},
+            {JsonField::U32, "n_anneal_steps", &s.n_anneal_steps},
+            {JsonField::U64, "seed", &s.seed},
+        };
+        int rc = decode_json_object(json, len, f, "sgd-stress settings");
+        if (rc < 0) return Status{JC_E_INVALID};
+        if (rc == 1) return Status::ok();
+        settings_ = s;
+        return Status::ok();
+    }
+
+    Status init(const CsrView &g, const float *xyz) override {
+        JC_TRY(require_device(nullptr));
+        JC_TRY(validate_csr(g));
+        if (g.n && !xyz) return Status::err(JC_E_INVALID, "positions pointer is NULL");
+        teardown();
+        n_ = g.n;
+        sweep_ = 0;
+        // sgd_stress.rs:143-175 select_pivots: first pivot rng.below(n), then max-min
+        std::vector<uint32_t> pivots;
+        std::vector<uint16_t> dist_pm;  // pivot-major [k][n]
+        dmax_ = 0;
+        if (n_) {
+            const uint32_t k = std::min<uint32_t>(std::max<uint32_t>(settings_.n_pivots, 1), n_);
+            Rng rng{settings_.seed};
+            std::vector<uint32_t> mind(n_, 0xFFFFFFFFu), d(n_), q(n_);
+            dist_pm.assign((size_t)k * n_, 0);
+            uint32_t next = (uint32_t)(rng.next() % n_);
+            for (uint32_t t = 0; t < k; t++) {
+                bfs(g, next, d, q);
+                uint32_t best_node = next, best = 0;
+                for (uint32_t v = 0; v < n_; v++) {
+                    const uint32_t dv = d[v];
+                    if (dv != 0xFFFFFFFFu && dv < mind[v]) mind[v] = dv;
+                    const uint32_t cover = mind[v];
+                    if (cover != 0xFFFFFFFFu && cover > best) {
+                        best = cover;
+                        best_node = v;
+                    }
+                    if (dv != 0xFFFFFFFFu && dv > 65535u)
+                        return Status::err(JC_E_INVALID, "hop distance exceeds the u16 range");
+                    if (dv != 0xFFFFFFFFu && dv != 0) {
+                        dist_pm[(size_t)t * n_ + v] = (uint16_t)dv;
+                        dmax_ = std::max(dmax_, dv);
+                    }
+                }
+                pivots.push_back(next);
+                next = best_node;
+            }
+        }
+        k_ = (uint32_t)pivots.size();
+        eta_max_ = settings_.eta_max > 0.0f ? settings_.eta_max : (float)dmax_ * (float)dmax_;
+        JC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        const uint32_t nn = n_ ? n_ : 1;
+        JC_TRY(pos_[0].alloc(nn));
+        JC_TRY(pos_[1].alloc(nn));
+        JC_TRY(pivots_.alloc(k_ ? k_ : 1));
+        JC_TRY(dist_.alloc(dist_pm.size()));
+        std::vector<float4> p4(nn, make_float4(0, 0, 0, 0));
+        for (uint32_t i = 0; i < n_; i++)
+            p4[i] = make_float4(xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2], 0.0f);
+        JC_HIP(hipMemcpyAsync(pos_[0].p, p4.data(), nn * sizeof(float4), hipMemcpyHostToDevice, stream_));
+        if (k_)
+            JC_HIP(hipMemcpyAsync(pivots_.p, pivots.data(), k_ * 4, hipMemcpyHostToDevice, stream_));
+        if (!dist_pm.empty())
+            JC_HIP(hipMemcpyAsync(dist_.p, dist_pm.data(), dist_pm.size() * 2, hipMemcpyHostToDevice,
+                                  stream_));
+        JC_HIP(hipStreamSynchronize(stream_));
+        h_pivots_ = pivots;
+        cur_ = 0;
+        inited_ = true;
+        return Status::ok();
+    }
+
+    // sgd_stress_gpu.rs:368-449: one eta per step, sweeps_per_step sweeps.
+    Status step(uint32_t n_iters, float *out_xyz) override {
+        if (!inited_) return Status::err(JC_E_STATE, "sgd-stress-hip step before init");
+        if (n_ == 0 || k_ == 0) return Status::ok();
+        const uint32_t sweeps = std::max<uint32_t>(settings_.sweeps_per_step, 1);
+        const size_t tab = dmax_ <= kMaxTable ? (size_t)(dmax_ + 1) * sizeof(float) : 0;
+        const uint32_t lds_piv = (k_ * sizeof(float4) + tab <= 60 * 1024) ? 1u : 0u;
+        const size_t shmem = (lds_piv ? k_ * sizeof(float4) : 0) + tab;
+        for (uint32_t it = 0; it < n_iters; it++) {
+            const float eta = anneal_eta(sweep_, eta_max_, settings_.eta_min, settings_.n_anneal_steps);
+            for (uint32_t s = 0; s < sweeps; s++) {
+                k_sgd_sweep<<<cdiv(n_, kSgdTB), kSgdTB, shmem, stream_>>>(
+                    n_, k_, pivots_.p, dist_.p, pos_[cur_].p, pos_[cur_ ^ 1].p, eta, dmax_, lds_piv);
+                cur_ ^= 1;
+            }
+            sweep_ += sweeps;
+        }
+        JC_HIP(hipGetLastError());
+        iteration_ += n_iters;
+        if (out_xyz) JC_TRY(read_positions(out_xyz));
+        return Status::ok();
+    }
+
+    Status sync() override {
+        if (stream_) JC_HIP(hipStreamSynchronize(stream_));
+        return Status::ok();
+    }
+
+    Status get_state(float *xyz, float *old_force, float *se2, uint64_t *it) override {
+        if (!inited_) return Status::err(JC_E_STATE, "get_state before init");
+        if (xyz) JC_TRY(read_positions(xyz));
+        if (old_force) std::fill(old_force, old_force + 3 * (size_t)n_, 0.0f);
+        if (se2) {
+            se2[0] = anneal_eta(sweep_, eta_max_, settings_.eta_min, settings_.n_anneal_steps);
+            se2[1] = (float)sweep_;
+        }
+        if (it) *it = iteration_;
+        return Status::ok();
+    }
+
+    Status set_state(const float *xyz, const float *, const float *se2, uint64_t it) override {
+        if (!inited_) return Status::err(JC_E_STATE, "set_state before init");
+        if (xyz) {
+            std::vector<float4> h(n_);
+            for (uint32_t i = 0; i < n_; i++)
+                h[i] = make_float4(xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2], 0.0f);
+            JC_HIP(hipMemcpy(pos_[cur_].p, h.data(), n_ * sizeof(float4), hipMemcpyHostToDevice));
+        }
+        if (se2) sweep_ = (uint64_t)se2[1];
+        iteration_ = it;
+        return Status::ok();
+    }
+
+    uint32_t n_pivots() const { return k_; }
+    const std::vector<uint32_t> &pivots() const { return h_pivots_; }
+    Status read_dist(uint16_t *out) {
+        // back to the reference's node-major layout dist[i*k + t]
+        std::vector<uint16_t> pm((size_t)k_ * n_);
+        if (!pm.empty())
+            JC_HIP(hipMemcpy(pm.data(), dist_.p, pm.size() * 2, hipMemcpyDeviceToHost));
+        for (uint32_t t = 0; t < k_; t++)
+            for (uint32_t i = 0; i < n_; i++) out[(size_t)i * k_ + t] = pm[(size_t)t * n_ + i];
+        return Status::ok();
+    }
+    float eta_max() const { return eta_max_; }
+
+    Status time_sweeps(uint32_t n, double *ms) {
+        if (!inited_) return Status::err(JC_E_STATE, "time before init");
+        hipEvent_t a, b;
+        JC_HIP(hipEventCreate(&a));
+        JC_HIP(hipEventCreate(&b));
+        JC_HIP(hipEventRecord(a, stream_));
+        Status s = step(n, nullptr);
+        JC_HIP(hipEventRecord(b, stream_));
+        JC_HIP(hipEventSynchronize(b));
+        float t = 0;
+        JC_HIP(hipEventElapsedTime(&t, a, b));
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        if (ms) *ms = t;
+        return s;
+    }
+
+  private:
+    Status read_positions(float *out) {
+        std::vector<float4> h(n_);
+        JC_HIP(hipMemcpyAsync(h.data(), pos_[cur_].p, n_ * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+        JC_HIP(hipStreamSynchronize(stream_));
+        for (uint32_t i = 0; i < n_; i++) {
+            out[3 * (size_t)i] = h[i].x;
+            out[3 * (size_t)i + 1] = h[i].y;
+            out[3 * (size_t)i + 2] = h[i].z;
+        }
+        return Status::ok();
+    }
+
+    void teardown() {
+        if (stream_) (void)hipStreamSynchronize(stream_);
+        pos_[0].release();
+        pos_[1].release();
+        pivots_.release();
+        dist_.release();
+        if (stream_) (void)hipStreamDestroy(stream_);
+        stream_ = nullptr;
+        inited_ = false;
+    }
+
+    jc_descriptor desc_{};
+    jc_sgd_settings settings_;
+    bool inited_ = false;
+    uint32_t n_ = 0, k_ = 0, cur_ = 0, dmax_ = 0;
+    uint64_t sweep_ = 0, iteration_ = 0;
+    float eta_max_ = 1.0f;
+    hipStream_t stream_ = nullptr;
+    DevBuf<float4> pos_[2];
+    DevBuf<uint32_t> pivots_;
+    DevBuf<uint16_t> dist_;
+    std::vector<uint32_t> h_pivots_;
+};
+
+}  // namespace
+
+LayoutEngine *make_sgd(const jc_sgd_settings &s) { return new SgdStressEngine(s); }
+
+Status sgd_pivots(LayoutEngine *e, uint32_t *k, uint32_t *pivots, uint16_t *dist, float *eta_max) {
+    auto *s = dynamic_cast<SgdStressEngine *>(e);
+    if (!s) return Status::err(JC_E_INVALID, "engine is not sgd-stress-hip");
+    if (k) *k = s->n_pivots();
+    if (pivots) std::copy(s->pivots().begin(), s->pivots().end(), pivots);
+    if (eta_max) *eta_max = s->eta_max();
+    if (dist) JC_TRY(s->read_dist(dist));
+    return Status::ok();
+}
+
+Status sgd_time(LayoutEngine *e, uint32_t n, double *ms) {
+    auto *s = dynamic_cast<SgdStressEngine *>(e);
+    if (!s) return Status::err(JC_E_INVALID, "engine is not sgd-stress-hip");
+    return s->time_sweeps(n, ms);
+}
+
 }  // namespace jc

@@ -95,8 +95,20 @@ SIGNATURES = {
     "jc_fa2bh_last_stats": (C.c_int, [_vp, _vp, _vp, _vp]),
     "jc_fa2bh_phase_name": (C.c_char_p, [C.c_uint32]),
     "jc_fa2bh_phase_count": (C.c_uint32, []),
+    "jc_sgd_pivots": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "jc_sgd_time": (C.c_int, [_vp, C.c_uint32, _vp]),
     "jc_pagerank": (C.c_int, [C.c_uint32, _vp, _vp, C.c_uint64, C.c_float, C.c_uint32,
                               C.c_double, _vp, _vp, _vp]),
+    "jc_pagerank_plan_create": (C.c_int, [C.c_uint32, _vp, _vp, C.c_uint64, C.c_float, _vp]),
+    "jc_pagerank_plan_run": (C.c_int, [_vp, C.c_uint32, C.c_double, _vp, _vp, _vp, _vp]),
+    "jc_pagerank_plan_time": (C.c_int, [_vp, C.c_uint32, _vp]),
+    "jc_pagerank_plan_destroy": (None, [_vp]),
+    "jc_fa2bh_batch_create": (C.c_int, [_vp, _vp]),
+    "jc_fa2bh_batch_init": (C.c_int, [_vp, C.c_uint32, _vp, _vp, _vp, _vp, _vp]),
+    "jc_fa2bh_batch_step": (C.c_int, [_vp, C.c_uint32, _vp]),
+    "jc_fa2bh_batch_speeds": (C.c_int, [_vp, _vp]),
+    "jc_fa2bh_batch_step_timed": (C.c_int, [_vp, C.c_uint32, _vp, _vp]),
+    "jc_fa2bh_batch_destroy": (None, [_vp]),
     "jc_gen_ba_pairs": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint64, _vp, C.c_uint64]),
     "jc_gen_sbm_pairs": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_double, C.c_double,
                                       C.c_uint64, _vp, C.c_uint64]),
@@ -306,6 +318,103 @@ class Fa2BhEngine(LayoutEngine):
         return {lib().jc_fa2bh_phase_name(i).decode(): float(out[i]) for i in range(k)}
 
 
+def default_sgd_settings(**overrides) -> SgdSettings:
+    s = SgdSettings()
+    lib().jc_sgd_default_settings(C.addressof(s))
+    for k, v in overrides.items():
+        setattr(s, k, v)
+    return s
+
+
+class SgdStressEngine(LayoutEngine):
+    """Sparse-pivot SGD stress ("sgd-stress-hip"), the drop-in for sgd_stress_gpu.rs."""
+
+    ID = "sgd-stress-hip"
+
+    def __init__(self, settings: SgdSettings | None = None, **overrides):
+        s = settings if settings is not None else default_sgd_settings()
+        for k, v in overrides.items():
+            setattr(s, k, v)
+        h = C.c_void_p()
+        _check(lib().jc_sgd_create(C.addressof(s), C.addressof(h)))
+        super().__init__(h.value)
+
+    def pivots(self):
+        """(pivots, node-major u16 distances, eta_max in force)."""
+        k = np.zeros(1, np.uint32)
+        em = np.zeros(1, np.float32)
+        _check(lib().jc_sgd_pivots(self._h, _ptr(k), None, None, _ptr(em)))
+        kk = int(k[0])
+        piv = np.zeros(max(kk, 1), np.uint32)
+        dist = np.zeros(max(kk * self.n, 1), np.uint16)
+        _check(lib().jc_sgd_pivots(self._h, _ptr(k), _ptr(piv), _ptr(dist), _ptr(em)))
+        return piv[:kk], dist[: kk * self.n], float(em[0])
+
+    def time_steps(self, n_steps: int) -> float:
+        ms = np.zeros(1, np.float64)
+        _check(lib().jc_sgd_time(self._h, n_steps, _ptr(ms)))
+        return float(ms[0])
+
+
+class Fa2BhBatch:
+    """Many independent graphs laid out together (one launch per phase for the
+    whole batch); each graph evolves exactly as a standalone Fa2BhEngine."""
+
+    def __init__(self, settings: Fa2BhSettings | None = None, **overrides):
+        s = settings if settings is not None else default_fa2bh_settings()
+        for k, v in overrides.items():
+            setattr(s, k, v)
+        h = C.c_void_p()
+        _check(lib().jc_fa2bh_batch_create(C.addressof(s), C.addressof(h)))
+        self._h = h.value
+        self.sizes = None
+
+    def init(self, graphs, positions):
+        """graphs: list of (offsets, neighbors); positions: list of xyz arrays."""
+        sizes = np.array([len(o) - 1 for o, _ in graphs], np.uint32)
+        nnz = np.array([len(nb) for _, nb in graphs], np.uint64)
+        off = np.concatenate([_u32(o) for o, _ in graphs])
+        nbs = [_u32(nb) for _, nb in graphs]
+        nb = np.concatenate(nbs) if sum(len(x) for x in nbs) else np.zeros(1, np.uint32)
+        xyz = np.concatenate([_f32(p) for p in positions]) if len(positions) else np.zeros(0, np.float32)
+        if len(xyz) != 3 * int(sizes.sum()):
+            raise JcError(JC_E_INVALID, "positions do not match the graph sizes")
+        self._keep = (sizes, nnz, off, nb)
+        _check(lib().jc_fa2bh_batch_init(self._h, len(sizes), _ptr(sizes), _ptr(nnz), _ptr(off),
+                                          _ptr(nb), _ptr(xyz) if len(xyz) else None))
+        self.sizes = sizes
+
+    def step(self, n_iters: int = 1, readback: bool = True):
+        """Returns the list of per-graph positions (or None)."""
+        tot = int(self.sizes.sum())
+        out = np.empty(3 * max(tot, 1), np.float32) if readback else None
+        _check(lib().jc_fa2bh_batch_step(self._h, n_iters, _ptr(out)))
+        if not readback:
+            return None
+        parts, o = [], 0
+        for n in self.sizes:
+            parts.append(out[3 * o: 3 * (o + int(n))])
+            o += int(n)
+        return parts
+
+    def speeds(self):
+        se = np.zeros(2 * len(self.sizes), np.float32)
+        _check(lib().jc_fa2bh_batch_speeds(self._h, _ptr(se)))
+        return se.reshape(-1, 2)
+
+    def step_timed(self, n_iters: int):
+        ms = np.zeros(1, np.float64)
+        bs = np.zeros(1, np.uint32)
+        _check(lib().jc_fa2bh_batch_step_timed(self._h, n_iters, _ptr(ms), _ptr(bs)))
+        return float(ms[0]), int(bs[0])
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().jc_fa2bh_batch_destroy(h)
+            self._h = None
+
+
 # ------------------------------------------------------------------ analytics
 def pagerank(offsets, neighbors, damping: float = 0.85, max_iters: int = 100, tol_l1: float = 1e-6):
     """gpu_pagerank (crates/graph-compute/src/analytics/pagerank.rs:67-287) with an
@@ -320,6 +429,41 @@ def pagerank(offsets, neighbors, damping: float = 0.85, max_iters: int = 100, to
                              len(neighbors), damping, max_iters, tol_l1, _ptr(ranks), _ptr(it),
                              _ptr(res)))
     return ranks[:n], int(it[0]), float(res[0])
+
+
+class PageRankPlan:
+    """A CSR graph resident on the device for repeated PageRank runs."""
+
+    def __init__(self, offsets, neighbors, damping: float = 0.85):
+        self._off = _u32(offsets)
+        self._nb = _u32(neighbors)
+        self.n = len(self._off) - 1
+        h = C.c_void_p()
+        _check(lib().jc_pagerank_plan_create(self.n, _ptr(self._off),
+                                             _ptr(self._nb) if len(self._nb) else None,
+                                             len(self._nb), damping, C.addressof(h)))
+        self._h = h.value
+
+    def run(self, max_iters: int = 100, tol_l1: float = 1e-6, readback: bool = True):
+        """Returns (ranks or None, iterations, residual, device ms)."""
+        ranks = np.empty(max(self.n, 1), np.float32) if readback else None
+        it = np.zeros(1, np.uint32)
+        res = np.zeros(1, np.float64)
+        ms = np.zeros(1, np.float64)
+        _check(lib().jc_pagerank_plan_run(self._h, max_iters, tol_l1, _ptr(ranks), _ptr(it),
+                                          _ptr(res), _ptr(ms)))
+        return (ranks[: self.n] if readback else None), int(it[0]), float(res[0]), float(ms[0])
+
+    def time_iterations(self, n_iters: int) -> float:
+        ms = np.zeros(1, np.float64)
+        _check(lib().jc_pagerank_plan_time(self._h, n_iters, _ptr(ms)))
+        return float(ms[0])
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().jc_pagerank_plan_destroy(h)
+            self._h = None
 
 
 # ------------------------------------------------------------------ fixtures
@@ -366,6 +510,19 @@ def rgg_pairs(n, radius, seed):
         if cnt <= cap:
             return xy[: 2 * n], pairs[: 2 * cnt]
         cap = int(cnt)
+
+
+def _splitmix_pair(seed: int):
+    """First two SplitMix64 outputs of `seed`: (u in [0,1) from the top 53 bits, h)."""
+    M = (1 << 64) - 1
+    st, outs = seed & M, []
+    for _ in range(2):
+        st = (st + 0x9E3779B97F4A7C15) & M
+        z = st
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        outs.append(z ^ (z >> 31))
+    return (outs[0] >> 11) / float(1 << 53), outs[1]
 
 
 def uniform_cube(n: int, half: float, seed: int):

@@ -872,6 +872,24 @@ static double acc_to_double(const uint64_t *in) {
     return ldexp((double)mant, lo - 149);
 }
 
+/* (p - q) * 2^-149 rounded to f64, for carry-normalised p >= q. */
+static double acc_diff_to_double(const uint64_t *pin, const uint64_t *qin) {
+    uint64_t p[JCO_LIMBS], q[JCO_LIMBS];
+    memcpy(p, pin, sizeof p);
+    memcpy(q, qin, sizeof q);
+    for (int k = 0; k + 1 < JCO_LIMBS; k++) {
+        p[k + 1] += p[k] >> 32; p[k] &= 0xFFFFFFFFull;
+        q[k + 1] += q[k] >> 32; q[k] &= 0xFFFFFFFFull;
+    }
+    int64_t borrow = 0;
+    for (int k = 0; k < JCO_LIMBS; k++) {
+        int64_t d = (int64_t)p[k] - (int64_t)q[k] - borrow;
+        borrow = d < 0;
+        p[k] = (uint64_t)(d + (borrow ? ((int64_t)1 << 32) : 0));
+    }
+    return acc_to_double(p);
+}
+
 void jco_exact_sum2(uint32_t n, const float *stats, double *out2) {
     uint64_t a[JCO_LIMBS + 1] = {0}, b[JCO_LIMBS + 1] = {0};
     for (uint32_t i = 0; i < n; i++) {
@@ -992,8 +1010,10 @@ int jco_fa2_run(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, float
 /* PageRank                                                                    */
 /* ------------------------------------------------------------------------- */
 /* Pull form of pagerank.wgsl:30-46 with the host set-up of pagerank.rs:92-124;
- * stops when the f64 L1 of (r_{t+1} - r_t), summed in node order, is < tol or
- * after max_iters.  Returns -1 on a dangling node (pagerank.rs:98-104). */
+ * stops when the L1 norm of (r_{t+1} - r_t) -- summed exactly and rounded once
+ * to f64 (the stopping rule is this build's addition; the reference runs a
+ * fixed count) -- is < tol, or after max_iters.  Returns -1 on a dangling node
+ * (pagerank.rs:98-104). */
 int jco_pagerank_pull(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, float damping,
                       uint32_t max_iters, double tol_l1, float *ranks, uint32_t *iters_done,
                       double *resid) {
@@ -1019,8 +1039,12 @@ int jco_pagerank_pull(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs,
             }
             b[v] = teleport + damping * acc;
         }
-        r = 0.0;
-        for (uint32_t v = 0; v < n; v++) r += fabs((double)b[v] - (double)a[v]);
+        uint64_t P[JCO_LIMBS + 1] = {0}, Q[JCO_LIMBS + 1] = {0};
+        for (uint32_t v = 0; v < n; v++) {
+            acc_add(P, fmaxf(a[v], b[v]));
+            acc_add(Q, fminf(a[v], b[v]));
+        }
+        r = acc_diff_to_double(P, Q);
         float *t = a; a = b; b = t;
         it++;
         if (r < tol_l1) break;

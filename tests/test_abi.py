@@ -1,0 +1,98 @@
+"""CPU checks of the C ABI (no GPU compute): the library loads and exports every
+entry point include/jc.h declares, the no-device path errs like the reference,
+settings decode with serde semantics, and the product's fixture generators
+agree with the oracle's restatement."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import jumpcannon as jc
+from jumpcannon import workloads
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "jc.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(jc_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(jc.LIB_PATH)
+    syms = _declared_symbols()
+    assert len(syms) >= 25
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+    # the Python front-end binds all of them
+    assert set(syms) <= set(jc.SIGNATURES)
+
+
+def test_no_device_init_errs_like_reference():
+    """fa2_bh.rs:228-231: init without a GPU returns an error (caller falls back)."""
+    if jc.device_count() > 0:
+        pytest.skip("a device is visible")
+    e = jc.Fa2BhEngine()
+    with pytest.raises(jc.NoDeviceError):
+        e.init(np.array([0, 1, 2], np.uint32), np.array([1, 0], np.uint32), np.zeros(6, np.float32))
+    with pytest.raises(jc.JcError) as ei:
+        e.step(1)
+    assert ei.value.code == jc.JC_E_STATE
+
+
+def test_descriptor():
+    d = jc.Fa2BhEngine().descriptor
+    assert d["id"] == "fa2-bh-hip" and d["kind"] == "physics"
+    assert d["needs_edges"] and d["needs_cpu_positions"] and not d["needs_gpu_positions_buffer"]
+
+
+def test_set_params_serde_semantics():
+    """fa2_bh.rs:212-220: null keeps; typed decode errors; unknown keys ignored."""
+    e = jc.Fa2BhEngine()
+    e.set_params(None)
+    e.set_params("null")
+    e.set_params({"theta": 1.2, "gravity": 3, "strong_gravity": True, "extra": {"x": [1, 2]}})
+    for bad in ({"theta": "1.0"}, {"strong_gravity": 1}, {"gravity": None}, [1, 2], "{",
+                '{"theta": 1.0,}', "3"):
+        with pytest.raises(jc.JcError) as ei:
+            e.set_params(bad)
+        assert ei.value.code == jc.JC_E_INVALID
+
+
+def test_default_settings_match_reference_defaults():
+    """Fa2BhSettings::default (fa2_bh.rs:109-125)."""
+    s = jc.default_fa2bh_settings()
+    assert (s.gravity, s.strong_gravity, s.scaling_ratio, s.edge_weight_influence,
+            s.jitter_tolerance, s.lin_log_mode, s.prevent_overlap, s.time_step,
+            round(s.theta, 6), s.max_displacement) == (1.0, 0, 2.0, 1.0, 1.0, 0, 0, 1.0, 0.7, 10.0)
+    g = jc.SgdSettings()
+    jc.lib().jc_sgd_default_settings(ctypes.addressof(g))
+    assert (g.n_pivots, g.sweeps_per_step, g.n_anneal_steps, g.seed) == (50, 1, 30, 0x5EEDC0DE12345678)
+    assert abs(g.eta_max - 1.0) < 1e-7 and abs(g.eta_min - 0.01) < 1e-7
+
+
+@pytest.mark.parametrize("n,m,seed", [(2, 1, 0), (50, 3, 1), (5000, 4, 0xC0FFEE)])
+def test_ba_generator_matches_oracle(n, m, seed):
+    a = jc.ba_graph(n, m, seed)
+    b = O.ba_graph(n, m, seed)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_sbm_rgg_cube_generators_match_oracle():
+    assert np.array_equal(jc.sbm_pairs(2000, 4, 0.02, 0.001, 5), O.sbm_pairs(2000, 4, 0.02, 0.001, 5))
+    xa, pa = jc.rgg_pairs(3000, 0.03, 9)
+    xb, pb = O.rgg_pairs(3000, 0.03, 9)
+    assert np.array_equal(xa, xb) and np.array_equal(pa, pb)
+    assert np.array_equal(jc.uniform_cube(777, 10.0, 3), O.uniform_cube(777, 10.0, 3))
+
+
+def test_sbm_2000_workload_shape():
+    off, nb, xyz = workloads.sbm_2000()
+    n = len(off) - 1
+    assert n == 2000 and xyz.shape == (6000,) and np.abs(xyz).max() <= 1.0
+    # E|E| ~ 4*C(500,2)*0.02 + 6*500*500*0.001 = 9980 + 1500
+    assert 10500 < len(nb) // 2 < 12500

@@ -1,0 +1,101 @@
+"""SGD sparse stress on the GPU against the oracle: pivots and u16 hop distances
+identical (sgd_stress.rs:143-175), every sweep bit-exact with the restated WGSL
+Jacobi update (sgd_stress.wgsl:34-62), plus the reference's solved-layout checks
+(solved_layouts.rs)."""
+import numpy as np
+import pytest
+
+import jumpcannon as jc
+from jumpcannon import workloads
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_oracle(off, nb, xyz, steps, k, seed, eta_max, eta_min, n_anneal):
+    piv, dist = O.select_pivots(off, nb, k, seed)
+    if eta_max <= 0:
+        eta_max = float(np.float32(dist.max()) * np.float32(dist.max()))
+    pos = np.asarray(xyz, np.float32)
+    for t in range(steps):
+        eta = O.anneal_eta(t, eta_max, eta_min, n_anneal)
+        pos = O.sgd_sweep(piv, dist, pos, eta, threads=16)
+    return piv, dist, pos
+
+
+def _grid(w, h):
+    pairs = []
+    for y in range(h):
+        for x in range(w):
+            v = y * w + x
+            if x + 1 < w:
+                pairs.append([v, v + 1])
+            if y + 1 < h:
+                pairs.append([v, v + w])
+    return O.csr_from_pairs(w * h, np.array(pairs, np.uint32).reshape(-1))
+
+
+CASES = {
+    "path24": lambda: O.csr_from_pairs(24, np.array([[i, i + 1] for i in range(23)], np.uint32).reshape(-1)),
+    "cycle16": lambda: O.csr_from_pairs(16, np.array([[i, (i + 1) % 16] for i in range(16)], np.uint32).reshape(-1)),
+    "grid4x4": lambda: _grid(4, 4),
+    "complete6": lambda: O.csr_from_pairs(6, np.array([[i, j] for i in range(6) for j in range(i + 1, 6)], np.uint32).reshape(-1)),
+    "ba3000": lambda: O.ba_graph(3000, 3, 8),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_pivots_and_sweeps_bit_exact(name):
+    off, nb = CASES[name]()
+    n = len(off) - 1
+    xyz = O.uniform_cube(n, 2.0, 31)
+    e = jc.SgdStressEngine(n_pivots=50, eta_max=0.0, eta_min=0.01, n_anneal_steps=30)
+    e.init(off, nb, xyz)
+    piv, dist, em = e.pivots()
+    opiv, odist, opos = _run_oracle(off, nb, xyz, 40, 50, 0x5EEDC0DE12345678, 0.0, 0.01, 30)
+    assert np.array_equal(piv, opiv) and np.array_equal(dist, odist)
+    pos = e.step(40)
+    assert np.array_equal(pos.view(np.uint32), opos.view(np.uint32)), np.abs(pos - opos).max()
+
+
+def test_path_zero_stress_fixed_point():
+    """solved_layouts.rs:254-306: the unit-spaced line is a fixed point."""
+    off, nb = CASES["path24"]()
+    pos = np.zeros(72, np.float32)
+    pos[0::3] = np.arange(24, dtype=np.float32)
+    e = jc.SgdStressEngine(n_pivots=24)
+    e.init(off, nb, pos)
+    out = e.step(20)
+    assert np.abs(out - pos).max() < 1e-5
+
+
+def test_cycle_stays_regular():
+    """solved_layouts.rs:360-391: cycle radius CV < 0.25 after relaxing."""
+    off, nb = CASES["cycle16"]()
+    e = jc.SgdStressEngine(n_pivots=16, eta_max=0.0)
+    e.init(off, nb, O.uniform_cube(16, 1.0, 5))
+    p = e.step(60).reshape(-1, 3)
+    r = np.linalg.norm(p - p.mean(0), axis=1)
+    assert r.std() / r.mean() < 0.25
+
+
+def test_rgg_200k_30_epochs_bit_exact_and_stress_drops():
+    """2-D RGG, 200k nodes (largest component), 200 max-min pivots, u16 hops, eta
+    from (max hop)^2 down to 0.01 over 30 epochs."""
+    off, nb, _ = workloads.rgg_200k()
+    n = len(off) - 1
+    xyz = workloads.plane_init(n)
+    e = jc.SgdStressEngine(n_pivots=200, eta_max=0.0, eta_min=0.01, n_anneal_steps=30)
+    e.init(off, nb, xyz)
+    piv, dist, em = e.pivots()
+    opiv, odist, opos = _run_oracle(off, nb, xyz, 30, 200, 0x5EEDC0DE12345678, 0.0, 0.01, 30)
+    assert np.array_equal(piv, opiv) and np.array_equal(dist, odist)
+    pos = e.step(30)
+    assert np.array_equal(pos.view(np.uint32), opos.view(np.uint32))
+    d = dist.reshape(n, len(piv))
+    ii, tt = np.nonzero(d > 0)
+    ti, tj, td = ii.astype(np.uint32), piv[tt].astype(np.uint32), d[ii, tt].astype(np.float32)
+    s0 = O.stress_normalized(xyz, ti, tj, td)
+    s1 = O.stress_normalized(pos, ti, tj, td)
+    print(f"normalised pivot stress {s0:.4f} -> {s1:.4f}")
+    assert s1 < 0.5 * s0 and np.abs(pos[2::3]).max() == 0.0
