@@ -140,6 +140,11 @@ int jc_fa2bh_step_timed(jc_engine *e, uint32_t n_iters, double *force_ms_sum,
 /* Octree size, number of interactions summed by the walk (accepted cells plus
  * leaf bodies) and the exact {sum swing, sum traction} of the last iteration. */
 int jc_fa2bh_last_stats(jc_engine *e, uint64_t *n_tree, uint64_t *interactions, double *sums2);
+/* Octree walk of the force kernel: 0 = every lane follows its own rope,
+ * 1 = wave-union walk (uniform node loads).  Both give bit-identical forces;
+ * by default the engine times both at its first step and keeps the faster.
+ * mode 0/1 forces one, -1 re-times; *current receives the mode in use. */
+int jc_fa2bh_walk_mode(jc_engine *e, int mode, int *current);
 /* Eager (uncaptured) iterations with events around every phase; phase_ms[k]
  * receives the mean ms of phase k over n_iters.  Phase names: jc_fa2bh_phase_name. */
 int jc_fa2bh_profile(jc_engine *e, uint32_t n_iters, double *phase_ms, uint32_t n_phases);

@@ -276,15 +276,14 @@ __global__ void k_b_graph_tree(const uint32_t *__restrict__ base, const uint32_t
     fr[g].n_tree = e - s;
 }
 
-__global__ __launch_bounds__(kBT) void k_b_emit(
+__device__ __forceinline__ void b_emit_body(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ perm,
     const float4 *__restrict__ pos, const uint32_t *__restrict__ deg,
     const uint32_t *__restrict__ gid, const uint32_t *__restrict__ base,
-    const uint32_t *__restrict__ count, uint32_t n, const GFrame *fr, BGlobal *gl,
+    const uint32_t *__restrict__ count, uint32_t n, const GFrame *fr, const BGlobal *gl,
     const uint32_t *__restrict__ loc, const uint32_t *__restrict__ blk,
     float4 *__restrict__ node_a, uint2 *__restrict__ node_b, uint8_t *__restrict__ node_depth,
-    uint64_t cap) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t cap, uint32_t i, uint32_t *bdepth) {
     if (i >= n) return;
     const Seg sg = seg_of(i, gid, base, count, fr);
     if (i >= sg.ve) return;
@@ -313,7 +312,7 @@ __global__ __launch_bounds__(kBT) void k_b_emit(
             node_depth[pos0 + t] = (uint8_t)d;
             h = h * 0.5f;
         }
-        atomicMax(&gl->max_depth, (uint32_t)(a0 + (int)c - 1));
+        atomicMax(bdepth, (uint32_t)(a0 + (int)c - 1));
     }
     const uint32_t leaf = pos0 + c;
     const uint32_t b = perm[i];
@@ -322,6 +321,23 @@ __global__ __launch_bounds__(kBT) void k_b_emit(
     node_a[leaf] = make_float4(p.x, p.y, p.z, fmaxf((float)deg[b] + 1.0f, 1e-3f));
     node_b[leaf] = make_uint2(kLeaf | b, (i + 1 < sg.ve) ? leaf + 1 : kEnd);
     node_depth[leaf] = (uint8_t)(0x80 | ((lp > li ? lp : li) + 1));
+}
+
+__global__ __launch_bounds__(kBT) void k_b_emit(
+    const uint64_t *__restrict__ keys, const uint32_t *__restrict__ perm,
+    const float4 *__restrict__ pos, const uint32_t *__restrict__ deg,
+    const uint32_t *__restrict__ gid, const uint32_t *__restrict__ base,
+    const uint32_t *__restrict__ count, uint32_t n, const GFrame *fr, BGlobal *gl,
+    const uint32_t *__restrict__ loc, const uint32_t *__restrict__ blk,
+    float4 *__restrict__ node_a, uint2 *__restrict__ node_b, uint8_t *__restrict__ node_depth,
+    uint64_t cap) {
+    __shared__ uint32_t bdepth;
+    if (threadIdx.x == 0) bdepth = 0;
+    __syncthreads();
+    b_emit_body(keys, perm, pos, deg, gid, base, count, n, fr, gl, loc, blk, node_a, node_b,
+                node_depth, cap, blockIdx.x * blockDim.x + threadIdx.x, &bdepth);
+    __syncthreads();
+    if (threadIdx.x == 0 && bdepth) atomicMax(&gl->max_depth, bdepth);
 }
 
 __global__ __launch_bounds__(kBT) void k_b_com_level(const BGlobal *gl, uint32_t d, uint32_t cap,

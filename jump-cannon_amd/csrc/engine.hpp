@@ -60,6 +60,7 @@ Status fa2bh_tree(LayoutEngine *e, uint32_t cap, uint32_t *n_tree, uint32_t *n_v
                   uint32_t *node_skip, uint8_t *node_depth, float *root4);
 Status fa2bh_step_timed(LayoutEngine *e, uint32_t n_iters, double *force_ms, double *iter_ms);
 Status fa2bh_last_stats(LayoutEngine *e, uint64_t *n_tree, uint64_t *interactions, double *sums2);
+Status fa2bh_walk_mode(LayoutEngine *e, int mode, int *current);
 Status fa2bh_profile(LayoutEngine *e, uint32_t n_iters, double *phase_ms, uint32_t n_phases);
 bool is_fa2bh(const LayoutEngine *e);
 

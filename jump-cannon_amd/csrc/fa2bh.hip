@@ -59,6 +59,15 @@ struct Frame {
     unsigned long long interactions;  // accepted cells + leaf bodies summed by the walk
 };
 
+// One octree node, 32 bytes: one cache-line-aligned record per visit.
+//   a = centre of mass xyz, mass
+//   b.x = internal: bits of (2*half)^2; leaf: 0x80000000 | body
+//   b.y = skip rope (first node after the subtree, kEnd at the end)
+struct __align__(32) Node {
+    float4 a;
+    uint4 b;
+};
+
 struct Params {
     uint32_t n;
     float gravity;
@@ -89,8 +98,11 @@ __global__ void k_frame_reset(Frame *fr) {
     }
 }
 
+// Min/max over finite coordinates: per-thread, wave shuffles, then across the
+// block's waves in LDS, so only one atomic per block and axis reaches L2.
 __global__ __launch_bounds__(kTB) void k_bbox(const float4 *__restrict__ pos, uint32_t n,
                                               Frame *fr) {
+    __shared__ float smn[3][kTB / 64], smx[3][kTB / 64];
     float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float4 p = pos[i];
@@ -103,6 +115,7 @@ __global__ __launch_bounds__(kTB) void k_bbox(const float4 *__restrict__ pos, ui
             }
         }
     }
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
 #pragma unroll
@@ -110,13 +123,21 @@ __global__ __launch_bounds__(kTB) void k_bbox(const float4 *__restrict__ pos, ui
             mn[k] = fminf(mn[k], __shfl_xor(mn[k], o, 64));
             mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], o, 64));
         }
-    }
-    if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            if (finitef(mn[k])) atomicMin(&fr->bbmin[k], f2ord(mn[k]));
-            if (finitef(mx[k])) atomicMax(&fr->bbmax[k], f2ord(mx[k]));
+        if (lane == 0) {
+            smn[k][w] = mn[k];
+            smx[k][w] = mx[k];
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int k = threadIdx.x;
+        float a = INFINITY, b = -INFINITY;
+        for (uint32_t q = 0; q < kTB / 64; q++) {
+            a = fminf(a, smn[k][q]);
+            b = fmaxf(b, smx[k][q]);
+        }
+        if (finitef(a)) atomicMin(&fr->bbmin[k], f2ord(a));
+        if (finitef(b)) atomicMax(&fr->bbmax[k], f2ord(b));
     }
 }
 
@@ -234,17 +255,16 @@ __device__ __forceinline__ uint32_t first_node_of(const uint32_t *loc, const uin
     return loc[i] + blk[i / kScanTile] + i;
 }
 
-__global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
-                                              const uint32_t *__restrict__ perm,
-                                              const float4 *__restrict__ pos,
-                                              const uint32_t *__restrict__ deg, uint32_t n,
-                                              Frame *fr, const uint32_t *__restrict__ loc,
-                                              const uint32_t *__restrict__ blk,
-                                              float4 *__restrict__ node_a,
-                                              uint2 *__restrict__ node_b,
-                                              uint8_t *__restrict__ node_depth, uint64_t cap) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nv = n - fr->n_invalid;
+__device__ __forceinline__ void emit_body(const uint64_t *__restrict__ keys,
+                                          const uint32_t *__restrict__ perm,
+                                          const float4 *__restrict__ pos,
+                                          const uint32_t *__restrict__ deg, uint32_t n, uint32_t nv,
+                                          uint32_t i, const Frame *fr,
+                                          const uint32_t *__restrict__ loc,
+                                          const uint32_t *__restrict__ blk,
+                                          Node *__restrict__ nodes,
+                                          uint8_t *__restrict__ node_depth, uint64_t cap,
+                                          uint32_t *bdepth) {
     if (i >= nv) return;
     const uint64_t k = keys[i];
     int lp, li;
@@ -268,37 +288,54 @@ __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
             const uint32_t skip = (j + 1 < nv) ? first_node_of(loc, blk, j + 1) : kEnd;
             const float s = h * 2.0f;
             const float s2 = s * s;
-            node_b[pos0 + t] = make_uint2(__float_as_uint(s2), skip);
+            nodes[pos0 + t].b = make_uint4(__float_as_uint(s2), skip, 0u, 0u);
             node_depth[pos0 + t] = (uint8_t)d;
             h = h * 0.5f;
         }
-        atomicMax(&fr->max_depth, (uint32_t)(a0 + (int)c - 1));
+        atomicMax(bdepth, (uint32_t)(a0 + (int)c - 1));
     }
     const uint32_t leaf = pos0 + c;
     const uint32_t b = perm[i];
     if (b >= n) return;
     const float4 p = pos[b];
     const float m = fmaxf((float)deg[b] + 1.0f, 1e-3f);
-    node_a[leaf] = make_float4(p.x, p.y, p.z, m);
-    node_b[leaf] = make_uint2(kLeaf | b, (i + 1 < nv) ? leaf + 1 : kEnd);
+    nodes[leaf].a = make_float4(p.x, p.y, p.z, m);
+    nodes[leaf].b = make_uint4(kLeaf | b, (i + 1 < nv) ? leaf + 1 : kEnd, 0u, 0u);
     const int ld = (lp > li ? lp : li) + 1;
     node_depth[leaf] = (uint8_t)(0x80 | ld);
+}
+
+__global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
+                                              const uint32_t *__restrict__ perm,
+                                              const float4 *__restrict__ pos,
+                                              const uint32_t *__restrict__ deg, uint32_t n,
+                                              Frame *fr, const uint32_t *__restrict__ loc,
+                                              const uint32_t *__restrict__ blk,
+                                              Node *__restrict__ nodes,
+                                              uint8_t *__restrict__ node_depth, uint64_t cap) {
+    __shared__ uint32_t bdepth;
+    if (threadIdx.x == 0) bdepth = 0;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nv = n - fr->n_invalid;
+    emit_body(keys, perm, pos, deg, n, nv, i, fr, loc, blk, nodes, node_depth, cap, &bdepth);
+    __syncthreads();
+    if (threadIdx.x == 0 && bdepth) atomicMax(&fr->max_depth, bdepth);
 }
 
 // Centre of mass of the internal cells at depth d from their children, in
 // octant (pre-order) order: m = sum m_c; com = sum(com_c * m_c) / m (fa2_bh.rs:944-965).
 __global__ __launch_bounds__(kTB) void k_com_level(const Frame *fr, uint32_t d, uint32_t cap,
-                                                   float4 *__restrict__ node_a,
-                                                   const uint2 *__restrict__ node_b,
+                                                   Node *__restrict__ nodes,
                                                    const uint8_t *__restrict__ node_depth) {
     if (d > fr->max_depth) return;
     const uint32_t N = min(fr->n_tree, cap);
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
         if (node_depth[p] != d) continue;
-        const uint32_t end = node_b[p].y;
+        const uint32_t end = nodes[p].b.y;
         float tm = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
-        for (uint32_t ch = p + 1; ch < N && ch != end; ch = node_b[ch].y) {
-            const float4 a = node_a[ch];
+        for (uint32_t ch = p + 1; ch < N && ch != end; ch = nodes[ch].b.y) {
+            const float4 a = nodes[ch].a;
             tm = tm + a.w;
             cx = cx + a.x * a.w;
             cy = cy + a.y * a.w;
@@ -309,10 +346,40 @@ __global__ __launch_bounds__(kTB) void k_com_level(const Frame *fr, uint32_t d, 
             cy = cy / tm;
             cz = cz / tm;
         }
-        node_a[p] = make_float4(cx, cy, cz, tm);
+        nodes[p].a = make_float4(cx, cy, cz, tm);
     }
 }
 
+// The walk of one body: the node sequence and the f32 operations of
+// fa2_barnes_hut.wgsl:108-156 (leaf: skip self; internal: accept when
+// (2*half)^2 < theta^2 * r^2, else descend to the next node in pre-order).
+__device__ __forceinline__ bool visit(const Node &nd, float px, float py, float pz, uint32_t b,
+                                      float krd, float theta2, float &fx, float &fy, float &fz,
+                                      bool &leaf) {
+    const float dx = px - nd.a.x, dy = py - nd.a.y, dz = pz - nd.a.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    leaf = (nd.b.x & kLeaf) != 0u;
+    bool take;
+    if (leaf) take = ((nd.b.x & ~kLeaf) != b) && nd.a.w > 0.0f;
+    else take = nd.a.w > 0.0f && r2 > 0.0f && __uint_as_float(nd.b.x) < theta2 * r2;
+    if (take) {
+        const float r2c = fmaxf(r2, 1.0e-4f);
+        const float cf = krd * nd.a.w / r2c;
+        fx = fx + dx * cf;
+        fy = fy + dy * cf;
+        fz = fz + dz * cf;
+    }
+    return take;
+}
+
+// MODE 0: every lane follows its own rope (64 node gathers per step).
+// MODE 1: wave-union walk.  The wave steps through the union of its lanes'
+// walks in pre-order: the current node v is wave-uniform (one scalar load),
+// lanes whose own walk is at v process it, the others wait.  Each lane still
+// sees exactly its own node sequence in its own order, so the sums are
+// bit-identical to MODE 0; the next v is v+1 when some lane descended, else
+// the minimum over the lanes' rope pointers.
+template <int MODE>
 __global__ __launch_bounds__(kTB) void k_force(const Params P, Frame *fr,
                                                const float4 *__restrict__ pos,
                                                const float4 *__restrict__ old_force,
@@ -321,8 +388,7 @@ __global__ __launch_bounds__(kTB) void k_force(const Params P, Frame *fr,
                                                const uint32_t *__restrict__ att_off,
                                                const uint32_t *__restrict__ att_adj,
                                                const uint32_t *__restrict__ perm,
-                                               const float4 *__restrict__ node_a,
-                                               const uint2 *__restrict__ node_b,
+                                               const Node *__restrict__ nodes,
                                                unsigned long long *__restrict__ partial) {
     __shared__ unsigned long long acc[2 * kLimbs];
     __shared__ unsigned long long inter;
@@ -332,37 +398,46 @@ __global__ __launch_bounds__(kTB) void k_force(const Params P, Frame *fr,
     uint32_t taken = 0;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t b = t < P.n ? perm[t] : P.n;
-    if (b < P.n) {
-        const float4 p = pos[b];
-        const float deg_i = (float)deg[b] + 1.0f;
-        const float krd = P.scaling_ratio * deg_i;
-        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-        const uint32_t N = min(fr->n_tree, P.cap);
-        if (N > 0) {
-            const uint64_t cap = max((uint64_t)N * 4ull, 16ull);
+    const bool live = b < P.n;
+    const float4 p = live ? pos[b] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float deg_i = live ? (float)deg[b] + 1.0f : 1.0f;
+    const float krd = P.scaling_ratio * deg_i;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    const uint32_t N = min(fr->n_tree, P.cap);
+    const uint64_t wcap = max((uint64_t)N * 4ull, 16ull);
+    if (MODE == 0) {
+        if (live && N > 0) {
             uint64_t walk = 0;
             uint32_t idx = 0;
-            while (idx < N && walk < cap) {
+            while (idx < N && walk < wcap) {
                 walk++;
-                const float4 A = node_a[idx];
-                const uint2 B = node_b[idx];
-                const float dx = p.x - A.x, dy = p.y - A.y, dz = p.z - A.z;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                const bool leaf = (B.x & kLeaf) != 0u;
-                bool take;
-                if (leaf) take = ((B.x & ~kLeaf) != b) && A.w > 0.0f;
-                else take = A.w > 0.0f && r2 > 0.0f && __uint_as_float(B.x) < P.theta2 * r2;
-                if (take) {
-                    taken++;
-                    const float r2c = fmaxf(r2, 1.0e-4f);
-                    const float cf = krd * A.w / r2c;
-                    fx = fx + dx * cf;
-                    fy = fy + dy * cf;
-                    fz = fz + dz * cf;
-                }
-                idx = (leaf || take) ? B.y : idx + 1u;
+                const Node nd = nodes[idx];
+                bool leaf;
+                const bool take = visit(nd, p.x, p.y, p.z, b, krd, P.theta2, fx, fy, fz, leaf);
+                taken += take;
+                idx = (leaf || take) ? nd.b.y : idx + 1u;
             }
         }
+    } else {
+        uint32_t ptr = (live && N > 0) ? 0u : kEnd;
+        uint64_t walk = 0;
+        uint32_t v = __builtin_amdgcn_readfirstlane(__reduce_min_sync(~0ull, ptr));
+        while (v != kEnd) {
+            const Node nd = nodes[v];
+            if (ptr == v) {
+                walk++;
+                bool leaf;
+                const bool take = visit(nd, p.x, p.y, p.z, b, krd, P.theta2, fx, fy, fz, leaf);
+                taken += take;
+                uint32_t nxt = (leaf || take) ? nd.b.y : v + 1u;
+                if (nxt >= N || walk >= wcap) nxt = kEnd;
+                ptr = nxt;
+            }
+            if (__any(ptr == v + 1u)) v = v + 1u;
+            else v = __builtin_amdgcn_readfirstlane(__reduce_min_sync(~0ull, ptr));
+        }
+    }
+    if (live) {
         // attraction: the other endpoint of every edge touching b, in the
         // reference's edge-list order (fa2_barnes_hut.wgsl:158-181)
         const uint32_t e0 = att_off[b], e1 = att_off[b + 1];
@@ -403,30 +478,38 @@ __global__ __launch_bounds__(kTB) void k_force(const Params P, Frame *fr,
     for (int o = 32; o > 0; o >>= 1) taken += __shfl_xor(taken, o, 64);
     if ((threadIdx.x & 63) == 0) atomicAdd(&inter, (unsigned long long)taken);
     __syncthreads();
-    if (threadIdx.x < 2 * kLimbs) partial[blockIdx.x * 2 * kLimbs + threadIdx.x] = acc[threadIdx.x];
+    // limb-major [2*kLimbs][gridDim.x] so the reduction reads coalesce
+    if (threadIdx.x < 2 * kLimbs) partial[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = acc[threadIdx.x];
     if (threadIdx.x == 0) atomicAdd(&fr->interactions, inter);
 }
 
-// Sums the per-block limbs, then runs AdaptiveSpeed::update (fa2_speed.rs:48-100)
-// on one thread.  update_speed = 0 only records the sums (parity hook).
-__global__ __launch_bounds__(256) void k_speed(const Params P, Frame *fr,
-                                               const unsigned long long *__restrict__ partial,
-                                               uint32_t nblocks, uint32_t update_speed) {
-    constexpr uint32_t G = 12;
-    __shared__ unsigned long long red[G][2 * kLimbs];
+// Sums the per-block limbs (limb-major, 51 threads per limb), then runs
+// AdaptiveSpeed::update (fa2_speed.rs:48-100) on one thread.  update_speed = 0
+// only records the sums (parity hook).
+__global__ __launch_bounds__(1024) void k_speed(const Params P, Frame *fr,
+                                                const unsigned long long *__restrict__ partial,
+                                                uint32_t nblocks, uint32_t update_speed) {
+    constexpr uint32_t G = 51;
+    __shared__ unsigned long long red[2 * kLimbs][G];
     const uint32_t t = threadIdx.x;
     if (t < G * 2 * kLimbs) {
-        const uint32_t k = t % (2 * kLimbs), g = t / (2 * kLimbs);
-        unsigned long long s = 0;
-        for (uint32_t b = g; b < nblocks; b += G) s += partial[b * 2 * kLimbs + k];
-        red[g][k] = s;
+        const uint32_t k = t / G, r = t % G;
+        const unsigned long long *row = partial + (size_t)k * nblocks;
+        unsigned long long s0 = 0, s1 = 0;
+        uint32_t b = r;
+        for (; b + G < nblocks; b += 2 * G) {
+            s0 += row[b];
+            s1 += row[b + G];
+        }
+        if (b < nblocks) s0 += row[b];
+        red[k][r] = s0 + s1;
     }
     __syncthreads();
     if (t != 0) return;
     unsigned long long tot[2 * kLimbs];
     for (uint32_t k = 0; k < 2 * kLimbs; k++) {
         unsigned long long s = 0;
-        for (uint32_t g = 0; g < G; g++) s += red[g][k];
+        for (uint32_t g = 0; g < G; g++) s += red[k][g];
         tot[k] = s;
     }
     exact_normalise(tot);
@@ -589,8 +672,7 @@ class Fa2BhEngine final : public LayoutEngine {
         // Internal cells at one depth are disjoint and hold >= 2 bodies each:
         // at most n/2 per depth over 22 depths, plus the n leaves.
         cap_ = nn + (uint64_t)(kLevels + 1) * (nn / 2 + 1) + 8;
-        JC_TRY(node_a_.alloc(cap_));
-        JC_TRY(node_b_.alloc(cap_));
+        JC_TRY(nodes_.alloc(cap_));
         JC_TRY(node_depth_.alloc(cap_));
         nb_force_ = cdiv(nn, kTB);
         JC_TRY(partial_.alloc((size_t)nb_force_ * 2 * kLimbs));
@@ -626,6 +708,7 @@ class Fa2BhEngine final : public LayoutEngine {
         if (!inited_) return Status::err(JC_E_STATE, "fa2-bh-hip step called before successful init");
         if (n_ == 0) return Status::ok();
         refresh_params();
+        JC_TRY(ensure_tuned());
         uint32_t left = n_iters;
         if (left >= 2) {
             JC_TRY(ensure_graph());
@@ -693,7 +776,7 @@ class Fa2BhEngine final : public LayoutEngine {
         JC_TRY(write_vec4(tmp.p, xyz));
         JC_TRY(enqueue_tree(tmp.p, nullptr));
         JC_TRY(enqueue_force(tmp.p, nullptr));
-        k_speed<<<1, 256, 0, stream_>>>(params_, frame_.p, partial_.p, nb_force_, 0u);
+        k_speed<<<1, 1024, 0, stream_>>>(params_, frame_.p, partial_.p, nb_force_, 0u);
         JC_HIP(hipGetLastError());
         if (out_force) JC_TRY(read_vec4(force_.p, out_force));
         Frame fr;
@@ -720,15 +803,12 @@ class Fa2BhEngine final : public LayoutEngine {
         if (perm) JC_HIP(hipMemcpy(perm, vals_.p, n_ * sizeof(uint32_t), hipMemcpyDeviceToHost));
         const uint32_t m = std::min<uint32_t>(cap, fr.n_tree);
         if (m) {
-            if (node_a)
-                JC_HIP(hipMemcpy(node_a, node_a_.p, m * sizeof(float4), hipMemcpyDeviceToHost));
-            if (info || skip) {
-                std::vector<uint2> b(m);
-                JC_HIP(hipMemcpy(b.data(), node_b_.p, m * sizeof(uint2), hipMemcpyDeviceToHost));
-                for (uint32_t q = 0; q < m; q++) {
-                    if (info) info[q] = b[q].x;
-                    if (skip) skip[q] = b[q].y;
-                }
+            std::vector<Node> h(m);
+            JC_HIP(hipMemcpy(h.data(), nodes_.p, m * sizeof(Node), hipMemcpyDeviceToHost));
+            for (uint32_t q = 0; q < m; q++) {
+                if (node_a) std::memcpy(node_a + 4 * (size_t)q, &h[q].a, sizeof(float4));
+                if (info) info[q] = h[q].b.x;
+                if (skip) skip[q] = h[q].b.y;
             }
             if (depth) JC_HIP(hipMemcpy(depth, node_depth_.p, m, hipMemcpyDeviceToHost));
         }
@@ -742,6 +822,7 @@ class Fa2BhEngine final : public LayoutEngine {
     Status step_timed(uint32_t n_iters, double *force_ms, double *iter_ms) {
         if (!inited_) return Status::err(JC_E_STATE, "step_timed before init");
         refresh_params();
+        if (n_) JC_TRY(ensure_tuned());
         double fsum = 0, isum = 0;
         if (n_ && n_iters) {
             JC_TRY(ensure_tree_graphs());
@@ -757,7 +838,7 @@ class Fa2BhEngine final : public LayoutEngine {
                     JC_HIP(hipEventRecord(tev_[2 * k], stream_));
                     JC_TRY(enqueue_force(pos_[c].p, nullptr));
                     JC_HIP(hipEventRecord(tev_[2 * k + 1], stream_));
-                    k_speed<<<1, 256, 0, stream_>>>(params_, frame_.p, partial_.p, nb_force_, 1u);
+                    k_speed<<<1, 1024, 0, stream_>>>(params_, frame_.p, partial_.p, nb_force_, 1u);
                     k_apply<<<cdiv(n_, kTB), kTB, 0, stream_>>>(params_, frame_.p, pos_[c].p,
                                                                  pos_[c ^ 1].p, force_.p,
                                                                  old_force_.p, deg_.p);
@@ -800,6 +881,7 @@ class Fa2BhEngine final : public LayoutEngine {
     Status profile(uint32_t n_iters, double *phase_ms, uint32_t n_phases) {
         if (!inited_) return Status::err(JC_E_STATE, "profile before init");
         refresh_params();
+        if (n_) JC_TRY(ensure_tuned());
         std::vector<hipEvent_t> ev(PH_N + 1);
         for (auto &e : ev) JC_HIP(hipEventCreate(&e));
         std::vector<double> acc(PH_N, 0.0);
@@ -820,7 +902,55 @@ class Fa2BhEngine final : public LayoutEngine {
         return Status::ok();
     }
 
+    // mode 0/1 forces a walk; -1 re-runs the timing; *current gets the mode in use.
+    Status set_walk_mode(int mode, int *current) {
+        if (mode == 0 || mode == 1) {
+            if (walk_mode_ != mode) drop_graphs();
+            walk_mode_ = mode;
+        } else if (mode == -1 && inited_ && n_) {
+            walk_mode_ = -1;
+            drop_graphs();
+            refresh_params();
+            JC_TRY(ensure_tuned());
+        }
+        if (current) *current = walk_mode_;
+        return Status::ok();
+    }
+
   private:
+    // Picks the faster walk on the current positions: builds the octree once,
+    // times three launches of each force kernel, keeps the faster (both give
+    // bit-identical forces).  Engine state is not advanced.
+    Status ensure_tuned() {
+        if (walk_mode_ >= 0) return Status::ok();
+        JC_TRY(enqueue_tree(pos_[cur_].p, nullptr));
+        hipEvent_t a, b;
+        JC_HIP(hipEventCreate(&a));
+        JC_HIP(hipEventCreate(&b));
+        float best = 1e30f;
+        int pick = 0;
+        for (int mode = 0; mode < 2; mode++) {
+            walk_mode_ = mode;
+            JC_TRY(enqueue_force(pos_[cur_].p, nullptr));
+            JC_HIP(hipEventRecord(a, stream_));
+            for (int r = 0; r < 3; r++) JC_TRY(enqueue_force(pos_[cur_].p, nullptr));
+            JC_HIP(hipEventRecord(b, stream_));
+            JC_HIP(hipEventSynchronize(b));
+            float ms = 0;
+            JC_HIP(hipEventElapsedTime(&ms, a, b));
+            tune_ms_[mode] = ms / 3.0f;
+            if (ms < best) {
+                best = ms;
+                pick = mode;
+            }
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        walk_mode_ = pick;
+        drop_graphs();
+        return Status::ok();
+    }
+
     void refresh_params() {
         if (!params_dirty_) return;
         const jc_fa2bh_settings &s = settings_;
@@ -867,7 +997,7 @@ class Fa2BhEngine final : public LayoutEngine {
         const uint32_t n = n_;
         if (ev) JC_HIP(hipEventRecord(ev[PH_BBOX], stream_));
         k_frame_reset<<<1, 64, 0, stream_>>>(frame_.p);
-        k_bbox<<<std::min<uint32_t>(cdiv(n, kTB), 1024), kTB, 0, stream_>>>(pos, n, frame_.p);
+        k_bbox<<<std::min<uint32_t>(cdiv(n, kTB), 256), kTB, 0, stream_>>>(pos, n, frame_.p);
         if (ev) JC_HIP(hipEventRecord(ev[PH_KEYS], stream_));
         k_keys<<<cdiv(n, kTB), kTB, 0, stream_>>>(pos, n, frame_.p, keys_.p, vals_.p);
         if (ev) JC_HIP(hipEventRecord(ev[PH_SORT], stream_));
@@ -877,22 +1007,26 @@ class Fa2BhEngine final : public LayoutEngine {
         k_scan_top<<<1, 1024, 0, stream_>>>(blk_.p, nb_scan_, n, frame_.p);
         if (ev) JC_HIP(hipEventRecord(ev[PH_EMIT], stream_));
         k_emit<<<cdiv(n, kTB), kTB, 0, stream_>>>(keys_.p, vals_.p, pos, deg_.p, n, frame_.p,
-                                                  loc_.p, blk_.p, node_a_.p, node_b_.p,
-                                                  node_depth_.p, cap_);
+                                                  loc_.p, blk_.p, nodes_.p, node_depth_.p, cap_);
         if (ev) JC_HIP(hipEventRecord(ev[PH_COM], stream_));
         const uint32_t gcom = std::min<uint32_t>(cdiv(cap_, kTB), 2048);
         for (int d = kLevels; d >= 0; d--)
             k_com_level<<<gcom, kTB, 0, stream_>>>(frame_.p, (uint32_t)d, (uint32_t)cap_,
-                                                   node_a_.p, node_b_.p, node_depth_.p);
+                                                   nodes_.p, node_depth_.p);
         JC_HIP(hipGetLastError());
         return Status::ok();
     }
 
     Status enqueue_force(const float4 *pos, hipEvent_t *ev) {
         if (ev) JC_HIP(hipEventRecord(ev[PH_FORCE], stream_));
-        k_force<<<nb_force_, kTB, 0, stream_>>>(params_, frame_.p, pos, old_force_.p, force_.p,
-                                                deg_.p, att_off_.p, att_adj_.p, vals_.p,
-                                                node_a_.p, node_b_.p, partial_.p);
+        if (walk_mode_ == 0)
+            k_force<0><<<nb_force_, kTB, 0, stream_>>>(params_, frame_.p, pos, old_force_.p,
+                                                       force_.p, deg_.p, att_off_.p, att_adj_.p,
+                                                       vals_.p, nodes_.p, partial_.p);
+        else
+            k_force<1><<<nb_force_, kTB, 0, stream_>>>(params_, frame_.p, pos, old_force_.p,
+                                                       force_.p, deg_.p, att_off_.p, att_adj_.p,
+                                                       vals_.p, nodes_.p, partial_.p);
         JC_HIP(hipGetLastError());
         return Status::ok();
     }
@@ -903,7 +1037,7 @@ class Fa2BhEngine final : public LayoutEngine {
         JC_TRY(enqueue_force(pos_[c].p, ev));
         if (ev) JC_HIP(hipEventRecord(ev[PH_SPEED], stream_));
         if (speed)
-            k_speed<<<1, 256, 0, stream_>>>(params_, frame_.p, partial_.p, nb_force_, 1u);
+            k_speed<<<1, 1024, 0, stream_>>>(params_, frame_.p, partial_.p, nb_force_, 1u);
         if (ev) JC_HIP(hipEventRecord(ev[PH_APPLY], stream_));
         if (apply)
             k_apply<<<cdiv(n_, kTB), kTB, 0, stream_>>>(params_, frame_.p, pos_[c].p,
@@ -977,8 +1111,7 @@ class Fa2BhEngine final : public LayoutEngine {
         vals_.release();
         loc_.release();
         blk_.release();
-        node_a_.release();
-        node_b_.release();
+        nodes_.release();
         node_depth_.release();
         partial_.release();
         frame_.release();
@@ -998,14 +1131,15 @@ class Fa2BhEngine final : public LayoutEngine {
     uint64_t iteration_ = 0;
     uint64_t cap_ = 0;
     uint32_t nb_scan_ = 0, nb_force_ = 0;
+    int walk_mode_ = -1;  // -1: pick at the first step by timing both walks
+    float tune_ms_[2] = {0.0f, 0.0f};
     hipStream_t stream_ = nullptr;
     DevBuf<float4> pos_[2], old_force_, force_;
     DevBuf<uint32_t> deg_, att_off_, att_adj_;
     DevBuf<uint64_t> keys_;
     DevBuf<uint32_t> vals_, loc_, blk_;
     RadixScratch sort_;
-    DevBuf<float4> node_a_;
-    DevBuf<uint2> node_b_;
+    DevBuf<Node> nodes_;
     DevBuf<uint8_t> node_depth_;
     DevBuf<unsigned long long> partial_;
     DevBuf<Frame> frame_;
@@ -1033,6 +1167,9 @@ Status fa2bh_step_timed(LayoutEngine *e, uint32_t n_iters, double *force_ms, dou
 }
 Status fa2bh_last_stats(LayoutEngine *e, uint64_t *n_tree, uint64_t *interactions, double *sums2) {
     return static_cast<Fa2BhEngine *>(e)->last_stats(n_tree, interactions, sums2);
+}
+Status fa2bh_walk_mode(LayoutEngine *e, int mode, int *current) {
+    return static_cast<Fa2BhEngine *>(e)->set_walk_mode(mode, current);
 }
 Status fa2bh_profile(LayoutEngine *e, uint32_t n_iters, double *phase_ms, uint32_t n_phases) {
     return static_cast<Fa2BhEngine *>(e)->profile(n_iters, phase_ms, n_phases);

@@ -419,6 +419,11 @@ int jc_fa2bh_last_stats(jc_engine *e, uint64_t *n_tree, uint64_t *interactions, 
     return finish(jc::fa2bh_last_stats(e->impl, n_tree, interactions, sums2));
 }
 
+int jc_fa2bh_walk_mode(jc_engine *e, int mode, int *current) {
+    if (int rc = need_fa2(e)) return rc;
+    return finish(jc::fa2bh_walk_mode(e->impl, mode, current));
+}
+
 int jc_fa2bh_profile(jc_engine *e, uint32_t n_iters, double *phase_ms, uint32_t n_phases) {
     if (int rc = need_fa2(e)) return rc;
     return finish(jc::fa2bh_profile(e->impl, n_iters, phase_ms, n_phases));

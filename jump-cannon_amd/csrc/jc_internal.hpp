@@ -79,7 +79,8 @@ struct DevBuf {
 struct RadixScratch {
     DevBuf<uint64_t> keys_alt;
     DevBuf<uint32_t> vals_alt;
-    DevBuf<uint32_t> hist;   // 256 * n_blocks
+    DevBuf<uint32_t> hist;   // 256 * n_blocks, digit-major
+    DevBuf<uint32_t> dbase;  // 256 digit bases
     uint32_t n = 0, n_blocks = 0;
     Status alloc(uint32_t n);
 };

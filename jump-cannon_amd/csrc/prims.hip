@@ -1,9 +1,10 @@
 // Device-wide primitives for gfx950: stable LSD radix sort of 64-bit keys with
-// 32-bit payloads.  Each 8-bit pass is histogram -> one-block scan -> stable
-// scatter.  Ranking inside a tile is wave-level (64 lanes): the lanes holding
-// the same digit are found with eight ballots, the lowest of them reserves a
-// run with one LDS atomic and broadcasts it, so equal digits keep their input
-// order (stability) without any block barrier between rounds.
+// 32-bit payloads.  Each 8-bit pass is: tile histograms -> per-digit scans of
+// the tile counts (256 blocks in parallel) and a scan of the digit totals ->
+// stable scatter.  Ranking inside a tile is wave-level (64 lanes): the lanes
+// holding the same digit are found with eight ballots, the lowest of them
+// reserves a run with one LDS atomic and broadcasts it, so equal digits keep
+// their input order (stability) without any block barrier between rounds.
 #include "jc_internal.hpp"
 
 namespace jc {
@@ -37,38 +38,56 @@ __device__ inline uint32_t wave_incl_scan(uint32_t v) {
     return v;
 }
 
-// Exclusive scan of m entries in place, one block of 1024 threads.
-__global__ __launch_bounds__(1024) void k_scan_inplace(uint32_t *__restrict__ a, uint32_t m) {
-    __shared__ uint32_t ws[16];
-    const uint32_t per = (m + 1023u) / 1024u;
-    const uint32_t b0 = min(threadIdx.x * per, m), b1 = min(b0 + per, m);
-    uint32_t s = 0;
-    for (uint32_t i = b0; i < b1; i++) s += a[i];
-    uint32_t incl = wave_incl_scan(s);
+// Per digit d (one block each): exclusive scan over the tile counts hist[d][*]
+// in place, and the digit's total into dtot[d].
+__global__ __launch_bounds__(256) void k_rs_digit_scan(uint32_t *__restrict__ hist, uint32_t nb,
+                                                       uint32_t *__restrict__ dtot) {
+    __shared__ uint32_t ws[4];
+    __shared__ uint32_t carry;
+    uint32_t *row = hist + (size_t)blockIdx.x * nb;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (uint32_t base = 0; base < nb; base += 256) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < nb ? row[i] : 0u;
+        const uint32_t incl = wave_incl_scan(v);
+        if (lane == 63) ws[w] = incl;
+        __syncthreads();
+        uint32_t wp = 0, tot = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < 4; q++) {
+            if (q < w) wp += ws[q];
+            tot += ws[q];
+        }
+        const uint32_t c0 = carry;
+        if (i < nb) row[i] = c0 + wp + incl - v;
+        __syncthreads();
+        if (threadIdx.x == 0) carry = c0 + tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dtot[blockIdx.x] = carry;
+}
+
+// Exclusive scan of the 256 digit totals (one block).
+__global__ __launch_bounds__(256) void k_rs_base_scan(uint32_t *__restrict__ dtot) {
+    __shared__ uint32_t ws[4];
+    const uint32_t v = dtot[threadIdx.x];
+    const uint32_t incl = wave_incl_scan(v);
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 63) ws[w] = incl;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t run = 0;
-        for (int k = 0; k < 16; k++) {
-            uint32_t t = ws[k];
-            ws[k] = run;
-            run += t;
-        }
-    }
-    __syncthreads();
-    uint32_t run = incl - s + ws[w];
-    for (uint32_t i = b0; i < b1; i++) {
-        uint32_t v = a[i];
-        a[i] = run;
-        run += v;
-    }
+    uint32_t wp = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < 4; q++)
+        if (q < w) wp += ws[q];
+    dtot[threadIdx.x] = wp + incl - v;
 }
 
 __global__ __launch_bounds__(kSortThreads) void k_rs_scatter(
     const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
     uint64_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t n, uint32_t shift,
-    const uint32_t *__restrict__ hist, uint32_t nb) {
+    const uint32_t *__restrict__ hist, const uint32_t *__restrict__ dbase, uint32_t nb) {
     constexpr uint32_t W = kSortThreads / 64;
     __shared__ uint32_t wcnt[W][256];
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -102,7 +121,7 @@ __global__ __launch_bounds__(kSortThreads) void k_rs_scatter(
     }
     __syncthreads();
     {
-        uint32_t run = hist[tid * nb + blockIdx.x];
+        uint32_t run = dbase[tid] + hist[tid * nb + blockIdx.x];
 #pragma unroll
         for (uint32_t q = 0; q < W; q++) {
             uint32_t c = wcnt[q][tid];
@@ -132,6 +151,7 @@ Status RadixScratch::alloc(uint32_t count) {
     JC_TRY(keys_alt.alloc(count));
     JC_TRY(vals_alt.alloc(count));
     JC_TRY(hist.alloc(256ull * n_blocks));
+    JC_TRY(dbase.alloc(256));
     return Status::ok();
 }
 
@@ -151,9 +171,10 @@ Status radix_sort_pairs_u64(uint64_t *keys, uint32_t *vals, RadixScratch &s, hip
     uint32_t *va = vals, *vb = s.vals_alt.p;
     for (uint32_t shift = begin_bit; shift < end_bit; shift += 8) {
         k_rs_hist<<<s.n_blocks, kSortThreads, 0, st>>>(ka, s.n, shift, s.hist.p, s.n_blocks);
-        k_scan_inplace<<<1, 1024, 0, st>>>(s.hist.p, 256u * s.n_blocks);
+        k_rs_digit_scan<<<256, 256, 0, st>>>(s.hist.p, s.n_blocks, s.dbase.p);
+        k_rs_base_scan<<<1, 256, 0, st>>>(s.dbase.p);
         k_rs_scatter<<<s.n_blocks, kSortThreads, 0, st>>>(ka, va, kb, vb, s.n, shift, s.hist.p,
-                                                          s.n_blocks);
+                                                          s.dbase.p, s.n_blocks);
         std::swap(ka, kb);
         std::swap(va, vb);
     }

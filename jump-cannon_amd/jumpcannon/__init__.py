@@ -93,6 +93,7 @@ SIGNATURES = {
     "jc_fa2bh_step_timed": (C.c_int, [_vp, C.c_uint32, _vp, _vp]),
     "jc_fa2bh_profile": (C.c_int, [_vp, C.c_uint32, _vp, C.c_uint32]),
     "jc_fa2bh_last_stats": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "jc_fa2bh_walk_mode": (C.c_int, [_vp, C.c_int, _vp]),
     "jc_fa2bh_phase_name": (C.c_char_p, [C.c_uint32]),
     "jc_fa2bh_phase_count": (C.c_uint32, []),
     "jc_sgd_pivots": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
@@ -302,6 +303,12 @@ class Fa2BhEngine(LayoutEngine):
         it = np.zeros(1, np.float64)
         _check(lib().jc_fa2bh_step_timed(self._h, n_iters, _ptr(f), _ptr(it)))
         return float(f[0]), float(it[0])
+
+    def walk_mode(self, mode: int | None = None) -> int:
+        """Query (None) or force (0 per-lane rope, 1 wave-union, -1 re-time) the walk."""
+        cur = np.zeros(1, np.int32)
+        _check(lib().jc_fa2bh_walk_mode(self._h, -2 if mode is None else mode, _ptr(cur)))
+        return int(cur[0])
 
     def last_stats(self) -> dict:
         nt = np.zeros(1, np.uint64)

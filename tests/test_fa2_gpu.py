@@ -67,10 +67,12 @@ def _tree_equal(g, t):
     assert np.array_equal(g["depth"], want_depth)
 
 
+@pytest.mark.parametrize("walk", [0, 1], ids=["lane-walk", "union-walk"])
 @pytest.mark.parametrize("name,off,nb,xyz", CASES, ids=[c[0] for c in CASES])
-def test_tree_and_forces_bit_exact(name, off, nb, xyz):
+def test_tree_and_forces_bit_exact(name, off, nb, xyz, walk):
     n = len(off) - 1
     e = _engine(off, nb, xyz, theta=0.8)
+    assert e.walk_mode(walk) == walk
     f_gpu, sums_gpu = e.forces_at(xyz)
     g = e.tree()
     mass = _mass(off, nb)
@@ -110,11 +112,13 @@ def test_forces_within_1e4_of_fp64(theta):
     assert rel.max() <= 1e-4
 
 
-def test_trajectory_bit_exact_sbm2000():
+@pytest.mark.parametrize("walk", [0, 1])
+def test_trajectory_bit_exact_sbm2000(walk):
     """50 iterations: positions, F_{t-1} and the speed state bit-exact with the
     oracle driving the same (Morton) octree and the exact f64 sums."""
     off, nb, xyz = workloads.sbm_2000()
     e = _engine(off, nb, xyz, theta=1.2)
+    e.walk_mode(walk)
     e.step(50, readback=False)
     st = e.get_state()
     x, old, se, _ = O.fa2_run(off, nb, xyz, 50, _osettings(theta=1.2), tree_kind=1, exact=True)
@@ -237,8 +241,12 @@ def test_ba_1m_tree_and_force_parity_at_full_size():
     off, nb, xyz = workloads.ba_1m()
     n = len(off) - 1
     e = _engine(off, nb, xyz, theta=0.8)
+    e.walk_mode(0)
+    f2, _ = e.forces_at(xyz)
+    e.walk_mode(1)
     f, sums = e.forces_at(xyz)
-    g = e.tree()
+    assert np.array_equal(f.view(np.uint32), f2.view(np.uint32))
+    g = e.tree(cap=4 * n)
     mass = _mass(off, nb)
     # invariants: one leaf per valid body, leaf mass sums to sum(deg+1), keys sorted
     leaf = (g["depth"] & 0x80) != 0
