@@ -141,9 +141,10 @@ int jc_fa2bh_step_timed(jc_engine *e, uint32_t n_iters, double *force_ms_sum,
  * leaf bodies) and the exact {sum swing, sum traction} of the last iteration. */
 int jc_fa2bh_last_stats(jc_engine *e, uint64_t *n_tree, uint64_t *interactions, double *sums2);
 /* Octree walk of the force kernel: 0 = every lane follows its own rope,
- * 1 = wave-union walk (uniform node loads).  Both give bit-identical forces;
- * by default the engine times both at its first step and keeps the faster.
- * mode 0/1 forces one, -1 re-times; *current receives the mode in use. */
+ * 1 = wave-union walk (uniform node loads), 2 / 3 = each lane interleaves the
+ * walks of 2 / 4 Morton-adjacent bodies.  All give bit-identical forces; by
+ * default the engine times them at its first step and keeps the fastest.
+ * mode 0..3 forces one, -1 re-times; *current receives the mode in use. */
 int jc_fa2bh_walk_mode(jc_engine *e, int mode, int *current);
 /* Eager (uncaptured) iterations with events around every phase; phase_ms[k]
  * receives the mean ms of phase k over n_iters.  Phase names: jc_fa2bh_phase_name. */
@@ -160,6 +161,16 @@ int jc_sgd_pivots(jc_engine *e, uint32_t *k, uint32_t *pivots, uint16_t *dist, f
 /* Runs n_steps steps (LayoutEngine::step without read-back) between HIP events
  * on the engine stream and returns their device time in ms. */
 int jc_sgd_time(jc_engine *e, uint32_t n_steps, double *gpu_ms);
+
+/* ---- layout quality: scale-normalized stress over pivot terms ----
+ * scale_normalized_stress (crates/graph-layouts/src/metrics.rs:31-59) over the
+ * terms (i, pivot t, hop distance d > 0) of a pivot distance table in the
+ * node-major layout dist[i*k + t] (as jc_sgd_pivots returns it), f64 sums. */
+typedef struct jc_stress_plan jc_stress_plan;
+int jc_stress_plan_create(uint32_t n, uint32_t k, const uint32_t *pivots, const uint16_t *dist,
+                          jc_stress_plan **out);
+int jc_stress_plan_eval(jc_stress_plan *p, const float *xyz, double *out_stress);
+void jc_stress_plan_destroy(jc_stress_plan *p);
 
 /* ---- PageRank (crates/graph-compute/src/analytics/pagerank.rs:67-287) ----
  * Pull power iteration: rank_{t+1}[v] = (1-d)/n + d * sum_{u in N(v)} rank_t[u]/deg(u),
@@ -200,6 +211,36 @@ int jc_fa2bh_batch_speeds(jc_batch *b, float *speed_eff);
 int jc_fa2bh_batch_step_timed(jc_batch *b, uint32_t n_iters, double *gpu_ms,
                               uint32_t *force_block);
 void jc_fa2bh_batch_destroy(jc_batch *b);
+
+/* ---- learned warm start: 3-layer message-passing network, hidden 128 ----
+ * No reference counterpart (the reference seeds at random or from a multilevel
+ * solve, engines/multilevel.rs:358-470).  BF16 weights/activations on MFMA
+ * with FP32 accumulation; FP32 master weights trained with Adam on the GPU.
+ * The CSR must be symmetric.  Parameter layout (floats): for l = 0..2,
+ * W_l (K_l x 128, K = 32, 256, 256) then b_l (128); then W_h (128 x 3), b_h (3). */
+typedef struct jc_mpnn jc_mpnn;
+uint64_t jc_mpnn_param_count(void);
+int jc_mpnn_create(uint64_t seed, jc_mpnn **out);
+void jc_mpnn_destroy(jc_mpnn *m);
+int jc_mpnn_get_params(jc_mpnn *m, float *out);
+int jc_mpnn_set_params(jc_mpnn *m, const float *in);
+int jc_mpnn_get_grads(jc_mpnn *m, float *out);
+/* Binds a graph and computes its input features (random-walk return
+ * probabilities from `walks` walks per node, degree terms, noise: 3n values
+ * in [-1,1] or NULL to draw them from feat_seed). */
+int jc_mpnn_set_graph(jc_mpnn *m, uint32_t n, const uint32_t *offsets, const uint32_t *neighbors,
+                      uint64_t nnz, uint64_t feat_seed, uint32_t walks, const float *noise);
+int jc_mpnn_features(jc_mpnn *m, float *out /* n x 16 */);
+int jc_mpnn_activations(jc_mpnn *m, int layer, float *out /* n x 128 */);
+/* Predicted coordinates (3n) and the device time of the forward pass. */
+int jc_mpnn_forward(jc_mpnn *m, float *out_xyz, double *gpu_ms);
+/* Target layout (3n) whose pair distances the loss matches: graph edges plus
+ * pairs with n_anchor seeded anchor nodes. */
+int jc_mpnn_set_target(jc_mpnn *m, const float *xyz, uint32_t n_anchor, uint64_t seed);
+/* Forward + loss + backward; the gradient is left in the network (get_grads). */
+int jc_mpnn_loss_grad(jc_mpnn *m, double *loss);
+int jc_mpnn_adam(jc_mpnn *m, float lr);
+int jc_mpnn_train(jc_mpnn *m, uint32_t steps, float lr, double *last_loss);
 
 /* ---- fixtures (crates/graph-compute/src/stability.rs:9-110) ---- */
 /* synthetic_scale_free(n, m, seed) as undirected pairs; returns the pair count

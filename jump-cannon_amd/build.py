@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libjc_hip.so")
-SOURCES = ["prims.hip", "fa2bh.hip", "batch.hip", "pagerank.hip", "sgd.hip", "host.cpp"]
+SOURCES = ["prims.hip", "fa2bh.hip", "batch.hip", "pagerank.hip", "sgd.hip", "mpnn.hip", "host.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("JC_OFFLOAD_ARCH", "gfx950")
 

@@ -379,7 +379,7 @@ __global__ __launch_bounds__(BS) void k_b_force(const BParams P, const GFrame *f
                                                 const float4 *__restrict__ node_a,
                                                 const uint2 *__restrict__ node_b,
                                                 float2 *__restrict__ stats) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
     const uint32_t b = perm[t];
     if (b >= P.n) return;

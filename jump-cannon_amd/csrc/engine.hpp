@@ -68,4 +68,12 @@ bool is_fa2bh(const LayoutEngine *e);
 Status sgd_pivots(LayoutEngine *e, uint32_t *k, uint32_t *pivots, uint16_t *dist, float *eta_max);
 Status sgd_time(LayoutEngine *e, uint32_t n, double *ms);
 
+// Pivot-term stress evaluator (sgd.hip)
+class StressPlan;
+StressPlan *make_stress_plan();
+Status stress_plan_create(StressPlan *p, uint32_t n, uint32_t k, const uint32_t *pivots,
+                          const uint16_t *dist);
+Status stress_plan_eval(StressPlan *p, const float *xyz, double *out);
+void free_stress_plan(StressPlan *p);
+
 }  // namespace jc

@@ -76,6 +76,17 @@ __device__ __forceinline__ uint32_t cell_end(const uint64_t *keys, uint32_t i, u
 }
 
 
+// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8),
+// and each XCD has its own L2.  This bijection gives XCD x the contiguous run
+// of logical blocks [x*G/8, (x+1)*G/8): with bodies in Morton order, each L2
+// then holds one region of the tree instead of all of it.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t G) {
+    constexpr uint32_t kXcd = 8;
+    const uint32_t q = G / kXcd, r = G % kXcd;
+    const uint32_t x = b % kXcd, slot = b / kXcd;
+    return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+}
+
 // Octant path of p through the reference cell chain for kLevels levels: digit
 // (x>=cx) | (y>=cy)<<1 | (z>=cz)<<2, child half = half*0.5, child centre =
 // centre +- child half (fa2_bh.rs:841-861).

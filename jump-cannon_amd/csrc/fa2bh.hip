@@ -42,6 +42,7 @@ namespace {
 constexpr uint32_t kTB = 256;             // threads per block for per-node kernels
 constexpr uint32_t kScanItems = 16;       // count/scan tile = 4096 bodies
 constexpr uint32_t kScanTile = kTB * kScanItems;
+constexpr int kWalkModes = 4;             // lane walk, wave-union, 2 and 4 chains per lane
 constexpr uint32_t kTimedIters = 10;      // timed iterations per host read-back of events
 
 struct Frame {
@@ -372,6 +373,53 @@ __device__ __forceinline__ bool visit(const Node &nd, float px, float py, float 
     return take;
 }
 
+// Attraction, gravity and the swing/traction stats of body b, after its walk
+// (fa2_barnes_hut.wgsl:158-197), accumulated exactly into the block's limbs.
+__device__ __forceinline__ void finish_body(const Params &P, uint32_t b, float4 p, float deg_i,
+                                            float fx, float fy, float fz,
+                                            const float4 *__restrict__ pos,
+                                            const float4 *__restrict__ old_force,
+                                            float4 *__restrict__ force,
+                                            const uint32_t *__restrict__ att_off,
+                                            const uint32_t *__restrict__ att_adj,
+                                            unsigned long long *acc) {
+        // attraction: the other endpoint of every edge touching b, in the
+        // reference's edge-list order (fa2_barnes_hut.wgsl:158-181)
+        const uint32_t e0 = att_off[b], e1 = att_off[b + 1];
+        for (uint32_t e = e0; e < e1; e++) {
+            const float4 q = pos[att_adj[e]];
+            const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+            float att = P.ewi_weight;
+            if (P.lin_log_mode) {
+                const float r = fmaxf(len3(dx, dy, dz), 1.0e-3f);
+                att = P.ewi_weight * logf(1.0f + r) / r;
+            }
+            fx = fx + dx * att;
+            fy = fy + dy * att;
+            fz = fz + dz * att;
+        }
+        // gravity (fa2_barnes_hut.wgsl:183-189)
+        const float gd = P.gravity * deg_i;
+        if (P.strong_gravity) {
+            fx = fx - p.x * gd;
+            fy = fy - p.y * gd;
+            fz = fz - p.z * gd;
+        } else {
+            const float r0 = fmaxf(len3(p.x, p.y, p.z), 1.0e-3f);
+            fx = fx - (p.x / r0) * gd;
+            fy = fy - (p.y / r0) * gd;
+            fz = fz - (p.z / r0) * gd;
+        }
+        const float4 o = old_force[b];
+        force[b] = make_float4(fx, fy, fz, 0.0f);
+        const float swing = deg_i * len3(o.x - fx, o.y - fy, o.z - fz);
+        const float tra = deg_i * 0.5f * len3(o.x + fx, o.y + fy, o.z + fz);
+        if (finitef(swing) && finitef(tra)) {
+            exact_acc_add(acc, swing);
+            exact_acc_add(acc + kLimbs, tra);
+        }
+    }
+
 // MODE 0: every lane follows its own rope (64 node gathers per step).
 // MODE 1: wave-union walk.  The wave steps through the union of its lanes'
 // walks in pre-order: the current node v is wave-uniform (one scalar load),
@@ -396,7 +444,7 @@ __global__ __launch_bounds__(kTB) void k_force(const Params P, Frame *fr,
     if (threadIdx.x == 0) inter = 0ull;
     __syncthreads();
     uint32_t taken = 0;
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const uint32_t b = t < P.n ? perm[t] : P.n;
     const bool live = b < P.n;
     const float4 p = live ? pos[b] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -437,48 +485,86 @@ __global__ __launch_bounds__(kTB) void k_force(const Params P, Frame *fr,
             else v = __builtin_amdgcn_readfirstlane(__reduce_min_sync(~0ull, ptr));
         }
     }
-    if (live) {
-        // attraction: the other endpoint of every edge touching b, in the
-        // reference's edge-list order (fa2_barnes_hut.wgsl:158-181)
-        const uint32_t e0 = att_off[b], e1 = att_off[b + 1];
-        for (uint32_t e = e0; e < e1; e++) {
-            const float4 q = pos[att_adj[e]];
-            const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
-            float att = P.ewi_weight;
-            if (P.lin_log_mode) {
-                const float r = fmaxf(len3(dx, dy, dz), 1.0e-3f);
-                att = P.ewi_weight * logf(1.0f + r) / r;
-            }
-            fx = fx + dx * att;
-            fy = fy + dy * att;
-            fz = fz + dz * att;
-        }
-        // gravity (fa2_barnes_hut.wgsl:183-189)
-        const float gd = P.gravity * deg_i;
-        if (P.strong_gravity) {
-            fx = fx - p.x * gd;
-            fy = fy - p.y * gd;
-            fz = fz - p.z * gd;
-        } else {
-            const float r0 = fmaxf(len3(p.x, p.y, p.z), 1.0e-3f);
-            fx = fx - (p.x / r0) * gd;
-            fy = fy - (p.y / r0) * gd;
-            fz = fz - (p.z / r0) * gd;
-        }
-        const float4 o = old_force[b];
-        force[b] = make_float4(fx, fy, fz, 0.0f);
-        const float swing = deg_i * len3(o.x - fx, o.y - fy, o.z - fz);
-        const float tra = deg_i * 0.5f * len3(o.x + fx, o.y + fy, o.z + fz);
-        if (finitef(swing) && finitef(tra)) {
-            exact_acc_add(acc, swing);
-            exact_acc_add(acc + kLimbs, tra);
-        }
-    }
+    if (live) finish_body(P, b, p, deg_i, fx, fy, fz, pos, old_force, force, att_off, att_adj, acc);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) taken += __shfl_xor(taken, o, 64);
     if ((threadIdx.x & 63) == 0) atomicAdd(&inter, (unsigned long long)taken);
     __syncthreads();
     // limb-major [2*kLimbs][gridDim.x] so the reduction reads coalesce
+    if (threadIdx.x < 2 * kLimbs) partial[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = acc[threadIdx.x];
+    if (threadIdx.x == 0) atomicAdd(&fr->interactions, inter);
+}
+
+// K bodies per thread (K consecutive bodies in Morton order), their walks
+// interleaved step by step: each step issues the K node loads back to back, so
+// K independent pointer chains are in flight per lane instead of one.  Every
+// body still performs exactly its own walk in its own order.
+template <int K>
+__global__ __launch_bounds__(kTB) void k_force_multi(const Params P, Frame *fr,
+                                                     const float4 *__restrict__ pos,
+                                                     const float4 *__restrict__ old_force,
+                                                     float4 *__restrict__ force,
+                                                     const uint32_t *__restrict__ deg,
+                                                     const uint32_t *__restrict__ att_off,
+                                                     const uint32_t *__restrict__ att_adj,
+                                                     const uint32_t *__restrict__ perm,
+                                                     const Node *__restrict__ nodes,
+                                                     unsigned long long *__restrict__ partial) {
+    __shared__ unsigned long long acc[2 * kLimbs];
+    __shared__ unsigned long long inter;
+    if (threadIdx.x < 2 * kLimbs) acc[threadIdx.x] = 0ull;
+    if (threadIdx.x == 0) inter = 0ull;
+    __syncthreads();
+    uint32_t taken = 0;
+    const uint32_t t0 = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) * K;
+    const uint32_t N = min(fr->n_tree, P.cap);
+    const uint64_t wcap = max((uint64_t)N * 4ull, 16ull);
+    uint32_t b[K], idx[K];
+    uint64_t walk[K];
+    float4 p[K];
+    float deg_i[K], krd[K], fx[K], fy[K], fz[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const uint32_t t = t0 + k;
+        b[k] = t < P.n ? perm[t] : P.n;
+        const bool live = b[k] < P.n;
+        p[k] = live ? pos[b[k]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        deg_i[k] = live ? (float)deg[b[k]] + 1.0f : 1.0f;
+        krd[k] = P.scaling_ratio * deg_i[k];
+        fx[k] = fy[k] = fz[k] = 0.0f;
+        idx[k] = (live && N > 0) ? 0u : kEnd;
+        walk[k] = 0;
+    }
+    for (;;) {
+        bool active = false;
+#pragma unroll
+        for (int k = 0; k < K; k++) active |= idx[k] < N;
+        if (!active) break;
+        Node nd[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) nd[k] = nodes[idx[k] < N ? idx[k] : 0u];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if (idx[k] < N) {
+                bool leaf;
+                const bool take =
+                    visit(nd[k], p[k].x, p[k].y, p[k].z, b[k], krd[k], P.theta2, fx[k], fy[k], fz[k], leaf);
+                taken += take;
+                uint32_t nxt = (leaf || take) ? nd[k].b.y : idx[k] + 1u;
+                if (++walk[k] >= wcap) nxt = kEnd;
+                idx[k] = nxt;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++)
+        if (b[k] < P.n)
+            finish_body(P, b[k], p[k], deg_i[k], fx[k], fy[k], fz[k], pos, old_force, force, att_off,
+                        att_adj, acc);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) taken += __shfl_xor(taken, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&inter, (unsigned long long)taken);
+    __syncthreads();
     if (threadIdx.x < 2 * kLimbs) partial[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = acc[threadIdx.x];
     if (threadIdx.x == 0) atomicAdd(&fr->interactions, inter);
 }
@@ -776,7 +862,7 @@ class Fa2BhEngine final : public LayoutEngine {
         JC_TRY(write_vec4(tmp.p, xyz));
         JC_TRY(enqueue_tree(tmp.p, nullptr));
         JC_TRY(enqueue_force(tmp.p, nullptr));
-        k_speed<<<1, 1024, 0, stream_>>>(params_, frame_.p, partial_.p, nb_force_, 0u);
+        k_speed<<<1, 1024, 0, stream_>>>(params_, frame_.p, partial_.p, force_grid_now(), 0u);
         JC_HIP(hipGetLastError());
         if (out_force) JC_TRY(read_vec4(force_.p, out_force));
         Frame fr;
@@ -838,7 +924,7 @@ class Fa2BhEngine final : public LayoutEngine {
                     JC_HIP(hipEventRecord(tev_[2 * k], stream_));
                     JC_TRY(enqueue_force(pos_[c].p, nullptr));
                     JC_HIP(hipEventRecord(tev_[2 * k + 1], stream_));
-                    k_speed<<<1, 1024, 0, stream_>>>(params_, frame_.p, partial_.p, nb_force_, 1u);
+                    k_speed<<<1, 1024, 0, stream_>>>(params_, frame_.p, partial_.p, force_grid_now(), 1u);
                     k_apply<<<cdiv(n_, kTB), kTB, 0, stream_>>>(params_, frame_.p, pos_[c].p,
                                                                  pos_[c ^ 1].p, force_.p,
                                                                  old_force_.p, deg_.p);
@@ -904,7 +990,7 @@ class Fa2BhEngine final : public LayoutEngine {
 
     // mode 0/1 forces a walk; -1 re-runs the timing; *current gets the mode in use.
     Status set_walk_mode(int mode, int *current) {
-        if (mode == 0 || mode == 1) {
+        if (mode >= 0 && mode < kWalkModes) {
             if (walk_mode_ != mode) drop_graphs();
             walk_mode_ = mode;
         } else if (mode == -1 && inited_ && n_) {
@@ -918,8 +1004,12 @@ class Fa2BhEngine final : public LayoutEngine {
     }
 
   private:
-    // Picks the faster walk on the current positions: builds the octree once,
-    // times three launches of each force kernel, keeps the faster (both give
+    static uint32_t chains_of(int mode) { return mode == 2 ? 2u : mode == 3 ? 4u : 1u; }
+    uint32_t force_grid(int mode) const { return cdiv(cdiv(n_ ? n_ : 1, chains_of(mode)), kTB); }
+    uint32_t force_grid_now() const { return force_grid(walk_mode_); }
+
+    // Picks the fastest walk on the current positions: builds the octree once,
+    // times five launches of each force kernel, keeps the fastest (all give
     // bit-identical forces).  Engine state is not advanced.
     Status ensure_tuned() {
         if (walk_mode_ >= 0) return Status::ok();
@@ -929,16 +1019,16 @@ class Fa2BhEngine final : public LayoutEngine {
         JC_HIP(hipEventCreate(&b));
         float best = 1e30f;
         int pick = 0;
-        for (int mode = 0; mode < 2; mode++) {
+        for (int mode = 0; mode < kWalkModes; mode++) {
             walk_mode_ = mode;
             JC_TRY(enqueue_force(pos_[cur_].p, nullptr));
             JC_HIP(hipEventRecord(a, stream_));
-            for (int r = 0; r < 3; r++) JC_TRY(enqueue_force(pos_[cur_].p, nullptr));
+            for (int r = 0; r < 5; r++) JC_TRY(enqueue_force(pos_[cur_].p, nullptr));
             JC_HIP(hipEventRecord(b, stream_));
             JC_HIP(hipEventSynchronize(b));
             float ms = 0;
             JC_HIP(hipEventElapsedTime(&ms, a, b));
-            tune_ms_[mode] = ms / 3.0f;
+            tune_ms_[mode] = ms / 5.0f;
             if (ms < best) {
                 best = ms;
                 pick = mode;
@@ -1019,14 +1109,17 @@ class Fa2BhEngine final : public LayoutEngine {
 
     Status enqueue_force(const float4 *pos, hipEvent_t *ev) {
         if (ev) JC_HIP(hipEventRecord(ev[PH_FORCE], stream_));
-        if (walk_mode_ == 0)
-            k_force<0><<<nb_force_, kTB, 0, stream_>>>(params_, frame_.p, pos, old_force_.p,
-                                                       force_.p, deg_.p, att_off_.p, att_adj_.p,
-                                                       vals_.p, nodes_.p, partial_.p);
-        else
-            k_force<1><<<nb_force_, kTB, 0, stream_>>>(params_, frame_.p, pos, old_force_.p,
-                                                       force_.p, deg_.p, att_off_.p, att_adj_.p,
-                                                       vals_.p, nodes_.p, partial_.p);
+        const uint32_t grid = force_grid(walk_mode_);
+#define JC_FORCE(KERNEL)                                                                    \
+    KERNEL<<<grid, kTB, 0, stream_>>>(params_, frame_.p, pos, old_force_.p, force_.p, deg_.p, \
+                                      att_off_.p, att_adj_.p, vals_.p, nodes_.p, partial_.p)
+        switch (walk_mode_) {
+            case 0: JC_FORCE(k_force<0>); break;
+            case 2: JC_FORCE(k_force_multi<2>); break;
+            case 3: JC_FORCE(k_force_multi<4>); break;
+            default: JC_FORCE(k_force<1>); break;
+        }
+#undef JC_FORCE
         JC_HIP(hipGetLastError());
         return Status::ok();
     }
@@ -1037,7 +1130,7 @@ class Fa2BhEngine final : public LayoutEngine {
         JC_TRY(enqueue_force(pos_[c].p, ev));
         if (ev) JC_HIP(hipEventRecord(ev[PH_SPEED], stream_));
         if (speed)
-            k_speed<<<1, 1024, 0, stream_>>>(params_, frame_.p, partial_.p, nb_force_, 1u);
+            k_speed<<<1, 1024, 0, stream_>>>(params_, frame_.p, partial_.p, force_grid_now(), 1u);
         if (ev) JC_HIP(hipEventRecord(ev[PH_APPLY], stream_));
         if (apply)
             k_apply<<<cdiv(n_, kTB), kTB, 0, stream_>>>(params_, frame_.p, pos_[c].p,
@@ -1132,7 +1225,7 @@ class Fa2BhEngine final : public LayoutEngine {
     uint64_t cap_ = 0;
     uint32_t nb_scan_ = 0, nb_force_ = 0;
     int walk_mode_ = -1;  // -1: pick at the first step by timing both walks
-    float tune_ms_[2] = {0.0f, 0.0f};
+    float tune_ms_[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     hipStream_t stream_ = nullptr;
     DevBuf<float4> pos_[2], old_force_, force_;
     DevBuf<uint32_t> deg_, att_off_, att_adj_;

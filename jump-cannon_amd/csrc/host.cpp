@@ -439,6 +439,35 @@ int jc_sgd_time(jc_engine *e, uint32_t n_steps, double *gpu_ms) {
     return finish(jc::sgd_time(e->impl, n_steps, gpu_ms));
 }
 
+struct jc_stress_plan {
+    jc::StressPlan *impl;
+};
+
+int jc_stress_plan_create(uint32_t n, uint32_t k, const uint32_t *pivots, const uint16_t *dist,
+                          jc_stress_plan **out) {
+    if (!out) return bad_handle();
+    auto *p = new jc_stress_plan{jc::make_stress_plan()};
+    jc::Status s = jc::stress_plan_create(p->impl, n, k, pivots, dist);
+    if (s.failed()) {
+        jc::free_stress_plan(p->impl);
+        delete p;
+        return s.code;
+    }
+    *out = p;
+    return finish(s);
+}
+
+int jc_stress_plan_eval(jc_stress_plan *p, const float *xyz, double *out) {
+    if (!p) return bad_handle();
+    return finish(jc::stress_plan_eval(p->impl, xyz, out));
+}
+
+void jc_stress_plan_destroy(jc_stress_plan *p) {
+    if (!p) return;
+    jc::free_stress_plan(p->impl);
+    delete p;
+}
+
 // ------------------------------------------------------------------ fixtures
 static inline uint64_t sm_next(uint64_t &s) {
     s += 0x9E3779B97F4A7C15ull;

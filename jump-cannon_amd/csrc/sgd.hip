@@ -73,6 +73,50 @@ __global__ __launch_bounds__(kSgdTB) void k_sgd_sweep(uint32_t n, uint32_t k,
 
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// Scale-normalized stress over pivot terms (crates/graph-layouts/src/metrics.rs:31-59):
+// terms (i, pivot t, d = hop distance > 0), w = 1/d^2.  Pass 0 accumulates
+// num = sum w d D, den = sum w D^2, wd2 = sum w d^2 (D = layout distance);
+// pass 1 accumulates sum w (alpha D - d)^2.  Per-block f64 partials in a fixed
+// order, summed in block order by the host.
+__global__ __launch_bounds__(256) void k_stress(uint32_t n, uint32_t k, const uint32_t *__restrict__ pivots,
+                                                const uint16_t *__restrict__ dist,
+                                                const float4 *__restrict__ pos, double alpha,
+                                                int pass, double *__restrict__ part) {
+    __shared__ double red[3][256];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 0, b = 0, c = 0;
+    if (i < n) {
+        const float4 p = pos[i];
+        for (uint32_t t = 0; t < k; t++) {
+            const uint32_t du = dist[(size_t)t * n + i];
+            if (!du) continue;
+            const float4 q = pos[pivots[t]];
+            const double dx = (double)(p.x - q.x), dy = (double)(p.y - q.y), dz = (double)(p.z - q.z);
+            const double D = sqrt(dx * dx + dy * dy + dz * dz);
+            const double d = (double)du, w = 1.0 / (d * d);
+            if (pass == 0) {
+                a += w * d * D;
+                b += w * D * D;
+                c += w * d * d;
+            } else {
+                const double r = alpha * D - d;
+                a += w * r * r;
+            }
+        }
+    }
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = b;
+    red[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int s2 = 128; s2 > 0; s2 >>= 1) {
+        if ((int)threadIdx.x < s2)
+            for (int q = 0; q < 3; q++) red[q][threadIdx.x] += red[q][threadIdx.x + s2];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; q++) part[3 * (size_t)blockIdx.x + q] = red[q][0];
+}
+
 // sgd_stress.rs:104-125 SplitMix64 (below(n) = next % n)
 struct Rng {
     uint64_t s;
@@ -333,6 +377,71 @@ class SgdStressEngine final : public LayoutEngine {
 
 }  // namespace
 
+class StressPlan {
+  public:
+    ~StressPlan() {
+        if (stream_) (void)hipStreamDestroy(stream_);
+    }
+    Status create(uint32_t n, uint32_t k, const uint32_t *pivots, const uint16_t *dist_node_major) {
+        JC_TRY(require_device(nullptr));
+        n_ = n;
+        k_ = k;
+        JC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        std::vector<uint16_t> pm((size_t)k * n);
+        for (uint32_t i = 0; i < n; i++)
+            for (uint32_t t = 0; t < k; t++) pm[(size_t)t * n + i] = dist_node_major[(size_t)i * k + t];
+        for (uint32_t t = 0; t < k; t++)
+            if (pivots[t] >= n) return Status::err(JC_E_INVALID, "pivot out of range");
+        JC_TRY(piv_.alloc(k ? k : 1));
+        JC_TRY(dist_.alloc(pm.size()));
+        JC_TRY(pos_.alloc(n ? n : 1));
+        JC_TRY(part_.alloc(3 * (size_t)cdiv(n ? n : 1, 256)));
+        if (k) JC_HIP(hipMemcpy(piv_.p, pivots, k * 4, hipMemcpyHostToDevice));
+        if (!pm.empty()) JC_HIP(hipMemcpy(dist_.p, pm.data(), pm.size() * 2, hipMemcpyHostToDevice));
+        return Status::ok();
+    }
+    Status eval(const float *xyz, double *out) {
+        if (!n_ || !k_) {
+            *out = 0.0;
+            return Status::ok();
+        }
+        std::vector<float4> h(n_);
+        for (uint32_t i = 0; i < n_; i++)
+            h[i] = make_float4(xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2], 0.0f);
+        JC_HIP(hipMemcpyAsync(pos_.p, h.data(), n_ * sizeof(float4), hipMemcpyHostToDevice, stream_));
+        const uint32_t nb = cdiv(n_, 256);
+        std::vector<double> part(3 * (size_t)nb);
+        auto run = [&](double alpha, int pass, double *acc) -> Status {
+            k_stress<<<nb, 256, 0, stream_>>>(n_, k_, piv_.p, dist_.p, pos_.p, alpha, pass, part_.p);
+            JC_HIP(hipGetLastError());
+            JC_HIP(hipMemcpyAsync(part.data(), part_.p, part.size() * 8, hipMemcpyDeviceToHost, stream_));
+            JC_HIP(hipStreamSynchronize(stream_));
+            acc[0] = acc[1] = acc[2] = 0.0;
+            for (uint32_t b = 0; b < nb; b++)
+                for (int q = 0; q < 3; q++) acc[q] += part[3 * (size_t)b + q];
+            return Status::ok();
+        };
+        double s0[3], s1[3];
+        JC_TRY(run(0.0, 0, s0));
+        if (s0[1] <= 0.0 || s0[2] <= 0.0) {
+            *out = 0.0;
+            return Status::ok();
+        }
+        const double alpha = s0[0] / s0[1];
+        JC_TRY(run(alpha, 1, s1));
+        *out = s1[0] / s0[2];
+        return Status::ok();
+    }
+
+  private:
+    uint32_t n_ = 0, k_ = 0;
+    hipStream_t stream_ = nullptr;
+    DevBuf<uint32_t> piv_;
+    DevBuf<uint16_t> dist_;
+    DevBuf<float4> pos_;
+    DevBuf<double> part_;
+};
+
 LayoutEngine *make_sgd(const jc_sgd_settings &s) { return new SgdStressEngine(s); }
 
 Status sgd_pivots(LayoutEngine *e, uint32_t *k, uint32_t *pivots, uint16_t *dist, float *eta_max) {
@@ -344,6 +453,14 @@ Status sgd_pivots(LayoutEngine *e, uint32_t *k, uint32_t *pivots, uint16_t *dist
     if (dist) JC_TRY(s->read_dist(dist));
     return Status::ok();
 }
+
+StressPlan *make_stress_plan() { return new StressPlan(); }
+Status stress_plan_create(StressPlan *p, uint32_t n, uint32_t k, const uint32_t *pivots,
+                          const uint16_t *dist) {
+    return p->create(n, k, pivots, dist);
+}
+Status stress_plan_eval(StressPlan *p, const float *xyz, double *out) { return p->eval(xyz, out); }
+void free_stress_plan(StressPlan *p) { delete p; }
 
 Status sgd_time(LayoutEngine *e, uint32_t n, double *ms) {
     auto *s = dynamic_cast<SgdStressEngine *>(e);

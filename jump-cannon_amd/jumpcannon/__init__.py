@@ -98,6 +98,9 @@ SIGNATURES = {
     "jc_fa2bh_phase_count": (C.c_uint32, []),
     "jc_sgd_pivots": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "jc_sgd_time": (C.c_int, [_vp, C.c_uint32, _vp]),
+    "jc_stress_plan_create": (C.c_int, [C.c_uint32, C.c_uint32, _vp, _vp, _vp]),
+    "jc_stress_plan_eval": (C.c_int, [_vp, _vp, _vp]),
+    "jc_stress_plan_destroy": (None, [_vp]),
     "jc_pagerank": (C.c_int, [C.c_uint32, _vp, _vp, C.c_uint64, C.c_float, C.c_uint32,
                               C.c_double, _vp, _vp, _vp]),
     "jc_pagerank_plan_create": (C.c_int, [C.c_uint32, _vp, _vp, C.c_uint64, C.c_float, _vp]),
@@ -110,6 +113,20 @@ SIGNATURES = {
     "jc_fa2bh_batch_speeds": (C.c_int, [_vp, _vp]),
     "jc_fa2bh_batch_step_timed": (C.c_int, [_vp, C.c_uint32, _vp, _vp]),
     "jc_fa2bh_batch_destroy": (None, [_vp]),
+    "jc_mpnn_param_count": (C.c_uint64, []),
+    "jc_mpnn_create": (C.c_int, [C.c_uint64, _vp]),
+    "jc_mpnn_destroy": (None, [_vp]),
+    "jc_mpnn_get_params": (C.c_int, [_vp, _vp]),
+    "jc_mpnn_set_params": (C.c_int, [_vp, _vp]),
+    "jc_mpnn_get_grads": (C.c_int, [_vp, _vp]),
+    "jc_mpnn_set_graph": (C.c_int, [_vp, C.c_uint32, _vp, _vp, C.c_uint64, C.c_uint64, C.c_uint32, _vp]),
+    "jc_mpnn_features": (C.c_int, [_vp, _vp]),
+    "jc_mpnn_activations": (C.c_int, [_vp, C.c_int, _vp]),
+    "jc_mpnn_forward": (C.c_int, [_vp, _vp, _vp]),
+    "jc_mpnn_set_target": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint64]),
+    "jc_mpnn_loss_grad": (C.c_int, [_vp, _vp]),
+    "jc_mpnn_adam": (C.c_int, [_vp, C.c_float]),
+    "jc_mpnn_train": (C.c_int, [_vp, C.c_uint32, C.c_float, _vp]),
     "jc_gen_ba_pairs": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint64, _vp, C.c_uint64]),
     "jc_gen_sbm_pairs": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_double, C.c_double,
                                       C.c_uint64, _vp, C.c_uint64]),
@@ -305,7 +322,8 @@ class Fa2BhEngine(LayoutEngine):
         return float(f[0]), float(it[0])
 
     def walk_mode(self, mode: int | None = None) -> int:
-        """Query (None) or force (0 per-lane rope, 1 wave-union, -1 re-time) the walk."""
+        """Query (None) or force the walk: 0 per-lane rope, 1 wave-union, 2/3 two/four
+        interleaved bodies per lane, -1 re-time and keep the fastest."""
         cur = np.zeros(1, np.int32)
         _check(lib().jc_fa2bh_walk_mode(self._h, -2 if mode is None else mode, _ptr(cur)))
         return int(cur[0])
@@ -419,6 +437,105 @@ class Fa2BhBatch:
         h = getattr(self, "_h", None)
         if h:
             lib().jc_fa2bh_batch_destroy(h)
+            self._h = None
+
+
+class StressJudge:
+    """Scale-normalized stress (metrics.rs:31-59) over pivot terms, on the GPU."""
+
+    def __init__(self, n: int, pivots, dist_node_major):
+        self._piv = _u32(pivots)
+        self._dist = np.ascontiguousarray(dist_node_major, np.uint16)
+        h = C.c_void_p()
+        _check(lib().jc_stress_plan_create(n, len(self._piv), _ptr(self._piv), _ptr(self._dist),
+                                           C.addressof(h)))
+        self._h = h.value
+
+    def __call__(self, xyz) -> float:
+        x = _f32(xyz)
+        out = np.zeros(1, np.float64)
+        _check(lib().jc_stress_plan_eval(self._h, _ptr(x), _ptr(out)))
+        return float(out[0])
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().jc_stress_plan_destroy(h)
+            self._h = None
+
+
+class Mpnn:
+    """The warm-start network (3 message-passing layers, hidden 128, BF16 MFMA)."""
+
+    def __init__(self, seed: int = 1):
+        h = C.c_void_p()
+        _check(lib().jc_mpnn_create(seed, C.addressof(h)))
+        self._h = h.value
+        self.n = 0
+        self.P = int(lib().jc_mpnn_param_count())
+
+    def params(self):
+        out = np.empty(self.P, np.float32)
+        _check(lib().jc_mpnn_get_params(self._h, _ptr(out)))
+        return out
+
+    def set_params(self, p):
+        p = _f32(p)
+        assert len(p) == self.P
+        _check(lib().jc_mpnn_set_params(self._h, _ptr(p)))
+
+    def grads(self):
+        out = np.empty(self.P, np.float32)
+        _check(lib().jc_mpnn_get_grads(self._h, _ptr(out)))
+        return out
+
+    def set_graph(self, offsets, neighbors, feat_seed: int = 7, walks: int = 16, noise=None):
+        off = _u32(offsets)
+        nb = _u32(neighbors)
+        self._keep = (off, nb)
+        self.n = len(off) - 1
+        nz = None if noise is None else _f32(noise)
+        _check(lib().jc_mpnn_set_graph(self._h, self.n, _ptr(off), _ptr(nb) if len(nb) else None,
+                                       len(nb), feat_seed, walks, _ptr(nz)))
+
+    def features(self):
+        out = np.empty(self.n * 16, np.float32)
+        _check(lib().jc_mpnn_features(self._h, _ptr(out)))
+        return out.reshape(self.n, 16)
+
+    def activations(self, layer: int):
+        out = np.empty(self.n * 128, np.float32)
+        _check(lib().jc_mpnn_activations(self._h, layer, _ptr(out)))
+        return out.reshape(self.n, 128)
+
+    def forward(self):
+        """Returns (coordinates 3n, device ms)."""
+        out = np.empty(3 * self.n, np.float32)
+        ms = np.zeros(1, np.float64)
+        _check(lib().jc_mpnn_forward(self._h, _ptr(out), _ptr(ms)))
+        return out, float(ms[0])
+
+    def set_target(self, xyz, n_anchor: int = 32, seed: int = 3):
+        x = _f32(xyz)
+        _check(lib().jc_mpnn_set_target(self._h, _ptr(x), n_anchor, seed))
+
+    def loss_grad(self) -> float:
+        L = np.zeros(1, np.float64)
+        _check(lib().jc_mpnn_loss_grad(self._h, _ptr(L)))
+        return float(L[0])
+
+    def adam(self, lr: float):
+        _check(lib().jc_mpnn_adam(self._h, lr))
+
+    def train(self, steps: int, lr: float) -> float:
+        L = np.zeros(1, np.float64)
+        _check(lib().jc_mpnn_train(self._h, steps, lr, _ptr(L)))
+        return float(L[0])
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().jc_mpnn_destroy(h)
             self._h = None
 
 

@@ -1,0 +1,183 @@
+"""Learned warm start for FA2+BH: train the message-passing network on layouts
+the plain engine produces, predict initial coordinates, refine with the plain
+engine, and judge both runs by normalized pivot stress.
+
+The plain method stays the drop-in and the fallback: a prediction whose
+refinement does not reach the plain run's target stress within the plain run's
+own iteration budget is replaced by the plain run.
+
+Training sample: (graph, random initial layout x0, FA2(x0) after `iters`
+iterations).  The network sees x0/scale as its noise input, so it learns to
+carry a random start towards the layout FA2 would reach from it; the loss is
+invariant to rotation, translation and scale (the predicted layout is rescaled
+with a size law fitted on the training targets).
+"""
+from __future__ import annotations
+
+import math
+import os
+import time
+
+import numpy as np
+
+from . import (Fa2BhBatch, Fa2BhEngine, Mpnn, SgdStressEngine, StressJudge, _splitmix_pair,
+               ba_graph, sbm_graph, uniform_cube)
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warmstart_weights.npz")
+INIT_HALF = 10.0     # initial layouts are uniform in [-10,10]^3 (the 1M-node workload's seed)
+
+
+def training_graphs(count: int, seed: int, n_min: int = 1000, n_max: int = 20000):
+    """Seeded SBM (even) / BA (odd) graphs of log-uniform size in [n_min, n_max]."""
+    out = []
+    for g in range(count):
+        u, h = _splitmix_pair(seed + g)
+        n = int(n_min * (n_max / n_min) ** u)
+        if g % 2 == 0:
+            B = 2 + h % 7
+            off, nb = sbm_graph(n, B, min(1.0, 8.0 / (n / B)), 0.5 / n, seed + g)
+        else:
+            off, nb = ba_graph(n, 2 + h % 4, seed + g)
+        x0 = uniform_cube(len(off) - 1, INIT_HALF, seed + 100_000 + g)
+        out.append((off, nb, x0))
+    return out
+
+
+def fa2_targets(graphs, iters: int = 300, theta: float = 1.0):
+    """Plain FA2 layouts of every graph from its x0, all graphs in one batch."""
+    b = Fa2BhBatch(theta=theta)
+    b.init([(o, nb) for o, nb, _ in graphs], [x0 for _, _, x0 in graphs])
+    return b.step(iters)
+
+
+def rms_radius(xyz):
+    p = np.asarray(xyz, np.float64).reshape(-1, 3)
+    return float(np.sqrt(((p - p.mean(0)) ** 2).sum(1).mean()))
+
+
+def fit_size_law(graphs, targets):
+    """log R_rms = a + b log n + c log(mean degree), least squares."""
+    X, y = [], []
+    for (off, nb, _), t in zip(graphs, targets):
+        n = len(off) - 1
+        X.append([1.0, math.log(n), math.log(max(len(nb) / n, 1e-9))])
+        y.append(math.log(rms_radius(t)))
+    coef, *_ = np.linalg.lstsq(np.array(X), np.array(y), rcond=None)
+    return coef
+
+
+def size_of(coef, n, nnz):
+    return float(math.exp(coef[0] + coef[1] * math.log(n) + coef[2] * math.log(max(nnz / n, 1e-9))))
+
+
+def train(net: Mpnn, graphs, targets, epochs: int = 4, steps_per_graph: int = 2,
+          lr: float = 2e-3, n_anchor: int = 32, walks: int = 16, log=None):
+    losses = []
+    for ep in range(epochs):
+        for g, ((off, nb, x0), t) in enumerate(zip(graphs, targets)):
+            net.set_graph(off, nb, feat_seed=7 + g, walks=walks, noise=x0 / INIT_HALF)
+            net.set_target(t, n_anchor=n_anchor, seed=ep * 1000 + g)
+            losses.append(net.train(steps_per_graph, lr))
+        if log:
+            log(f"epoch {ep}: mean loss {np.mean(losses[-len(graphs):]):.4f}")
+    return losses
+
+
+def save(net: Mpnn, coef, path=WEIGHTS, **meta):
+    np.savez(path, params=net.params(), size_law=np.asarray(coef, np.float64),
+             **{k: np.asarray(v) for k, v in meta.items()})
+
+
+def load(path=WEIGHTS, seed: int = 1):
+    with np.load(path, allow_pickle=False) as z:
+        net = Mpnn(seed)
+        net.set_params(z["params"])
+        return net, z["size_law"].copy()
+
+
+def predict(net: Mpnn, coef, off, nb, x0, walks: int = 16, feat_seed: int = 7):
+    """Predicted initial layout (3n) and the device time of features + forward (ms)."""
+    t0 = time.perf_counter()
+    net.set_graph(off, nb, feat_seed=feat_seed, walks=walks, noise=np.asarray(x0) / INIT_HALF)
+    y, fwd_ms = net.forward()
+    p = y.reshape(-1, 3).astype(np.float64)
+    p -= p.mean(0)
+    r = rms_radius(p)
+    n = len(off) - 1
+    p *= size_of(coef, n, len(nb)) / max(r, 1e-30)
+    wall_ms = (time.perf_counter() - t0) * 1e3
+    return p.astype(np.float32).reshape(-1), fwd_ms, wall_ms
+
+
+def stress_judge(off, nb, k: int = 50, seed: int = 0x5EEDC0DE12345678):
+    """Pivot terms of k max-min pivots (sgd_stress.rs:143-175) on the GPU stress plan."""
+    n = len(off) - 1
+    e = SgdStressEngine(n_pivots=k, seed=seed)
+    e.init(off, nb, np.zeros(3 * n, np.float32))
+    piv, dist, _ = e.pivots()
+    return StressJudge(n, piv, dist)
+
+
+def stress_curve(off, nb, init, judge, iters: int, every: int, theta: float,
+                 stop_below: float | None = None):
+    """Runs the plain engine from `init`; stress after every `every` iterations.
+    Returns (list of (iteration, stress), ms per iteration measured on the device)."""
+    e = Fa2BhEngine(theta=theta)
+    e.init(off, nb, init)
+    curve = [(0, judge(init))]
+    dev_ms = 0.0
+    done = 0
+    while done < iters:
+        k = min(every, iters - done)
+        _, it_ms = e.step_timed(k)
+        dev_ms += it_ms
+        done += k
+        s = judge(e.get_state()["positions"])
+        curve.append((done, s))
+        if stop_below is not None and s <= stop_below:
+            break
+    return curve, dev_ms / max(done, 1)
+
+
+def first_below(curve, target):
+    for it, s in curve:
+        if s <= target:
+            return it
+    return None
+
+
+def evaluate(net: Mpnn, coef, off, nb, x0, iters: int = 200, every: int = 5, theta: float = 0.8,
+             k_pivots: int = 50, log=None):
+    """Time to reach stress <= 1.01 x the plain run's stress at `iters`, plain vs learned."""
+    judge = stress_judge(off, nb, k_pivots)
+    plain, ms_it = stress_curve(off, nb, x0, judge, iters, every, theta)
+    s_star = plain[-1][1]
+    target = 1.01 * s_star
+    t_plain_it = first_below(plain, target)
+    init, fwd_ms, pred_wall_ms = predict(net, coef, off, nb, x0)
+    learned, ms_it2 = stress_curve(off, nb, init, judge, iters, every, theta, stop_below=target)
+    t_learn_it = first_below(learned, target)
+    fallback = t_learn_it is None
+    res = {
+        "plain_stress_final": s_star,
+        "target_stress": target,
+        "plain_iters_to_target": t_plain_it,
+        "plain_ms_to_target": t_plain_it * ms_it,
+        "learned_init_stress": learned[0][1],
+        "learned_iters_to_target": t_learn_it,
+        "inference_ms": pred_wall_ms,
+        "inference_forward_device_ms": fwd_ms,
+        "learned_ms_to_target": None if fallback else pred_wall_ms + t_learn_it * ms_it2,
+        "ms_per_iteration": ms_it,
+        "fallback_to_plain": fallback,
+        "plain_curve": plain,
+        "learned_curve": learned,
+    }
+    if not fallback and res["learned_ms_to_target"] > 0:
+        res["speedup"] = res["plain_ms_to_target"] / res["learned_ms_to_target"]
+    else:
+        res["speedup"] = None if fallback else float("inf")
+    if log:
+        log(f"plain: S*={s_star:.4f}, target reached at it {t_plain_it}; learned: init S="
+            f"{learned[0][1]:.4f}, target at it {t_learn_it}, speedup {res['speedup']}")
+    return res

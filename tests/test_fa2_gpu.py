@@ -67,7 +67,7 @@ def _tree_equal(g, t):
     assert np.array_equal(g["depth"], want_depth)
 
 
-@pytest.mark.parametrize("walk", [0, 1], ids=["lane-walk", "union-walk"])
+@pytest.mark.parametrize("walk", [0, 1, 2, 3], ids=["lane", "union", "chains2", "chains4"])
 @pytest.mark.parametrize("name,off,nb,xyz", CASES, ids=[c[0] for c in CASES])
 def test_tree_and_forces_bit_exact(name, off, nb, xyz, walk):
     n = len(off) - 1
@@ -112,7 +112,7 @@ def test_forces_within_1e4_of_fp64(theta):
     assert rel.max() <= 1e-4
 
 
-@pytest.mark.parametrize("walk", [0, 1])
+@pytest.mark.parametrize("walk", [0, 1, 2, 3])
 def test_trajectory_bit_exact_sbm2000(walk):
     """50 iterations: positions, F_{t-1} and the speed state bit-exact with the
     oracle driving the same (Morton) octree and the exact f64 sums."""
@@ -241,11 +241,14 @@ def test_ba_1m_tree_and_force_parity_at_full_size():
     off, nb, xyz = workloads.ba_1m()
     n = len(off) - 1
     e = _engine(off, nb, xyz, theta=0.8)
-    e.walk_mode(0)
-    f2, _ = e.forces_at(xyz)
-    e.walk_mode(1)
-    f, sums = e.forces_at(xyz)
-    assert np.array_equal(f.view(np.uint32), f2.view(np.uint32))
+    fs = []
+    for mode in (0, 1, 2, 3):
+        e.walk_mode(mode)
+        fs.append(e.forces_at(xyz))
+    f, sums = fs[0]
+    for f2, s2 in fs[1:]:
+        assert np.array_equal(f.view(np.uint32), f2.view(np.uint32))
+        assert np.array_equal(sums.view(np.uint64), s2.view(np.uint64))
     g = e.tree(cap=4 * n)
     mass = _mass(off, nb)
     # invariants: one leaf per valid body, leaf mass sums to sum(deg+1), keys sorted
