@@ -3,13 +3,27 @@
 workload (m=4, positions uniform in [-10,10]^3, theta 0.8).
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--no-cpu-baseline]
+                    [--no-warm-start] [--copy-probe]
+                    [--workload ba_1m|batch_4096|pagerank_ba1m|sgd_rgg200k]
 
 A step is one full FA2+BH iteration (octree build, walk, attraction, gravity,
 adaptive speed, apply) with inputs already resident in HBM.  The timed region
 replays the captured iteration graph; HIP events on the engine stream bracket
 every force kernel inside it, which gives the roofline figure.  The CPU
 baseline is the oracle's restatement of the reference path timed on this
-host on a bounded sample of the same workload.  Prints one JSON line.
+host on a bounded sample of the same workload.  The second half of the metric,
+time to reach 1.01x the plain run's stress at iteration 200 from the learned
+warm start versus from the random start, is measured on the same graph and
+start (normalized stress over 50 max-min pivot BFS rows).  Prints one JSON line.
+
+--copy-probe copies 256 MiB three times with torch and exits: the HBM-counter
+calibration run (FETCH_SIZE / WRITE_SIZE of a copy of known size).
+
+The other workloads print one JSON line each for the sibling paths:
+  batch_4096     4,096 graphs (50..20,000 nodes) in one batch, 500 iterations,
+                 against the same graphs run one engine at a time (sampled);
+  pagerank_ba1m  PageRank on the 1M-node BA CSR, d=0.85, L1 tol 1e-6;
+  sgd_rgg200k    SGD sparse stress on the 200k-node 2-D RGG, 200 pivots, 30 epochs.
 """
 from __future__ import annotations
 
@@ -67,6 +81,135 @@ def cpu_baseline(off, nb, xyz, theta: float, budget_s: float) -> dict:
                       f"under OpenMP ({threads} threads), f64 node-order speed sums"}
 
 
+def copy_probe() -> int:
+    import torch
+
+    a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("copied 256 MiB three times")
+    return 0
+
+
+def warm_start(off, nb, xyz, theta: float) -> dict | None:
+    from jumpcannon import warmstart as W
+
+    if not os.path.exists(W.WEIGHTS):
+        log("no trained warm-start weights; skipping the learned half of the metric")
+        return None
+    net, coef = W.load()
+    r = W.evaluate(net, coef, off, nb, xyz, theta=theta, log=log)
+    keep = ("horizon", "check_every", "plain_init_stress", "plain_stress_final", "target_stress",
+            "plain_iters_to_target", "plain_ms_to_target", "learned_init_stress",
+            "learned_stress_final", "learned_iters_to_target", "inference_ms",
+            "learned_ms_to_target", "fallback_to_plain", "speedup",
+            "scaled_random_iters_to_target", "scaled_random_ms_to_target")
+    return {k: (round(r[k], 5) if isinstance(r[k], float) else r[k]) for k in keep}
+
+
+def _line(metric, value, unit, steps, warmup, ms_per_step, dtype, config, **extra) -> int:
+    out = {"metric": metric, "value": round(value, 4), "unit": unit, "n_gpus": 1,
+           "steps": steps, "warmup": warmup, "ms_per_step": round(ms_per_step, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+           "data": "synthetic", "config": config}
+    out.update(extra)
+    print(json.dumps(out))
+    return 0
+
+
+def bench_batch(args) -> int:
+    """4,096 graphs, 500 iterations in one batch; the same graphs one engine at a
+    time, timed on every 16th graph and scaled by 16."""
+    import jumpcannon as jc
+    from jumpcannon import workloads
+
+    iters = 500
+    t = time.perf_counter()
+    graphs, pos = workloads.batch_4096()
+    total = sum(len(o) - 1 for o, _ in graphs)
+    log(f"workload: {len(graphs)} graphs, {total} nodes ({time.perf_counter() - t:.1f}s)")
+    b = jc.Fa2BhBatch(theta=1.0)
+    b.init(graphs, pos)
+    b.step(2, readback=False)
+    b_ms, bs = b.step_timed(iters)
+    log(f"batch: {iters} iterations in {b_ms:.1f} ms (force block {bs})")
+    seq_ms = 0.0
+    sample = range(0, len(graphs), 16)
+    for g in sample:
+        off, nb = graphs[g]
+        e = jc.Fa2BhEngine(theta=1.0)
+        e.init(off, nb, pos[g])
+        e.step(2, readback=False)
+        _, it_ms = e.step_timed(20)
+        seq_ms += it_ms / 20 * iters
+        e.close()
+    seq_ms *= len(graphs) / len(sample)
+    log(f"one engine per graph (sampled): {seq_ms:.1f} ms for {iters} iterations")
+    return _line("batched FA2+BH node-iterations/s (4,096 graphs)", total * iters / (b_ms / 1e3),
+                 "node-iterations/s", iters, 2, b_ms / iters, "f32",
+                 {"workload": "batch_4096_fa2bh", "graphs": len(graphs), "total_nodes": total,
+                  "sizes": "log-uniform 50..20000, half SBM half BA", "theta": 1.0,
+                  "parallelism": "single-gpu"},
+                 batch_ms=round(b_ms, 2), force_block=bs,
+                 sequential_ms_estimate=round(seq_ms, 1),
+                 speedup_vs_sequential=round(seq_ms / b_ms, 2),
+                 sequential_sample="every 16th graph, its own engine, 20 timed iterations, x16")
+
+
+def bench_pagerank(args) -> int:
+    """PageRank on the 1M-node BA CSR to L1 < 1e-6; iterations/s of the fixed
+    per-iteration kernels and the HBM figure of one iteration."""
+    import jumpcannon as jc
+    from jumpcannon import workloads
+
+    off, nb, _ = workloads.ba_1m(args.nodes)
+    n, nnz = len(off) - 1, len(nb)
+    plan = jc.PageRankPlan(off, nb, 0.85)
+    plan.run(100, 1e-6, readback=False)
+    _, iters, resid, run_ms = plan.run(100, 1e-6, readback=False)
+    ms_it = plan.time_iterations(100)
+    algo = 4 * (n + 1) + 4 * nnz + 24 * n
+    achieved = algo / (ms_it / 1e3) / 1e9
+    log(f"pagerank: {iters} iterations to L1 {resid:.3e} in {run_ms:.3f} ms; {ms_it:.4f} ms/iteration")
+    return _line("PageRank iterations/s at 1M nodes (BA m=4)", 1e3 / ms_it, "iterations/s", 100, 100,
+                 ms_it, "f32",
+                 {"workload": "pagerank_ba1m", "n_nodes": n, "csr_nnz": nnz, "damping": 0.85,
+                  "tol_l1": 1e-6, "parallelism": "single-gpu"},
+                 iterations_to_tol=iters, residual_l1=resid, ms_to_tol=round(run_ms, 4),
+                 roofline={"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                           "traffic": None, "bytes_per_iteration": algo})
+
+
+def bench_sgd(args) -> int:
+    """SGD sparse stress on the 200k-node RGG: 200 pivots, 30 epochs."""
+    import jumpcannon as jc
+    from jumpcannon import workloads
+
+    off, nb, _ = workloads.rgg_200k()
+    n = len(off) - 1
+    xyz = workloads.plane_init(n)
+    e = jc.SgdStressEngine(n_pivots=200, eta_max=0.0, eta_min=0.01, n_anneal_steps=30)
+    t = time.perf_counter()
+    e.init(off, nb, xyz)
+    init_s = time.perf_counter() - t
+    ms = e.time_steps(30)
+    k = 200
+    algo = 2 * n * k + 32 * n + 16 * k
+    achieved = algo / (ms / 30 / 1e3) / 1e9
+    log(f"sgd: 30 epochs in {ms:.3f} ms (init incl. BFS {init_s:.2f} s)")
+    return _line("SGD sparse-stress epochs/s at 200k nodes (RGG, 200 pivots)", 30 / (ms / 1e3),
+                 "epochs/s", 30, 0, ms / 30, "f32",
+                 {"workload": "sgd_rgg200k", "n_nodes": n, "csr_nnz": len(nb), "pivots": k,
+                  "parallelism": "single-gpu"},
+                 init_seconds_incl_bfs=round(init_s, 3),
+                 roofline={"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                           "traffic": None, "bytes_per_epoch": algo})
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,11 +219,20 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--profile-iters", type=int, default=4)
+    ap.add_argument("--no-warm-start", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--workload", default="ba_1m",
+                    choices=["ba_1m", "batch_4096", "pagerank_ba1m", "sgd_rgg200k"])
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the layout path is single-GPU in this build).", file=sys.stderr)
         return 2
+    if args.workload != "ba_1m":
+        return {"batch_4096": bench_batch, "pagerank_ba1m": bench_pagerank,
+                "sgd_rgg200k": bench_sgd}[args.workload](args)
 
     import jumpcannon as jc
     from jumpcannon import workloads
@@ -128,6 +280,8 @@ def main() -> int:
         cpu = cpu_baseline(off, nb, xyz, theta, args.cpu_seconds)
         log(f"cpu baseline {cpu['value']:.3f} it/s on {cpu['cores']} threads")
 
+    ws = None if args.no_warm_start else warm_start(off, nb, xyz, theta)
+
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -151,6 +305,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(value / cpu["value"], 2) if cpu else None,
+        "warm_start": ws,
     }
     print(json.dumps(out))
     return 0

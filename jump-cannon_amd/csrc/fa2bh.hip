@@ -44,6 +44,8 @@ constexpr uint32_t kScanItems = 16;       // count/scan tile = 4096 bodies
 constexpr uint32_t kScanTile = kTB * kScanItems;
 constexpr int kWalkModes = 4;             // lane walk, wave-union, 2 and 4 chains per lane
 constexpr uint32_t kTimedIters = 10;      // timed iterations per host read-back of events
+constexpr uint32_t kHeavyAtt = 32;        // attraction rows longer than this go to k_force_heavy
+constexpr uint32_t kHeavyChunk = 256;     // attraction terms staged in LDS per round (4 per lane)
 
 struct Frame {
     uint32_t bbmin[3];
@@ -375,6 +377,48 @@ __device__ __forceinline__ bool visit(const Node &nd, float px, float py, float 
 
 // Attraction, gravity and the swing/traction stats of body b, after its walk
 // (fa2_barnes_hut.wgsl:158-197), accumulated exactly into the block's limbs.
+// One attraction term d * att (fa2_barnes_hut.wgsl:158-181), d = q - p.
+__device__ __forceinline__ void attraction_term(const Params &P, float4 p, float4 q, float &tx,
+                                                float &ty, float &tz) {
+    const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+    float att = P.ewi_weight;
+    if (P.lin_log_mode) {
+        const float r = fmaxf(len3(dx, dy, dz), 1.0e-3f);
+        att = P.ewi_weight * logf(1.0f + r) / r;
+    }
+    tx = dx * att;
+    ty = dy * att;
+    tz = dz * att;
+}
+
+// Gravity (fa2_barnes_hut.wgsl:183-189), the force write and the swing /
+// traction stats (:191-197) into the exact accumulator.
+__device__ __forceinline__ void gravity_and_stats(const Params &P, uint32_t b, float4 p, float deg_i,
+                                                  float fx, float fy, float fz,
+                                                  const float4 *__restrict__ old_force,
+                                                  float4 *__restrict__ force,
+                                                  unsigned long long *acc) {
+    const float gd = P.gravity * deg_i;
+    if (P.strong_gravity) {
+        fx = fx - p.x * gd;
+        fy = fy - p.y * gd;
+        fz = fz - p.z * gd;
+    } else {
+        const float r0 = fmaxf(len3(p.x, p.y, p.z), 1.0e-3f);
+        fx = fx - (p.x / r0) * gd;
+        fy = fy - (p.y / r0) * gd;
+        fz = fz - (p.z / r0) * gd;
+    }
+    const float4 o = old_force[b];
+    force[b] = make_float4(fx, fy, fz, 0.0f);
+    const float swing = deg_i * len3(o.x - fx, o.y - fy, o.z - fz);
+    const float tra = deg_i * 0.5f * len3(o.x + fx, o.y + fy, o.z + fz);
+    if (finitef(swing) && finitef(tra)) {
+        exact_acc_add(acc, swing);
+        exact_acc_add(acc + kLimbs, tra);
+    }
+}
+
 __device__ __forceinline__ void finish_body(const Params &P, uint32_t b, float4 p, float deg_i,
                                             float fx, float fy, float fz,
                                             const float4 *__restrict__ pos,
@@ -384,40 +428,37 @@ __device__ __forceinline__ void finish_body(const Params &P, uint32_t b, float4 
                                             const uint32_t *__restrict__ att_adj,
                                             unsigned long long *acc) {
         // attraction: the other endpoint of every edge touching b, in the
-        // reference's edge-list order (fa2_barnes_hut.wgsl:158-181)
+        // reference's edge-list order (fa2_barnes_hut.wgsl:158-181).  Rows
+        // longer than kHeavyAtt leave the repulsion sum in force[b] and are
+        // finished by k_force_heavy, one wave per row.
         const uint32_t e0 = att_off[b], e1 = att_off[b + 1];
-        for (uint32_t e = e0; e < e1; e++) {
-            const float4 q = pos[att_adj[e]];
-            const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
-            float att = P.ewi_weight;
-            if (P.lin_log_mode) {
-                const float r = fmaxf(len3(dx, dy, dz), 1.0e-3f);
-                att = P.ewi_weight * logf(1.0f + r) / r;
+        if (e1 - e0 > kHeavyAtt) {
+            force[b] = make_float4(fx, fy, fz, 0.0f);
+            return;
+        }
+        uint32_t e = e0;
+        // four neighbours per round: the index loads, then the position
+        // gathers, are issued back to back; the sum stays in list order
+        for (; e + 4 <= e1; e += 4) {
+            const uint32_t j0 = att_adj[e], j1 = att_adj[e + 1], j2 = att_adj[e + 2], j3 = att_adj[e + 3];
+            const float4 q[4] = {pos[j0], pos[j1], pos[j2], pos[j3]};
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                float tx, ty, tz;
+                attraction_term(P, p, q[u], tx, ty, tz);
+                fx = fx + tx;
+                fy = fy + ty;
+                fz = fz + tz;
             }
-            fx = fx + dx * att;
-            fy = fy + dy * att;
-            fz = fz + dz * att;
         }
-        // gravity (fa2_barnes_hut.wgsl:183-189)
-        const float gd = P.gravity * deg_i;
-        if (P.strong_gravity) {
-            fx = fx - p.x * gd;
-            fy = fy - p.y * gd;
-            fz = fz - p.z * gd;
-        } else {
-            const float r0 = fmaxf(len3(p.x, p.y, p.z), 1.0e-3f);
-            fx = fx - (p.x / r0) * gd;
-            fy = fy - (p.y / r0) * gd;
-            fz = fz - (p.z / r0) * gd;
+        for (; e < e1; e++) {
+            float tx, ty, tz;
+            attraction_term(P, p, pos[att_adj[e]], tx, ty, tz);
+            fx = fx + tx;
+            fy = fy + ty;
+            fz = fz + tz;
         }
-        const float4 o = old_force[b];
-        force[b] = make_float4(fx, fy, fz, 0.0f);
-        const float swing = deg_i * len3(o.x - fx, o.y - fy, o.z - fz);
-        const float tra = deg_i * 0.5f * len3(o.x + fx, o.y + fy, o.z + fz);
-        if (finitef(swing) && finitef(tra)) {
-            exact_acc_add(acc, swing);
-            exact_acc_add(acc + kLimbs, tra);
-        }
+        gravity_and_stats(P, b, p, deg_i, fx, fy, fz, old_force, force, acc);
     }
 
 // MODE 0: every lane follows its own rope (64 node gathers per step).
@@ -437,7 +478,8 @@ __global__ __launch_bounds__(kTB) void k_force(const Params P, Frame *fr,
                                                const uint32_t *__restrict__ att_adj,
                                                const uint32_t *__restrict__ perm,
                                                const Node *__restrict__ nodes,
-                                               unsigned long long *__restrict__ partial) {
+                                               unsigned long long *__restrict__ partial,
+                                               uint32_t pstride) {
     __shared__ unsigned long long acc[2 * kLimbs];
     __shared__ unsigned long long inter;
     if (threadIdx.x < 2 * kLimbs) acc[threadIdx.x] = 0ull;
@@ -490,8 +532,9 @@ __global__ __launch_bounds__(kTB) void k_force(const Params P, Frame *fr,
     for (int o = 32; o > 0; o >>= 1) taken += __shfl_xor(taken, o, 64);
     if ((threadIdx.x & 63) == 0) atomicAdd(&inter, (unsigned long long)taken);
     __syncthreads();
-    // limb-major [2*kLimbs][gridDim.x] so the reduction reads coalesce
-    if (threadIdx.x < 2 * kLimbs) partial[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = acc[threadIdx.x];
+    // limb-major [2*kLimbs][pstride] so the reduction reads coalesce; columns
+    // from gridDim.x on belong to k_force_heavy
+    if (threadIdx.x < 2 * kLimbs) partial[(size_t)threadIdx.x * pstride + blockIdx.x] = acc[threadIdx.x];
     if (threadIdx.x == 0) atomicAdd(&fr->interactions, inter);
 }
 
@@ -509,7 +552,8 @@ __global__ __launch_bounds__(kTB) void k_force_multi(const Params P, Frame *fr,
                                                      const uint32_t *__restrict__ att_adj,
                                                      const uint32_t *__restrict__ perm,
                                                      const Node *__restrict__ nodes,
-                                                     unsigned long long *__restrict__ partial) {
+                                                     unsigned long long *__restrict__ partial,
+                                                     uint32_t pstride) {
     __shared__ unsigned long long acc[2 * kLimbs];
     __shared__ unsigned long long inter;
     if (threadIdx.x < 2 * kLimbs) acc[threadIdx.x] = 0ull;
@@ -565,8 +609,67 @@ __global__ __launch_bounds__(kTB) void k_force_multi(const Params P, Frame *fr,
     for (int o = 32; o > 0; o >>= 1) taken += __shfl_xor(taken, o, 64);
     if ((threadIdx.x & 63) == 0) atomicAdd(&inter, (unsigned long long)taken);
     __syncthreads();
-    if (threadIdx.x < 2 * kLimbs) partial[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = acc[threadIdx.x];
+    if (threadIdx.x < 2 * kLimbs) partial[(size_t)threadIdx.x * pstride + blockIdx.x] = acc[threadIdx.x];
     if (threadIdx.x == 0) atomicAdd(&fr->interactions, inter);
+}
+
+// Attraction rows longer than kHeavyAtt, one 64-lane wave per row.  force[b]
+// holds the repulsion sum the walk left there.  Each round stages up to
+// kHeavyChunk terms d*att in LDS (four gathers in flight per lane), then lanes
+// 0, 1, 2 add the x, y, z terms to the running sum in list order -- the same
+// f32 additions, in the same order, as the per-lane loop -- and lane 0 finishes
+// the body.  Its swing / traction go to partial column col0 + blockIdx.x.
+__global__ __launch_bounds__(64) void k_force_heavy(const Params P,
+                                                   const uint32_t *__restrict__ heavy,
+                                                   const float4 *__restrict__ pos,
+                                                   const float4 *__restrict__ old_force,
+                                                   float4 *__restrict__ force,
+                                                   const uint32_t *__restrict__ deg,
+                                                   const uint32_t *__restrict__ att_off,
+                                                   const uint32_t *__restrict__ att_adj,
+                                                   unsigned long long *__restrict__ partial,
+                                                   uint32_t pstride, uint32_t col0) {
+    __shared__ float term[3][kHeavyChunk];
+    __shared__ unsigned long long acc[2 * kLimbs];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t b = heavy[blockIdx.x];
+    if (lane < 2 * kLimbs) acc[lane] = 0ull;
+    const float4 p = pos[b];
+    const float4 f0 = force[b];
+    float run = lane == 0 ? f0.x : lane == 1 ? f0.y : f0.z;
+    const uint32_t e0 = att_off[b], e1 = att_off[b + 1];
+    for (uint32_t base = e0; base < e1; base += kHeavyChunk) {
+        const uint32_t cnt = min(kHeavyChunk, e1 - base);
+        uint32_t j[kHeavyChunk / 64];
+#pragma unroll
+        for (uint32_t r = 0; r < kHeavyChunk / 64; r++) {
+            const uint32_t k = lane + 64u * r;
+            j[r] = k < cnt ? att_adj[base + k] : 0u;
+        }
+        float4 q[kHeavyChunk / 64];
+#pragma unroll
+        for (uint32_t r = 0; r < kHeavyChunk / 64; r++) q[r] = pos[j[r]];
+#pragma unroll
+        for (uint32_t r = 0; r < kHeavyChunk / 64; r++) {
+            const uint32_t k = lane + 64u * r;
+            if (k < cnt) {
+                float tx, ty, tz;
+                attraction_term(P, p, q[r], tx, ty, tz);
+                term[0][k] = tx;
+                term[1][k] = ty;
+                term[2][k] = tz;
+            }
+        }
+        __syncthreads();
+        if (lane < 3)
+            for (uint32_t k = 0; k < cnt; k++) run = run + term[lane][k];
+        __syncthreads();
+    }
+    const float fx = __shfl(run, 0, 64), fy = __shfl(run, 1, 64), fz = __shfl(run, 2, 64);
+    if (lane == 0)
+        gravity_and_stats(P, b, p, (float)deg[b] + 1.0f, fx, fy, fz, old_force, force, acc);
+    __syncthreads();
+    if (lane < 2 * kLimbs) partial[(size_t)lane * pstride + col0 + blockIdx.x] = acc[lane];
 }
 
 // Sums the per-block limbs (limb-major, 51 threads per limb), then runs
@@ -740,6 +843,10 @@ class Fa2BhEngine final : public LayoutEngine {
                 }
             n_att_ = aoff[n_];
         }
+        std::vector<uint32_t> heavy;
+        for (uint32_t v = 0; v < n_; v++)
+            if (aoff[v + 1] - aoff[v] > kHeavyAtt) heavy.push_back(v);
+        n_heavy_ = (uint32_t)heavy.size();
         JC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         const uint32_t nn = n_ ? n_ : 1;
         JC_TRY(pos_[0].alloc(nn));
@@ -761,7 +868,11 @@ class Fa2BhEngine final : public LayoutEngine {
         JC_TRY(nodes_.alloc(cap_));
         JC_TRY(node_depth_.alloc(cap_));
         nb_force_ = cdiv(nn, kTB);
-        JC_TRY(partial_.alloc((size_t)nb_force_ * 2 * kLimbs));
+        JC_TRY(partial_.alloc(((size_t)nb_force_ + n_heavy_) * 2 * kLimbs));
+        JC_TRY(heavy_.alloc(n_heavy_ ? n_heavy_ : 1));
+        if (n_heavy_)
+            JC_HIP(hipMemcpyAsync(heavy_.p, heavy.data(), n_heavy_ * sizeof(uint32_t),
+                                  hipMemcpyHostToDevice, stream_));
         JC_TRY(frame_.alloc(1));
         std::vector<float4> p4(nn, make_float4(0, 0, 0, 0));
         for (uint32_t i = 0; i < n_; i++)
@@ -862,7 +973,7 @@ class Fa2BhEngine final : public LayoutEngine {
         JC_TRY(write_vec4(tmp.p, xyz));
         JC_TRY(enqueue_tree(tmp.p, nullptr));
         JC_TRY(enqueue_force(tmp.p, nullptr));
-        k_speed<<<1, 1024, 0, stream_>>>(params_, frame_.p, partial_.p, force_grid_now(), 0u);
+        k_speed<<<1, 1024, 0, stream_>>>(params_, frame_.p, partial_.p, speed_cols(), 0u);
         JC_HIP(hipGetLastError());
         if (out_force) JC_TRY(read_vec4(force_.p, out_force));
         Frame fr;
@@ -924,7 +1035,7 @@ class Fa2BhEngine final : public LayoutEngine {
                     JC_HIP(hipEventRecord(tev_[2 * k], stream_));
                     JC_TRY(enqueue_force(pos_[c].p, nullptr));
                     JC_HIP(hipEventRecord(tev_[2 * k + 1], stream_));
-                    k_speed<<<1, 1024, 0, stream_>>>(params_, frame_.p, partial_.p, force_grid_now(), 1u);
+                    k_speed<<<1, 1024, 0, stream_>>>(params_, frame_.p, partial_.p, speed_cols(), 1u);
                     k_apply<<<cdiv(n_, kTB), kTB, 0, stream_>>>(params_, frame_.p, pos_[c].p,
                                                                  pos_[c ^ 1].p, force_.p,
                                                                  old_force_.p, deg_.p);
@@ -1007,6 +1118,8 @@ class Fa2BhEngine final : public LayoutEngine {
     static uint32_t chains_of(int mode) { return mode == 2 ? 2u : mode == 3 ? 4u : 1u; }
     uint32_t force_grid(int mode) const { return cdiv(cdiv(n_ ? n_ : 1, chains_of(mode)), kTB); }
     uint32_t force_grid_now() const { return force_grid(walk_mode_); }
+    // columns of the limb-major partial sums: the force blocks, then one per heavy row
+    uint32_t speed_cols() const { return force_grid_now() + n_heavy_; }
 
     // Picks the fastest walk on the current positions: builds the octree once,
     // times five launches of each force kernel, keeps the fastest (all give
@@ -1112,7 +1225,8 @@ class Fa2BhEngine final : public LayoutEngine {
         const uint32_t grid = force_grid(walk_mode_);
 #define JC_FORCE(KERNEL)                                                                    \
     KERNEL<<<grid, kTB, 0, stream_>>>(params_, frame_.p, pos, old_force_.p, force_.p, deg_.p, \
-                                      att_off_.p, att_adj_.p, vals_.p, nodes_.p, partial_.p)
+                                      att_off_.p, att_adj_.p, vals_.p, nodes_.p, partial_.p,   \
+                                      grid + n_heavy_)
         switch (walk_mode_) {
             case 0: JC_FORCE(k_force<0>); break;
             case 2: JC_FORCE(k_force_multi<2>); break;
@@ -1120,6 +1234,10 @@ class Fa2BhEngine final : public LayoutEngine {
             default: JC_FORCE(k_force<1>); break;
         }
 #undef JC_FORCE
+        if (n_heavy_)
+            k_force_heavy<<<n_heavy_, 64, 0, stream_>>>(params_, heavy_.p, pos, old_force_.p,
+                                                       force_.p, deg_.p, att_off_.p, att_adj_.p,
+                                                       partial_.p, grid + n_heavy_, grid);
         JC_HIP(hipGetLastError());
         return Status::ok();
     }
@@ -1130,7 +1248,7 @@ class Fa2BhEngine final : public LayoutEngine {
         JC_TRY(enqueue_force(pos_[c].p, ev));
         if (ev) JC_HIP(hipEventRecord(ev[PH_SPEED], stream_));
         if (speed)
-            k_speed<<<1, 1024, 0, stream_>>>(params_, frame_.p, partial_.p, force_grid_now(), 1u);
+            k_speed<<<1, 1024, 0, stream_>>>(params_, frame_.p, partial_.p, speed_cols(), 1u);
         if (ev) JC_HIP(hipEventRecord(ev[PH_APPLY], stream_));
         if (apply)
             k_apply<<<cdiv(n_, kTB), kTB, 0, stream_>>>(params_, frame_.p, pos_[c].p,
@@ -1207,6 +1325,8 @@ class Fa2BhEngine final : public LayoutEngine {
         nodes_.release();
         node_depth_.release();
         partial_.release();
+        heavy_.release();
+        n_heavy_ = 0;
         frame_.release();
         if (stream_) (void)hipStreamDestroy(stream_);
         stream_ = nullptr;
@@ -1220,6 +1340,8 @@ class Fa2BhEngine final : public LayoutEngine {
     bool inited_ = false;
     uint32_t n_ = 0;
     uint64_t n_att_ = 0;
+    uint32_t n_heavy_ = 0;          // bodies whose attraction row exceeds kHeavyAtt
+    DevBuf<uint32_t> heavy_;
     uint32_t cur_ = 0;
     uint64_t iteration_ = 0;
     uint64_t cap_ = 0;

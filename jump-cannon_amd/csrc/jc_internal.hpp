@@ -49,6 +49,7 @@ template <typename T>
 struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
+    size_t cap = 0;
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
@@ -57,6 +58,7 @@ struct DevBuf {
         if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
+        cap = 0;
     }
     Status alloc(size_t count) {
         release();
@@ -68,7 +70,16 @@ struct DevBuf {
                                              " bytes failed: " + hipGetErrorString(e));
         }
         n = count;
+        cap = count ? count : 1;
         return Status::ok();
+    }
+    // Keeps the allocation when it already holds `count` elements.
+    Status fit(size_t count) {
+        if (p && count <= cap) {
+            n = count;
+            return Status::ok();
+        }
+        return alloc(count);
     }
 };
 

@@ -533,20 +533,20 @@ class Mpnn {
         n_ = n;
         has_target_ = false;
         const size_t nn = n ? n : 1;
-        JC_TRY(off_.alloc((size_t)n + 1));
-        JC_TRY(nbr_.alloc(nnz ? nnz : 1));
-        JC_TRY(x_.alloc(nn * kIn));
-        JC_TRY(a0_.alloc(nn * kIn));
-        for (int l = 0; l < kLayers; l++) JC_TRY(h_[l].alloc(nn * kHid));
-        for (int l = 1; l < kLayers; l++) JC_TRY(a_[l].alloc(nn * kHid));
-        JC_TRY(y_.alloc(nn * 3));
-        JC_TRY(dy_.alloc(nn * 3));
-        JC_TRY(dzf_.alloc(nn * kHid));
-        JC_TRY(dz_.alloc(nn * kHid));
-        JC_TRY(dx_.alloc(nn * 2 * kHid));
-        JC_TRY(dh_.alloc(nn * kHid));
-        JC_TRY(rw_.alloc(nn * kRw));
-        JC_TRY(parts_.alloc((size_t)cd(nn, kXtgRows) * (2 * kHid + 1) * kHid));
+        JC_TRY(off_.fit((size_t)n + 1));
+        JC_TRY(nbr_.fit(nnz ? nnz : 1));
+        JC_TRY(x_.fit(nn * kIn));
+        JC_TRY(a0_.fit(nn * kIn));
+        for (int l = 0; l < kLayers; l++) JC_TRY(h_[l].fit(nn * kHid));
+        for (int l = 1; l < kLayers; l++) JC_TRY(a_[l].fit(nn * kHid));
+        JC_TRY(y_.fit(nn * 3));
+        JC_TRY(dy_.fit(nn * 3));
+        JC_TRY(dzf_.fit(nn * kHid));
+        JC_TRY(dz_.fit(nn * kHid));
+        JC_TRY(dx_.fit(nn * 2 * kHid));
+        JC_TRY(dh_.fit(nn * kHid));
+        JC_TRY(rw_.fit(nn * kRw));
+        JC_TRY(parts_.fit((size_t)cd(nn, kXtgRows) * (2 * kHid + 1) * kHid));
         DevBuf<float> noise_d;
         JC_TRY(noise_d.alloc(nn * 3));
         std::vector<float> nz(nn * 3, 0.0f);

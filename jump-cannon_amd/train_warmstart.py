@@ -4,10 +4,11 @@
     python jump-cannon_amd/train_warmstart.py [--graphs 256] [--epochs 6] [--out PATH]
 
 Training set: seeded SBM/BA graphs of 1k-20k nodes, each with a random initial
-layout x0 in [-10,10]^3 and its plain FA2+BH layout after 300 iterations
-(produced by the batched engine).  Every forward/backward/Adam step runs in the
-library's own HIP kernels.  The default output is the file the warm start
-loads (jumpcannon/warmstart_weights.npz).
+layout x0 in [-10,10]^3.  Shape targets are the layouts the SGD sparse-stress
+engine reaches from x0; the size law comes from the plain FA2+BH layouts after
+--fa2-iters iterations (all graphs in one batched run).  Every forward,
+backward and Adam step runs in the library's own HIP kernels.  The default
+output is the file the warm start loads (jumpcannon/warmstart_weights.npz).
 """
 from __future__ import annotations
 
@@ -31,7 +32,9 @@ def main() -> int:
     ap.add_argument("--graphs", type=int, default=256)
     ap.add_argument("--n-min", type=int, default=1000)
     ap.add_argument("--n-max", type=int, default=20000)
-    ap.add_argument("--target-iters", type=int, default=300)
+    ap.add_argument("--fa2-iters", type=int, default=1500)
+    ap.add_argument("--pivots", type=int, default=50)
+    ap.add_argument("--sgd-epochs", type=int, default=30)
     ap.add_argument("--epochs", type=int, default=6)
     ap.add_argument("--steps-per-graph", type=int, default=2)
     ap.add_argument("--lr", type=float, default=2e-3)
@@ -45,9 +48,12 @@ def main() -> int:
     sizes = [len(o) - 1 for o, _, _ in graphs]
     log(f"{len(graphs)} graphs, {sum(sizes)} nodes ({time.perf_counter() - t0:.1f}s)")
     t1 = time.perf_counter()
-    targets = W.fa2_targets(graphs, args.target_iters, theta=1.0)
-    log(f"targets: {args.target_iters} FA2 iterations on the batch ({time.perf_counter() - t1:.1f}s)")
-    coef = W.fit_size_law(graphs, targets)
+    fa2 = W.fa2_targets(graphs, args.fa2_iters, theta=1.0)
+    log(f"size-law layouts: {args.fa2_iters} FA2 iterations on the batch ({time.perf_counter() - t1:.1f}s)")
+    coef = W.fit_size_law(graphs, fa2)
+    t1 = time.perf_counter()
+    targets = W.stress_targets(graphs, args.pivots, args.sgd_epochs)
+    log(f"shape targets: {args.sgd_epochs} SGD stress epochs per graph ({time.perf_counter() - t1:.1f}s)")
     log(f"size law log R = {coef[0]:.3f} + {coef[1]:.3f} log n + {coef[2]:.3f} log(mean degree)")
     net = jc.Mpnn(seed=args.seed)
     t2 = time.perf_counter()
@@ -55,7 +61,8 @@ def main() -> int:
     log(f"trained {len(losses) * args.steps_per_graph} steps ({time.perf_counter() - t2:.1f}s), "
         f"first-epoch loss {np.mean(losses[:len(graphs)]):.4f}, last {np.mean(losses[-len(graphs):]):.4f}")
     W.save(net, coef, args.out, graphs=args.graphs, epochs=args.epochs, seed=args.seed,
-           target_iters=args.target_iters, n_min=args.n_min, n_max=args.n_max)
+           fa2_iters=args.fa2_iters, pivots=args.pivots, sgd_epochs=args.sgd_epochs,
+           n_min=args.n_min, n_max=args.n_max)
     log(f"wrote {args.out}")
     print(json.dumps({"graphs": args.graphs, "nodes": int(sum(sizes)),
                       "loss_first_epoch": float(np.mean(losses[:len(graphs)])),
