@@ -249,7 +249,8 @@ def main() -> int:
 
     e.step(args.warmup, readback=False)
     e.sync()
-    log(f"warmup {args.warmup} done")
+    walk = e.walk_mode(-1)   # re-time the walk variants on the warmed-up layout
+    log(f"warmup {args.warmup} done; walk mode {walk}")
 
     t0 = time.perf_counter()
     force_ms, iter_ms = e.step_timed(args.steps)   # synchronizes at the end
@@ -271,7 +272,7 @@ def main() -> int:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "kernel": "k_force", "bytes_per_launch": fb,
+                "kernel": "k_force (+ k_force_heavy)", "bytes_per_launch": fb,
                 "avg_launch_ms": round(force_ms / args.steps, 4)}
 
     cpu = None
@@ -301,6 +302,7 @@ def main() -> int:
         "interactions_per_iter": st["interactions"],
         "interactions_per_s": round(st["interactions"] * value, 1),
         "octree_nodes": st["n_tree"],
+        "walk_mode": walk,
         "phase_ms": {k: round(v, 4) for k, v in phases.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,

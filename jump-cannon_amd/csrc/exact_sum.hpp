@@ -36,26 +36,33 @@ __device__ __forceinline__ void exact_normalise(unsigned long long *l) {
 }
 
 // Nearest f64 (ties to even) of a normalised non-negative limb integer * 2^-149.
-__device__ inline double exact_to_double(const unsigned long long *l) {
-    int top = -1;
+// Word-level: the three limbs from the top nonzero one form a 96-bit window
+// H:L; the 53-bit mantissa, round bit and sticky bit come from shifts of the
+// window plus an OR of the limbs below it.  All limb indices are compile-time
+// constants after unrolling, so the limbs stay in registers.
+__host__ __device__ inline double exact_to_double(const unsigned long long *l) {
+    int tk = -1;
+#pragma unroll
     for (int k = (int)kLimbs - 1; k >= 0; k--)
-        if (l[k]) {
-            top = 32 * k + 63 - __clzll((long long)l[k]);
-            break;
-        }
-    if (top < 0) return 0.0;
-    auto bit = [&](int q) -> uint32_t {
-        return q < 0 ? 0u : (uint32_t)((l[q >> 5] >> (q & 31)) & 1ull);
-    };
-    unsigned long long mant = 0;
-    const int lo = top - 52;
-    for (int q = top; q >= (lo > 0 ? lo : 0); q--) mant = (mant << 1) | bit(q);
-    if (lo <= 0) return ldexp((double)mant, -149);
-    const uint32_t rb = bit(lo - 1);
-    uint32_t sticky = 0;
-    for (int q = lo - 2; q >= 0 && !sticky; q--) sticky |= bit(q);
+        if (tk < 0 && l[k] != 0ull) tk = k;
+    if (tk < 0) return 0.0;
+    unsigned long long hi = 0, mid = 0, lo = 0;
+    bool below = false;
+#pragma unroll
+    for (int k = 0; k < (int)kLimbs; k++) {
+        if (k == tk) hi = l[k];
+        if (k == tk - 1) mid = l[k];
+        if (k == tk - 2) lo = l[k];
+        if (k < tk - 2) below |= l[k] != 0ull;
+    }
+    const unsigned long long L = (mid << 32) | lo;      // window bits 0..63; hi is bits 64..95
+    const int h = 63 - __builtin_clzll(hi);                // top bit of hi (0..31)
+    const int sh = 12 + h;                                 // window bits dropped below the mantissa
+    unsigned long long mant = (hi << (52 - h)) | (L >> sh);
+    const unsigned long long rb = (L >> (sh - 1)) & 1ull;
+    const bool sticky = below || (L & ((1ull << (sh - 1)) - 1ull)) != 0ull;
     if (rb && (sticky || (mant & 1ull))) mant++;
-    return ldexp((double)mant, lo - 149);
+    return ldexp((double)mant, sh + 32 * (tk - 2) - 149);
 }
 
 // p - q for normalised limb integers with p >= q, result normalised in p.

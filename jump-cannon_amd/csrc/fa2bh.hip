@@ -45,7 +45,10 @@ constexpr uint32_t kScanTile = kTB * kScanItems;
 constexpr int kWalkModes = 4;             // lane walk, wave-union, 2 and 4 chains per lane
 constexpr uint32_t kTimedIters = 10;      // timed iterations per host read-back of events
 constexpr uint32_t kHeavyAtt = 32;        // attraction rows longer than this go to k_force_heavy
-constexpr uint32_t kHeavyChunk = 256;     // attraction terms staged in LDS per round (4 per lane)
+constexpr uint32_t kHeavyStage = 1024;    // heavy-row terms staged in LDS per round
+constexpr uint32_t kEmitTasks = 1024;     // cell tasks per k_emit block held in LDS
+constexpr uint32_t kAccCols = 64;         // columns of the swing/traction limb accumulator
+constexpr uint32_t kSpeedThreads = 320;   // k_speed: 16 threads per limb
 
 struct Frame {
     uint32_t bbmin[3];
@@ -88,7 +91,10 @@ struct Params {
 };
 
 // ---------------------------------------------------------------- kernels
-__global__ void k_frame_reset(Frame *fr) {
+// Also clears the limb accumulator the force kernels add into (every force
+// launch follows a tree build, which starts with this kernel).
+__global__ void k_frame_reset(Frame *fr, unsigned long long *__restrict__ acc_cols) {
+    for (uint32_t q = threadIdx.x; q < 2 * kLimbs * kAccCols; q += blockDim.x) acc_cols[q] = 0ull;
     if (threadIdx.x == 0) {
         const uint32_t pinf = 0xFF800000u;  // f2ord(+inf)
         for (int k = 0; k < 3; k++) {
@@ -258,56 +264,31 @@ __device__ __forceinline__ uint32_t first_node_of(const uint32_t *loc, const uin
     return loc[i] + blk[i / kScanTile] + i;
 }
 
-__device__ __forceinline__ void emit_body(const uint64_t *__restrict__ keys,
-                                          const uint32_t *__restrict__ perm,
-                                          const float4 *__restrict__ pos,
-                                          const uint32_t *__restrict__ deg, uint32_t n, uint32_t nv,
-                                          uint32_t i, const Frame *fr,
+// One internal cell: depth d, opened by sorted body i at node index `node`.
+// Its half-size is the root's halved d times (fa2_bh.rs:855-861); its skip rope
+// is the first node of the body after the last one sharing its d-digit prefix.
+__device__ __forceinline__ void emit_cell(const uint64_t *__restrict__ keys, uint32_t nv,
+                                          uint32_t i, int d, uint32_t node, const Frame *fr,
                                           const uint32_t *__restrict__ loc,
                                           const uint32_t *__restrict__ blk,
                                           Node *__restrict__ nodes,
-                                          uint8_t *__restrict__ node_depth, uint64_t cap,
-                                          uint32_t *bdepth) {
-    if (i >= nv) return;
-    const uint64_t k = keys[i];
-    int lp, li;
-    if (nv == 1) {
-        lp = -1;
-        li = 0;  // the lone body hangs below an internal root
-    } else {
-        lp = i > 0 ? lcp_digits(keys[i - 1], k) : -1;
-        li = i + 1 < nv ? lcp_digits(k, keys[i + 1]) : -1;
-    }
-    const uint32_t c = li > lp ? (uint32_t)(li - lp) : 0u;
-    const int a0 = lp + 1;
-    const uint32_t pos0 = first_node_of(loc, blk, i);
-    if ((uint64_t)pos0 + c >= cap) return;  // cannot happen for a consistent scan
-    if (c) {
-        float h = fr->root[3];
-        for (int d = 0; d < a0; d++) h = h * 0.5f;
-        for (uint32_t t = 0; t < c; t++) {
-            const int d = a0 + (int)t;
-            const uint32_t j = cell_end(keys, i, nv, k, d);
-            const uint32_t skip = (j + 1 < nv) ? first_node_of(loc, blk, j + 1) : kEnd;
-            const float s = h * 2.0f;
-            const float s2 = s * s;
-            nodes[pos0 + t].b = make_uint4(__float_as_uint(s2), skip, 0u, 0u);
-            node_depth[pos0 + t] = (uint8_t)d;
-            h = h * 0.5f;
-        }
-        atomicMax(bdepth, (uint32_t)(a0 + (int)c - 1));
-    }
-    const uint32_t leaf = pos0 + c;
-    const uint32_t b = perm[i];
-    if (b >= n) return;
-    const float4 p = pos[b];
-    const float m = fmaxf((float)deg[b] + 1.0f, 1e-3f);
-    nodes[leaf].a = make_float4(p.x, p.y, p.z, m);
-    nodes[leaf].b = make_uint4(kLeaf | b, (i + 1 < nv) ? leaf + 1 : kEnd, 0u, 0u);
-    const int ld = (lp > li ? lp : li) + 1;
-    node_depth[leaf] = (uint8_t)(0x80 | ld);
+                                          uint8_t *__restrict__ node_depth) {
+    float h = fr->root[3];
+    for (int q = 0; q < d; q++) h = h * 0.5f;
+    const uint32_t j = cell_end(keys, i, nv, keys[i], d);
+    const uint32_t skip = (j + 1 < nv) ? first_node_of(loc, blk, j + 1) : kEnd;
+    const float s = h * 2.0f;
+    const float s2 = s * s;
+    nodes[node].b = make_uint4(__float_as_uint(s2), skip, 0u, 0u);
+    node_depth[node] = (uint8_t)d;
 }
 
+// Pre-order emission.  Body i (sorted) opens the cells at depths lp+1 .. li
+// (lp, li: common octal digits with its neighbours), then its leaf at depth
+// max(lp, li)+1.  The leaves are written by their body's thread; the cells go
+// through a block-wide task list, so the prefix searches of a body that opens
+// many cells (the first bodies open the top of the tree) run on many threads
+// instead of one after another.
 __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
                                               const uint32_t *__restrict__ perm,
                                               const float4 *__restrict__ pos,
@@ -316,13 +297,59 @@ __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
                                               const uint32_t *__restrict__ blk,
                                               Node *__restrict__ nodes,
                                               uint8_t *__restrict__ node_depth, uint64_t cap) {
-    __shared__ uint32_t bdepth;
-    if (threadIdx.x == 0) bdepth = 0;
+    __shared__ uint32_t bdepth, ntask;
+    __shared__ uint32_t t_node[kEmitTasks], t_body[kEmitTasks];
+    __shared__ uint8_t t_depth[kEmitTasks];
+    if (threadIdx.x == 0) {
+        bdepth = 0;
+        ntask = 0;
+    }
     __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nv = n - fr->n_invalid;
-    emit_body(keys, perm, pos, deg, n, nv, i, fr, loc, blk, nodes, node_depth, cap, &bdepth);
+    if (i < nv) {
+        const uint64_t k = keys[i];
+        int lp, li;
+        if (nv == 1) {
+            lp = -1;
+            li = 0;  // the lone body hangs below an internal root
+        } else {
+            lp = i > 0 ? lcp_digits(keys[i - 1], k) : -1;
+            li = i + 1 < nv ? lcp_digits(k, keys[i + 1]) : -1;
+        }
+        const uint32_t c = li > lp ? (uint32_t)(li - lp) : 0u;
+        const int a0 = lp + 1;
+        const uint32_t pos0 = first_node_of(loc, blk, i);
+        if ((uint64_t)pos0 + c < cap) {  // always, for a consistent scan
+            if (c) {
+                const uint32_t slot = atomicAdd(&ntask, c);
+                for (uint32_t t = 0; t < c; t++) {
+                    if (slot + t < kEmitTasks) {
+                        t_node[slot + t] = pos0 + t;
+                        t_body[slot + t] = i;
+                        t_depth[slot + t] = (uint8_t)(a0 + (int)t);
+                    } else {
+                        emit_cell(keys, nv, i, a0 + (int)t, pos0 + t, fr, loc, blk, nodes, node_depth);
+                    }
+                }
+                atomicMax(&bdepth, (uint32_t)(a0 + (int)c - 1));
+            }
+            const uint32_t leaf = pos0 + c;
+            const uint32_t b = perm[i];
+            if (b < n) {
+                const float4 p = pos[b];
+                const float m = fmaxf((float)deg[b] + 1.0f, 1e-3f);
+                nodes[leaf].a = make_float4(p.x, p.y, p.z, m);
+                nodes[leaf].b = make_uint4(kLeaf | b, (i + 1 < nv) ? leaf + 1 : kEnd, 0u, 0u);
+                const int ld = (lp > li ? lp : li) + 1;
+                node_depth[leaf] = (uint8_t)(0x80 | ld);
+            }
+        }
+    }
     __syncthreads();
+    const uint32_t nt = min(ntask, kEmitTasks);
+    for (uint32_t q = threadIdx.x; q < nt; q += blockDim.x)
+        emit_cell(keys, nv, t_body[q], t_depth[q], t_node[q], fr, loc, blk, nodes, node_depth);
     if (threadIdx.x == 0 && bdepth) atomicMax(&fr->max_depth, bdepth);
 }
 
@@ -478,8 +505,7 @@ __global__ __launch_bounds__(kTB) void k_force(const Params P, Frame *fr,
                                                const uint32_t *__restrict__ att_adj,
                                                const uint32_t *__restrict__ perm,
                                                const Node *__restrict__ nodes,
-                                               unsigned long long *__restrict__ partial,
-                                               uint32_t pstride) {
+                                               unsigned long long *__restrict__ partial) {
     __shared__ unsigned long long acc[2 * kLimbs];
     __shared__ unsigned long long inter;
     if (threadIdx.x < 2 * kLimbs) acc[threadIdx.x] = 0ull;
@@ -532,9 +558,10 @@ __global__ __launch_bounds__(kTB) void k_force(const Params P, Frame *fr,
     for (int o = 32; o > 0; o >>= 1) taken += __shfl_xor(taken, o, 64);
     if ((threadIdx.x & 63) == 0) atomicAdd(&inter, (unsigned long long)taken);
     __syncthreads();
-    // limb-major [2*kLimbs][pstride] so the reduction reads coalesce; columns
-    // from gridDim.x on belong to k_force_heavy
-    if (threadIdx.x < 2 * kLimbs) partial[(size_t)threadIdx.x * pstride + blockIdx.x] = acc[threadIdx.x];
+    // the block's limbs go into one of kAccCols columns of the limb-major
+    // accumulator; integer adds, so the total is exact in any order
+    if (threadIdx.x < 2 * kLimbs && acc[threadIdx.x])
+        atomicAdd(&partial[threadIdx.x * kAccCols + blockIdx.x % kAccCols], acc[threadIdx.x]);
     if (threadIdx.x == 0) atomicAdd(&fr->interactions, inter);
 }
 
@@ -552,8 +579,7 @@ __global__ __launch_bounds__(kTB) void k_force_multi(const Params P, Frame *fr,
                                                      const uint32_t *__restrict__ att_adj,
                                                      const uint32_t *__restrict__ perm,
                                                      const Node *__restrict__ nodes,
-                                                     unsigned long long *__restrict__ partial,
-                                                     uint32_t pstride) {
+                                                     unsigned long long *__restrict__ partial) {
     __shared__ unsigned long long acc[2 * kLimbs];
     __shared__ unsigned long long inter;
     if (threadIdx.x < 2 * kLimbs) acc[threadIdx.x] = 0ull;
@@ -609,98 +635,116 @@ __global__ __launch_bounds__(kTB) void k_force_multi(const Params P, Frame *fr,
     for (int o = 32; o > 0; o >>= 1) taken += __shfl_xor(taken, o, 64);
     if ((threadIdx.x & 63) == 0) atomicAdd(&inter, (unsigned long long)taken);
     __syncthreads();
-    if (threadIdx.x < 2 * kLimbs) partial[(size_t)threadIdx.x * pstride + blockIdx.x] = acc[threadIdx.x];
+    if (threadIdx.x < 2 * kLimbs && acc[threadIdx.x])
+        atomicAdd(&partial[threadIdx.x * kAccCols + blockIdx.x % kAccCols], acc[threadIdx.x]);
     if (threadIdx.x == 0) atomicAdd(&fr->interactions, inter);
 }
 
-// Attraction rows longer than kHeavyAtt, one 64-lane wave per row.  force[b]
-// holds the repulsion sum the walk left there.  Each round stages up to
-// kHeavyChunk terms d*att in LDS (four gathers in flight per lane), then lanes
-// 0, 1, 2 add the x, y, z terms to the running sum in list order -- the same
-// f32 additions, in the same order, as the per-lane loop -- and lane 0 finishes
-// the body.  Its swing / traction go to partial column col0 + blockIdx.x.
+// Attraction rows longer than kHeavyAtt.  Their terms d*att are independent
+// of the walk, so k_heavy_terms computes all of them before the force kernel
+// (one thread per entry, every gather in flight at once) into hterm, three
+// component arrays over the heavy rows' entries laid end to end (row r owns
+// [hoff[r], hoff[r] + hlen[r]), hoff[r] a multiple of 4).  After the force kernel, k_force_heavy adds them to
+// the repulsion sum the walk left in force[b]: lanes 0, 1, 2 of one wave per
+// row run the x, y, z sums in list order -- the same f32 additions in the same
+// order as the per-lane loop -- and lane 0 finishes the body.
+__global__ __launch_bounds__(kTB) void k_heavy_terms(const Params P, uint32_t n_ent,
+                                                     const uint32_t *__restrict__ hbody,
+                                                     const uint32_t *__restrict__ hadj,
+                                                     const float4 *__restrict__ pos,
+                                                     float *__restrict__ hterm, uint32_t stride) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_ent) return;
+    float tx, ty, tz;
+    attraction_term(P, pos[hbody[q]], pos[hadj[q]], tx, ty, tz);
+    hterm[q] = tx;
+    hterm[stride + q] = ty;
+    hterm[2 * (size_t)stride + q] = tz;
+}
+
+// Per row: the wave copies up to kHeavyStage terms of each component into LDS
+// with coalesced 16-byte loads (rows start on 16-byte boundaries in hterm),
+// then lanes 0, 1, 2 add them from LDS.  Swing / traction limbs are added
+// (integer atomics, so the sum stays exact and order-free) into accumulator
+// column blockIdx.x % kAccCols.
 __global__ __launch_bounds__(64) void k_force_heavy(const Params P,
                                                    const uint32_t *__restrict__ heavy,
+                                                   const uint32_t *__restrict__ hoff,
+                                                   const uint32_t *__restrict__ hlen,
+                                                   const float *__restrict__ hterm,
+                                                   uint32_t stride,
                                                    const float4 *__restrict__ pos,
                                                    const float4 *__restrict__ old_force,
                                                    float4 *__restrict__ force,
                                                    const uint32_t *__restrict__ deg,
-                                                   const uint32_t *__restrict__ att_off,
-                                                   const uint32_t *__restrict__ att_adj,
-                                                   unsigned long long *__restrict__ partial,
-                                                   uint32_t pstride, uint32_t col0) {
-    __shared__ float term[3][kHeavyChunk];
+                                                   unsigned long long *__restrict__ partial) {
+    __shared__ __align__(16) float st[3][kHeavyStage];
     __shared__ unsigned long long acc[2 * kLimbs];
     const uint32_t lane = threadIdx.x;
     const uint32_t b = heavy[blockIdx.x];
     if (lane < 2 * kLimbs) acc[lane] = 0ull;
-    const float4 p = pos[b];
     const float4 f0 = force[b];
     float run = lane == 0 ? f0.x : lane == 1 ? f0.y : f0.z;
-    const uint32_t e0 = att_off[b], e1 = att_off[b + 1];
-    for (uint32_t base = e0; base < e1; base += kHeavyChunk) {
-        const uint32_t cnt = min(kHeavyChunk, e1 - base);
-        uint32_t j[kHeavyChunk / 64];
+    const uint32_t e0 = hoff[blockIdx.x], len = hlen[blockIdx.x];
+    for (uint32_t c0 = 0; c0 < len; c0 += kHeavyStage) {
+        const uint32_t cnt = min(kHeavyStage, len - c0);
+        const uint32_t nv4 = (cnt + 3u) >> 2;   // padded entries are loaded, never added
 #pragma unroll
-        for (uint32_t r = 0; r < kHeavyChunk / 64; r++) {
-            const uint32_t k = lane + 64u * r;
-            j[r] = k < cnt ? att_adj[base + k] : 0u;
-        }
-        float4 q[kHeavyChunk / 64];
-#pragma unroll
-        for (uint32_t r = 0; r < kHeavyChunk / 64; r++) q[r] = pos[j[r]];
-#pragma unroll
-        for (uint32_t r = 0; r < kHeavyChunk / 64; r++) {
-            const uint32_t k = lane + 64u * r;
-            if (k < cnt) {
-                float tx, ty, tz;
-                attraction_term(P, p, q[r], tx, ty, tz);
-                term[0][k] = tx;
-                term[1][k] = ty;
-                term[2][k] = tz;
-            }
+        for (uint32_t comp = 0; comp < 3; comp++) {
+            const float4 *src = reinterpret_cast<const float4 *>(hterm + (size_t)comp * stride + e0 + c0);
+            float4 *dst = reinterpret_cast<float4 *>(st[comp]);
+            for (uint32_t v = lane; v < nv4; v += 64) dst[v] = src[v];
         }
         __syncthreads();
-        if (lane < 3)
-            for (uint32_t k = 0; k < cnt; k++) run = run + term[lane][k];
+        if (lane < 3) {
+            const float4 *t4 = reinterpret_cast<const float4 *>(st[lane]);
+            const uint32_t full = cnt >> 2;
+#pragma unroll 8
+            for (uint32_t q = 0; q < full; q++) {
+                const float4 v = t4[q];
+                run = run + v.x;
+                run = run + v.y;
+                run = run + v.z;
+                run = run + v.w;
+            }
+            for (uint32_t k = full << 2; k < cnt; k++) run = run + st[lane][k];
+        }
         __syncthreads();
     }
     const float fx = __shfl(run, 0, 64), fy = __shfl(run, 1, 64), fz = __shfl(run, 2, 64);
     if (lane == 0)
-        gravity_and_stats(P, b, p, (float)deg[b] + 1.0f, fx, fy, fz, old_force, force, acc);
+        gravity_and_stats(P, b, pos[b], (float)deg[b] + 1.0f, fx, fy, fz, old_force, force, acc);
     __syncthreads();
-    if (lane < 2 * kLimbs) partial[(size_t)lane * pstride + col0 + blockIdx.x] = acc[lane];
+    if (lane < 2 * kLimbs && acc[lane])
+        atomicAdd(&partial[lane * kAccCols + blockIdx.x % kAccCols], acc[lane]);
 }
 
-// Sums the per-block limbs (limb-major, 51 threads per limb), then runs
+// Sums the kAccCols accumulator columns per limb, then runs
 // AdaptiveSpeed::update (fa2_speed.rs:48-100) on one thread.  update_speed = 0
 // only records the sums (parity hook).
-__global__ __launch_bounds__(1024) void k_speed(const Params P, Frame *fr,
-                                                const unsigned long long *__restrict__ partial,
-                                                uint32_t nblocks, uint32_t update_speed) {
-    constexpr uint32_t G = 51;
-    __shared__ unsigned long long red[2 * kLimbs][G];
+__global__ __launch_bounds__(kSpeedThreads) void k_speed(const Params P, Frame *fr,
+                                                         const unsigned long long *__restrict__ partial,
+                                                         uint32_t update_speed) {
+    constexpr uint32_t G = kSpeedThreads / (2 * kLimbs);   // threads per limb
+    static_assert(kAccCols % G == 0, "columns split evenly over a limb's threads");
+    static_assert(64 % G == 0, "a limb's threads sit inside one wave");
+    __shared__ unsigned long long red[2 * kLimbs];
     const uint32_t t = threadIdx.x;
-    if (t < G * 2 * kLimbs) {
+    {
         const uint32_t k = t / G, r = t % G;
-        const unsigned long long *row = partial + (size_t)k * nblocks;
-        unsigned long long s0 = 0, s1 = 0;
-        uint32_t b = r;
-        for (; b + G < nblocks; b += 2 * G) {
-            s0 += row[b];
-            s1 += row[b + G];
-        }
-        if (b < nblocks) s0 += row[b];
-        red[k][r] = s0 + s1;
+        const unsigned long long *row = partial + (size_t)k * kAccCols;
+        unsigned long long s = 0;
+#pragma unroll
+        for (uint32_t c = r; c < kAccCols; c += G) s += row[c];
+#pragma unroll
+        for (uint32_t o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, (int)o, 64);
+        if (r == 0) red[k] = s;
     }
     __syncthreads();
     if (t != 0) return;
     unsigned long long tot[2 * kLimbs];
-    for (uint32_t k = 0; k < 2 * kLimbs; k++) {
-        unsigned long long s = 0;
-        for (uint32_t g = 0; g < G; g++) s += red[k][g];
-        tot[k] = s;
-    }
+#pragma unroll
+    for (uint32_t k = 0; k < 2 * kLimbs; k++) tot[k] = red[k];
     exact_normalise(tot);
     exact_normalise(tot + kLimbs);
     const double ts = exact_to_double(tot), tt = exact_to_double(tot + kLimbs);
@@ -846,7 +890,31 @@ class Fa2BhEngine final : public LayoutEngine {
         std::vector<uint32_t> heavy;
         for (uint32_t v = 0; v < n_; v++)
             if (aoff[v + 1] - aoff[v] > kHeavyAtt) heavy.push_back(v);
+        // longest rows first, so the longest wave starts with the launch
+        std::stable_sort(heavy.begin(), heavy.end(), [&](uint32_t a, uint32_t b) {
+            return aoff[a + 1] - aoff[a] > aoff[b + 1] - aoff[b];
+        });
         n_heavy_ = (uint32_t)heavy.size();
+        // rows laid end to end, each starting on a multiple of 4 entries; the
+        // padding entries repeat the row's body as its own neighbour (a zero
+        // term that is loaded but never added)
+        std::vector<uint32_t> hoff(n_heavy_ + 1, 0), hlen(n_heavy_ ? n_heavy_ : 1, 0), hbody, hadj;
+        for (uint32_t r = 0; r < n_heavy_; r++) {
+            const uint32_t v = heavy[r];
+            hoff[r] = (uint32_t)hbody.size();
+            hlen[r] = aoff[v + 1] - aoff[v];
+            for (uint32_t e = aoff[v]; e < aoff[v + 1]; e++) {
+                hbody.push_back(v);
+                hadj.push_back(adj[e]);
+            }
+            while (hbody.size() & 3u) {
+                hbody.push_back(v);
+                hadj.push_back(v);
+            }
+        }
+        hoff[n_heavy_] = (uint32_t)hbody.size();
+        n_hent_ = (uint32_t)hbody.size();
+        hstride_ = (n_hent_ + 3u) & ~3u;
         JC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         const uint32_t nn = n_ ? n_ : 1;
         JC_TRY(pos_[0].alloc(nn));
@@ -868,11 +936,25 @@ class Fa2BhEngine final : public LayoutEngine {
         JC_TRY(nodes_.alloc(cap_));
         JC_TRY(node_depth_.alloc(cap_));
         nb_force_ = cdiv(nn, kTB);
-        JC_TRY(partial_.alloc(((size_t)nb_force_ + n_heavy_) * 2 * kLimbs));
+        JC_TRY(partial_.alloc((size_t)kAccCols * 2 * kLimbs));
         JC_TRY(heavy_.alloc(n_heavy_ ? n_heavy_ : 1));
-        if (n_heavy_)
+        JC_TRY(hoff_.alloc((size_t)n_heavy_ + 1));
+        JC_TRY(hlen_.alloc(n_heavy_ ? n_heavy_ : 1));
+        JC_TRY(hbody_.alloc(n_hent_ ? n_hent_ : 1));
+        JC_TRY(hadj_.alloc(n_hent_ ? n_hent_ : 1));
+        JC_TRY(hterm_.alloc(3 * (size_t)(hstride_ ? hstride_ : 4)));
+        if (n_heavy_) {
             JC_HIP(hipMemcpyAsync(heavy_.p, heavy.data(), n_heavy_ * sizeof(uint32_t),
                                   hipMemcpyHostToDevice, stream_));
+            JC_HIP(hipMemcpyAsync(hoff_.p, hoff.data(), hoff.size() * sizeof(uint32_t),
+                                  hipMemcpyHostToDevice, stream_));
+            JC_HIP(hipMemcpyAsync(hlen_.p, hlen.data(), n_heavy_ * sizeof(uint32_t),
+                                  hipMemcpyHostToDevice, stream_));
+            JC_HIP(hipMemcpyAsync(hbody_.p, hbody.data(), n_hent_ * sizeof(uint32_t),
+                                  hipMemcpyHostToDevice, stream_));
+            JC_HIP(hipMemcpyAsync(hadj_.p, hadj.data(), n_hent_ * sizeof(uint32_t),
+                                  hipMemcpyHostToDevice, stream_));
+        }
         JC_TRY(frame_.alloc(1));
         std::vector<float4> p4(nn, make_float4(0, 0, 0, 0));
         for (uint32_t i = 0; i < n_; i++)
@@ -973,7 +1055,7 @@ class Fa2BhEngine final : public LayoutEngine {
         JC_TRY(write_vec4(tmp.p, xyz));
         JC_TRY(enqueue_tree(tmp.p, nullptr));
         JC_TRY(enqueue_force(tmp.p, nullptr));
-        k_speed<<<1, 1024, 0, stream_>>>(params_, frame_.p, partial_.p, speed_cols(), 0u);
+        k_speed<<<1, kSpeedThreads, 0, stream_>>>(params_, frame_.p, partial_.p, 0u);
         JC_HIP(hipGetLastError());
         if (out_force) JC_TRY(read_vec4(force_.p, out_force));
         Frame fr;
@@ -1035,7 +1117,7 @@ class Fa2BhEngine final : public LayoutEngine {
                     JC_HIP(hipEventRecord(tev_[2 * k], stream_));
                     JC_TRY(enqueue_force(pos_[c].p, nullptr));
                     JC_HIP(hipEventRecord(tev_[2 * k + 1], stream_));
-                    k_speed<<<1, 1024, 0, stream_>>>(params_, frame_.p, partial_.p, speed_cols(), 1u);
+                    k_speed<<<1, kSpeedThreads, 0, stream_>>>(params_, frame_.p, partial_.p, 1u);
                     k_apply<<<cdiv(n_, kTB), kTB, 0, stream_>>>(params_, frame_.p, pos_[c].p,
                                                                  pos_[c ^ 1].p, force_.p,
                                                                  old_force_.p, deg_.p);
@@ -1118,35 +1200,34 @@ class Fa2BhEngine final : public LayoutEngine {
     static uint32_t chains_of(int mode) { return mode == 2 ? 2u : mode == 3 ? 4u : 1u; }
     uint32_t force_grid(int mode) const { return cdiv(cdiv(n_ ? n_ : 1, chains_of(mode)), kTB); }
     uint32_t force_grid_now() const { return force_grid(walk_mode_); }
-    // columns of the limb-major partial sums: the force blocks, then one per heavy row
-    uint32_t speed_cols() const { return force_grid_now() + n_heavy_; }
 
-    // Picks the fastest walk on the current positions: builds the octree once,
-    // times five launches of each force kernel, keeps the fastest (all give
-    // bit-identical forces).  Engine state is not advanced.
+    // Picks the fastest walk on the current positions (all give bit-identical
+    // forces).  Engine state is not advanced.
     Status ensure_tuned() {
         if (walk_mode_ >= 0) return Status::ok();
-        JC_TRY(enqueue_tree(pos_[cur_].p, nullptr));
         hipEvent_t a, b;
         JC_HIP(hipEventCreate(&a));
         JC_HIP(hipEventCreate(&b));
-        float best = 1e30f;
-        int pick = 0;
-        for (int mode = 0; mode < kWalkModes; mode++) {
-            walk_mode_ = mode;
-            JC_TRY(enqueue_force(pos_[cur_].p, nullptr));
-            JC_HIP(hipEventRecord(a, stream_));
-            for (int r = 0; r < 5; r++) JC_TRY(enqueue_force(pos_[cur_].p, nullptr));
-            JC_HIP(hipEventRecord(b, stream_));
-            JC_HIP(hipEventSynchronize(b));
-            float ms = 0;
-            JC_HIP(hipEventElapsedTime(&ms, a, b));
-            tune_ms_[mode] = ms / 5.0f;
-            if (ms < best) {
-                best = ms;
-                pick = mode;
+        // Each timed launch follows a tree build, as in a real iteration (the
+        // build leaves the caches the way the walk finds them then; back-to-back
+        // walks would favour the variants that reuse a warm L2 most).  Four
+        // interleaved rounds over the modes, best round per mode.
+        for (int mode = 0; mode < kWalkModes; mode++) tune_ms_[mode] = 1e30f;
+        for (int round = 0; round < 4; round++)
+            for (int mode = 0; mode < kWalkModes; mode++) {
+                walk_mode_ = mode;
+                JC_TRY(enqueue_tree(pos_[cur_].p, nullptr));
+                JC_HIP(hipEventRecord(a, stream_));
+                JC_TRY(enqueue_force(pos_[cur_].p, nullptr));
+                JC_HIP(hipEventRecord(b, stream_));
+                JC_HIP(hipEventSynchronize(b));
+                float ms = 0;
+                JC_HIP(hipEventElapsedTime(&ms, a, b));
+                tune_ms_[mode] = std::min(tune_ms_[mode], ms);
             }
-        }
+        int pick = 0;
+        for (int mode = 1; mode < kWalkModes; mode++)
+            if (tune_ms_[mode] < tune_ms_[pick]) pick = mode;
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
         walk_mode_ = pick;
@@ -1199,7 +1280,7 @@ class Fa2BhEngine final : public LayoutEngine {
     Status enqueue_tree(const float4 *pos, hipEvent_t *ev) {
         const uint32_t n = n_;
         if (ev) JC_HIP(hipEventRecord(ev[PH_BBOX], stream_));
-        k_frame_reset<<<1, 64, 0, stream_>>>(frame_.p);
+        k_frame_reset<<<1, 256, 0, stream_>>>(frame_.p, partial_.p);
         k_bbox<<<std::min<uint32_t>(cdiv(n, kTB), 256), kTB, 0, stream_>>>(pos, n, frame_.p);
         if (ev) JC_HIP(hipEventRecord(ev[PH_KEYS], stream_));
         k_keys<<<cdiv(n, kTB), kTB, 0, stream_>>>(pos, n, frame_.p, keys_.p, vals_.p);
@@ -1212,7 +1293,7 @@ class Fa2BhEngine final : public LayoutEngine {
         k_emit<<<cdiv(n, kTB), kTB, 0, stream_>>>(keys_.p, vals_.p, pos, deg_.p, n, frame_.p,
                                                   loc_.p, blk_.p, nodes_.p, node_depth_.p, cap_);
         if (ev) JC_HIP(hipEventRecord(ev[PH_COM], stream_));
-        const uint32_t gcom = std::min<uint32_t>(cdiv(cap_, kTB), 2048);
+        const uint32_t gcom = std::min<uint32_t>(cdiv(cap_, kTB), 4096);
         for (int d = kLevels; d >= 0; d--)
             k_com_level<<<gcom, kTB, 0, stream_>>>(frame_.p, (uint32_t)d, (uint32_t)cap_,
                                                    nodes_.p, node_depth_.p);
@@ -1223,10 +1304,12 @@ class Fa2BhEngine final : public LayoutEngine {
     Status enqueue_force(const float4 *pos, hipEvent_t *ev) {
         if (ev) JC_HIP(hipEventRecord(ev[PH_FORCE], stream_));
         const uint32_t grid = force_grid(walk_mode_);
+        if (n_heavy_)
+            k_heavy_terms<<<cdiv(n_hent_, kTB), kTB, 0, stream_>>>(params_, n_hent_, hbody_.p,
+                                                                  hadj_.p, pos, hterm_.p, hstride_);
 #define JC_FORCE(KERNEL)                                                                    \
     KERNEL<<<grid, kTB, 0, stream_>>>(params_, frame_.p, pos, old_force_.p, force_.p, deg_.p, \
-                                      att_off_.p, att_adj_.p, vals_.p, nodes_.p, partial_.p,   \
-                                      grid + n_heavy_)
+                                      att_off_.p, att_adj_.p, vals_.p, nodes_.p, partial_.p)
         switch (walk_mode_) {
             case 0: JC_FORCE(k_force<0>); break;
             case 2: JC_FORCE(k_force_multi<2>); break;
@@ -1235,9 +1318,9 @@ class Fa2BhEngine final : public LayoutEngine {
         }
 #undef JC_FORCE
         if (n_heavy_)
-            k_force_heavy<<<n_heavy_, 64, 0, stream_>>>(params_, heavy_.p, pos, old_force_.p,
-                                                       force_.p, deg_.p, att_off_.p, att_adj_.p,
-                                                       partial_.p, grid + n_heavy_, grid);
+            k_force_heavy<<<n_heavy_, 64, 0, stream_>>>(params_, heavy_.p, hoff_.p, hlen_.p,
+                                                       hterm_.p, hstride_, pos, old_force_.p,
+                                                       force_.p, deg_.p, partial_.p);
         JC_HIP(hipGetLastError());
         return Status::ok();
     }
@@ -1248,7 +1331,7 @@ class Fa2BhEngine final : public LayoutEngine {
         JC_TRY(enqueue_force(pos_[c].p, ev));
         if (ev) JC_HIP(hipEventRecord(ev[PH_SPEED], stream_));
         if (speed)
-            k_speed<<<1, 1024, 0, stream_>>>(params_, frame_.p, partial_.p, speed_cols(), 1u);
+            k_speed<<<1, kSpeedThreads, 0, stream_>>>(params_, frame_.p, partial_.p, 1u);
         if (ev) JC_HIP(hipEventRecord(ev[PH_APPLY], stream_));
         if (apply)
             k_apply<<<cdiv(n_, kTB), kTB, 0, stream_>>>(params_, frame_.p, pos_[c].p,
@@ -1326,7 +1409,14 @@ class Fa2BhEngine final : public LayoutEngine {
         node_depth_.release();
         partial_.release();
         heavy_.release();
+        hoff_.release();
+        hlen_.release();
+        hbody_.release();
+        hadj_.release();
+        hterm_.release();
         n_heavy_ = 0;
+        n_hent_ = 0;
+        hstride_ = 0;
         frame_.release();
         if (stream_) (void)hipStreamDestroy(stream_);
         stream_ = nullptr;
@@ -1341,7 +1431,10 @@ class Fa2BhEngine final : public LayoutEngine {
     uint32_t n_ = 0;
     uint64_t n_att_ = 0;
     uint32_t n_heavy_ = 0;          // bodies whose attraction row exceeds kHeavyAtt
-    DevBuf<uint32_t> heavy_;
+    uint32_t n_hent_ = 0;           // entries of those rows
+    uint32_t hstride_ = 0;          // n_hent_ rounded up to 4: one component array of hterm_
+    DevBuf<uint32_t> heavy_, hoff_, hlen_, hbody_, hadj_;
+    DevBuf<float> hterm_;
     uint32_t cur_ = 0;
     uint64_t iteration_ = 0;
     uint64_t cap_ = 0;

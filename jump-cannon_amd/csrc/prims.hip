@@ -1,7 +1,7 @@
 // Device-wide primitives for gfx950: stable LSD radix sort of 64-bit keys with
-// 32-bit payloads.  Each 8-bit pass is: tile histograms -> per-digit scans of
-// the tile counts (256 blocks in parallel) and a scan of the digit totals ->
-// stable scatter.  Ranking inside a tile is wave-level (64 lanes): the lanes
+// 32-bit payloads.  Each 8-bit pass is three launches: tile histograms ->
+// per-digit scans of the tile counts (256 blocks in parallel) -> stable scatter,
+// which also scans the 256 digit totals into digit bases in LDS.  Ranking inside a tile is wave-level (64 lanes): the lanes
 // holding the same digit are found with eight ballots, the lowest of them
 // reserves a run with one LDS atomic and broadcasts it, so equal digits keep
 // their input order (stability) without any block barrier between rounds.
@@ -69,10 +69,9 @@ __global__ __launch_bounds__(256) void k_rs_digit_scan(uint32_t *__restrict__ hi
     if (threadIdx.x == 0) dtot[blockIdx.x] = carry;
 }
 
-// Exclusive scan of the 256 digit totals (one block).
-__global__ __launch_bounds__(256) void k_rs_base_scan(uint32_t *__restrict__ dtot) {
-    __shared__ uint32_t ws[4];
-    const uint32_t v = dtot[threadIdx.x];
+// Exclusive scan of one value per thread over a 256-thread block (the caller
+// synchronises before reusing ws).
+__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t *ws) {
     const uint32_t incl = wave_incl_scan(v);
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 63) ws[w] = incl;
@@ -81,21 +80,32 @@ __global__ __launch_bounds__(256) void k_rs_base_scan(uint32_t *__restrict__ dto
 #pragma unroll
     for (uint32_t q = 0; q < 4; q++)
         if (q < w) wp += ws[q];
-    dtot[threadIdx.x] = wp + incl - v;
+    return wp + incl - v;
 }
 
+// Stable scatter of one tile.  Keys are ranked per wave and digit (ballots),
+// placed in LDS in (digit, input order) order, and then written out by
+// consecutive threads, so each digit's run of the tile goes to global memory
+// as one contiguous, coalesced span.
 __global__ __launch_bounds__(kSortThreads) void k_rs_scatter(
     const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
     uint64_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t n, uint32_t shift,
-    const uint32_t *__restrict__ hist, const uint32_t *__restrict__ dbase, uint32_t nb) {
+    const uint32_t *__restrict__ hist, const uint32_t *__restrict__ dtot, uint32_t nb) {
+    static_assert(kSortThreads == 256, "one thread per digit");
     constexpr uint32_t W = kSortThreads / 64;
     __shared__ uint32_t wcnt[W][256];
+    __shared__ uint32_t gbase[256];
+    __shared__ uint32_t ws[4];
+    __shared__ uint64_t skey[kSortTile];
+    __shared__ uint32_t sval[kSortTile];
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
 #pragma unroll
     for (uint32_t q = 0; q < W; q++) wcnt[q][tid] = 0;
+    const uint32_t dbase = block_excl_scan(dtot[tid], ws);   // digit base over all tiles
     __syncthreads();
     const uint64_t lt = (1ull << lane) - 1ull;
-    const uint32_t base = blockIdx.x * kSortTile + w * (64 * kSortItems);
+    const uint32_t tile0 = blockIdx.x * kSortTile;
+    const uint32_t base = tile0 + w * (64 * kSortItems);
     uint64_t k[kSortItems];
     uint32_t v[kSortItems], rank[kSortItems];
 #pragma unroll
@@ -121,24 +131,40 @@ __global__ __launch_bounds__(kSortThreads) void k_rs_scatter(
     }
     __syncthreads();
     {
-        uint32_t run = dbase[tid] + hist[tid * nb + blockIdx.x];
+        // thread tid owns digit tid: the tile's count, its local start (scan over
+        // digits) and the per-wave local bases
+        uint32_t c[W], tot = 0;
 #pragma unroll
         for (uint32_t q = 0; q < W; q++) {
-            uint32_t c = wcnt[q][tid];
-            wcnt[q][tid] = run;
-            run += c;
+            c[q] = wcnt[q][tid];
+            tot += c[q];
         }
+        const uint32_t lstart = block_excl_scan(tot, ws);
+        uint32_t run = lstart;
+#pragma unroll
+        for (uint32_t q = 0; q < W; q++) {
+            wcnt[q][tid] = run;
+            run += c[q];
+        }
+        gbase[tid] = dbase + hist[tid * nb + blockIdx.x] - lstart;
     }
     __syncthreads();
 #pragma unroll
     for (uint32_t r = 0; r < kSortItems; r++) {
-        const uint32_t i = base + r * 64 + lane;
-        if (i < n) {
+        if (base + r * 64 + lane < n) {
             const uint32_t d = (uint32_t)(k[r] >> shift) & 255u;
-            const uint32_t dst = wcnt[w][d] + rank[r];
-            kout[dst] = k[r];
-            vout[dst] = v[r];
+            const uint32_t loc = wcnt[w][d] + rank[r];
+            skey[loc] = k[r];
+            sval[loc] = v[r];
         }
+    }
+    __syncthreads();
+    const uint32_t cnt = min(kSortTile, n - tile0);
+    for (uint32_t j = tid; j < cnt; j += kSortThreads) {
+        const uint64_t kk = skey[j];
+        const uint32_t dst = gbase[(uint32_t)(kk >> shift) & 255u] + j;
+        kout[dst] = kk;
+        vout[dst] = sval[j];
     }
 }
 
@@ -172,7 +198,6 @@ Status radix_sort_pairs_u64(uint64_t *keys, uint32_t *vals, RadixScratch &s, hip
     for (uint32_t shift = begin_bit; shift < end_bit; shift += 8) {
         k_rs_hist<<<s.n_blocks, kSortThreads, 0, st>>>(ka, s.n, shift, s.hist.p, s.n_blocks);
         k_rs_digit_scan<<<256, 256, 0, st>>>(s.hist.p, s.n_blocks, s.dbase.p);
-        k_rs_base_scan<<<1, 256, 0, st>>>(s.dbase.p);
         k_rs_scatter<<<s.n_blocks, kSortThreads, 0, st>>>(ka, va, kb, vb, s.n, shift, s.hist.p,
                                                           s.dbase.p, s.n_blocks);
         std::swap(ka, kb);
