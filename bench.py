@@ -11,10 +11,12 @@ adaptive speed, apply) with inputs already resident in HBM.  The timed region
 replays the captured iteration graph; HIP events on the engine stream bracket
 every force kernel inside it, which gives the roofline figure.  The CPU
 baseline is the oracle's restatement of the reference path timed on this
-host on a bounded sample of the same workload.  The second half of the metric,
-time to reach 1.01x the plain run's stress at iteration 200 from the learned
-warm start versus from the random start, is measured on the same graph and
-start (normalized stress over 50 max-min pivot BFS rows).  Prints one JSON line.
+host on a bounded sample of the same workload.  The second half of the metric
+is measured on the same graph and start: time to reach 1.01x the plain run's
+converged normalized stress (its stress after 2,000 iterations; 50 max-min
+pivot BFS rows), from the learned warm start (prediction wall time included)
+versus from the random start, with the plain run as fallback.  Prints one
+JSON line.
 
 --copy-probe copies 256 MiB three times with torch and exits: the HBM-counter
 calibration run (FETCH_SIZE / WRITE_SIZE of a copy of known size).

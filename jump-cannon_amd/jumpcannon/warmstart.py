@@ -6,14 +6,18 @@ The plain method stays the drop-in and the fallback: a prediction whose
 refinement does not reach the plain run's target stress within the plain run's
 own iteration budget is replaced by the plain run.
 
-Training sample: (graph, random initial layout x0, target layout).  The shape
-target is the layout the project's SGD sparse-stress engine reaches from x0
-(max-min pivots, BFS hop distances, annealed sweeps); the loss is invariant to
-rotation, translation and scale.  The scale comes from a size law fitted on the
-converged plain FA2 layouts of the same graphs, so the predicted layout starts
-at the size FA2 settles at and the refinement does not spend hundreds of
-max-displacement-limited iterations expanding a small cube.  The network sees
-x0/10 as its noise input (symmetry breaking).
+Training sample: (graph, random initial layout x0, target layout).  The target
+keeps what the network can know and the stress engine decides: each node's
+distance from the centre of the layout the project's SGD sparse-stress engine
+reaches from x0 (max-min pivots, BFS hop distances, annealed sweeps), along the
+direction of the node's own noise averaged over its 3-hop neighbourhood (the
+network's receptive field; the stress layout's global orientation of a region
+is not predictable from 3 hops).  On Barabasi-Albert graphs this radial profile
+is also where the normalized stress lives: with random directions it already
+scores lower than the stress layout it came from.  The loss is invariant to
+rotation, translation and scale; the scale comes from a size law fitted on the
+converged plain FA2 layouts of the same graphs, so the prediction starts at the
+size FA2 settles at.  The network sees x0/10 as its noise input.
 """
 from __future__ import annotations
 
@@ -55,15 +59,34 @@ def fa2_targets(graphs, iters: int = 1500, theta: float = 1.0):
     return b.step(iters)
 
 
+def smooth_directions(off, nb, x0, hops: int = 3):
+    """Unit vectors of the centred x0 averaged `hops` times over each node and
+    its neighbours."""
+    off = np.asarray(off, np.int64)
+    nb = np.asarray(nb, np.int64)
+    deg = np.diff(off)
+    src = np.repeat(np.arange(len(deg)), deg)
+    u = np.asarray(x0, np.float64).reshape(-1, 3)
+    u = u - u.mean(0)
+    for _ in range(hops):
+        acc = u.copy()
+        np.add.at(acc, src, u[nb])
+        u = acc / (deg + 1)[:, None]
+    return u / np.maximum(np.linalg.norm(u, axis=1, keepdims=True), 1e-30)
+
+
 def stress_targets(graphs, k: int = 50, epochs: int = 30, seed: int = TRAIN_PIVOT_SEED):
-    """SGD sparse-stress layouts (k max-min pivots, eta from (max hop)^2 down to
-    0.01 over `epochs` sweeps) of every graph from its x0."""
+    """Per graph: the SGD sparse-stress layout from x0 (k max-min pivots, eta from
+    (max hop)^2 down to 0.01 over `epochs` sweeps); the target is its radial
+    profile along the smoothed noise directions."""
     out = []
     for off, nb, x0 in graphs:
         e = SgdStressEngine(n_pivots=k, eta_max=0.0, eta_min=0.01, n_anneal_steps=epochs, seed=seed)
         e.init(off, nb, x0)
-        out.append(e.step(epochs))
+        sg = e.step(epochs).reshape(-1, 3).astype(np.float64)
         e.close()
+        r = np.linalg.norm(sg - sg.mean(0), axis=1)
+        out.append((smooth_directions(off, nb, x0) * r[:, None]).astype(np.float32).reshape(-1))
     return out
 
 

@@ -1,0 +1,48 @@
+"""The learned warm start on the GPU: the committed network, run by the
+library's own kernels, against the plain FA2+BH engine from the same random
+start on a held-out Barabasi-Albert graph (not among the training graphs:
+other sizes and seeds).  The gain is measured the way the headline metric is:
+time to reach normalized stress <= 1.01 x the plain run's converged stress,
+with the plain run as the fallback; the ablation starts the plain engine from
+the same start rescaled to the size law, without the network."""
+import numpy as np
+import pytest
+
+import jumpcannon as jc
+from jumpcannon import warmstart as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def net():
+    return W.load()
+
+
+def test_weights_load_and_match_the_network(net):
+    m, coef = net
+    assert m.params().shape == (jc.Mpnn(1).P,)
+    assert np.all(np.isfinite(m.params())) and coef.shape == (3,)
+
+
+def test_warm_start_beats_plain_and_the_scale_only_ablation(net):
+    m, coef = net
+    off, nb = jc.ba_graph(40_000, 4, 0xBADC0DE)
+    x0 = jc.uniform_cube(40_000, 10.0, 0xBADC0DE + 1)
+    r = W.evaluate(m, coef, off, nb, x0, horizon=1200, every=50, theta=0.8)
+    print({k: v for k, v in r.items() if not k.endswith("curve")})
+    assert not r["fallback_to_plain"]
+    assert r["learned_init_stress"] < r["plain_init_stress"]
+    assert r["speedup"] >= 3.0
+    sr = r["scaled_random_ms_to_target"]
+    assert sr is None or r["learned_ms_to_target"] < sr
+
+
+def test_prediction_is_deterministic_and_finite(net):
+    m, coef = net
+    off, nb = jc.ba_graph(5_000, 3, 77)
+    x0 = jc.uniform_cube(5_000, 10.0, 78)
+    a, _, _ = W.predict(m, coef, off, nb, x0)
+    b, _, _ = W.predict(m, coef, off, nb, x0)
+    assert np.isfinite(a).all()
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
