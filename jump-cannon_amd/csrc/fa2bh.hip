@@ -47,8 +47,11 @@ constexpr uint32_t kTimedIters = 10;      // timed iterations per host read-back
 constexpr uint32_t kHeavyAtt = 32;        // attraction rows longer than this go to k_force_heavy
 constexpr uint32_t kHeavyStage = 1024;    // heavy-row terms staged in LDS per round
 constexpr uint32_t kEmitTasks = 1024;     // cell tasks per k_emit block held in LDS
-constexpr uint32_t kAccCols = 64;         // columns of the swing/traction limb accumulator
-constexpr uint32_t kSpeedThreads = 320;   // k_speed: 16 threads per limb
+// Columns of the swing/traction limb accumulator.  Blocks add into column
+// blockIdx.x % kAccCols; many columns keep the device-scope atomics on any one
+// address few (they serialise at the memory side across XCDs).
+constexpr uint32_t kAccCols = 1024;
+constexpr uint32_t kSpeedThreads = 640;   // k_speed: 32 threads per limb
 
 struct Frame {
     uint32_t bbmin[3];
@@ -1280,7 +1283,7 @@ class Fa2BhEngine final : public LayoutEngine {
     Status enqueue_tree(const float4 *pos, hipEvent_t *ev) {
         const uint32_t n = n_;
         if (ev) JC_HIP(hipEventRecord(ev[PH_BBOX], stream_));
-        k_frame_reset<<<1, 256, 0, stream_>>>(frame_.p, partial_.p);
+        k_frame_reset<<<1, 1024, 0, stream_>>>(frame_.p, partial_.p);
         k_bbox<<<std::min<uint32_t>(cdiv(n, kTB), 256), kTB, 0, stream_>>>(pos, n, frame_.p);
         if (ev) JC_HIP(hipEventRecord(ev[PH_KEYS], stream_));
         k_keys<<<cdiv(n, kTB), kTB, 0, stream_>>>(pos, n, frame_.p, keys_.p, vals_.p);

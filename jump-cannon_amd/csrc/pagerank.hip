@@ -30,8 +30,9 @@ namespace jc {
 namespace {
 
 constexpr uint32_t kPrTB = 256;
-constexpr uint32_t kHeavy = 96;     // rows with more CSR entries go to the hub kernel
-constexpr uint32_t kHubChunk = 1024;
+constexpr uint32_t kHeavy = 32;     // rows with more CSR entries go to the hub kernel
+constexpr uint32_t kHubChunk = 1024;  // x values per hub-row round: 16 gathers per lane
+constexpr uint32_t kAccCols = 1024;   // residual limb accumulator columns (few atomics per address)
 constexpr uint32_t kChunkIters = 10;
 
 struct PrState {
@@ -47,11 +48,15 @@ __device__ __forceinline__ void resid_add(unsigned long long *acc, float r_new, 
     exact_acc_add(acc + kLimbs, fminf(r_new, r_old));
 }
 
+// Also clears the residual accumulator (k_pr_reduce clears it again after
+// every read, so each iteration starts from zero).
 __global__ __launch_bounds__(kPrTB) void k_pr_init(uint32_t n, float inv_n,
                                                    const float *__restrict__ inv,
                                                    float *__restrict__ rank0,
-                                                   float *__restrict__ x0, PrState *st, double tol) {
+                                                   float *__restrict__ x0, PrState *st, double tol,
+                                                   unsigned long long *__restrict__ acc_cols) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < 2 * kLimbs * kAccCols) acc_cols[v] = 0ull;
     if (v == 0) {
         st->converged = 0;
         st->iters = 0;
@@ -63,41 +68,63 @@ __global__ __launch_bounds__(kPrTB) void k_pr_init(uint32_t n, float inv_n,
     x0[v] = inv_n * inv[v];
 }
 
+// Rows of at most kHeavy entries.  Each wave takes 64 consecutive rows: their
+// entries are contiguous in the light CSR (offl/nbrl, where hub rows are empty),
+// so the wave reads the index run coalesced, gathers all of its x values at
+// once into LDS, and then every lane adds its own row from LDS in CSR order.
 __global__ __launch_bounds__(kPrTB) void k_pr_light(
-    uint32_t n, const uint32_t *__restrict__ off, const uint32_t *__restrict__ nbr,
-    const float *__restrict__ inv, const float *__restrict__ x_in,
-    const float *__restrict__ rank_in, float *__restrict__ rank_out, float *__restrict__ x_out,
-    float teleport, float damping, const PrState *st, unsigned long long *__restrict__ partial) {
+    uint32_t n, const uint32_t *__restrict__ off, const uint32_t *__restrict__ offl,
+    const uint32_t *__restrict__ nbrl, const float *__restrict__ inv,
+    const float *__restrict__ x_in, const float *__restrict__ rank_in,
+    float *__restrict__ rank_out, float *__restrict__ x_out, float teleport, float damping,
+    const PrState *st, unsigned long long *__restrict__ partial) {
     if (st->converged) return;
+    constexpr uint32_t W = kPrTB / 64, SEG = 64 * kHeavy;
+    __shared__ float xs[W][SEG];
     __shared__ unsigned long long acc[2 * kLimbs];
     if (threadIdx.x < 2 * kLimbs) acc[threadIdx.x] = 0ull;
-    __syncthreads();
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < n) {
-        const uint32_t s = off[v], e = off[v + 1];
-        if (e - s <= kHeavy) {
-            float sum = 0.0f;
-            uint32_t k = s;
-            for (; k + 4 <= e; k += 4) {  // four independent gathers in flight
-                const uint32_t u0 = nbr[k], u1 = nbr[k + 1], u2 = nbr[k + 2], u3 = nbr[k + 3];
-                const float a0 = x_in[u0], a1 = x_in[u1], a2 = x_in[u2], a3 = x_in[u3];
-                sum = sum + a0;
-                sum = sum + a1;
-                sum = sum + a2;
-                sum = sum + a3;
-            }
-            for (; k < e; k++) sum = sum + x_in[nbr[k]];
-            const float r = teleport + damping * sum;
-            rank_out[v] = r;
-            x_out[v] = r * inv[v];
-            resid_add(acc, r, rank_in[v]);
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t r0 = (blockIdx.x * W + w) * 64;
+    const uint32_t v = r0 + lane;
+    uint32_t seg0 = 0, seg1 = 0;
+    if (r0 < n) {
+        seg0 = offl[r0];
+        seg1 = offl[min(r0 + 64u, n)];
+    }
+    for (uint32_t base = seg0; base < seg1; base += 8 * 64) {
+        uint32_t u[8];
+        float a[8];
+#pragma unroll
+        for (uint32_t q = 0; q < 8; q++) {
+            const uint32_t k = base + lane + 64u * q;
+            u[q] = k < seg1 ? nbrl[k] : 0u;
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < 8; q++) a[q] = x_in[u[q]];
+#pragma unroll
+        for (uint32_t q = 0; q < 8; q++) {
+            const uint32_t k = base + lane + 64u * q;
+            if (k < seg1) xs[w][k - seg0] = a[q];
         }
     }
     __syncthreads();
-    if (threadIdx.x < 2 * kLimbs) partial[(size_t)blockIdx.x * 2 * kLimbs + threadIdx.x] = acc[threadIdx.x];
+    if (v < n && off[v + 1] - off[v] <= kHeavy) {
+        const uint32_t s = offl[v] - seg0, e = offl[v + 1] - seg0;
+        float sum = 0.0f;
+        for (uint32_t k = s; k < e; k++) sum = sum + xs[w][k];
+        const float r = teleport + damping * sum;
+        rank_out[v] = r;
+        x_out[v] = r * inv[v];
+        resid_add(acc, r, rank_in[v]);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * kLimbs && acc[threadIdx.x])
+        atomicAdd(&partial[threadIdx.x * kAccCols + blockIdx.x % kAccCols], acc[threadIdx.x]);
 }
 
-// One 64-lane block per hub row.
+// One 64-lane block per hub row: each round gathers up to kHubChunk x values
+// into LDS (all sixteen index loads of a lane, then all sixteen gathers, in
+// flight together), then lane 0 adds them in CSR order.
 __global__ __launch_bounds__(64) void k_pr_hub(
     const uint32_t *__restrict__ hubs, const uint32_t *__restrict__ off,
     const uint32_t *__restrict__ nbr, const float *__restrict__ inv,
@@ -112,10 +139,25 @@ __global__ __launch_bounds__(64) void k_pr_hub(
     float sum = 0.0f;
     for (uint32_t c = s; c < e; c += kHubChunk) {
         const uint32_t len = min(kHubChunk, e - c);
-        for (uint32_t k = threadIdx.x; k < len; k += 64) buf[k] = x_in[nbr[c + k]];
+        constexpr uint32_t R = kHubChunk / 64;
+        uint32_t j[R];
+#pragma unroll
+        for (uint32_t r = 0; r < R; r++) {
+            const uint32_t k = threadIdx.x + 64u * r;
+            j[r] = k < len ? nbr[c + k] : 0u;
+        }
+        float a[R];
+#pragma unroll
+        for (uint32_t r = 0; r < R; r++) a[r] = x_in[j[r]];
+#pragma unroll
+        for (uint32_t r = 0; r < R; r++) {
+            const uint32_t k = threadIdx.x + 64u * r;
+            if (k < len) buf[k] = a[r];
+        }
         __syncthreads();
         if (threadIdx.x == 0) {
             uint32_t k = 0;
+#pragma unroll 8
             for (; k + 4 <= len; k += 4) {
                 const float4 q = *reinterpret_cast<const float4 *>(&buf[k]);
                 sum = sum + q.x;
@@ -136,32 +178,39 @@ __global__ __launch_bounds__(64) void k_pr_hub(
         resid_add(acc, r, rank_in[v]);
     }
     __syncthreads();
-    if (threadIdx.x < 2 * kLimbs) partial[(size_t)blockIdx.x * 2 * kLimbs + threadIdx.x] = acc[threadIdx.x];
+    if (threadIdx.x < 2 * kLimbs && acc[threadIdx.x])
+        atomicAdd(&partial[threadIdx.x * kAccCols + blockIdx.x % kAccCols], acc[threadIdx.x]);
 }
 
-__global__ __launch_bounds__(256) void k_pr_reduce(const unsigned long long *__restrict__ partial,
-                                                   uint32_t nparts, PrState *st) {
+// Sums the accumulator columns per limb (32 threads per limb, then wave
+// shuffles), clears them for the next iteration, and updates the state.
+__global__ __launch_bounds__(640) void k_pr_reduce(unsigned long long *__restrict__ partial,
+                                                   PrState *st) {
     if (st->converged) return;
-    constexpr uint32_t G = 12;
-    __shared__ unsigned long long red[G][2 * kLimbs];
+    constexpr uint32_t G = 32;
+    static_assert(2 * kLimbs * G == 640 && kAccCols % G == 0 && 64 % G == 0, "layout");
+    __shared__ unsigned long long red[2 * kLimbs];
     const uint32_t t = threadIdx.x;
-    if (t < G * 2 * kLimbs) {
-        const uint32_t k = t % (2 * kLimbs), g = t / (2 * kLimbs);
-        unsigned long long s = 0;
-        for (uint32_t b = g; b < nparts; b += G) s += partial[(size_t)b * 2 * kLimbs + k];
-        red[g][k] = s;
+    {
+        const uint32_t k = t / G, r = t % G;
+        unsigned long long *row = partial + (size_t)k * kAccCols;
+        unsigned long long sum = 0;
+#pragma unroll
+        for (uint32_t c = r; c < kAccCols; c += G) {
+            sum += row[c];
+            row[c] = 0ull;
+        }
+#pragma unroll
+        for (uint32_t o = G / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, (int)o, 64);
+        if (r == 0) red[k] = sum;
     }
     __syncthreads();
     if (t != 0) return;
     unsigned long long p[kLimbs], q[kLimbs];
+#pragma unroll
     for (uint32_t k = 0; k < kLimbs; k++) {
-        unsigned long long a = 0, b = 0;
-        for (uint32_t g = 0; g < G; g++) {
-            a += red[g][k];
-            b += red[g][kLimbs + k];
-        }
-        p[k] = a;
-        q[k] = b;
+        p[k] = red[k];
+        q[k] = red[kLimbs + k];
     }
     exact_normalise(p);
     exact_normalise(q);
@@ -185,7 +234,7 @@ class PageRankPlan {
         n_ = n;
         damping_ = damping;
         std::vector<float> inv(n ? n : 1, 0.0f);
-        std::vector<uint32_t> hubs;
+        std::vector<uint32_t> hubs, offl((size_t)n + 1, 0), nbrl;
         for (uint32_t v = 0; v < n; v++) {
             const uint32_t d = off[v + 1] - off[v];
             if (d == 0)
@@ -194,6 +243,8 @@ class PageRankPlan {
                                        std::to_string(v) + " has degree 0)");
             inv[v] = 1.0f / (float)d;  // pagerank.rs:105
             if (d > kHeavy) hubs.push_back(v);
+            else nbrl.insert(nbrl.end(), nbr + off[v], nbr + off[v + 1]);
+            offl[v + 1] = (uint32_t)nbrl.size();
         }
         // longest hub rows first: they are the critical path of an iteration
         std::stable_sort(hubs.begin(), hubs.end(), [&](uint32_t a, uint32_t b) {
@@ -206,16 +257,21 @@ class PageRankPlan {
         JC_TRY(nbr_.alloc(nnz));
         JC_TRY(inv_.alloc(n));
         JC_TRY(hubs_.alloc(hubs.size()));
+        JC_TRY(offl_.alloc((size_t)n + 1));
+        JC_TRY(nbrl_.alloc(nbrl.size() ? nbrl.size() : 1));
         for (int k = 0; k < 2; k++) {
             JC_TRY(rank_[k].alloc(n));
             JC_TRY(x_[k].alloc(n));
         }
         nb_light_ = cdiv32(n, kPrTB);
-        JC_TRY(partial_.alloc((size_t)(nb_light_ + n_hubs_) * 2 * kLimbs));
+        JC_TRY(partial_.alloc((size_t)kAccCols * 2 * kLimbs));
         JC_TRY(st_.alloc(1));
         JC_HIP(hipMemcpyAsync(off_.p, off, ((size_t)n + 1) * 4, hipMemcpyHostToDevice, stream_));
         if (nnz) JC_HIP(hipMemcpyAsync(nbr_.p, nbr, nnz * 4, hipMemcpyHostToDevice, stream_));
         JC_HIP(hipMemcpyAsync(inv_.p, inv.data(), (size_t)n * 4, hipMemcpyHostToDevice, stream_));
+        JC_HIP(hipMemcpyAsync(offl_.p, offl.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, stream_));
+        if (!nbrl.empty())
+            JC_HIP(hipMemcpyAsync(nbrl_.p, nbrl.data(), nbrl.size() * 4, hipMemcpyHostToDevice, stream_));
         if (!hubs.empty())
             JC_HIP(hipMemcpyAsync(hubs_.p, hubs.data(), hubs.size() * 4, hipMemcpyHostToDevice, stream_));
         JC_HIP(hipStreamSynchronize(stream_));
@@ -244,8 +300,8 @@ class PageRankPlan {
         for (auto &e : ev_)
             if (!e) JC_HIP(hipEventCreate(&e));
         JC_HIP(hipEventRecord(ev_[0], stream_));
-        k_pr_init<<<cdiv32(n_, kPrTB), kPrTB, 0, stream_>>>(n_, inv_n, inv_.p, rank_[0].p, x_[0].p,
-                                                            st_.p, tol);
+        k_pr_init<<<cdiv32(std::max<uint32_t>(n_, 2 * kLimbs * kAccCols), kPrTB), kPrTB, 0, stream_>>>(
+            n_, inv_n, inv_.p, rank_[0].p, x_[0].p, st_.p, tol, partial_.p);
         JC_HIP(hipGetLastError());
         PrState hs{};
         uint32_t launched = 0;
@@ -282,8 +338,8 @@ class PageRankPlan {
             if (!e) JC_HIP(hipEventCreate(&e));
         const float inv_n = 1.0f / (float)n_;
         teleport_ = (1.0f - damping_) * inv_n;
-        k_pr_init<<<cdiv32(n_, kPrTB), kPrTB, 0, stream_>>>(n_, inv_n, inv_.p, rank_[0].p, x_[0].p,
-                                                            st_.p, -1.0);
+        k_pr_init<<<cdiv32(std::max<uint32_t>(n_, 2 * kLimbs * kAccCols), kPrTB), kPrTB, 0, stream_>>>(
+            n_, inv_n, inv_.p, rank_[0].p, x_[0].p, st_.p, -1.0, partial_.p);
         JC_HIP(hipGetLastError());
         JC_HIP(hipEventRecord(ev_[0], stream_));
         for (uint32_t k = 0; k < n_iters; k++) JC_TRY(enqueue_iteration(k & 1u));
@@ -300,12 +356,11 @@ class PageRankPlan {
         if (n_hubs_)
             k_pr_hub<<<n_hubs_, 64, 0, stream_>>>(hubs_.p, off_.p, nbr_.p, inv_.p, x_[c].p,
                                                    rank_[c].p, rank_[c ^ 1].p, x_[c ^ 1].p,
-                                                   teleport_, damping_, st_.p,
-                                                   partial_.p + (size_t)nb_light_ * 2 * kLimbs);
-        k_pr_light<<<nb_light_, kPrTB, 0, stream_>>>(n_, off_.p, nbr_.p, inv_.p, x_[c].p,
+                                                   teleport_, damping_, st_.p, partial_.p);
+        k_pr_light<<<nb_light_, kPrTB, 0, stream_>>>(n_, off_.p, offl_.p, nbrl_.p, inv_.p, x_[c].p,
                                                      rank_[c].p, rank_[c ^ 1].p, x_[c ^ 1].p,
                                                      teleport_, damping_, st_.p, partial_.p);
-        k_pr_reduce<<<1, 256, 0, stream_>>>(partial_.p, nb_light_ + n_hubs_, st_.p);
+        k_pr_reduce<<<1, 640, 0, stream_>>>(partial_.p, st_.p);
         JC_HIP(hipGetLastError());
         return Status::ok();
     }
@@ -327,7 +382,7 @@ class PageRankPlan {
     uint32_t n_ = 0, n_hubs_ = 0, nb_light_ = 0;
     float damping_ = 0.85f, teleport_ = 0.0f;
     hipStream_t stream_ = nullptr;
-    DevBuf<uint32_t> off_, nbr_, hubs_;
+    DevBuf<uint32_t> off_, nbr_, hubs_, offl_, nbrl_;
     DevBuf<float> inv_, rank_[2], x_[2];
     DevBuf<unsigned long long> partial_;
     DevBuf<PrState> st_;
