@@ -33,6 +33,9 @@ namespace {
 constexpr uint32_t kBT = 256;
 constexpr uint32_t kBScanItems = 16;
 constexpr uint32_t kBScanTile = kBT * kBScanItems;
+constexpr uint32_t kBEmitTasks = 1024;    // cell tasks per k_b_emit block held in LDS
+constexpr uint32_t kBHeavyAtt = 32;       // attraction rows longer than this go to k_b_force_heavy
+constexpr uint32_t kBHeavyStage = 1024;   // heavy-row terms staged in LDS per round
 
 struct GFrame {
     uint32_t bbmin[3];
@@ -80,45 +83,54 @@ __global__ void k_b_reset(GFrame *fr, uint32_t G, BGlobal *gl) {
     fr[g].n_invalid = 0;
 }
 
+// Per-graph box over finite coordinates: one block per graph reduces the
+// graph's contiguous body range (wave shuffles, then LDS) and writes the box
+// with plain stores -- no atomics on the graph's frame, which large graphs
+// would otherwise serialise.
 __global__ __launch_bounds__(kBT) void k_b_bbox(const float4 *__restrict__ pos,
-                                                const uint32_t *__restrict__ gid, uint32_t n,
+                                                const uint32_t *__restrict__ base,
+                                                const uint32_t *__restrict__ count,
                                                 GFrame *fr) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < n;
-    const uint32_t g = live ? gid[i] : 0xFFFFFFFFu;
+    __shared__ float red[kBT / 64][6];
+    const uint32_t g = blockIdx.x;
+    const uint32_t b0 = base[g], ng = count[g];
     float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    if (live) {
-        const float4 p = pos[i];
+    for (uint32_t i = threadIdx.x; i < ng; i += kBT) {
+        const float4 p = pos[b0 + i];
         const float v[3] = {p.x + 0.0f, p.y + 0.0f, p.z + 0.0f};
 #pragma unroll
         for (int k = 0; k < 3; k++)
-            if (finitef(v[k])) mn[k] = mx[k] = v[k];
+            if (finitef(v[k])) {
+                mn[k] = fminf(mn[k], v[k]);
+                mx[k] = fmaxf(mx[k], v[k]);
+            }
     }
-    const uint32_t g0 = __shfl(g, 0, 64);
-    const bool uniform = __all(g == g0);
-    if (uniform) {
-        if (g0 == 0xFFFFFFFFu) return;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            mn[k] = fminf(mn[k], __shfl_xor(mn[k], o, 64));
+            mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], o, 64));
+        }
+    }
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                mn[k] = fminf(mn[k], __shfl_xor(mn[k], o, 64));
-                mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], o, 64));
-            }
+            red[w][k] = mn[k];
+            red[w][3 + k] = mx[k];
         }
-        if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                if (finitef(mn[k])) atomicMin(&fr[g0].bbmin[k], f2ord(mn[k]));
-                if (finitef(mx[k])) atomicMax(&fr[g0].bbmax[k], f2ord(mx[k]));
-            }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int k = threadIdx.x;
+        float a = red[0][k], c = red[0][3 + k];
+        for (uint32_t q = 1; q < kBT / 64; q++) {
+            a = fminf(a, red[q][k]);
+            c = fmaxf(c, red[q][3 + k]);
         }
-    } else if (live) {
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            if (finitef(mn[k])) atomicMin(&fr[g].bbmin[k], f2ord(mn[k]));
-            if (finitef(mx[k])) atomicMax(&fr[g].bbmax[k], f2ord(mx[k]));
-        }
+        // the single-graph engine's box: +inf / -inf (as f2ord) when no finite value
+        fr[g].bbmin[k] = finitef(a) ? f2ord(a) : 0xFF800000u;
+        fr[g].bbmax[k] = finitef(c) ? f2ord(c) : ~0xFF800000u;
     }
 }
 
@@ -174,16 +186,8 @@ __device__ __forceinline__ Seg seg_of(uint32_t i, const uint32_t *gid, const uin
     return s;
 }
 
-// Nodes written by sorted body i: its opened cells plus its leaf (0 if invalid).
-__device__ __forceinline__ uint32_t nodes_of(const uint64_t *keys, uint32_t i, const Seg &s) {
-    if (i >= s.ve) return 0;
-    if (s.ve - s.lo == 1) return 2;
-    const uint64_t k = keys[i];
-    const int lp = i > s.lo ? lcp_digits(keys[i - 1], k) : -1;
-    const int li = i + 1 < s.ve ? lcp_digits(k, keys[i + 1]) : -1;
-    return (li > lp ? (uint32_t)(li - lp) : 0u) + 1u;
-}
-
+// Per sorted body i, the nodes it writes: its opened cells plus its leaf (0 if
+// invalid, 2 for a graph's lone body); then the block-local exclusive scan.
 __global__ __launch_bounds__(kBT) void k_b_count_scan(const uint64_t *__restrict__ keys,
                                                       const uint32_t *__restrict__ gid,
                                                       const uint32_t *__restrict__ base,
@@ -195,13 +199,34 @@ __global__ __launch_bounds__(kBT) void k_b_count_scan(const uint64_t *__restrict
     const uint32_t b0 = blockIdx.x * kBScanTile + threadIdx.x * kBScanItems;
     uint32_t c[kBScanItems];
     uint32_t s = 0;
+    // the thread's keys and both neighbours, loaded together; the graph segment
+    // is looked up again only when the items cross into the next graph
+    uint64_t kk[kBScanItems + 2];
+#pragma unroll
+    for (uint32_t j = 0; j < kBScanItems + 2; j++) {
+        const uint32_t i = b0 + j - 1;   // wraps to 0xFFFFFFFF for b0 = 0: never read
+        kk[j] = (b0 + j >= 1 && i < n) ? keys[i] : 0ull;
+    }
+    Seg sg{0u, 0u, 0u};
+    uint32_t seg_end = 0;
 #pragma unroll
     for (uint32_t j = 0; j < kBScanItems; j++) {
         const uint32_t i = b0 + j;
         c[j] = 0;
         if (i < n) {
-            const Seg sg = seg_of(i, gid, base, count, fr);
-            c[j] = nodes_of(keys, i, sg);
+            if (j == 0 || i >= seg_end) {
+                sg = seg_of(i, gid, base, count, fr);
+                seg_end = base[sg.g] + count[sg.g];
+            }
+            if (i < sg.ve) {
+                if (sg.ve - sg.lo == 1) {
+                    c[j] = 2;
+                } else {
+                    const int lp = i > sg.lo ? lcp_digits(kk[j], kk[j + 1]) : -1;
+                    const int li = i + 1 < sg.ve ? lcp_digits(kk[j + 1], kk[j + 2]) : -1;
+                    c[j] = (li > lp ? (uint32_t)(li - lp) : 0u) + 1u;
+                }
+            }
         }
         s += c[j];
     }
@@ -276,82 +301,105 @@ __global__ void k_b_graph_tree(const uint32_t *__restrict__ base, const uint32_t
     fr[g].n_tree = e - s;
 }
 
-__device__ __forceinline__ void b_emit_body(
-    const uint64_t *__restrict__ keys, const uint32_t *__restrict__ perm,
-    const float4 *__restrict__ pos, const uint32_t *__restrict__ deg,
-    const uint32_t *__restrict__ gid, const uint32_t *__restrict__ base,
-    const uint32_t *__restrict__ count, uint32_t n, const GFrame *fr, const BGlobal *gl,
-    const uint32_t *__restrict__ loc, const uint32_t *__restrict__ blk,
-    float4 *__restrict__ node_a, uint2 *__restrict__ node_b, uint8_t *__restrict__ node_depth,
-    uint64_t cap, uint32_t i, uint32_t *bdepth) {
-    if (i >= n) return;
-    const Seg sg = seg_of(i, gid, base, count, fr);
-    if (i >= sg.ve) return;
-    const uint64_t k = keys[i];
-    int lp, li;
-    if (sg.ve - sg.lo == 1) {
-        lp = -1;
-        li = 0;
-    } else {
-        lp = i > sg.lo ? lcp_digits(keys[i - 1], k) : -1;
-        li = i + 1 < sg.ve ? lcp_digits(k, keys[i + 1]) : -1;
-    }
-    const uint32_t c = li > lp ? (uint32_t)(li - lp) : 0u;
-    const int a0 = lp + 1;
-    const uint32_t pos0 = node_start(loc, blk, i, n, gl);
-    if ((uint64_t)pos0 + c >= cap) return;
-    if (c) {
-        float h = fr[sg.g].root[3];
-        for (int d = 0; d < a0; d++) h = h * 0.5f;
-        for (uint32_t t = 0; t < c; t++) {
-            const int d = a0 + (int)t;
-            const uint32_t j = cell_end(keys, i, sg.ve, k, d);
-            const uint32_t skip = (j + 1 < sg.ve) ? node_start(loc, blk, j + 1, n, gl) : kEnd;
-            const float s = h * 2.0f;
-            node_b[pos0 + t] = make_uint2(__float_as_uint(s * s), skip);
-            node_depth[pos0 + t] = (uint8_t)d;
-            h = h * 0.5f;
-        }
-        atomicMax(bdepth, (uint32_t)(a0 + (int)c - 1));
-    }
-    const uint32_t leaf = pos0 + c;
-    const uint32_t b = perm[i];
-    if (b >= n) return;
-    const float4 p = pos[b];
-    node_a[leaf] = make_float4(p.x, p.y, p.z, fmaxf((float)deg[b] + 1.0f, 1e-3f));
-    node_b[leaf] = make_uint2(kLeaf | b, (i + 1 < sg.ve) ? leaf + 1 : kEnd);
-    node_depth[leaf] = (uint8_t)(0x80 | ((lp > li ? lp : li) + 1));
+// One internal cell of graph sg.g at depth d, opened by sorted body i (its
+// half-size: the graph's root halved d times; its skip: the first node of the
+// body after the last one sharing the d-digit prefix, inside the graph).
+__device__ __forceinline__ void b_emit_cell(const uint64_t *__restrict__ keys, const Seg &sg,
+                                            uint32_t i, int d, uint32_t node, uint32_t n,
+                                            const GFrame *fr, const BGlobal *gl,
+                                            const uint32_t *__restrict__ loc,
+                                            const uint32_t *__restrict__ blk,
+                                            Node *__restrict__ nodes,
+                                            uint8_t *__restrict__ node_depth) {
+    float h = fr[sg.g].root[3];
+    for (int q = 0; q < d; q++) h = h * 0.5f;
+    const uint32_t j = cell_end(keys, i, sg.ve, keys[i], d);
+    const uint32_t skip = (j + 1 < sg.ve) ? node_start(loc, blk, j + 1, n, gl) : kEnd;
+    const float s = h * 2.0f;
+    nodes[node].b = make_uint4(__float_as_uint(s * s), skip, 0u, 0u);
+    node_depth[node] = (uint8_t)d;
 }
 
+// Pre-order emission per graph segment (as k_emit of the single-graph engine):
+// leaves by their body's thread, cells through a block-wide task list.
 __global__ __launch_bounds__(kBT) void k_b_emit(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ perm,
     const float4 *__restrict__ pos, const uint32_t *__restrict__ deg,
     const uint32_t *__restrict__ gid, const uint32_t *__restrict__ base,
     const uint32_t *__restrict__ count, uint32_t n, const GFrame *fr, BGlobal *gl,
     const uint32_t *__restrict__ loc, const uint32_t *__restrict__ blk,
-    float4 *__restrict__ node_a, uint2 *__restrict__ node_b, uint8_t *__restrict__ node_depth,
-    uint64_t cap) {
-    __shared__ uint32_t bdepth;
-    if (threadIdx.x == 0) bdepth = 0;
+    Node *__restrict__ nodes, uint8_t *__restrict__ node_depth, uint64_t cap) {
+    __shared__ uint32_t bdepth, ntask;
+    __shared__ uint32_t t_node[kBEmitTasks], t_body[kBEmitTasks];
+    __shared__ uint8_t t_depth[kBEmitTasks];
+    if (threadIdx.x == 0) {
+        bdepth = 0;
+        ntask = 0;
+    }
     __syncthreads();
-    b_emit_body(keys, perm, pos, deg, gid, base, count, n, fr, gl, loc, blk, node_a, node_b,
-                node_depth, cap, blockIdx.x * blockDim.x + threadIdx.x, &bdepth);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const Seg sg = seg_of(i, gid, base, count, fr);
+        if (i < sg.ve) {
+            const uint64_t k = keys[i];
+            int lp, li;
+            if (sg.ve - sg.lo == 1) {
+                lp = -1;
+                li = 0;
+            } else {
+                lp = i > sg.lo ? lcp_digits(keys[i - 1], k) : -1;
+                li = i + 1 < sg.ve ? lcp_digits(k, keys[i + 1]) : -1;
+            }
+            const uint32_t c = li > lp ? (uint32_t)(li - lp) : 0u;
+            const int a0 = lp + 1;
+            const uint32_t pos0 = node_start(loc, blk, i, n, gl);
+            if ((uint64_t)pos0 + c < cap) {
+                if (c) {
+                    const uint32_t slot = atomicAdd(&ntask, c);
+                    for (uint32_t t = 0; t < c; t++) {
+                        if (slot + t < kBEmitTasks) {
+                            t_node[slot + t] = pos0 + t;
+                            t_body[slot + t] = i;
+                            t_depth[slot + t] = (uint8_t)(a0 + (int)t);
+                        } else {
+                            b_emit_cell(keys, sg, i, a0 + (int)t, pos0 + t, n, fr, gl, loc, blk,
+                                        nodes, node_depth);
+                        }
+                    }
+                    atomicMax(&bdepth, (uint32_t)(a0 + (int)c - 1));
+                }
+                const uint32_t leaf = pos0 + c;
+                const uint32_t b = perm[i];
+                if (b < n) {
+                    const float4 p = pos[b];
+                    nodes[leaf].a = make_float4(p.x, p.y, p.z, fmaxf((float)deg[b] + 1.0f, 1e-3f));
+                    nodes[leaf].b = make_uint4(kLeaf | b, (i + 1 < sg.ve) ? leaf + 1 : kEnd, 0u, 0u);
+                    node_depth[leaf] = (uint8_t)(0x80 | ((lp > li ? lp : li) + 1));
+                }
+            }
+        }
+    }
     __syncthreads();
+    const uint32_t nt = min(ntask, kBEmitTasks);
+    for (uint32_t q = threadIdx.x; q < nt; q += blockDim.x) {
+        const uint32_t ib = t_body[q];
+        const Seg sg = seg_of(ib, gid, base, count, fr);
+        b_emit_cell(keys, sg, ib, t_depth[q], t_node[q], n, fr, gl, loc, blk, nodes, node_depth);
+    }
     if (threadIdx.x == 0 && bdepth) atomicMax(&gl->max_depth, bdepth);
 }
 
 __global__ __launch_bounds__(kBT) void k_b_com_level(const BGlobal *gl, uint32_t d, uint32_t cap,
-                                                     float4 *__restrict__ node_a,
-                                                     const uint2 *__restrict__ node_b,
+                                                     Node *__restrict__ nodes,
                                                      const uint8_t *__restrict__ node_depth) {
     if (d > gl->max_depth) return;
     const uint32_t N = min(gl->n_tree_total, cap);
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
         if (node_depth[p] != d) continue;
-        const uint32_t end = node_b[p].y;
+        const uint32_t end = nodes[p].b.y;
         float tm = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
-        for (uint32_t ch = p + 1; ch < N && ch != end; ch = node_b[ch].y) {
-            const float4 a = node_a[ch];
+        for (uint32_t ch = p + 1; ch < N && ch != end; ch = nodes[ch].b.y) {
+            const float4 a = nodes[ch].a;
             tm = tm + a.w;
             cx = cx + a.x * a.w;
             cy = cy + a.y * a.w;
@@ -362,10 +410,52 @@ __global__ __launch_bounds__(kBT) void k_b_com_level(const BGlobal *gl, uint32_t
             cy = cy / tm;
             cz = cz / tm;
         }
-        node_a[p] = make_float4(cx, cy, cz, tm);
+        nodes[p].a = make_float4(cx, cy, cz, tm);
     }
 }
 
+// One attraction term d * att (fa2_barnes_hut.wgsl:158-181), d = q - p.
+__device__ __forceinline__ void b_attraction_term(const BParams &P, float4 p, float4 q, float &tx,
+                                                  float &ty, float &tz) {
+    const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+    float att = P.ewi_weight;
+    if (P.lin_log_mode) {
+        const float r = fmaxf(len3(dx, dy, dz), 1.0e-3f);
+        att = P.ewi_weight * logf(1.0f + r) / r;
+    }
+    tx = dx * att;
+    ty = dy * att;
+    tz = dz * att;
+}
+
+// Gravity (fa2_barnes_hut.wgsl:183-189), the force write and the per-body
+// swing / traction stats (:191-197).
+__device__ __forceinline__ void b_gravity_and_stats(const BParams &P, uint32_t b, float4 p,
+                                                    float deg_i, float fx, float fy, float fz,
+                                                    const float4 *__restrict__ old_force,
+                                                    float4 *__restrict__ force,
+                                                    float2 *__restrict__ stats) {
+    const float gd = P.gravity * deg_i;
+    if (P.strong_gravity) {
+        fx = fx - p.x * gd;
+        fy = fy - p.y * gd;
+        fz = fz - p.z * gd;
+    } else {
+        const float r0 = fmaxf(len3(p.x, p.y, p.z), 1.0e-3f);
+        fx = fx - (p.x / r0) * gd;
+        fy = fy - (p.y / r0) * gd;
+        fz = fz - (p.z / r0) * gd;
+    }
+    const float4 o = old_force[b];
+    force[b] = make_float4(fx, fy, fz, 0.0f);
+    stats[b] = make_float2(deg_i * len3(o.x - fx, o.y - fy, o.z - fz),
+                           deg_i * 0.5f * len3(o.x + fx, o.y + fy, o.z + fz));
+}
+
+// Walk of the body's own graph tree (fa2_barnes_hut.wgsl:108-156), attraction
+// over rows of at most kBHeavyAtt entries (four gathers in flight, list order
+// kept), gravity and stats.  Longer rows leave the repulsion sum in force[b]
+// for k_b_force_heavy.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_b_force(const BParams P, const GFrame *fr,
                                                 const uint32_t *__restrict__ gid,
@@ -376,8 +466,7 @@ __global__ __launch_bounds__(BS) void k_b_force(const BParams P, const GFrame *f
                                                 const uint32_t *__restrict__ att_off,
                                                 const uint32_t *__restrict__ att_adj,
                                                 const uint32_t *__restrict__ perm,
-                                                const float4 *__restrict__ node_a,
-                                                const uint2 *__restrict__ node_b,
+                                                const Node *__restrict__ nodes,
                                                 float2 *__restrict__ stats) {
     const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= P.n) return;
@@ -396,78 +485,154 @@ __global__ __launch_bounds__(BS) void k_b_force(const BParams P, const GFrame *f
         uint32_t idx = start;
         while (idx < P.cap && walk < wcap) {
             walk++;
-            const float4 A = node_a[idx];
-            const uint2 B = node_b[idx];
-            const float dx = p.x - A.x, dy = p.y - A.y, dz = p.z - A.z;
+            const Node nd = nodes[idx];
+            const float dx = p.x - nd.a.x, dy = p.y - nd.a.y, dz = p.z - nd.a.z;
             const float r2 = dx * dx + dy * dy + dz * dz;
-            const bool leaf = (B.x & kLeaf) != 0u;
+            const bool leaf = (nd.b.x & kLeaf) != 0u;
             bool take;
-            if (leaf) take = ((B.x & ~kLeaf) != b) && A.w > 0.0f;
-            else take = A.w > 0.0f && r2 > 0.0f && __uint_as_float(B.x) < P.theta2 * r2;
+            if (leaf) take = ((nd.b.x & ~kLeaf) != b) && nd.a.w > 0.0f;
+            else take = nd.a.w > 0.0f && r2 > 0.0f && __uint_as_float(nd.b.x) < P.theta2 * r2;
             if (take) {
                 const float r2c = fmaxf(r2, 1.0e-4f);
-                const float cf = krd * A.w / r2c;
+                const float cf = krd * nd.a.w / r2c;
                 fx = fx + dx * cf;
                 fy = fy + dy * cf;
                 fz = fz + dz * cf;
             }
-            idx = (leaf || take) ? B.y : idx + 1u;
+            idx = (leaf || take) ? nd.b.y : idx + 1u;
         }
     }
     const uint32_t e0 = att_off[b], e1 = att_off[b + 1];
-    for (uint32_t e = e0; e < e1; e++) {
-        const float4 q = pos[att_adj[e]];
-        const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
-        float att = P.ewi_weight;
-        if (P.lin_log_mode) {
-            const float r = fmaxf(len3(dx, dy, dz), 1.0e-3f);
-            att = P.ewi_weight * logf(1.0f + r) / r;
+    if (e1 - e0 > kBHeavyAtt) {
+        force[b] = make_float4(fx, fy, fz, 0.0f);
+        return;
+    }
+    uint32_t e = e0;
+    for (; e + 4 <= e1; e += 4) {
+        const uint32_t j0 = att_adj[e], j1 = att_adj[e + 1], j2 = att_adj[e + 2], j3 = att_adj[e + 3];
+        const float4 q[4] = {pos[j0], pos[j1], pos[j2], pos[j3]};
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            float tx, ty, tz;
+            b_attraction_term(P, p, q[u], tx, ty, tz);
+            fx = fx + tx;
+            fy = fy + ty;
+            fz = fz + tz;
         }
-        fx = fx + dx * att;
-        fy = fy + dy * att;
-        fz = fz + dz * att;
     }
-    const float gd = P.gravity * deg_i;
-    if (P.strong_gravity) {
-        fx = fx - p.x * gd;
-        fy = fy - p.y * gd;
-        fz = fz - p.z * gd;
-    } else {
-        const float r0 = fmaxf(len3(p.x, p.y, p.z), 1.0e-3f);
-        fx = fx - (p.x / r0) * gd;
-        fy = fy - (p.y / r0) * gd;
-        fz = fz - (p.z / r0) * gd;
+    for (; e < e1; e++) {
+        float tx, ty, tz;
+        b_attraction_term(P, p, pos[att_adj[e]], tx, ty, tz);
+        fx = fx + tx;
+        fy = fy + ty;
+        fz = fz + tz;
     }
-    const float4 o = old_force[b];
-    force[b] = make_float4(fx, fy, fz, 0.0f);
-    stats[b] = make_float2(deg_i * len3(o.x - fx, o.y - fy, o.z - fz),
-                           deg_i * 0.5f * len3(o.x + fx, o.y + fy, o.z + fz));
+    b_gravity_and_stats(P, b, p, deg_i, fx, fy, fz, old_force, force, stats);
 }
 
-// One block per graph: exact swing/traction sums over its bodies, then that
-// graph's AdaptiveSpeed::update (fa2_speed.rs:48-100).
+// Attraction terms of the rows longer than kBHeavyAtt, one thread per entry,
+// into three component arrays (rows start on multiples of 4 entries).
+__global__ __launch_bounds__(kBT) void k_b_heavy_terms(const BParams P, uint32_t n_ent,
+                                                       const uint32_t *__restrict__ hbody,
+                                                       const uint32_t *__restrict__ hadj,
+                                                       const float4 *__restrict__ pos,
+                                                       float *__restrict__ hterm, uint32_t stride) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_ent) return;
+    float tx, ty, tz;
+    b_attraction_term(P, pos[hbody[q]], pos[hadj[q]], tx, ty, tz);
+    hterm[q] = tx;
+    hterm[stride + q] = ty;
+    hterm[2 * (size_t)stride + q] = tz;
+}
+
+// One wave per long row: the terms are staged in LDS with coalesced loads and
+// lanes 0, 1, 2 add the x, y, z terms to the repulsion sum in list order.
+__global__ __launch_bounds__(64) void k_b_force_heavy(const BParams P,
+                                                     const uint32_t *__restrict__ heavy,
+                                                     const uint32_t *__restrict__ hoff,
+                                                     const uint32_t *__restrict__ hlen,
+                                                     const float *__restrict__ hterm,
+                                                     uint32_t stride,
+                                                     const float4 *__restrict__ pos,
+                                                     const float4 *__restrict__ old_force,
+                                                     float4 *__restrict__ force,
+                                                     const uint32_t *__restrict__ deg,
+                                                     float2 *__restrict__ stats) {
+    __shared__ __align__(16) float st[3][kBHeavyStage];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t b = heavy[blockIdx.x];
+    const float4 f0 = force[b];
+    float run = lane == 0 ? f0.x : lane == 1 ? f0.y : f0.z;
+    const uint32_t e0 = hoff[blockIdx.x], len = hlen[blockIdx.x];
+    for (uint32_t c0 = 0; c0 < len; c0 += kBHeavyStage) {
+        const uint32_t cnt = min(kBHeavyStage, len - c0);
+        const uint32_t nv4 = (cnt + 3u) >> 2;
+#pragma unroll
+        for (uint32_t comp = 0; comp < 3; comp++) {
+            const float4 *src = reinterpret_cast<const float4 *>(hterm + (size_t)comp * stride + e0 + c0);
+            float4 *dst = reinterpret_cast<float4 *>(st[comp]);
+            for (uint32_t v = lane; v < nv4; v += 64) dst[v] = src[v];
+        }
+        __syncthreads();
+        if (lane < 3) {
+            const float4 *t4 = reinterpret_cast<const float4 *>(st[lane]);
+            const uint32_t full = cnt >> 2;
+#pragma unroll 8
+            for (uint32_t q = 0; q < full; q++) {
+                const float4 v = t4[q];
+                run = run + v.x;
+                run = run + v.y;
+                run = run + v.z;
+                run = run + v.w;
+            }
+            for (uint32_t k = full << 2; k < cnt; k++) run = run + st[lane][k];
+        }
+        __syncthreads();
+    }
+    const float fx = __shfl(run, 0, 64), fy = __shfl(run, 1, 64), fz = __shfl(run, 2, 64);
+    if (lane == 0)
+        b_gravity_and_stats(P, b, pos[b], (float)deg[b] + 1.0f, fx, fy, fz, old_force, force, stats);
+}
+
 __global__ __launch_bounds__(256) void k_b_speed(const BParams P, GFrame *fr,
                                                  const uint32_t *__restrict__ base,
                                                  const uint32_t *__restrict__ count,
                                                  const double *__restrict__ est_jt,
                                                  const double *__restrict__ min_jt,
                                                  const float2 *__restrict__ stats) {
-    __shared__ unsigned long long acc[2 * kLimbs];
+    // Each thread sums its share of the graph's stats exactly in registers (no
+    // atomics), then the limbs are added across the block.
+    __shared__ unsigned long long red[4][2 * kLimbs];
     const uint32_t g = blockIdx.x;
-    if (threadIdx.x < 2 * kLimbs) acc[threadIdx.x] = 0ull;
-    __syncthreads();
     const uint32_t b0 = base[g], ng = count[g];
+    unsigned long long sw[kLimbs] = {}, tr[kLimbs] = {};
     for (uint32_t i = threadIdx.x; i < ng; i += blockDim.x) {
         const float2 s = stats[b0 + i];
         if (finitef(s.x) && finitef(s.y)) {
-            exact_acc_add(acc, s.x);
-            exact_acc_add(acc + kLimbs, s.y);
+            exact_reg_add(sw, s.x);
+            exact_reg_add(tr, s.y);
+        }
+    }
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (uint32_t k = 0; k < kLimbs; k++) {
+        unsigned long long a = sw[k], c = tr[k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o, 64);
+            c += __shfl_xor(c, o, 64);
+        }
+        if (lane == 0) {
+            red[w][k] = a;
+            red[w][kLimbs + k] = c;
         }
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
     unsigned long long tot[2 * kLimbs];
-    for (uint32_t k = 0; k < 2 * kLimbs; k++) tot[k] = acc[k];
+#pragma unroll
+    for (uint32_t k = 0; k < 2 * kLimbs; k++) tot[k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
     exact_normalise(tot);
     exact_normalise(tot + kLimbs);
     const double ts = exact_to_double(tot), tt = exact_to_double(tot + kLimbs);
@@ -598,6 +763,33 @@ class Fa2BhBatch {
             off_pos += (uint64_t)ng + 1;
             nb_pos += nnz[g];
         }
+        // rows longer than kBHeavyAtt, longest first, laid end to end from
+        // multiples of 4 entries (padding: the body as its own neighbour, a zero
+        // term loaded but never added)
+        std::vector<uint32_t> heavy;
+        for (uint32_t v2 = 0; v2 < n_; v2++)
+            if (aoff[v2 + 1] - aoff[v2] > kBHeavyAtt) heavy.push_back(v2);
+        std::stable_sort(heavy.begin(), heavy.end(), [&](uint32_t a, uint32_t b) {
+            return aoff[a + 1] - aoff[a] > aoff[b + 1] - aoff[b];
+        });
+        n_heavy_ = (uint32_t)heavy.size();
+        std::vector<uint32_t> hoff(n_heavy_ ? n_heavy_ : 1, 0), hlen(n_heavy_ ? n_heavy_ : 1, 0),
+            hbody, hadj;
+        for (uint32_t r = 0; r < n_heavy_; r++) {
+            const uint32_t v2 = heavy[r];
+            hoff[r] = (uint32_t)hbody.size();
+            hlen[r] = aoff[v2 + 1] - aoff[v2];
+            for (uint32_t e = aoff[v2]; e < aoff[v2 + 1]; e++) {
+                hbody.push_back(v2);
+                hadj.push_back(adj[e]);
+            }
+            while (hbody.size() & 3u) {
+                hbody.push_back(v2);
+                hadj.push_back(v2);
+            }
+        }
+        n_hent_ = (uint32_t)hbody.size();
+        hstride_ = n_hent_;
         gbits_ = 0;
         while ((1u << gbits_) < G) gbits_++;
         gbits_ = (gbits_ + 7) / 8 * 8;
@@ -629,8 +821,13 @@ class Fa2BhBatch {
         for (uint32_t g = 0; g < G; g++) cap_ += (uint64_t)count[g] + (uint64_t)(kLevels + 1) * (count[g] / 2 + 1) + 1;
         cap_ += 8;
         if (cap_ >= 0xFFFFFFF0ull) return Status::err(JC_E_INVALID, "batch too large for u32 node ids");
-        JC_TRY(node_a_.alloc(cap_));
-        JC_TRY(node_b_.alloc(cap_));
+        JC_TRY(nodes_.alloc(cap_));
+        JC_TRY(heavy_.alloc(n_heavy_ ? n_heavy_ : 1));
+        JC_TRY(hoff_.alloc(n_heavy_ ? n_heavy_ : 1));
+        JC_TRY(hlen_.alloc(n_heavy_ ? n_heavy_ : 1));
+        JC_TRY(hbody_.alloc(n_hent_ ? n_hent_ : 1));
+        JC_TRY(hadj_.alloc(n_hent_ ? n_hent_ : 1));
+        JC_TRY(hterm_.alloc(3 * (size_t)(hstride_ ? hstride_ : 4)));
         JC_TRY(node_depth_.alloc(cap_));
         JC_TRY(frame_.alloc(G));
         JC_TRY(global_.alloc(1));
@@ -652,6 +849,13 @@ class Fa2BhBatch {
         JC_TRY(up(mjt_.p, mjt.data(), G * 8));
         JC_TRY(up(att_off_.p, aoff.data(), ((size_t)n_ + 1) * 4));
         JC_TRY(up(att_adj_.p, adj.data(), adj.size() * 4));
+        if (n_heavy_) {
+            JC_TRY(up(heavy_.p, heavy.data(), n_heavy_ * 4));
+            JC_TRY(up(hoff_.p, hoff.data(), n_heavy_ * 4));
+            JC_TRY(up(hlen_.p, hlen.data(), n_heavy_ * 4));
+            JC_TRY(up(hbody_.p, hbody.data(), n_hent_ * 4));
+            JC_TRY(up(hadj_.p, hadj.data(), n_hent_ * 4));
+        }
         JC_HIP(hipMemsetAsync(frame_.p, 0, G * sizeof(GFrame), stream_));
         JC_HIP(hipMemsetAsync(global_.p, 0, sizeof(BGlobal), stream_));
         k_b_init_speed<<<cdivb(G, 256), 256, 0, stream_>>>(frame_.p, G);
@@ -749,7 +953,7 @@ class Fa2BhBatch {
     Status enqueue_tree(const float4 *pos) {
         const uint32_t n = n_;
         k_b_reset<<<cdivb(G_, 256), 256, 0, stream_>>>(frame_.p, G_, global_.p);
-        k_b_bbox<<<cdivb(n, kBT), kBT, 0, stream_>>>(pos, gid_.p, n, frame_.p);
+        k_b_bbox<<<G_, kBT, 0, stream_>>>(pos, base_.p, count_.p, frame_.p);
         k_b_keys<<<cdivb(n, kBT), kBT, 0, stream_>>>(pos, gid_.p, base_.p, n, frame_.p,
                                                      keys_raw_.p, keys_.p, vals_.p);
         JC_TRY(radix_sort_pairs_u64(keys_.p, vals_.p, sort_, stream_, 0, 64));
@@ -765,28 +969,35 @@ class Fa2BhBatch {
                                                             blk_.p, global_.p, frame_.p);
         k_b_emit<<<cdivb(n, kBT), kBT, 0, stream_>>>(keys_.p, vals_.p, pos, deg_.p, gid_.p,
                                                      base_.p, count_.p, n, frame_.p, global_.p,
-                                                     loc_.p, blk_.p, node_a_.p, node_b_.p,
-                                                     node_depth_.p, cap_);
+                                                     loc_.p, blk_.p, nodes_.p, node_depth_.p,
+                                                     cap_);
         const uint32_t gcom = std::min<uint32_t>(cdivb(cap_, kBT), 4096);
         for (int d = kLevels; d >= 0; d--)
             k_b_com_level<<<gcom, kBT, 0, stream_>>>(global_.p, (uint32_t)d, (uint32_t)cap_,
-                                                     node_a_.p, node_b_.p, node_depth_.p);
+                                                     nodes_.p, node_depth_.p);
         JC_HIP(hipGetLastError());
         return Status::ok();
     }
 
     Status launch_force(const float4 *pos, uint32_t bs) {
         const uint32_t grid = cdivb(n_, bs);
+        if (n_heavy_)
+            k_b_heavy_terms<<<cdivb(n_hent_, kBT), kBT, 0, stream_>>>(P_, n_hent_, hbody_.p, hadj_.p,
+                                                                    pos, hterm_.p, hstride_);
 #define JC_BF(BS)                                                                           \
     k_b_force<BS><<<grid, BS, 0, stream_>>>(P_, frame_.p, gid_.p, pos, old_force_.p, force_.p, \
-                                            deg_.p, att_off_.p, att_adj_.p, vals_.p,          \
-                                            node_a_.p, node_b_.p, stats_.p)
+                                            deg_.p, att_off_.p, att_adj_.p, vals_.p, nodes_.p, \
+                                            stats_.p)
         switch (bs) {
             case 64: JC_BF(64); break;
             case 128: JC_BF(128); break;
             default: JC_BF(256); break;
         }
 #undef JC_BF
+        if (n_heavy_)
+            k_b_force_heavy<<<n_heavy_, 64, 0, stream_>>>(P_, heavy_.p, hoff_.p, hlen_.p, hterm_.p,
+                                                         hstride_, pos, old_force_.p, force_.p,
+                                                         deg_.p, stats_.p);
         JC_HIP(hipGetLastError());
         return Status::ok();
     }
@@ -887,8 +1098,10 @@ class Fa2BhBatch {
     DevBuf<double> est_, mjt_;
     DevBuf<uint64_t> keys_raw_, keys_;
     RadixScratch sort_;
-    DevBuf<float4> node_a_;
-    DevBuf<uint2> node_b_;
+    DevBuf<Node> nodes_;
+    uint32_t n_heavy_ = 0, n_hent_ = 0, hstride_ = 0;   // rows longer than kBHeavyAtt
+    DevBuf<uint32_t> heavy_, hoff_, hlen_, hbody_, hadj_;
+    DevBuf<float> hterm_;
     DevBuf<uint8_t> node_depth_;
     DevBuf<GFrame> frame_;
     DevBuf<BGlobal> global_;

@@ -27,6 +27,25 @@ __device__ __forceinline__ void exact_acc_add(unsigned long long *acc, float v) 
     if (t >> 32) atomicAdd(&acc[(sh >> 5) + 1], (unsigned long long)(t >> 32));
 }
 
+// Adds |v| (v finite) to a thread-private accumulator held in registers: the
+// same two 32-bit pieces exact_acc_add adds, placed by comparisons against
+// constant limb indices (no dynamic indexing, no atomics).
+__device__ __forceinline__ void exact_reg_add(unsigned long long (&acc)[kLimbs], float v) {
+    const uint32_t u = __float_as_uint(v) & 0x7FFFFFFFu;
+    const uint32_t e = u >> 23;
+    uint32_t m = u & 0x7FFFFFu;
+    if (e) m |= 0x800000u;
+    const uint32_t sh = (e ? e : 1u) - 1u;
+    const uint64_t t = (uint64_t)m << (sh & 31u);
+    const uint32_t k = sh >> 5;
+    const unsigned long long lo = t & 0xFFFFFFFFull, hi = t >> 32;
+#pragma unroll
+    for (uint32_t j = 0; j < kLimbs; j++) {
+        acc[j] += (j == k) ? lo : 0ull;
+        acc[j] += (j == k + 1) ? hi : 0ull;
+    }
+}
+
 // Carry-normalises l[0..kLimbs) in place (each limb < 2^32 afterwards).
 __device__ __forceinline__ void exact_normalise(unsigned long long *l) {
     for (uint32_t k = 0; k + 1 < kLimbs; k++) {

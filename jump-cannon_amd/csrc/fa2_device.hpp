@@ -10,6 +10,15 @@ constexpr int kLevels = 21;            // octree key depth (3 bits per level)
 constexpr uint32_t kLeaf = 0x80000000u;  // node_info flag of a leaf
 constexpr uint32_t kEnd = 0xFFFFFFFFu;   // end of the rope
 
+// One octree node, 32 bytes: one aligned record per visit.
+//   a = centre of mass xyz, mass
+//   b.x = internal: bits of (2*half)^2; leaf: 0x80000000 | body
+//   b.y = skip rope (first node after the subtree, kEnd at the end)
+struct __align__(32) Node {
+    float4 a;
+    uint4 b;
+};
+
 __device__ __forceinline__ uint32_t f2ord(float f) {
     uint32_t u = __float_as_uint(f);
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);

@@ -68,15 +68,6 @@ struct Frame {
     unsigned long long interactions;  // accepted cells + leaf bodies summed by the walk
 };
 
-// One octree node, 32 bytes: one cache-line-aligned record per visit.
-//   a = centre of mass xyz, mass
-//   b.x = internal: bits of (2*half)^2; leaf: 0x80000000 | body
-//   b.y = skip rope (first node after the subtree, kEnd at the end)
-struct __align__(32) Node {
-    float4 a;
-    uint4 b;
-};
-
 struct Params {
     uint32_t n;
     float gravity;
