@@ -257,6 +257,23 @@ uint64_t jc_csr_from_pairs(uint32_t n, const uint32_t *pairs, uint64_t npairs, u
 /* Positions uniform in [-half, half]^3 ((2u-1)*half per coordinate). */
 void jc_gen_uniform_cube(uint32_t n, float half, uint64_t seed, float *xyz);
 
+/* ---- binary CSR wire format (CsrGraph::from_bin_bytes / to_bin,
+ * crates/graph-compute/src/sim.rs:52-128; the graph-api /graph/csr.bin payload
+ * and the LoadGraph gRPC body) ----
+ * Little-endian [u32 n_nodes][u32 n_edges][u32 x (n_nodes+1) offsets]
+ * [u32 x n_edges neighbors].  Like the reference parser, the contents are not
+ * validated here (engine init validates).  No device is needed. */
+/* Header and length check: JC_E_INVALID with the reference's messages
+ * ("CSR truncated: ..." / "CSR length mismatch: ..."). */
+int jc_csr_bin_header(const uint8_t *bytes, size_t len, uint32_t *n_nodes, uint32_t *n_edges);
+/* Copies the arrays out (offsets: n_nodes+1, neighbors: n_edges u32s). */
+int jc_csr_from_bin(const uint8_t *bytes, size_t len, uint32_t *offsets, uint32_t *neighbors);
+/* Size in bytes of the encoding of a graph (8 + 4(n_nodes+1) + 4 n_edges). */
+uint64_t jc_csr_bin_size(uint32_t n_nodes, uint64_t n_edges);
+/* Encodes into out[0..cap); JC_E_INVALID when cap is too small or n_edges > 2^32-1. */
+int jc_csr_to_bin(uint32_t n_nodes, const uint32_t *offsets, const uint32_t *neighbors,
+                  uint64_t n_edges, uint8_t *out, uint64_t cap);
+
 #ifdef __cplusplus
 }
 #endif

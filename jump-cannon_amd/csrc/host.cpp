@@ -657,4 +657,70 @@ void jc_gen_uniform_cube(uint32_t n, float half, uint64_t seed, float *xyz) {
     }
 }
 
+namespace {
+inline uint32_t rd_le32(const uint8_t *b) {
+    return (uint32_t)b[0] | ((uint32_t)b[1] << 8) | ((uint32_t)b[2] << 16) | ((uint32_t)b[3] << 24);
+}
+inline void wr_le32(uint8_t *b, uint32_t v) {
+    b[0] = (uint8_t)v;
+    b[1] = (uint8_t)(v >> 8);
+    b[2] = (uint8_t)(v >> 16);
+    b[3] = (uint8_t)(v >> 24);
+}
+}  // namespace
+
+// sim.rs:72-104 (from_bin_bytes): header, then an exact length check.
+int jc_csr_bin_header(const uint8_t *bytes, size_t len, uint32_t *n_nodes, uint32_t *n_edges) {
+    if (!bytes && len) return finish(jc::Status::err(JC_E_INVALID, "CSR bytes pointer is NULL"));
+    if (len < 8)
+        return finish(jc::Status::err(JC_E_INVALID, "CSR truncated: " + std::to_string(len) +
+                                                        " bytes < 8-byte header"));
+    const uint32_t n = rd_le32(bytes), m = rd_le32(bytes + 4);
+    const uint64_t expected = 8ull + 4ull * ((uint64_t)n + 1) + 4ull * (uint64_t)m;
+    if ((uint64_t)len != expected)
+        return finish(jc::Status::err(
+            JC_E_INVALID, "CSR length mismatch: got " + std::to_string(len) + " bytes, expected " +
+                              std::to_string(expected) + " (n_nodes=" + std::to_string(n) +
+                              ", n_edges=" + std::to_string(m) + ")"));
+    if (n_nodes) *n_nodes = n;
+    if (n_edges) *n_edges = m;
+    return finish(jc::Status::ok());
+}
+
+int jc_csr_from_bin(const uint8_t *bytes, size_t len, uint32_t *offsets, uint32_t *neighbors) {
+    uint32_t n = 0, m = 0;
+    const int rc = jc_csr_bin_header(bytes, len, &n, &m);
+    if (rc) return rc;
+    if ((!offsets) || (m && !neighbors))
+        return finish(jc::Status::err(JC_E_INVALID, "CSR output pointer is NULL"));
+    const uint8_t *p = bytes + 8;
+    for (uint64_t i = 0; i <= n; i++, p += 4) offsets[i] = rd_le32(p);
+    for (uint64_t i = 0; i < m; i++, p += 4) neighbors[i] = rd_le32(p);
+    return finish(jc::Status::ok());
+}
+
+uint64_t jc_csr_bin_size(uint32_t n_nodes, uint64_t n_edges) {
+    return 8ull + 4ull * ((uint64_t)n_nodes + 1) + 4ull * n_edges;
+}
+
+// sim.rs:118-127 (to_bin).
+int jc_csr_to_bin(uint32_t n_nodes, const uint32_t *offsets, const uint32_t *neighbors,
+                  uint64_t n_edges, uint8_t *out, uint64_t cap) {
+    if (n_edges > 0xFFFFFFFFull)
+        return finish(jc::Status::err(JC_E_INVALID, "CSR has more than 2^32-1 edges"));
+    if (!offsets || (n_edges && !neighbors) || !out)
+        return finish(jc::Status::err(JC_E_INVALID, "CSR pointer is NULL"));
+    const uint64_t need = jc_csr_bin_size(n_nodes, n_edges);
+    if (cap < need)
+        return finish(jc::Status::err(JC_E_INVALID, "CSR output buffer too small: " +
+                                                        std::to_string(cap) + " < " +
+                                                        std::to_string(need)));
+    wr_le32(out, n_nodes);
+    wr_le32(out + 4, (uint32_t)n_edges);
+    uint8_t *p = out + 8;
+    for (uint64_t i = 0; i <= n_nodes; i++, p += 4) wr_le32(p, offsets[i]);
+    for (uint64_t i = 0; i < n_edges; i++, p += 4) wr_le32(p, neighbors[i]);
+    return finish(jc::Status::ok());
+}
+
 }  // extern "C"

@@ -133,6 +133,10 @@ SIGNATURES = {
     "jc_gen_rgg_pairs": (C.c_uint64, [C.c_uint32, C.c_double, C.c_uint64, _vp, _vp, C.c_uint64]),
     "jc_csr_from_pairs": (C.c_uint64, [C.c_uint32, _vp, C.c_uint64, _vp, _vp]),
     "jc_gen_uniform_cube": (None, [C.c_uint32, C.c_float, C.c_uint64, _vp]),
+    "jc_csr_bin_header": (C.c_int, [_vp, C.c_size_t, _vp, _vp]),
+    "jc_csr_from_bin": (C.c_int, [_vp, C.c_size_t, _vp, _vp]),
+    "jc_csr_bin_size": (C.c_uint64, [C.c_uint32, C.c_uint64]),
+    "jc_csr_to_bin": (C.c_int, [C.c_uint32, _vp, _vp, C.c_uint64, _vp, C.c_uint64]),
 }
 
 _lib = None
@@ -653,3 +657,39 @@ def uniform_cube(n: int, half: float, seed: int):
     out = np.empty(3 * max(n, 1), np.float32)
     lib().jc_gen_uniform_cube(n, half, seed, _ptr(out))
     return out[: 3 * n]
+
+
+def csr_from_bin(data) -> tuple:
+    """CsrGraph::from_bin_bytes (sim.rs:72-104): (offsets, neighbors) from the
+    little-endian binary CSR; raises JcError on a truncated or mis-sized buffer."""
+    buf = np.frombuffer(bytes(data), np.uint8)
+    n = np.zeros(1, np.uint32)
+    m = np.zeros(1, np.uint32)
+    _check(lib().jc_csr_bin_header(_ptr(buf) if len(buf) else None, len(buf), _ptr(n), _ptr(m)))
+    off = np.empty(int(n[0]) + 1, np.uint32)
+    nb = np.empty(max(int(m[0]), 1), np.uint32)
+    _check(lib().jc_csr_from_bin(_ptr(buf), len(buf), _ptr(off), _ptr(nb)))
+    return off, nb[: int(m[0])]
+
+
+def csr_to_bin(offsets, neighbors) -> bytes:
+    """CsrGraph::to_bin (sim.rs:118-127)."""
+    off = _u32(offsets)
+    nb = _u32(neighbors)
+    n = len(off) - 1
+    size = int(lib().jc_csr_bin_size(n, len(nb)))
+    out = np.empty(size, np.uint8)
+    _check(lib().jc_csr_to_bin(n, _ptr(off), _ptr(nb) if len(nb) else None, len(nb), _ptr(out), size))
+    return out.tobytes()
+
+
+def load_csr_bin(path) -> tuple:
+    """CsrGraph::load_bin (sim.rs:61-66)."""
+    with open(path, "rb") as fh:
+        return csr_from_bin(fh.read())
+
+
+def write_csr_bin(path, offsets, neighbors) -> None:
+    """CsrGraph::write_bin (sim.rs:107-113)."""
+    with open(path, "wb") as fh:
+        fh.write(csr_to_bin(offsets, neighbors))

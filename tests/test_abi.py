@@ -96,3 +96,47 @@ def test_sbm_2000_workload_shape():
     assert n == 2000 and xyz.shape == (6000,) and np.abs(xyz).max() <= 1.0
     # E|E| ~ 4*C(500,2)*0.02 + 6*500*500*0.001 = 9980 + 1500
     assert 10500 < len(nb) // 2 < 12500
+
+
+def _path_csr(n):
+    """CsrGraph::path(n) (sim.rs:33-51): 0-1-2-...-(n-1)."""
+    off, nb = [0], []
+    for i in range(n):
+        if i > 0:
+            nb.append(i - 1)
+        if i + 1 < n:
+            nb.append(i + 1)
+        off.append(len(nb))
+    return np.array(off, np.uint32), np.array(nb, np.uint32)
+
+
+def test_csr_bin_roundtrip(tmp_path):
+    """sim.rs:135-149: write_bin then load_bin gives the same graph; the bytes are
+    the little-endian header plus both arrays."""
+    off, nb = _path_csr(8)
+    p = tmp_path / "path8.bin"
+    jc.write_csr_bin(p, off, nb)
+    raw = p.read_bytes()
+    want = np.concatenate([np.array([8, len(nb)], "<u4"), off.astype("<u4"), nb.astype("<u4")])
+    assert raw == want.tobytes()
+    o2, n2 = jc.load_csr_bin(p)
+    assert np.array_equal(o2, off) and np.array_equal(n2, nb)
+    bo, bn = jc.ba_graph(3000, 3, 11)
+    o3, n3 = jc.csr_from_bin(jc.csr_to_bin(bo, bn))
+    assert np.array_equal(o3, bo) and np.array_equal(n3, bn)
+    o4, n4 = jc.csr_from_bin(jc.csr_to_bin(np.zeros(1, np.uint32), np.zeros(0, np.uint32)))
+    assert list(o4) == [0] and len(n4) == 0
+
+
+def test_csr_bin_rejects_truncated_and_length_mismatch():
+    """sim.rs:151-179: a 4-byte file is truncated; a header claiming 4 nodes and 8
+    edges with no payload is a length mismatch."""
+    with pytest.raises(jc.JcError) as ei:
+        jc.csr_from_bin(bytes(4))
+    assert ei.value.code == jc.JC_E_INVALID and "truncated" in str(ei.value)
+    hdr = np.array([4, 8], "<u4").tobytes()
+    with pytest.raises(jc.JcError) as ei:
+        jc.csr_from_bin(hdr)
+    assert "length mismatch" in str(ei.value) and "expected 60" in str(ei.value)
+    with pytest.raises(jc.JcError):
+        jc.csr_from_bin(jc.csr_to_bin(*_path_csr(5)) + b"\0")
