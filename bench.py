@@ -258,12 +258,13 @@ def main() -> int:
     force_ms, iter_ms = e.step_timed(args.steps)   # synchronizes at the end
     wall = time.perf_counter() - t0
     value = args.steps / wall
-    st = e.last_stats()
     log(f"timed {args.steps} iterations: {wall * 1e3:.1f} ms, force kernels {force_ms:.1f} ms")
 
-    phases = e.profile(args.profile_iters) if args.profile_iters else {}
-    if phases:
-        log("phases (ms): " + ", ".join(f"{k}={v:.3f}" for k, v in phases.items()))
+    # the profiled iterations also count the walk's interactions (a separate
+    # counting walk after them; plain iterations do not count)
+    phases = e.profile(max(args.profile_iters, 1))
+    log("phases (ms): " + ", ".join(f"{k}={v:.3f}" for k, v in phases.items()))
+    st = e.last_stats()
 
     fb = force_kernel_bytes(n, st["n_tree"], nnz_att)
     f_avg_s = force_ms / args.steps / 1e3

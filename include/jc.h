@@ -137,8 +137,10 @@ int jc_fa2bh_tree(jc_engine *e, uint32_t cap, uint32_t *n_tree, uint32_t *n_vali
  * time (ms). */
 int jc_fa2bh_step_timed(jc_engine *e, uint32_t n_iters, double *force_ms_sum,
                         double *iter_ms_sum);
-/* Octree size, number of interactions summed by the walk (accepted cells plus
- * leaf bodies) and the exact {sum swing, sum traction} of the last iteration. */
+/* Octree size and the exact {sum swing, sum traction} of the last iteration, and
+ * the number of interactions summed by the walk (accepted cells plus leaf bodies)
+ * in the last iteration run by jc_fa2bh_profile or jc_fa2bh_forces_at (plain
+ * steps do not count them: the count is a separate walk, 0 after a plain step). */
 int jc_fa2bh_last_stats(jc_engine *e, uint64_t *n_tree, uint64_t *interactions, double *sums2);
 /* Octree walk of the force kernel: 0 = every lane follows its own rope,
  * 1 = wave-union walk (uniform node loads), 2 / 3 = each lane interleaves the

@@ -69,6 +69,7 @@ struct BParams {
     float max_displacement;
     float ewi_weight;
     uint32_t cap;
+    uint32_t fastdiv;    // scaling ratio inside kr_in_fastdiv_range
 };
 
 __global__ void k_b_reset(GFrame *fr, uint32_t G, BGlobal *gl) {
@@ -316,7 +317,7 @@ __device__ __forceinline__ void b_emit_cell(const uint64_t *__restrict__ keys, c
     const uint32_t j = cell_end(keys, i, sg.ve, keys[i], d);
     const uint32_t skip = (j + 1 < sg.ve) ? node_start(loc, blk, j + 1, n, gl) : kEnd;
     const float s = h * 2.0f;
-    nodes[node].b = make_uint4(__float_as_uint(s * s), skip, 0u, 0u);
+    nodes[node].b = cell_b(s * s, skip);
     node_depth[node] = (uint8_t)d;
 }
 
@@ -373,7 +374,7 @@ __global__ __launch_bounds__(kBT) void k_b_emit(
                 if (b < n) {
                     const float4 p = pos[b];
                     nodes[leaf].a = make_float4(p.x, p.y, p.z, fmaxf((float)deg[b] + 1.0f, 1e-3f));
-                    nodes[leaf].b = make_uint4(kLeaf | b, (i + 1 < sg.ve) ? leaf + 1 : kEnd, 0u, 0u);
+                    nodes[leaf].b = leaf_b(b, (i + 1 < sg.ve) ? leaf + 1 : kEnd);
                     node_depth[leaf] = (uint8_t)(0x80 | ((lp > li ? lp : li) + 1));
                 }
             }
@@ -396,9 +397,9 @@ __global__ __launch_bounds__(kBT) void k_b_com_level(const BGlobal *gl, uint32_t
     const uint32_t N = min(gl->n_tree_total, cap);
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
         if (node_depth[p] != d) continue;
-        const uint32_t end = nodes[p].b.y;
+        const uint32_t end = nodes[p].b.w;
         float tm = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
-        for (uint32_t ch = p + 1; ch < N && ch != end; ch = nodes[ch].b.y) {
+        for (uint32_t ch = p + 1; ch < N && ch != end; ch = nodes[ch].b.w) {
             const float4 a = nodes[ch].a;
             tm = tm + a.w;
             cx = cx + a.x * a.w;
@@ -479,27 +480,15 @@ __global__ __launch_bounds__(BS) void k_b_force(const BParams P, const GFrame *f
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     const uint32_t N = fr[g].n_tree;
     if (N > 0) {
-        const uint32_t start = fr[g].tree_start;
-        const uint64_t wcap = max((uint64_t)N * 4ull, 16ull);
-        uint64_t walk = 0;
-        uint32_t idx = start;
-        while (idx < P.cap && walk < wcap) {
-            walk++;
-            const Node nd = nodes[idx];
-            const float dx = p.x - nd.a.x, dy = p.y - nd.a.y, dz = p.z - nd.a.z;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            const bool leaf = (nd.b.x & kLeaf) != 0u;
-            bool take;
-            if (leaf) take = ((nd.b.x & ~kLeaf) != b) && nd.a.w > 0.0f;
-            else take = nd.a.w > 0.0f && r2 > 0.0f && __uint_as_float(nd.b.x) < P.theta2 * r2;
-            if (take) {
-                const float r2c = fmaxf(r2, 1.0e-4f);
-                const float cf = krd * nd.a.w / r2c;
-                fx = fx + dx * cf;
-                fy = fy + dy * cf;
-                fz = fz + dz * cf;
-            }
-            idx = (leaf || take) ? nd.b.y : idx + 1u;
+        // the graph's tree is nodes [start, start + N); its last rope is kEnd
+        const uint32_t start = fr[g].tree_start, end = start + N;
+        if (end <= P.cap && tree_reducible(end, fr[g].root[3], P.theta2) && finite3(p)) {
+            if (P.fastdiv)
+                (void)walk_reduced<true>(nodes, start, end, p.x, p.y, p.z, b, krd, P.theta2, fx, fy, fz);
+            else
+                (void)walk_reduced<false>(nodes, start, end, p.x, p.y, p.z, b, krd, P.theta2, fx, fy, fz);
+        } else {
+            (void)walk_exact(nodes, start, min(end, P.cap), p.x, p.y, p.z, b, krd, P.theta2, fx, fy, fz);
         }
     }
     const uint32_t e0 = att_off[b], e1 = att_off[b + 1];
@@ -946,6 +935,7 @@ class Fa2BhBatch {
         P_.max_displacement = s.max_displacement;
         P_.ewi_weight = std::pow(std::fmax(1.0f, 1.0e-6f), s.edge_weight_influence);
         P_.cap = (uint32_t)cap_;
+        P_.fastdiv = kr_in_fastdiv_range(s.scaling_ratio) ? 1u : 0u;
         params_dirty_ = false;
         drop_graph();
     }

@@ -7,17 +7,53 @@
 namespace jc {
 
 constexpr int kLevels = 21;            // octree key depth (3 bits per level)
-constexpr uint32_t kLeaf = 0x80000000u;  // node_info flag of a leaf
+constexpr uint32_t kLeaf = 0x80000000u;  // node_info flag of a leaf (exported tree form)
 constexpr uint32_t kEnd = 0xFFFFFFFFu;   // end of the rope
+constexpr uint32_t kCellId = 0xFFFFFFFFu;   // Node::b.z of a cell: never a live body
+constexpr uint32_t kLeafSize = 0xFF800000u;  // Node::b.y of a leaf: bits of -inf
+// Byte offsets of nodes fit 32 bits below this many nodes (the fast walk's addressing).
+constexpr uint32_t kMaxNodes32 = 1u << 27;
+// Word 3 of a gfx9-family buffer resource for raw (untyped) dword loads.
+constexpr int kBufferRsrcWord3 = 0x00020000;
 
 // One octree node, 32 bytes: one aligned record per visit.
-//   a = centre of mass xyz, mass
-//   b.x = internal: bits of (2*half)^2; leaf: 0x80000000 | body
-//   b.y = skip rope (first node after the subtree, kEnd at the end)
+//   a   = centre of mass xyz, mass
+//   b.x = skip rope as a byte offset (b.w * 32; 0xFFFFFFFF at the end or when
+//         b.w * 32 would not fit 32 bits)
+//   b.y = cell: bits of the opening size (2*half)^2; leaf: bits of -inf
+//   b.z = leaf: its body; cell: kCellId
+//   b.w = skip rope (first node after the subtree, kEnd at the end)
 struct __align__(32) Node {
     float4 a;
     uint4 b;
 };
+
+__device__ __forceinline__ uint32_t skip_bytes(uint32_t skip) {
+    return skip < kMaxNodes32 ? skip << 5 : 0xFFFFFFFFu;
+}
+__device__ __forceinline__ uint4 cell_b(float s2, uint32_t skip) {
+    return make_uint4(skip_bytes(skip), __float_as_uint(s2), kCellId, skip);
+}
+__device__ __forceinline__ uint4 leaf_b(uint32_t body, uint32_t skip) {
+    return make_uint4(skip_bytes(skip), kLeafSize, body, skip);
+}
+
+// Correctly rounded a / b in 8 instructions: the reciprocal refinement and the
+// two residual corrections of the compiler's IEEE f32 division, without its
+// operand scaling (v_div_scale) and special-case fixup (v_div_fixup).  Both are
+// identities when a and b are positive normals, a >= 2^-100, b < 2^126,
+// a / b is a normal and the exponents differ by less than 96; there this returns
+// the bits of a / b.  Callers guarantee that range.
+__device__ __forceinline__ float div_in_range(float a, float b) {
+    const float y0 = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, y0, 1.0f);
+    const float y = __builtin_fmaf(e, y0, y0);
+    const float q0 = a * y;
+    const float r0 = __builtin_fmaf(-b, q0, a);
+    const float q1 = __builtin_fmaf(r0, y, q0);
+    const float r1 = __builtin_fmaf(-b, q1, a);
+    return __builtin_fmaf(r1, y, q1);
+}
 
 __device__ __forceinline__ uint32_t f2ord(float f) {
     uint32_t u = __float_as_uint(f);
@@ -28,6 +64,9 @@ __device__ __forceinline__ float ord2f(uint32_t o) {
     return __uint_as_float(u);
 }
 __device__ __forceinline__ bool finitef(float x) { return __builtin_isfinite(x); }
+__device__ __forceinline__ bool finite3(float4 p) {
+    return finitef(p.x) && finitef(p.y) && finitef(p.z);
+}
 __device__ __forceinline__ float len3(float x, float y, float z) {
     return sqrtf(x * x + y * y + z * z);
 }
@@ -94,6 +133,159 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t G) {
     const uint32_t q = G / kXcd, r = G % kXcd;
     const uint32_t x = b % kXcd, slot = b / kXcd;
     return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+}
+
+// One visit of the walk in the reference form (fa2_barnes_hut.wgsl:108-156):
+// leaf: skip self, take any other body; cell: accept when (2*half)^2 <
+// theta^2 * r^2 and r^2 > 0, else descend to the next node in pre-order.
+// Returns whether the node was taken; *next is the node visited after it.
+__device__ __forceinline__ bool visit_exact(const Node &nd, uint32_t idx, float px, float py,
+                                            float pz, uint32_t b, float krd, float theta2,
+                                            float &fx, float &fy, float &fz, uint32_t &next) {
+    const float dx = px - nd.a.x, dy = py - nd.a.y, dz = pz - nd.a.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const bool leaf = nd.b.z != kCellId;
+    bool take;
+    if (leaf) take = nd.b.z != b && nd.a.w > 0.0f;
+    else take = nd.a.w > 0.0f && r2 > 0.0f && __uint_as_float(nd.b.y) < theta2 * r2;
+    if (take) {
+        const float r2c = fmaxf(r2, 1.0e-4f);
+        const float cf = krd * nd.a.w / r2c;
+        fx = fx + dx * cf;
+        fy = fy + dy * cf;
+        fz = fz + dz * cf;
+    }
+    next = (leaf || take) ? nd.b.w : idx + 1u;
+    return take;
+}
+
+// The walk of one body in the reference form, nodes [start, end) (end: one past
+// the tree's last node, where the ropes also stop).  Returns the nodes taken.
+__device__ __forceinline__ uint32_t walk_exact(const Node *__restrict__ nodes, uint32_t start,
+                                               uint32_t end, float px, float py, float pz,
+                                               uint32_t b, float krd, float theta2, float &fx,
+                                               float &fy, float &fz) {
+    uint32_t taken = 0, idx = start;
+    // ropes only move forward, so a walk visits at most end - start nodes
+    for (uint32_t steps = 0; idx < end && steps < end - start; steps++) {
+        uint32_t next;
+        taken += visit_exact(nodes[idx], idx, px, py, pz, b, krd, theta2, fx, fy, fz, next);
+        idx = next;
+    }
+    return taken;
+}
+
+// Host-side range check of the scaling ratio for div_in_range in the repulsion
+// quotient kr*deg*mass / max(r2, 1e-4): with masses in [1, 2^34], degrees below
+// 2^31 and squared distances below 2^84 (tree_reducible), kr in [2^-10, 2^10]
+// keeps numerator, denominator and quotient inside div_in_range's range.
+inline bool kr_in_fastdiv_range(float kr) { return kr >= 0x1p-10f && kr <= 0x1p10f; }
+
+// Where walk_reduced may stand in for walk_exact: node byte offsets fit 32
+// bits, the root half-size is below 2^40 (so every squared distance between
+// finite bodies of the tree is finite and below 2^84), theta^2 is finite, and
+// the walking body is finite.
+constexpr float kReducedRootHalf = 1099511627776.0f;  // 2^40
+__device__ __forceinline__ bool tree_reducible(uint32_t end, float root_half, float theta2) {
+    return end < kMaxNodes32 && root_half < kReducedRootHalf && finitef(theta2);
+}
+
+// The same walk in reduced form: same nodes in the same order, same f32
+// operations, same sums.  Every mass is >= 1 (leaves: deg+1; cells: sums of
+// those) and every squared distance r2 is finite, so the reference tests become
+//     take = size < theta^2 * r2  &&  id != b
+// with size = (2*half)^2 >= 0 for a cell (size < theta^2*r2 then implies
+// r2 > 0) and -inf for a leaf (theta^2*r2 is finite or +inf, never NaN), and
+// id = the leaf's body, kCellId for a cell.  A leaf's skip rope is the next
+// node, so next = take ? skip : this + 1 covers both kinds; the ropes are byte
+// offsets, which saves the index shift per step.  r2 >= +0 is never -0 or NaN,
+// so max(r2, 1e-4) is the unsigned maximum of the bit patterns.  FASTDIV uses
+// div_in_range: the caller has checked that krd * mass / max(r2, 1e-4) stays in
+// its range.  Interactions are not counted here (walk_exact counts them).
+template <bool FASTDIV>
+__device__ __forceinline__ void walk_reduced(const Node *__restrict__ nodes, uint32_t start,
+                                             uint32_t end, float px, float py, float pz,
+                                             uint32_t b, float krd, float theta2, float &fx,
+                                             float &fy, float &fz) {
+    // Both halves of a node come in as one 16-byte buffer load each (a plain
+    // pointer load is split by the compiler into the fields each branch uses).
+    const uint32_t end_b = end << 5;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<Node *>(nodes), (short)0, (int)end_b,
+                                          kBufferRsrcWord3);
+    uint32_t off = start << 5;
+    for (uint32_t steps = 0; off < end_b && steps < end - start; steps++) {
+        const auto av = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+        const auto bv = __builtin_amdgcn_raw_buffer_load_b128(rs, off + 16u, 0, 0);
+        const float4 a = make_float4(__uint_as_float(av[0]), __uint_as_float(av[1]),
+                                     __uint_as_float(av[2]), __uint_as_float(av[3]));
+        const uint32_t skip_b = bv[0], size = bv[1], id = bv[2];
+        const float dx = px - a.x, dy = py - a.y, dz = pz - a.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const bool take = __uint_as_float(size) < theta2 * r2 && id != b;
+        const uint32_t next = take ? skip_b : off + 32u;
+        if (take) {
+            const float r2c = __uint_as_float(max(__float_as_uint(r2), 0x38D1B717u));  // 1e-4f
+            const float cf = FASTDIV ? div_in_range(krd * a.w, r2c) : krd * a.w / r2c;
+            fx = fx + dx * cf;
+            fy = fy + dy * cf;
+            fz = fz + dz * cf;
+        }
+        off = next;
+    }
+}
+
+// K reduced walks interleaved step by step in one lane (K independent
+// pointer chains in flight): each walk is walk_reduced's, nodes [0, end).  A
+// finished walk parks at end * 32, past the buffer, where its loads return
+// zeros and its visits are skipped.
+template <int K, bool FASTDIV>
+__device__ __forceinline__ void walk_reduced_multi(const Node *__restrict__ nodes, uint32_t end,
+                                                   const bool (&live)[K], const float4 (&p)[K],
+                                                   const uint32_t (&b)[K], const float (&krd)[K],
+                                                   float theta2, float (&fx)[K], float (&fy)[K],
+                                                   float (&fz)[K]) {
+    const uint32_t end_b = end << 5;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<Node *>(nodes), (short)0, (int)end_b,
+                                          kBufferRsrcWord3);
+    uint32_t off[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) off[k] = live[k] ? 0u : end_b;
+    for (uint32_t steps = 0; steps < end; steps++) {
+        bool active = false;
+#pragma unroll
+        for (int k = 0; k < K; k++) active |= off[k] < end_b;
+        if (!active) break;
+        uint4 av[K], bv[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const auto x = __builtin_amdgcn_raw_buffer_load_b128(rs, off[k], 0, 0);
+            const auto y = __builtin_amdgcn_raw_buffer_load_b128(rs, off[k] + 16u, 0, 0);
+            av[k] = make_uint4(x[0], x[1], x[2], x[3]);
+            bv[k] = make_uint4(y[0], y[1], y[2], y[3]);
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if (off[k] < end_b) {
+                const float dx = p[k].x - __uint_as_float(av[k].x);
+                const float dy = p[k].y - __uint_as_float(av[k].y);
+                const float dz = p[k].z - __uint_as_float(av[k].z);
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                const bool take = __uint_as_float(bv[k].y) < theta2 * r2 && bv[k].z != b[k];
+                const uint32_t next = take ? bv[k].x : off[k] + 32u;
+                if (take) {
+                    const float r2c = __uint_as_float(max(__float_as_uint(r2), 0x38D1B717u));
+                    const float m = krd[k] * __uint_as_float(av[k].w);
+                    const float cf = FASTDIV ? div_in_range(m, r2c) : m / r2c;
+                    fx[k] = fx[k] + dx * cf;
+                    fy[k] = fy[k] + dy * cf;
+                    fz[k] = fz[k] + dz * cf;
+                }
+                off[k] = next;
+            }
+        }
+    }
 }
 
 // Octant path of p through the reference cell chain for kLevels levels: digit

@@ -82,6 +82,11 @@ struct Params {
     double est_jt;     // 0.05 * sqrt(n)           (fa2_speed.rs:72)
     double min_jt;     // sqrt(est_jt)             (fa2_speed.rs:73)
     uint32_t cap;      // node capacity of the octree arrays
+    // scaling_ratio in [2^-10, 2^10]: with masses in [1, 2^34], degrees below
+    // 2^31 and squared distances in [1e-4, 2^84] (root half-size below 2^40,
+    // checked per launch), every repulsion quotient kr*deg*mass / r2 is in the
+    // range of div_in_range
+    uint32_t fastdiv;
 };
 
 // ---------------------------------------------------------------- kernels
@@ -273,7 +278,7 @@ __device__ __forceinline__ void emit_cell(const uint64_t *__restrict__ keys, uin
     const uint32_t skip = (j + 1 < nv) ? first_node_of(loc, blk, j + 1) : kEnd;
     const float s = h * 2.0f;
     const float s2 = s * s;
-    nodes[node].b = make_uint4(__float_as_uint(s2), skip, 0u, 0u);
+    nodes[node].b = cell_b(s2, skip);
     node_depth[node] = (uint8_t)d;
 }
 
@@ -334,7 +339,7 @@ __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
                 const float4 p = pos[b];
                 const float m = fmaxf((float)deg[b] + 1.0f, 1e-3f);
                 nodes[leaf].a = make_float4(p.x, p.y, p.z, m);
-                nodes[leaf].b = make_uint4(kLeaf | b, (i + 1 < nv) ? leaf + 1 : kEnd, 0u, 0u);
+                nodes[leaf].b = leaf_b(b, (i + 1 < nv) ? leaf + 1 : kEnd);
                 const int ld = (lp > li ? lp : li) + 1;
                 node_depth[leaf] = (uint8_t)(0x80 | ld);
             }
@@ -356,9 +361,9 @@ __global__ __launch_bounds__(kTB) void k_com_level(const Frame *fr, uint32_t d, 
     const uint32_t N = min(fr->n_tree, cap);
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
         if (node_depth[p] != d) continue;
-        const uint32_t end = nodes[p].b.y;
+        const uint32_t end = nodes[p].b.w;
         float tm = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
-        for (uint32_t ch = p + 1; ch < N && ch != end; ch = nodes[ch].b.y) {
+        for (uint32_t ch = p + 1; ch < N && ch != end; ch = nodes[ch].b.w) {
             const float4 a = nodes[ch].a;
             tm = tm + a.w;
             cx = cx + a.x * a.w;
@@ -372,28 +377,6 @@ __global__ __launch_bounds__(kTB) void k_com_level(const Frame *fr, uint32_t d, 
         }
         nodes[p].a = make_float4(cx, cy, cz, tm);
     }
-}
-
-// The walk of one body: the node sequence and the f32 operations of
-// fa2_barnes_hut.wgsl:108-156 (leaf: skip self; internal: accept when
-// (2*half)^2 < theta^2 * r^2, else descend to the next node in pre-order).
-__device__ __forceinline__ bool visit(const Node &nd, float px, float py, float pz, uint32_t b,
-                                      float krd, float theta2, float &fx, float &fy, float &fz,
-                                      bool &leaf) {
-    const float dx = px - nd.a.x, dy = py - nd.a.y, dz = pz - nd.a.z;
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    leaf = (nd.b.x & kLeaf) != 0u;
-    bool take;
-    if (leaf) take = ((nd.b.x & ~kLeaf) != b) && nd.a.w > 0.0f;
-    else take = nd.a.w > 0.0f && r2 > 0.0f && __uint_as_float(nd.b.x) < theta2 * r2;
-    if (take) {
-        const float r2c = fmaxf(r2, 1.0e-4f);
-        const float cf = krd * nd.a.w / r2c;
-        fx = fx + dx * cf;
-        fy = fy + dy * cf;
-        fz = fz + dz * cf;
-    }
-    return take;
 }
 
 // Attraction, gravity and the swing/traction stats of body b, after its walk
@@ -482,6 +465,61 @@ __device__ __forceinline__ void finish_body(const Params &P, uint32_t b, float4 
         gravity_and_stats(P, b, p, deg_i, fx, fy, fz, old_force, force, acc);
     }
 
+// Whether this launch may use the reduced walk and the in-range division
+// (uniform over the launch: the tree size and root come from the frame, the
+// scaling ratio from the host's range check in Params::fastdiv).
+struct WalkForm {
+    bool reduced;
+    bool fastdiv;
+};
+__device__ __forceinline__ WalkForm walk_form(const Params &P, const Frame *fr, uint32_t N) {
+    WalkForm w;
+    w.reduced = tree_reducible(N, fr->root[3], P.theta2);
+    w.fastdiv = w.reduced && P.fastdiv != 0u;
+    return w;
+}
+
+// The walk of body b over nodes [0, N) (fa2_barnes_hut.wgsl:108-156), in the
+// reduced form where that is exact for this body.
+__device__ __forceinline__ void walk_body(const Node *__restrict__ nodes, uint32_t N, WalkForm w,
+                                          float4 p, uint32_t b, float krd, float theta2, float &fx,
+                                          float &fy, float &fz) {
+    if (w.reduced && finite3(p)) {
+        if (w.fastdiv) walk_reduced<true>(nodes, 0u, N, p.x, p.y, p.z, b, krd, theta2, fx, fy, fz);
+        else walk_reduced<false>(nodes, 0u, N, p.x, p.y, p.z, b, krd, theta2, fx, fy, fz);
+    } else {
+        (void)walk_exact(nodes, 0u, N, p.x, p.y, p.z, b, krd, theta2, fx, fy, fz);
+    }
+}
+
+// Interactions of the walks over the current tree (accepted cells plus leaf
+// bodies, summed over all bodies) into fr->interactions.  A measurement hook:
+// the force kernels do not count.
+__global__ __launch_bounds__(kTB) void k_count_interactions(const Params P, Frame *fr,
+                                                            const float4 *__restrict__ pos,
+                                                            const uint32_t *__restrict__ deg,
+                                                            const uint32_t *__restrict__ perm,
+                                                            const Node *__restrict__ nodes) {
+    __shared__ unsigned long long inter;
+    if (threadIdx.x == 0) inter = 0ull;
+    __syncthreads();
+    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const uint32_t b = t < P.n ? perm[t] : P.n;
+    const uint32_t N = min(fr->n_tree, P.cap);
+    uint32_t taken = 0;
+    if (b < P.n && N > 0) {
+        const float4 p = pos[b];
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        taken = walk_exact(nodes, 0u, N, p.x, p.y, p.z, b, P.scaling_ratio * ((float)deg[b] + 1.0f),
+                           P.theta2, fx, fy, fz);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) taken += __shfl_xor(taken, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&inter, (unsigned long long)taken);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(&fr->interactions, inter);
+}
+
 // MODE 0: every lane follows its own rope (64 node gathers per step).
 // MODE 1: wave-union walk.  The wave steps through the union of its lanes'
 // walks in pre-order: the current node v is wave-uniform (one scalar load),
@@ -501,11 +539,8 @@ __global__ __launch_bounds__(kTB) void k_force(const Params P, Frame *fr,
                                                const Node *__restrict__ nodes,
                                                unsigned long long *__restrict__ partial) {
     __shared__ unsigned long long acc[2 * kLimbs];
-    __shared__ unsigned long long inter;
     if (threadIdx.x < 2 * kLimbs) acc[threadIdx.x] = 0ull;
-    if (threadIdx.x == 0) inter = 0ull;
     __syncthreads();
-    uint32_t taken = 0;
     const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const uint32_t b = t < P.n ? perm[t] : P.n;
     const bool live = b < P.n;
@@ -514,21 +549,11 @@ __global__ __launch_bounds__(kTB) void k_force(const Params P, Frame *fr,
     const float krd = P.scaling_ratio * deg_i;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     const uint32_t N = min(fr->n_tree, P.cap);
-    const uint64_t wcap = max((uint64_t)N * 4ull, 16ull);
     if (MODE == 0) {
-        if (live && N > 0) {
-            uint64_t walk = 0;
-            uint32_t idx = 0;
-            while (idx < N && walk < wcap) {
-                walk++;
-                const Node nd = nodes[idx];
-                bool leaf;
-                const bool take = visit(nd, p.x, p.y, p.z, b, krd, P.theta2, fx, fy, fz, leaf);
-                taken += take;
-                idx = (leaf || take) ? nd.b.y : idx + 1u;
-            }
-        }
+        const WalkForm w = walk_form(P, fr, N);
+        if (live && N > 0) walk_body(nodes, N, w, p, b, krd, P.theta2, fx, fy, fz);
     } else {
+        const uint64_t wcap = max((uint64_t)N * 4ull, 16ull);
         uint32_t ptr = (live && N > 0) ? 0u : kEnd;
         uint64_t walk = 0;
         uint32_t v = __builtin_amdgcn_readfirstlane(__reduce_min_sync(~0ull, ptr));
@@ -536,10 +561,8 @@ __global__ __launch_bounds__(kTB) void k_force(const Params P, Frame *fr,
             const Node nd = nodes[v];
             if (ptr == v) {
                 walk++;
-                bool leaf;
-                const bool take = visit(nd, p.x, p.y, p.z, b, krd, P.theta2, fx, fy, fz, leaf);
-                taken += take;
-                uint32_t nxt = (leaf || take) ? nd.b.y : v + 1u;
+                uint32_t nxt;
+                (void)visit_exact(nd, v, p.x, p.y, p.z, b, krd, P.theta2, fx, fy, fz, nxt);
                 if (nxt >= N || walk >= wcap) nxt = kEnd;
                 ptr = nxt;
             }
@@ -548,21 +571,18 @@ __global__ __launch_bounds__(kTB) void k_force(const Params P, Frame *fr,
         }
     }
     if (live) finish_body(P, b, p, deg_i, fx, fy, fz, pos, old_force, force, att_off, att_adj, acc);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) taken += __shfl_xor(taken, o, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&inter, (unsigned long long)taken);
     __syncthreads();
     // the block's limbs go into one of kAccCols columns of the limb-major
     // accumulator; integer adds, so the total is exact in any order
     if (threadIdx.x < 2 * kLimbs && acc[threadIdx.x])
         atomicAdd(&partial[threadIdx.x * kAccCols + blockIdx.x % kAccCols], acc[threadIdx.x]);
-    if (threadIdx.x == 0) atomicAdd(&fr->interactions, inter);
 }
 
 // K bodies per thread (K consecutive bodies in Morton order), their walks
 // interleaved step by step: each step issues the K node loads back to back, so
 // K independent pointer chains are in flight per lane instead of one.  Every
-// body still performs exactly its own walk in its own order.
+// body still performs exactly its own walk in its own order, in the reduced
+// form when the launch and all K bodies allow it.
 template <int K>
 __global__ __launch_bounds__(kTB) void k_force_multi(const Params P, Frame *fr,
                                                      const float4 *__restrict__ pos,
@@ -575,16 +595,11 @@ __global__ __launch_bounds__(kTB) void k_force_multi(const Params P, Frame *fr,
                                                      const Node *__restrict__ nodes,
                                                      unsigned long long *__restrict__ partial) {
     __shared__ unsigned long long acc[2 * kLimbs];
-    __shared__ unsigned long long inter;
     if (threadIdx.x < 2 * kLimbs) acc[threadIdx.x] = 0ull;
-    if (threadIdx.x == 0) inter = 0ull;
     __syncthreads();
-    uint32_t taken = 0;
     const uint32_t t0 = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) * K;
     const uint32_t N = min(fr->n_tree, P.cap);
-    const uint64_t wcap = max((uint64_t)N * 4ull, 16ull);
     uint32_t b[K], idx[K];
-    uint64_t walk[K];
     float4 p[K];
     float deg_i[K], krd[K], fx[K], fy[K], fz[K];
 #pragma unroll
@@ -597,26 +612,35 @@ __global__ __launch_bounds__(kTB) void k_force_multi(const Params P, Frame *fr,
         krd[k] = P.scaling_ratio * deg_i[k];
         fx[k] = fy[k] = fz[k] = 0.0f;
         idx[k] = (live && N > 0) ? 0u : kEnd;
-        walk[k] = 0;
     }
-    for (;;) {
-        bool active = false;
+    const WalkForm w = walk_form(P, fr, N);
+    bool reduced = w.reduced, live[K];
 #pragma unroll
-        for (int k = 0; k < K; k++) active |= idx[k] < N;
-        if (!active) break;
-        Node nd[K];
+    for (int k = 0; k < K; k++) {
+        live[k] = idx[k] < N;
+        reduced = reduced && finite3(p[k]);
+    }
+    if (reduced) {
+        if (w.fastdiv) walk_reduced_multi<K, true>(nodes, N, live, p, b, krd, P.theta2, fx, fy, fz);
+        else walk_reduced_multi<K, false>(nodes, N, live, p, b, krd, P.theta2, fx, fy, fz);
+    } else {
+        // ropes only move forward: at most N steps per walk
+        for (uint32_t steps = 0; steps < N; steps++) {
+            bool active = false;
 #pragma unroll
-        for (int k = 0; k < K; k++) nd[k] = nodes[idx[k] < N ? idx[k] : 0u];
+            for (int k = 0; k < K; k++) active |= idx[k] < N;
+            if (!active) break;
+            Node nd[K];
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            if (idx[k] < N) {
-                bool leaf;
-                const bool take =
-                    visit(nd[k], p[k].x, p[k].y, p[k].z, b[k], krd[k], P.theta2, fx[k], fy[k], fz[k], leaf);
-                taken += take;
-                uint32_t nxt = (leaf || take) ? nd[k].b.y : idx[k] + 1u;
-                if (++walk[k] >= wcap) nxt = kEnd;
-                idx[k] = nxt;
+            for (int k = 0; k < K; k++) nd[k] = nodes[idx[k] < N ? idx[k] : 0u];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                if (idx[k] < N) {
+                    uint32_t nxt;
+                    (void)visit_exact(nd[k], idx[k], p[k].x, p[k].y, p[k].z, b[k], krd[k],
+                                      P.theta2, fx[k], fy[k], fz[k], nxt);
+                    idx[k] = nxt;
+                }
             }
         }
     }
@@ -625,13 +649,9 @@ __global__ __launch_bounds__(kTB) void k_force_multi(const Params P, Frame *fr,
         if (b[k] < P.n)
             finish_body(P, b[k], p[k], deg_i[k], fx[k], fy[k], fz[k], pos, old_force, force, att_off,
                         att_adj, acc);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) taken += __shfl_xor(taken, o, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&inter, (unsigned long long)taken);
     __syncthreads();
     if (threadIdx.x < 2 * kLimbs && acc[threadIdx.x])
         atomicAdd(&partial[threadIdx.x * kAccCols + blockIdx.x % kAccCols], acc[threadIdx.x]);
-    if (threadIdx.x == 0) atomicAdd(&fr->interactions, inter);
 }
 
 // Attraction rows longer than kHeavyAtt.  Their terms d*att are independent
@@ -1049,6 +1069,8 @@ class Fa2BhEngine final : public LayoutEngine {
         JC_TRY(write_vec4(tmp.p, xyz));
         JC_TRY(enqueue_tree(tmp.p, nullptr));
         JC_TRY(enqueue_force(tmp.p, nullptr));
+        k_count_interactions<<<cdiv(n_, kTB), kTB, 0, stream_>>>(params_, frame_.p, tmp.p, deg_.p,
+                                                                 vals_.p, nodes_.p);
         k_speed<<<1, kSpeedThreads, 0, stream_>>>(params_, frame_.p, partial_.p, 0u);
         JC_HIP(hipGetLastError());
         if (out_force) JC_TRY(read_vec4(force_.p, out_force));
@@ -1080,8 +1102,9 @@ class Fa2BhEngine final : public LayoutEngine {
             JC_HIP(hipMemcpy(h.data(), nodes_.p, m * sizeof(Node), hipMemcpyDeviceToHost));
             for (uint32_t q = 0; q < m; q++) {
                 if (node_a) std::memcpy(node_a + 4 * (size_t)q, &h[q].a, sizeof(float4));
-                if (info) info[q] = h[q].b.x;
-                if (skip) skip[q] = h[q].b.y;
+                // exported form: leaf 0x80000000 | body, cell the bits of (2*half)^2
+                if (info) info[q] = h[q].b.z != kCellId ? (kLeaf | h[q].b.z) : h[q].b.y;
+                if (skip) skip[q] = h[q].b.w;
             }
             if (depth) JC_HIP(hipMemcpy(depth, node_depth_.p, m, hipMemcpyDeviceToHost));
         }
@@ -1170,6 +1193,15 @@ class Fa2BhEngine final : public LayoutEngine {
             }
         }
         for (auto &e : ev) (void)hipEventDestroy(e);
+        if (n_iters && n_) {
+            // interactions of the last profiled iteration: its tree is still in
+            // nodes_, built from the positions that iteration read
+            k_count_interactions<<<cdiv(n_, kTB), kTB, 0, stream_>>>(params_, frame_.p,
+                                                                     pos_[cur_ ^ 1].p, deg_.p,
+                                                                     vals_.p, nodes_.p);
+            JC_HIP(hipGetLastError());
+            JC_HIP(hipStreamSynchronize(stream_));
+        }
         for (uint32_t k = 0; k < n_phases && k < PH_N; k++)
             phase_ms[k] = n_iters ? acc[k] / n_iters : 0.0;
         return Status::ok();
@@ -1246,6 +1278,7 @@ class Fa2BhEngine final : public LayoutEngine {
         params_.est_jt = 0.05 * std::sqrt((double)n_);
         params_.min_jt = std::sqrt(params_.est_jt);
         params_.cap = (uint32_t)cap_;
+        params_.fastdiv = kr_in_fastdiv_range(s.scaling_ratio) ? 1u : 0u;
         params_dirty_ = false;
         drop_graphs();  // kernel arguments changed
     }

@@ -87,6 +87,40 @@ def test_tree_and_forces_bit_exact(name, off, nb, xyz, walk):
     assert np.array_equal(sums_gpu.view(np.uint64), want.view(np.uint64))
 
 
+@pytest.mark.parametrize("form", ["reduced_fastdiv", "reduced_ieee_div", "reference"])
+def test_walk_forms_bit_exact(form):
+    """The walk runs in one of three forms, chosen per launch and per body: the
+    reduced walk with the 8-instruction division (scaling ratio in [2^-10, 2^10],
+    root half-size below 2^40), the reduced walk with IEEE division (scaling
+    ratio 2048), and the reference-form walk (root half-size above 2^40, here
+    positions of order 3e12).  Each gives the oracle's forces and sums bit for
+    bit."""
+    off, nb = O.ba_graph(5000, 3, 9)
+    n = len(off) - 1
+    xyz = O.uniform_cube(n, 10.0, 10)
+    kr = 2.0
+    if form == "reduced_ieee_div":
+        kr = 2048.0
+    elif form == "reference":
+        xyz = (xyz * np.float32(2.0 ** 38)).astype(np.float32)
+    e = _engine(off, nb, xyz, theta=0.8, scaling_ratio=kr)
+    assert e.walk_mode(0) == 0
+    f_gpu, sums_gpu = e.forces_at(xyz)
+    g = e.tree()
+    if form == "reference":
+        assert g["root4"][3] >= 2.0 ** 40
+    else:
+        assert g["root4"][3] < 2.0 ** 40
+    t = O.octree_morton(xyz, _mass(off, nb))
+    _tree_equal(g, t)
+    deg, aoff, aadj = O.fa2_prepare(n, off, nb)
+    f_cpu, stats = O.fa2_forces(xyz, None, deg, aoff, aadj, t,
+                                _osettings(theta=0.8, scaling_ratio=kr))
+    assert np.array_equal(f_gpu.view(np.uint32), f_cpu.view(np.uint32)), \
+        f"max |diff| {np.abs(f_gpu - f_cpu).max()}"
+    assert np.array_equal(sums_gpu.view(np.uint64), O.exact_sum2(stats).view(np.uint64))
+
+
 @pytest.mark.parametrize("theta", [0.0, 0.8, 1.2])
 def test_forces_within_1e4_of_fp64(theta):
     """Per-node |F32 - F64| / max(|F64|, floor) <= 1e-4 with floor = 1e-3 * rms|F64|,
