@@ -142,11 +142,12 @@ int jc_fa2bh_step_timed(jc_engine *e, uint32_t n_iters, double *force_ms_sum,
  * in the last iteration run by jc_fa2bh_profile or jc_fa2bh_forces_at (plain
  * steps do not count them: the count is a separate walk, 0 after a plain step). */
 int jc_fa2bh_last_stats(jc_engine *e, uint64_t *n_tree, uint64_t *interactions, double *sums2);
-/* Octree walk of the force kernel: 0 = every lane follows its own rope,
- * 1 = wave-union walk (uniform node loads), 2 / 3 = each lane interleaves the
- * walks of 2 / 4 Morton-adjacent bodies.  All give bit-identical forces; by
- * default the engine times them at its first step and keeps the fastest.
- * mode 0..3 forces one, -1 re-times; *current receives the mode in use. */
+/* Octree walk of the force kernel: 0 = every lane follows its own rope (the
+ * top three octree levels read from LDS), 1 = wave-union walk (uniform node
+ * loads), 2 / 3 = each lane interleaves the walks of 2 / 4 Morton-adjacent
+ * bodies, 4 / 5 = mode 0 in 512- / 1024-thread blocks.  All give bit-identical
+ * forces; by default the engine times them at its first step and keeps the
+ * fastest.  mode 0..5 forces one, -1 re-times; *current receives the mode in use. */
 int jc_fa2bh_walk_mode(jc_engine *e, int mode, int *current);
 /* Eager (uncaptured) iterations with events around every phase; phase_ms[k]
  * receives the mean ms of phase k over n_iters.  Phase names: jc_fa2bh_phase_name. */

@@ -288,6 +288,111 @@ __device__ __forceinline__ void walk_reduced_multi(const Node *__restrict__ node
     }
 }
 
+// ---- top of the tree in LDS ----
+// The nodes of depth <= kTopDepth (at most kTopSlots of them) are visited by
+// nearly every walk.  They are gathered once per tree build into a compact
+// pre-order array ("top tree") that each force block copies into LDS, so those
+// visits read LDS instead of going through the vector-memory address path.
+// A node's place in the gathering is its slot in the pre-order enumeration of
+// the complete octree of depth kTopDepth: slot(root) = 0 and a child with
+// octant o of a node at depth j-1 sits at slot(parent) + 1 + o * top_sub(j),
+// where top_sub(j) is the size of a complete subtree rooted at depth j.
+constexpr int kTopDepth = 3;
+constexpr uint32_t kTopSlots = 585;  // 1 + 8 + 64 + 512
+constexpr uint32_t kTopAbsent = 0xFFFFFFFFu;
+__host__ __device__ constexpr uint32_t top_sub(int j) {
+    return j > kTopDepth ? 0u : (j == kTopDepth ? 1u : 1u + 8u * top_sub(j + 1));
+}
+static_assert(top_sub(0) == kTopSlots, "complete octree of depth kTopDepth");
+// Slot of the depth-d (d <= kTopDepth) prefix of a key.
+__device__ __forceinline__ uint32_t top_slot_of(uint64_t key, int d) {
+    uint32_t s = 0;
+    for (int j = 1; j <= d; j++) s += 1u + ((uint32_t)(key >> (3 * (kLevels - j))) & 7u) * top_sub(j);
+    return s;
+}
+// Depth of a slot of the enumeration.
+__device__ __forceinline__ int top_slot_depth(uint32_t s) {
+    int d = 0;
+    while (s != 0u && d < kTopDepth) {
+        d++;
+        s = (s - 1u) % top_sub(d);
+    }
+    return d;
+}
+// Top-tree record b: x = opening size bits, y = id (as Node::b.y, b.z),
+// z = rank of the first top node after the subtree | 0x80000000 when the node
+// is a cell of depth kTopDepth (its children live only in the global array),
+// w = the node's byte offset in the global array.
+constexpr uint32_t kTopFrontier = 0x80000000u;
+
+// The walk of one body with the top tree read from LDS (tA, tB: K records)
+// and everything below it from the global array (nodes [0, N)); the node
+// sequence, the operations and the sums are walk_reduced's.  In the top tree
+// the walk moves by rank: a taken node or a leaf goes to its subtree's end
+// rank, an opened cell above the frontier to rank + 1, an opened frontier cell
+// walks its subtree in the global array (from the cell's offset + 32 up to the
+// next top node's offset) and then resumes at its subtree's end rank.  Callers
+// guarantee what walk_reduced needs (tree_reducible, finite body).
+template <bool FASTDIV>
+__device__ __forceinline__ void walk_top(const float4 *tA, const uint4 *tB, uint32_t K,
+                                         const Node *__restrict__ nodes, uint32_t N, float px,
+                                         float py, float pz, uint32_t b, float krd, float theta2,
+                                         float &fx, float &fy, float &fz) {
+    const uint32_t end_b = N << 5;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<Node *>(nodes), (short)0, (int)end_b,
+                                          kBufferRsrcWord3);
+    bool in_top = true;
+    uint32_t u = 0, off = 0, sub_end = 0;
+    for (uint32_t steps = 0; steps < N + K; steps++) {
+        float4 a;
+        uint32_t size, id, link;
+        if (in_top) {
+            if (u >= K) break;
+            a = tA[u];
+            const uint4 q = tB[u];
+            size = q.x;
+            id = q.y;
+            link = q.z;
+        } else {
+            const auto av = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+            const auto bv = __builtin_amdgcn_raw_buffer_load_b128(rs, off + 16u, 0, 0);
+            a = make_float4(__uint_as_float(av[0]), __uint_as_float(av[1]), __uint_as_float(av[2]),
+                            __uint_as_float(av[3]));
+            size = bv[1];
+            id = bv[2];
+            link = bv[0];
+        }
+        const float dx = px - a.x, dy = py - a.y, dz = pz - a.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const bool take = __uint_as_float(size) < theta2 * r2 && id != b;
+        if (take) {
+            const float r2c = __uint_as_float(max(__float_as_uint(r2), 0x38D1B717u));  // 1e-4f
+            const float cf = FASTDIV ? div_in_range(krd * a.w, r2c) : krd * a.w / r2c;
+            fx = fx + dx * cf;
+            fy = fy + dy * cf;
+            fz = fz + dz * cf;
+        }
+        if (in_top) {
+            const uint32_t ts = link & ~kTopFrontier;
+            if (take) {
+                u = ts;
+            } else if (link & kTopFrontier) {
+                off = tB[u].w + 32u;
+                sub_end = ts < K ? tB[ts].w : end_b;
+                u = ts;
+                in_top = false;
+            } else {
+                u = u + 1u;
+            }
+        } else {
+            const uint32_t next = take ? link : off + 32u;
+            if (next >= sub_end) in_top = true;
+            else off = next;
+        }
+    }
+}
+
 // Octant path of p through the reference cell chain for kLevels levels: digit
 // (x>=cx) | (y>=cy)<<1 | (z>=cz)<<2, child half = half*0.5, child centre =
 // centre +- child half (fa2_bh.rs:841-861).

@@ -42,7 +42,8 @@ namespace {
 constexpr uint32_t kTB = 256;             // threads per block for per-node kernels
 constexpr uint32_t kScanItems = 16;       // count/scan tile = 4096 bodies
 constexpr uint32_t kScanTile = kTB * kScanItems;
-constexpr int kWalkModes = 4;             // lane walk, wave-union, 2 and 4 chains per lane
+// lane walk, wave-union, 2 and 4 chains per lane, lane walk in 512- and 1024-thread blocks
+constexpr int kWalkModes = 6;
 constexpr uint32_t kTimedIters = 10;      // timed iterations per host read-back of events
 constexpr uint32_t kHeavyAtt = 32;        // attraction rows longer than this go to k_force_heavy
 constexpr uint32_t kHeavyStage = 1024;    // heavy-row terms staged in LDS per round
@@ -64,7 +65,7 @@ struct Frame {
     double sum_swing;
     double sum_traction;
     uint32_t max_depth;
-    uint32_t walk_cap_hits;
+    uint32_t n_top;                   // nodes in the top tree (depth <= kTopDepth)
     unsigned long long interactions;  // accepted cells + leaf bodies summed by the walk
 };
 
@@ -91,9 +92,12 @@ struct Params {
 
 // ---------------------------------------------------------------- kernels
 // Also clears the limb accumulator the force kernels add into (every force
-// launch follows a tree build, which starts with this kernel).
-__global__ void k_frame_reset(Frame *fr, unsigned long long *__restrict__ acc_cols) {
+// launch follows a tree build, which starts with this kernel) and the top-tree
+// slot table that k_emit fills.
+__global__ void k_frame_reset(Frame *fr, unsigned long long *__restrict__ acc_cols,
+                              uint32_t *__restrict__ top_slot) {
     for (uint32_t q = threadIdx.x; q < 2 * kLimbs * kAccCols; q += blockDim.x) acc_cols[q] = 0ull;
+    for (uint32_t q = threadIdx.x; q < kTopSlots; q += blockDim.x) top_slot[q] = kTopAbsent;
     if (threadIdx.x == 0) {
         const uint32_t pinf = 0xFF800000u;  // f2ord(+inf)
         for (int k = 0; k < 3; k++) {
@@ -271,7 +275,9 @@ __device__ __forceinline__ void emit_cell(const uint64_t *__restrict__ keys, uin
                                           const uint32_t *__restrict__ loc,
                                           const uint32_t *__restrict__ blk,
                                           Node *__restrict__ nodes,
-                                          uint8_t *__restrict__ node_depth) {
+                                          uint8_t *__restrict__ node_depth,
+                                          uint32_t *__restrict__ top_slot) {
+    if (d <= kTopDepth) top_slot[top_slot_of(keys[i], d)] = node;
     float h = fr->root[3];
     for (int q = 0; q < d; q++) h = h * 0.5f;
     const uint32_t j = cell_end(keys, i, nv, keys[i], d);
@@ -295,7 +301,8 @@ __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
                                               Frame *fr, const uint32_t *__restrict__ loc,
                                               const uint32_t *__restrict__ blk,
                                               Node *__restrict__ nodes,
-                                              uint8_t *__restrict__ node_depth, uint64_t cap) {
+                                              uint8_t *__restrict__ node_depth, uint64_t cap,
+                                              uint32_t *__restrict__ top_slot) {
     __shared__ uint32_t bdepth, ntask;
     __shared__ uint32_t t_node[kEmitTasks], t_body[kEmitTasks];
     __shared__ uint8_t t_depth[kEmitTasks];
@@ -328,7 +335,8 @@ __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
                         t_body[slot + t] = i;
                         t_depth[slot + t] = (uint8_t)(a0 + (int)t);
                     } else {
-                        emit_cell(keys, nv, i, a0 + (int)t, pos0 + t, fr, loc, blk, nodes, node_depth);
+                        emit_cell(keys, nv, i, a0 + (int)t, pos0 + t, fr, loc, blk, nodes, node_depth,
+                                  top_slot);
                     }
                 }
                 atomicMax(&bdepth, (uint32_t)(a0 + (int)c - 1));
@@ -342,13 +350,15 @@ __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
                 nodes[leaf].b = leaf_b(b, (i + 1 < nv) ? leaf + 1 : kEnd);
                 const int ld = (lp > li ? lp : li) + 1;
                 node_depth[leaf] = (uint8_t)(0x80 | ld);
+                if (ld <= kTopDepth) top_slot[top_slot_of(k, ld)] = leaf;
             }
         }
     }
     __syncthreads();
     const uint32_t nt = min(ntask, kEmitTasks);
     for (uint32_t q = threadIdx.x; q < nt; q += blockDim.x)
-        emit_cell(keys, nv, t_body[q], t_depth[q], t_node[q], fr, loc, blk, nodes, node_depth);
+        emit_cell(keys, nv, t_body[q], t_depth[q], t_node[q], fr, loc, blk, nodes, node_depth,
+                  top_slot);
     if (threadIdx.x == 0 && bdepth) atomicMax(&fr->max_depth, bdepth);
 }
 
@@ -377,6 +387,44 @@ __global__ __launch_bounds__(kTB) void k_com_level(const Frame *fr, uint32_t d, 
         }
         nodes[p].a = make_float4(cx, cy, cz, tm);
     }
+}
+
+// Gathers the top tree (the nodes of depth <= kTopDepth, in pre-order) from
+// the slot table that k_emit filled, once the centres of mass are final.  One
+// block: an exclusive scan over slot presence gives each node its rank, and a
+// node's subtree ends at the first present slot after its complete subtree.
+__global__ __launch_bounds__(1024) void k_top_build(Frame *fr, const uint32_t *__restrict__ top_slot,
+                                                    const Node *__restrict__ nodes,
+                                                    float4 *__restrict__ top_a,
+                                                    uint4 *__restrict__ top_b) {
+    static_assert(kTopSlots < 1024, "one thread per slot");
+    __shared__ uint32_t rk[kTopSlots + 1];
+    __shared__ uint32_t ws[16];
+    const uint32_t s = threadIdx.x, lane = s & 63, w = s >> 6;
+    const uint32_t g = s < kTopSlots ? top_slot[s] : kTopAbsent;
+    const uint32_t e = g != kTopAbsent ? 1u : 0u;
+    uint32_t incl = e;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o, 64);
+        if ((int)lane >= o) incl += t;
+    }
+    if (lane == 63) ws[w] = incl;
+    __syncthreads();
+    uint32_t wp = 0;
+    for (uint32_t q = 0; q < w; q++) wp += ws[q];
+    if (s <= kTopSlots) rk[s] = wp + incl - e;
+    __syncthreads();
+    if (e) {
+        const uint32_t t = rk[s];
+        const int d = top_slot_depth(s);
+        const Node nd = nodes[g];
+        const uint32_t ts = rk[s + top_sub(d)];
+        const bool frontier = d == kTopDepth && nd.b.z == kCellId;
+        top_a[t] = nd.a;
+        top_b[t] = make_uint4(nd.b.y, nd.b.z, ts | (frontier ? kTopFrontier : 0u), g << 5);
+    }
+    if (s == kTopSlots) fr->n_top = rk[kTopSlots];
 }
 
 // Attraction, gravity and the swing/traction stats of body b, after its walk
@@ -479,19 +527,6 @@ __device__ __forceinline__ WalkForm walk_form(const Params &P, const Frame *fr, 
     return w;
 }
 
-// The walk of body b over nodes [0, N) (fa2_barnes_hut.wgsl:108-156), in the
-// reduced form where that is exact for this body.
-__device__ __forceinline__ void walk_body(const Node *__restrict__ nodes, uint32_t N, WalkForm w,
-                                          float4 p, uint32_t b, float krd, float theta2, float &fx,
-                                          float &fy, float &fz) {
-    if (w.reduced && finite3(p)) {
-        if (w.fastdiv) walk_reduced<true>(nodes, 0u, N, p.x, p.y, p.z, b, krd, theta2, fx, fy, fz);
-        else walk_reduced<false>(nodes, 0u, N, p.x, p.y, p.z, b, krd, theta2, fx, fy, fz);
-    } else {
-        (void)walk_exact(nodes, 0u, N, p.x, p.y, p.z, b, krd, theta2, fx, fy, fz);
-    }
-}
-
 // Interactions of the walks over the current tree (accepted cells plus leaf
 // bodies, summed over all bodies) into fr->interactions.  A measurement hook:
 // the force kernels do not count.
@@ -520,15 +555,15 @@ __global__ __launch_bounds__(kTB) void k_count_interactions(const Params P, Fram
     if (threadIdx.x == 0) atomicAdd(&fr->interactions, inter);
 }
 
-// MODE 0: every lane follows its own rope (64 node gathers per step).
+// MODE 0: every lane follows its own rope, the top tree from LDS.
 // MODE 1: wave-union walk.  The wave steps through the union of its lanes'
 // walks in pre-order: the current node v is wave-uniform (one scalar load),
 // lanes whose own walk is at v process it, the others wait.  Each lane still
 // sees exactly its own node sequence in its own order, so the sums are
 // bit-identical to MODE 0; the next v is v+1 when some lane descended, else
 // the minimum over the lanes' rope pointers.
-template <int MODE>
-__global__ __launch_bounds__(kTB) void k_force(const Params P, Frame *fr,
+template <int MODE, int BS = kTB>
+__global__ __launch_bounds__(BS) void k_force(const Params P, Frame *fr,
                                                const float4 *__restrict__ pos,
                                                const float4 *__restrict__ old_force,
                                                float4 *__restrict__ force,
@@ -537,7 +572,9 @@ __global__ __launch_bounds__(kTB) void k_force(const Params P, Frame *fr,
                                                const uint32_t *__restrict__ att_adj,
                                                const uint32_t *__restrict__ perm,
                                                const Node *__restrict__ nodes,
-                                               unsigned long long *__restrict__ partial) {
+                                               unsigned long long *__restrict__ partial,
+                                               const float4 *__restrict__ top_a,
+                                               const uint4 *__restrict__ top_b) {
     __shared__ unsigned long long acc[2 * kLimbs];
     if (threadIdx.x < 2 * kLimbs) acc[threadIdx.x] = 0ull;
     __syncthreads();
@@ -549,9 +586,27 @@ __global__ __launch_bounds__(kTB) void k_force(const Params P, Frame *fr,
     const float krd = P.scaling_ratio * deg_i;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     const uint32_t N = min(fr->n_tree, P.cap);
-    if (MODE == 0) {
+    if constexpr (MODE == 0) {
+        // the top tree goes to LDS; each lane then walks it and the global array
+        __shared__ float4 tA[kTopSlots];
+        __shared__ uint4 tB[kTopSlots];
         const WalkForm w = walk_form(P, fr, N);
-        if (live && N > 0) walk_body(nodes, N, w, p, b, krd, P.theta2, fx, fy, fz);
+        const uint32_t K = w.reduced ? min(fr->n_top, kTopSlots) : 0u;
+        for (uint32_t q = threadIdx.x; q < K; q += blockDim.x) {
+            tA[q] = top_a[q];
+            tB[q] = top_b[q];
+        }
+        __syncthreads();
+        if (live && N > 0) {
+            if (w.reduced && finite3(p)) {
+                if (w.fastdiv)
+                    walk_top<true>(tA, tB, K, nodes, N, p.x, p.y, p.z, b, krd, P.theta2, fx, fy, fz);
+                else
+                    walk_top<false>(tA, tB, K, nodes, N, p.x, p.y, p.z, b, krd, P.theta2, fx, fy, fz);
+            } else {
+                (void)walk_exact(nodes, 0u, N, p.x, p.y, p.z, b, krd, P.theta2, fx, fy, fz);
+            }
+        }
     } else {
         const uint64_t wcap = max((uint64_t)N * 4ull, 16ull);
         uint32_t ptr = (live && N > 0) ? 0u : kEnd;
@@ -949,6 +1004,9 @@ class Fa2BhEngine final : public LayoutEngine {
         cap_ = nn + (uint64_t)(kLevels + 1) * (nn / 2 + 1) + 8;
         JC_TRY(nodes_.alloc(cap_));
         JC_TRY(node_depth_.alloc(cap_));
+        JC_TRY(top_slot_.alloc(kTopSlots));
+        JC_TRY(top_a_.alloc(kTopSlots));
+        JC_TRY(top_b_.alloc(kTopSlots));
         nb_force_ = cdiv(nn, kTB);
         JC_TRY(partial_.alloc((size_t)kAccCols * 2 * kLimbs));
         JC_TRY(heavy_.alloc(n_heavy_ ? n_heavy_ : 1));
@@ -1224,7 +1282,12 @@ class Fa2BhEngine final : public LayoutEngine {
 
   private:
     static uint32_t chains_of(int mode) { return mode == 2 ? 2u : mode == 3 ? 4u : 1u; }
-    uint32_t force_grid(int mode) const { return cdiv(cdiv(n_ ? n_ : 1, chains_of(mode)), kTB); }
+    // modes 4 and 5 are the lane walk in bigger blocks: fewer, larger Morton
+    // ranges share a CU, so its L1 serves more of their common nodes
+    static uint32_t block_of(int mode) { return mode == 4 ? 512u : mode == 5 ? 1024u : kTB; }
+    uint32_t force_grid(int mode) const {
+        return cdiv(cdiv(n_ ? n_ : 1, chains_of(mode)), block_of(mode));
+    }
     uint32_t force_grid_now() const { return force_grid(walk_mode_); }
 
     // Picks the fastest walk on the current positions (all give bit-identical
@@ -1307,7 +1370,7 @@ class Fa2BhEngine final : public LayoutEngine {
     Status enqueue_tree(const float4 *pos, hipEvent_t *ev) {
         const uint32_t n = n_;
         if (ev) JC_HIP(hipEventRecord(ev[PH_BBOX], stream_));
-        k_frame_reset<<<1, 1024, 0, stream_>>>(frame_.p, partial_.p);
+        k_frame_reset<<<1, 1024, 0, stream_>>>(frame_.p, partial_.p, top_slot_.p);
         k_bbox<<<std::min<uint32_t>(cdiv(n, kTB), 256), kTB, 0, stream_>>>(pos, n, frame_.p);
         if (ev) JC_HIP(hipEventRecord(ev[PH_KEYS], stream_));
         k_keys<<<cdiv(n, kTB), kTB, 0, stream_>>>(pos, n, frame_.p, keys_.p, vals_.p);
@@ -1318,12 +1381,14 @@ class Fa2BhEngine final : public LayoutEngine {
         k_scan_top<<<1, 1024, 0, stream_>>>(blk_.p, nb_scan_, n, frame_.p);
         if (ev) JC_HIP(hipEventRecord(ev[PH_EMIT], stream_));
         k_emit<<<cdiv(n, kTB), kTB, 0, stream_>>>(keys_.p, vals_.p, pos, deg_.p, n, frame_.p,
-                                                  loc_.p, blk_.p, nodes_.p, node_depth_.p, cap_);
+                                                  loc_.p, blk_.p, nodes_.p, node_depth_.p, cap_,
+                                                  top_slot_.p);
         if (ev) JC_HIP(hipEventRecord(ev[PH_COM], stream_));
         const uint32_t gcom = std::min<uint32_t>(cdiv(cap_, kTB), 4096);
         for (int d = kLevels; d >= 0; d--)
             k_com_level<<<gcom, kTB, 0, stream_>>>(frame_.p, (uint32_t)d, (uint32_t)cap_,
                                                    nodes_.p, node_depth_.p);
+        k_top_build<<<1, 1024, 0, stream_>>>(frame_.p, top_slot_.p, nodes_.p, top_a_.p, top_b_.p);
         JC_HIP(hipGetLastError());
         return Status::ok();
     }
@@ -1334,14 +1399,18 @@ class Fa2BhEngine final : public LayoutEngine {
         if (n_heavy_)
             k_heavy_terms<<<cdiv(n_hent_, kTB), kTB, 0, stream_>>>(params_, n_hent_, hbody_.p,
                                                                   hadj_.p, pos, hterm_.p, hstride_);
-#define JC_FORCE(KERNEL)                                                                    \
-    KERNEL<<<grid, kTB, 0, stream_>>>(params_, frame_.p, pos, old_force_.p, force_.p, deg_.p, \
-                                      att_off_.p, att_adj_.p, vals_.p, nodes_.p, partial_.p)
+        const uint32_t bs = block_of(walk_mode_);
+#define JC_FORCE(KERNEL, ...)                                                              \
+    KERNEL<<<grid, bs, 0, stream_>>>(params_, frame_.p, pos, old_force_.p, force_.p, deg_.p, \
+                                     att_off_.p, att_adj_.p, vals_.p, nodes_.p, partial_.p   \
+                                     __VA_ARGS__)
         switch (walk_mode_) {
-            case 0: JC_FORCE(k_force<0>); break;
+            case 0: JC_FORCE(k_force<0>, , top_a_.p, top_b_.p); break;
             case 2: JC_FORCE(k_force_multi<2>); break;
             case 3: JC_FORCE(k_force_multi<4>); break;
-            default: JC_FORCE(k_force<1>); break;
+            case 4: JC_FORCE((k_force<0, 512>), , top_a_.p, top_b_.p); break;
+            case 5: JC_FORCE((k_force<0, 1024>), , top_a_.p, top_b_.p); break;
+            default: JC_FORCE(k_force<1>, , top_a_.p, top_b_.p); break;
         }
 #undef JC_FORCE
         if (n_heavy_)
@@ -1434,6 +1503,9 @@ class Fa2BhEngine final : public LayoutEngine {
         blk_.release();
         nodes_.release();
         node_depth_.release();
+        top_slot_.release();
+        top_a_.release();
+        top_b_.release();
         partial_.release();
         heavy_.release();
         hoff_.release();
@@ -1467,7 +1539,7 @@ class Fa2BhEngine final : public LayoutEngine {
     uint64_t cap_ = 0;
     uint32_t nb_scan_ = 0, nb_force_ = 0;
     int walk_mode_ = -1;  // -1: pick at the first step by timing both walks
-    float tune_ms_[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float tune_ms_[kWalkModes] = {};
     hipStream_t stream_ = nullptr;
     DevBuf<float4> pos_[2], old_force_, force_;
     DevBuf<uint32_t> deg_, att_off_, att_adj_;
@@ -1476,6 +1548,9 @@ class Fa2BhEngine final : public LayoutEngine {
     RadixScratch sort_;
     DevBuf<Node> nodes_;
     DevBuf<uint8_t> node_depth_;
+    DevBuf<uint32_t> top_slot_;   // kTopSlots: node of each slot of the top-tree enumeration
+    DevBuf<float4> top_a_;        // top tree in pre-order: centre of mass, mass
+    DevBuf<uint4> top_b_;         // and size, id, subtree end rank | frontier, byte offset
     DevBuf<unsigned long long> partial_;
     DevBuf<Frame> frame_;
     hipGraph_t graph_ = nullptr, tree_graph_[2] = {};

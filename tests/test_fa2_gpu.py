@@ -67,7 +67,8 @@ def _tree_equal(g, t):
     assert np.array_equal(g["depth"], want_depth)
 
 
-@pytest.mark.parametrize("walk", [0, 1, 2, 3], ids=["lane", "union", "chains2", "chains4"])
+@pytest.mark.parametrize("walk", [0, 1, 2, 3, 4, 5],
+                         ids=["lane", "union", "chains2", "chains4", "lane512", "lane1024"])
 @pytest.mark.parametrize("name,off,nb,xyz", CASES, ids=[c[0] for c in CASES])
 def test_tree_and_forces_bit_exact(name, off, nb, xyz, walk):
     n = len(off) - 1
@@ -146,7 +147,7 @@ def test_forces_within_1e4_of_fp64(theta):
     assert rel.max() <= 1e-4
 
 
-@pytest.mark.parametrize("walk", [0, 1, 2, 3])
+@pytest.mark.parametrize("walk", [0, 1, 2, 3, 4, 5])
 def test_trajectory_bit_exact_sbm2000(walk):
     """50 iterations: positions, F_{t-1} and the speed state bit-exact with the
     oracle driving the same (Morton) octree and the exact f64 sums."""
@@ -316,7 +317,7 @@ def test_ba_1m_tree_and_force_parity_at_full_size():
     n = len(off) - 1
     e = _engine(off, nb, xyz, theta=0.8)
     fs = []
-    for mode in (0, 1, 2, 3):
+    for mode in (0, 1, 2, 3, 4, 5):
         e.walk_mode(mode)
         fs.append(e.forces_at(xyz))
     f, sums = fs[0]
