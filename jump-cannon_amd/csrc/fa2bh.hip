@@ -15,7 +15,9 @@
 //               the skip rope (first node after a subtree) found by a galloping
 //               search on the sorted keys
 //   com         centre of mass level by level from the deepest cells, children
-//               in octant order (fa2_bh.rs:925-967): no float atomics
+//               in octant order (fa2_bh.rs:925-967): no float atomics; the top
+//               levels (depth <= 3) in one block, which also sets their ropes
+//               and gathers them into the top tree the walk reads from LDS
 //   force       rope walk (fa2_barnes_hut.wgsl:108-156), attraction over the
 //               reference's edge-scan order (:158-181), gravity, stats; bodies
 //               processed in Morton order so a wave walks one region of the tree
@@ -66,6 +68,7 @@ struct Frame {
     double sum_traction;
     uint32_t max_depth;
     uint32_t n_top;                   // nodes in the top tree (depth <= kTopDepth)
+    uint32_t n_valid;                 // finite bodies of the last tree build
     unsigned long long interactions;  // accepted cells + leaf bodies summed by the walk
 };
 
@@ -259,6 +262,7 @@ __global__ __launch_bounds__(1024) void k_scan_top(uint32_t *__restrict__ blk, u
     if (threadIdx.x == 0) {
         const uint32_t nv = n - fr->n_invalid;
         fr->n_tree = nv ? tot + nv : 0u;
+        fr->n_valid = nv;
     }
 }
 
@@ -269,31 +273,35 @@ __device__ __forceinline__ uint32_t first_node_of(const uint32_t *loc, const uin
 
 // One internal cell: depth d, opened by sorted body i at node index `node`.
 // Its half-size is the root's halved d times (fa2_bh.rs:855-861); its skip rope
-// is the first node of the body after the last one sharing its d-digit prefix.
-__device__ __forceinline__ void emit_cell(const uint64_t *__restrict__ keys, uint32_t nv,
-                                          uint32_t i, int d, uint32_t node, const Frame *fr,
-                                          const uint32_t *__restrict__ loc,
-                                          const uint32_t *__restrict__ blk,
-                                          Node *__restrict__ nodes,
-                                          uint8_t *__restrict__ node_depth,
-                                          uint32_t *__restrict__ top_slot) {
-    if (d <= kTopDepth) top_slot[top_slot_of(keys[i], d)] = node;
+// is the first node of the body after the last one sharing its d-digit prefix
+// (for a cell of depth <= kTopDepth k_top_build sets it from the slot table).
+// Returns the cell's Node::b.
+__device__ __forceinline__ uint4 cell_record(const uint64_t *__restrict__ keys, uint32_t nv,
+                                             uint32_t i, int d, uint32_t node, const Frame *fr,
+                                             const uint32_t *__restrict__ loc,
+                                             const uint32_t *__restrict__ blk,
+                                             uint32_t *__restrict__ top_slot) {
     float h = fr->root[3];
     for (int q = 0; q < d; q++) h = h * 0.5f;
-    const uint32_t j = cell_end(keys, i, nv, keys[i], d);
-    const uint32_t skip = (j + 1 < nv) ? first_node_of(loc, blk, j + 1) : kEnd;
+    uint32_t skip = kEnd;
+    if (d <= kTopDepth) {
+        top_slot[top_slot_of(keys[i], d)] = node;
+    } else {
+        const uint32_t j = cell_end(keys, i, nv, keys[i], d);
+        skip = (j + 1 < nv) ? first_node_of(loc, blk, j + 1) : kEnd;
+    }
     const float s = h * 2.0f;
     const float s2 = s * s;
-    nodes[node].b = cell_b(s2, skip);
-    node_depth[node] = (uint8_t)d;
+    return cell_b(s2, skip);
 }
 
 // Pre-order emission.  Body i (sorted) opens the cells at depths lp+1 .. li
 // (lp, li: common octal digits with its neighbours), then its leaf at depth
 // max(lp, li)+1.  The leaves are written by their body's thread; the cells go
 // through a block-wide task list, so the prefix searches of a body that opens
-// many cells (the first bodies open the top of the tree) run on many threads
-// instead of one after another.
+// many cells run on many threads instead of one after another.  (Assembling a
+// block's node range in LDS and writing it out as one span was measured: it is
+// slower, the partial-line writes are not what bounds this kernel.)
 __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
                                               const uint32_t *__restrict__ perm,
                                               const float4 *__restrict__ pos,
@@ -330,13 +338,14 @@ __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
             if (c) {
                 const uint32_t slot = atomicAdd(&ntask, c);
                 for (uint32_t t = 0; t < c; t++) {
+                    const int d = a0 + (int)t;
                     if (slot + t < kEmitTasks) {
                         t_node[slot + t] = pos0 + t;
                         t_body[slot + t] = i;
-                        t_depth[slot + t] = (uint8_t)(a0 + (int)t);
+                        t_depth[slot + t] = (uint8_t)d;
                     } else {
-                        emit_cell(keys, nv, i, a0 + (int)t, pos0 + t, fr, loc, blk, nodes, node_depth,
-                                  top_slot);
+                        nodes[pos0 + t].b = cell_record(keys, nv, i, d, pos0 + t, fr, loc, blk, top_slot);
+                        node_depth[pos0 + t] = (uint8_t)d;
                     }
                 }
                 atomicMax(&bdepth, (uint32_t)(a0 + (int)c - 1));
@@ -356,9 +365,10 @@ __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
     }
     __syncthreads();
     const uint32_t nt = min(ntask, kEmitTasks);
-    for (uint32_t q = threadIdx.x; q < nt; q += blockDim.x)
-        emit_cell(keys, nv, t_body[q], t_depth[q], t_node[q], fr, loc, blk, nodes, node_depth,
-                  top_slot);
+    for (uint32_t q = threadIdx.x; q < nt; q += blockDim.x) {
+        nodes[t_node[q]].b = cell_record(keys, nv, t_body[q], t_depth[q], t_node[q], fr, loc, blk, top_slot);
+        node_depth[t_node[q]] = t_depth[q];
+    }
     if (threadIdx.x == 0 && bdepth) atomicMax(&fr->max_depth, bdepth);
 }
 
@@ -389,17 +399,31 @@ __global__ __launch_bounds__(kTB) void k_com_level(const Frame *fr, uint32_t d, 
     }
 }
 
-// Gathers the top tree (the nodes of depth <= kTopDepth, in pre-order) from
-// the slot table that k_emit filled, once the centres of mass are final.  One
-// block: an exclusive scan over slot presence gives each node its rank, and a
-// node's subtree ends at the first present slot after its complete subtree.
+// The top of the tree (the nodes of depth <= kTopDepth), one block, once the
+// centres of mass of the deeper cells are final:
+//  1. an exclusive scan over slot presence gives each top node its pre-order
+//     rank; a node's subtree ends at the first present slot after its complete
+//     subtree (rank ts);
+//  2. the skip ropes of the top cells, which k_emit leaves to this kernel: the
+//     node of rank ts, kEnd past the last (the cells' long prefix searches are
+//     the longest dependent chains of k_emit);
+//  3. the centres of mass of the top cells, deepest first, children in octant
+//     order, m = sum m_c, com = sum com_c * m_c / m (fa2_bh.rs:944-965): a
+//     frontier cell's children are read from the global array along their
+//     ropes, a shallower cell's children are the top nodes u+1, ts(u+1), ...;
+//  4. the top tree for the force kernels: centre of mass and mass; size, id,
+//     subtree end rank | frontier flag, byte offset.
 __global__ __launch_bounds__(1024) void k_top_build(Frame *fr, const uint32_t *__restrict__ top_slot,
-                                                    const Node *__restrict__ nodes,
+                                                    Node *__restrict__ nodes, uint32_t cap,
                                                     float4 *__restrict__ top_a,
                                                     uint4 *__restrict__ top_b) {
     static_assert(kTopSlots < 1024, "one thread per slot");
     __shared__ uint32_t rk[kTopSlots + 1];
     __shared__ uint32_t ws[16];
+    __shared__ uint32_t gof[kTopSlots];   // global node of each rank
+    __shared__ uint32_t tsk[kTopSlots];   // subtree end rank of each rank
+    __shared__ uint8_t dep[kTopSlots];
+    __shared__ float4 A[kTopSlots];       // centre of mass and mass of each rank
     const uint32_t s = threadIdx.x, lane = s & 63, w = s >> 6;
     const uint32_t g = s < kTopSlots ? top_slot[s] : kTopAbsent;
     const uint32_t e = g != kTopAbsent ? 1u : 0u;
@@ -418,13 +442,63 @@ __global__ __launch_bounds__(1024) void k_top_build(Frame *fr, const uint32_t *_
     if (e) {
         const uint32_t t = rk[s];
         const int d = top_slot_depth(s);
-        const Node nd = nodes[g];
-        const uint32_t ts = rk[s + top_sub(d)];
-        const bool frontier = d == kTopDepth && nd.b.z == kCellId;
-        top_a[t] = nd.a;
-        top_b[t] = make_uint4(nd.b.y, nd.b.z, ts | (frontier ? kTopFrontier : 0u), g << 5);
+        gof[t] = g;
+        tsk[t] = rk[s + top_sub(d)];
+        dep[t] = (uint8_t)d;
     }
-    if (s == kTopSlots) fr->n_top = rk[kTopSlots];
+    __syncthreads();
+    const uint32_t K = rk[kTopSlots];
+    const uint32_t N = min(fr->n_tree, cap);
+    const uint32_t u = threadIdx.x;
+    uint4 nb = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t skip = kEnd;
+    if (u < K) {
+        const Node nd = nodes[gof[u]];
+        nb = nd.b;
+        A[u] = nd.a;  // final for leaves; cells are recomputed below
+        if (nb.z == kCellId) {
+            skip = tsk[u] < K ? gof[tsk[u]] : kEnd;
+            nodes[gof[u]].b = make_uint4(skip_bytes(skip), nb.y, nb.z, skip);
+        }
+    }
+    __syncthreads();
+    for (int lvl = kTopDepth; lvl >= 0; lvl--) {
+        if (u < K && dep[u] == lvl && nb.z == kCellId) {
+            float tm = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+            if (lvl == kTopDepth) {
+                for (uint32_t ch = gof[u] + 1; ch < N && ch != skip; ch = nodes[ch].b.w) {
+                    const float4 a = nodes[ch].a;
+                    tm = tm + a.w;
+                    cx = cx + a.x * a.w;
+                    cy = cy + a.y * a.w;
+                    cz = cz + a.z * a.w;
+                }
+            } else {
+                for (uint32_t c = u + 1; c < K && c != tsk[u]; c = tsk[c]) {
+                    const float4 a = A[c];
+                    tm = tm + a.w;
+                    cx = cx + a.x * a.w;
+                    cy = cy + a.y * a.w;
+                    cz = cz + a.z * a.w;
+                }
+            }
+            if (tm > 0.0f) {
+                cx = cx / tm;
+                cy = cy / tm;
+                cz = cz / tm;
+            }
+            const float4 r = make_float4(cx, cy, cz, tm);
+            A[u] = r;
+            nodes[gof[u]].a = r;
+        }
+        __syncthreads();
+    }
+    if (u < K) {
+        const bool frontier = dep[u] == kTopDepth && nb.z == kCellId;
+        top_a[u] = A[u];
+        top_b[u] = make_uint4(nb.y, nb.z, tsk[u] | (frontier ? kTopFrontier : 0u), gof[u] << 5);
+    }
+    if (u == 0) fr->n_top = K;
 }
 
 // Attraction, gravity and the swing/traction stats of body b, after its walk
@@ -788,26 +862,44 @@ __global__ __launch_bounds__(64) void k_force_heavy(const Params P,
         atomicAdd(&partial[lane * kAccCols + blockIdx.x % kAccCols], acc[lane]);
 }
 
-// Sums the kAccCols accumulator columns per limb, then runs
+// Sums the kAccCols accumulator columns per limb (and clears them), then runs
 // AdaptiveSpeed::update (fa2_speed.rs:48-100) on one thread.  update_speed = 0
-// only records the sums (parity hook).
+// only records the sums (parity hook).  It also resets what the next tree
+// build accumulates into: bounding box, invalid count, depth, top-tree slots.
 __global__ __launch_bounds__(kSpeedThreads) void k_speed(const Params P, Frame *fr,
-                                                         const unsigned long long *__restrict__ partial,
+                                                         unsigned long long *__restrict__ partial,
+                                                         uint32_t *__restrict__ top_slot,
                                                          uint32_t update_speed) {
     constexpr uint32_t G = kSpeedThreads / (2 * kLimbs);   // threads per limb
     static_assert(kAccCols % G == 0, "columns split evenly over a limb's threads");
     static_assert(64 % G == 0, "a limb's threads sit inside one wave");
+    static_assert(kTopSlots <= kSpeedThreads, "one thread per top slot");
     __shared__ unsigned long long red[2 * kLimbs];
     const uint32_t t = threadIdx.x;
     {
         const uint32_t k = t / G, r = t % G;
-        const unsigned long long *row = partial + (size_t)k * kAccCols;
+        unsigned long long *row = partial + (size_t)k * kAccCols;
         unsigned long long s = 0;
 #pragma unroll
-        for (uint32_t c = r; c < kAccCols; c += G) s += row[c];
+        for (uint32_t c = r; c < kAccCols; c += G) {
+            s += row[c];
+            row[c] = 0ull;  // cleared for the next force pass
+        }
 #pragma unroll
         for (uint32_t o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, (int)o, 64);
         if (r == 0) red[k] = s;
+    }
+    // the per-build state of the next tree build (k_apply accumulates the box
+    // of the positions it writes)
+    if (t < kTopSlots) top_slot[t] = kTopAbsent;
+    if (t < 3) {
+        fr->bbmin[t] = 0xFF800000u;   // f2ord(+inf)
+        fr->bbmax[t] = ~0xFF800000u;  // f2ord(-inf)
+    }
+    if (t == 0) {
+        fr->n_invalid = 0;
+        fr->max_depth = 0;
+        fr->interactions = 0ull;
     }
     __syncthreads();
     if (t != 0) return;
@@ -841,26 +933,64 @@ __global__ __launch_bounds__(kSpeedThreads) void k_speed(const Params P, Frame *
     fr->speed_eff = eff;
 }
 
-__global__ __launch_bounds__(kTB) void k_apply(const Params P, const Frame *fr,
+// fa2_barnes_hut.wgsl:203-226, and the bounding box of the positions it writes
+// (finite coordinates, as k_bbox) for the next tree build: per thread over a
+// grid-stride loop, then the block, then one atomic per block and axis.
+__global__ __launch_bounds__(kTB) void k_apply(const Params P, Frame *fr,
                                                const float4 *__restrict__ pos_in,
                                                float4 *__restrict__ pos_out,
                                                const float4 *__restrict__ force,
                                                float4 *__restrict__ old_force,
                                                const uint32_t *__restrict__ deg) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.n) return;
+    __shared__ float smn[3][kTB / 64], smx[3][kTB / 64];
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     const float speed = fr->speed;
-    const float4 f = force[i];
-    const float4 o = old_force[i];
-    const float deg_i = (float)deg[i] + 1.0f;
-    const float swinging = deg_i * len3(o.x - f.x, o.y - f.y, o.z - f.z);
-    float factor = speed / (1.0f + sqrtf(speed * swinging));
-    const float df = len3(f.x, f.y, f.z);
-    if (P.max_displacement > 0.0f && df > 1.0e-9f) factor = fminf(factor, P.max_displacement / df);
-    const float k = factor * P.time_step;
-    const float4 p = pos_in[i];
-    pos_out[i] = make_float4(p.x + f.x * k, p.y + f.y * k, p.z + f.z * k, 0.0f);
-    old_force[i] = make_float4(f.x, f.y, f.z, 0.0f);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P.n; i += gridDim.x * blockDim.x) {
+        const float4 f = force[i];
+        const float4 o = old_force[i];
+        const float deg_i = (float)deg[i] + 1.0f;
+        const float swinging = deg_i * len3(o.x - f.x, o.y - f.y, o.z - f.z);
+        float factor = speed / (1.0f + sqrtf(speed * swinging));
+        const float df = len3(f.x, f.y, f.z);
+        if (P.max_displacement > 0.0f && df > 1.0e-9f) factor = fminf(factor, P.max_displacement / df);
+        const float k = factor * P.time_step;
+        const float4 p = pos_in[i];
+        const float4 q = make_float4(p.x + f.x * k, p.y + f.y * k, p.z + f.z * k, 0.0f);
+        pos_out[i] = q;
+        old_force[i] = make_float4(f.x, f.y, f.z, 0.0f);
+        const float v[3] = {q.x + 0.0f, q.y + 0.0f, q.z + 0.0f};  // +0: canonical zero sign
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            if (finitef(v[c])) {
+                mn[c] = fminf(mn[c], v[c]);
+                mx[c] = fmaxf(mx[c], v[c]);
+            }
+        }
+    }
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            mn[c] = fminf(mn[c], __shfl_xor(mn[c], o, 64));
+            mx[c] = fmaxf(mx[c], __shfl_xor(mx[c], o, 64));
+        }
+        if (lane == 0) {
+            smn[c][w] = mn[c];
+            smx[c][w] = mx[c];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int c = threadIdx.x;
+        float a = INFINITY, b = -INFINITY;
+        for (uint32_t q = 0; q < kTB / 64; q++) {
+            a = fminf(a, smn[c][q]);
+            b = fmaxf(b, smx[c][q]);
+        }
+        if (finitef(a)) atomicMin(&fr->bbmin[c], f2ord(a));
+        if (finitef(b)) atomicMax(&fr->bbmax[c], f2ord(b));
+    }
 }
 
 __global__ void k_set_speed(Frame *fr, float speed, float eff) {
@@ -1051,6 +1181,7 @@ class Fa2BhEngine final : public LayoutEngine {
         cur_ = 0;
         iteration_ = 0;
         params_dirty_ = true;
+        ready_ = false;
         inited_ = true;
         return Status::ok();
     }
@@ -1060,6 +1191,7 @@ class Fa2BhEngine final : public LayoutEngine {
         if (n_ == 0) return Status::ok();
         refresh_params();
         JC_TRY(ensure_tuned());
+        JC_TRY(ensure_ready());
         uint32_t left = n_iters;
         if (left >= 2) {
             JC_TRY(ensure_graph());
@@ -1108,6 +1240,7 @@ class Fa2BhEngine final : public LayoutEngine {
         if (!inited_) return Status::err(JC_E_STATE, "set_state before init");
         if (xyz) JC_TRY(write_vec4(pos_[cur_].p, xyz));
         if (old_force) JC_TRY(write_vec4(old_force_.p, old_force));
+        ready_ = false;
         if (se2) {
             k_set_speed<<<1, 64, 0, stream_>>>(frame_.p, se2[0], se2[1]);
             JC_HIP(hipGetLastError());
@@ -1125,11 +1258,13 @@ class Fa2BhEngine final : public LayoutEngine {
         DevBuf<float4> tmp;
         JC_TRY(tmp.alloc(n_));
         JC_TRY(write_vec4(tmp.p, xyz));
+        ready_ = false;
+        JC_TRY(enqueue_fresh(tmp.p));
         JC_TRY(enqueue_tree(tmp.p, nullptr));
         JC_TRY(enqueue_force(tmp.p, nullptr));
         k_count_interactions<<<cdiv(n_, kTB), kTB, 0, stream_>>>(params_, frame_.p, tmp.p, deg_.p,
                                                                  vals_.p, nodes_.p);
-        k_speed<<<1, kSpeedThreads, 0, stream_>>>(params_, frame_.p, partial_.p, 0u);
+        k_speed<<<1, kSpeedThreads, 0, stream_>>>(params_, frame_.p, partial_.p, top_slot_.p, 0u);
         JC_HIP(hipGetLastError());
         if (out_force) JC_TRY(read_vec4(force_.p, out_force));
         Frame fr;
@@ -1149,7 +1284,7 @@ class Fa2BhEngine final : public LayoutEngine {
         JC_HIP(hipMemcpyAsync(&fr, frame_.p, sizeof(Frame), hipMemcpyDeviceToHost, stream_));
         JC_HIP(hipStreamSynchronize(stream_));
         if (n_tree) *n_tree = fr.n_tree;
-        if (n_valid) *n_valid = n_ - fr.n_invalid;
+        if (n_valid) *n_valid = fr.n_valid;
         if (root4) std::memcpy(root4, fr.root, sizeof fr.root);
         if (n_ == 0) return Status::ok();
         if (keys) JC_HIP(hipMemcpy(keys, keys_.p, n_ * sizeof(uint64_t), hipMemcpyDeviceToHost));
@@ -1179,6 +1314,7 @@ class Fa2BhEngine final : public LayoutEngine {
         if (n_) JC_TRY(ensure_tuned());
         double fsum = 0, isum = 0;
         if (n_ && n_iters) {
+            JC_TRY(ensure_ready());
             JC_TRY(ensure_tree_graphs());
             for (auto &e : tev_)
                 if (!e) JC_HIP(hipEventCreate(&e));
@@ -1192,10 +1328,10 @@ class Fa2BhEngine final : public LayoutEngine {
                     JC_HIP(hipEventRecord(tev_[2 * k], stream_));
                     JC_TRY(enqueue_force(pos_[c].p, nullptr));
                     JC_HIP(hipEventRecord(tev_[2 * k + 1], stream_));
-                    k_speed<<<1, kSpeedThreads, 0, stream_>>>(params_, frame_.p, partial_.p, 1u);
-                    k_apply<<<cdiv(n_, kTB), kTB, 0, stream_>>>(params_, frame_.p, pos_[c].p,
-                                                                 pos_[c ^ 1].p, force_.p,
-                                                                 old_force_.p, deg_.p);
+                    k_speed<<<1, kSpeedThreads, 0, stream_>>>(params_, frame_.p, partial_.p, top_slot_.p, 1u);
+                    k_apply<<<apply_grid(), kTB, 0, stream_>>>(params_, frame_.p, pos_[c].p,
+                                                                pos_[c ^ 1].p, force_.p,
+                                                                old_force_.p, deg_.p);
                     JC_HIP(hipGetLastError());
                     cur_ ^= 1;
                 }
@@ -1236,6 +1372,7 @@ class Fa2BhEngine final : public LayoutEngine {
         if (!inited_) return Status::err(JC_E_STATE, "profile before init");
         refresh_params();
         if (n_) JC_TRY(ensure_tuned());
+        if (n_ && n_iters) JC_TRY(ensure_ready());
         std::vector<hipEvent_t> ev(PH_N + 1);
         for (auto &e : ev) JC_HIP(hipEventCreate(&e));
         std::vector<double> acc(PH_N, 0.0);
@@ -1305,6 +1442,8 @@ class Fa2BhEngine final : public LayoutEngine {
         for (int round = 0; round < 4; round++)
             for (int mode = 0; mode < kWalkModes; mode++) {
                 walk_mode_ = mode;
+                ready_ = false;
+                JC_TRY(enqueue_fresh(pos_[cur_].p));
                 JC_TRY(enqueue_tree(pos_[cur_].p, nullptr));
                 JC_HIP(hipEventRecord(a, stream_));
                 JC_TRY(enqueue_force(pos_[cur_].p, nullptr));
@@ -1367,11 +1506,28 @@ class Fa2BhEngine final : public LayoutEngine {
     }
     Status read_positions(float *out) { return read_vec4(pos_[cur_].p, out); }
 
+    // Resets the per-build state and takes the bounding box of pos: needed when
+    // pos was not written by k_apply right after k_speed (init, set_state, the
+    // parity hooks, the tuner's force passes without k_speed).
+    Status enqueue_fresh(const float4 *pos) {
+        k_frame_reset<<<1, 1024, 0, stream_>>>(frame_.p, partial_.p, top_slot_.p);
+        k_bbox<<<std::min<uint32_t>(cdiv(n_, kTB), 256), kTB, 0, stream_>>>(pos, n_, frame_.p);
+        JC_HIP(hipGetLastError());
+        return Status::ok();
+    }
+    Status ensure_ready() {
+        if (!ready_) JC_TRY(enqueue_fresh(pos_[cur_].p));
+        ready_ = true;
+        return Status::ok();
+    }
+    uint32_t apply_grid() const { return std::min<uint32_t>(cdiv(n_ ? n_ : 1, kTB), 512); }
+
+    // The tree build of pos; the frame holds pos's bounding box and a reset
+    // per-build state (enqueue_fresh, or k_speed + k_apply of the previous
+    // iteration).
     Status enqueue_tree(const float4 *pos, hipEvent_t *ev) {
         const uint32_t n = n_;
         if (ev) JC_HIP(hipEventRecord(ev[PH_BBOX], stream_));
-        k_frame_reset<<<1, 1024, 0, stream_>>>(frame_.p, partial_.p, top_slot_.p);
-        k_bbox<<<std::min<uint32_t>(cdiv(n, kTB), 256), kTB, 0, stream_>>>(pos, n, frame_.p);
         if (ev) JC_HIP(hipEventRecord(ev[PH_KEYS], stream_));
         k_keys<<<cdiv(n, kTB), kTB, 0, stream_>>>(pos, n, frame_.p, keys_.p, vals_.p);
         if (ev) JC_HIP(hipEventRecord(ev[PH_SORT], stream_));
@@ -1385,10 +1541,11 @@ class Fa2BhEngine final : public LayoutEngine {
                                                   top_slot_.p);
         if (ev) JC_HIP(hipEventRecord(ev[PH_COM], stream_));
         const uint32_t gcom = std::min<uint32_t>(cdiv(cap_, kTB), 4096);
-        for (int d = kLevels; d >= 0; d--)
+        for (int d = kLevels; d > kTopDepth; d--)
             k_com_level<<<gcom, kTB, 0, stream_>>>(frame_.p, (uint32_t)d, (uint32_t)cap_,
                                                    nodes_.p, node_depth_.p);
-        k_top_build<<<1, 1024, 0, stream_>>>(frame_.p, top_slot_.p, nodes_.p, top_a_.p, top_b_.p);
+        k_top_build<<<1, 1024, 0, stream_>>>(frame_.p, top_slot_.p, nodes_.p, (uint32_t)cap_,
+                                             top_a_.p, top_b_.p);
         JC_HIP(hipGetLastError());
         return Status::ok();
     }
@@ -1427,12 +1584,12 @@ class Fa2BhEngine final : public LayoutEngine {
         JC_TRY(enqueue_force(pos_[c].p, ev));
         if (ev) JC_HIP(hipEventRecord(ev[PH_SPEED], stream_));
         if (speed)
-            k_speed<<<1, kSpeedThreads, 0, stream_>>>(params_, frame_.p, partial_.p, 1u);
+            k_speed<<<1, kSpeedThreads, 0, stream_>>>(params_, frame_.p, partial_.p, top_slot_.p, 1u);
         if (ev) JC_HIP(hipEventRecord(ev[PH_APPLY], stream_));
         if (apply)
-            k_apply<<<cdiv(n_, kTB), kTB, 0, stream_>>>(params_, frame_.p, pos_[c].p,
-                                                         pos_[c ^ 1].p, force_.p, old_force_.p,
-                                                         deg_.p);
+            k_apply<<<apply_grid(), kTB, 0, stream_>>>(params_, frame_.p, pos_[c].p,
+                                                        pos_[c ^ 1].p, force_.p, old_force_.p,
+                                                        deg_.p);
         if (ev) JC_HIP(hipEventRecord(ev[PH_N], stream_));
         JC_HIP(hipGetLastError());
         return Status::ok();
@@ -1527,6 +1684,8 @@ class Fa2BhEngine final : public LayoutEngine {
     Params params_{};
     bool params_dirty_ = true;
     bool inited_ = false;
+    // the frame holds the bounding box of pos_[cur_] and a reset per-build state
+    bool ready_ = false;
     uint32_t n_ = 0;
     uint64_t n_att_ = 0;
     uint32_t n_heavy_ = 0;          // bodies whose attraction row exceeds kHeavyAtt
