@@ -275,8 +275,10 @@ def main() -> int:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "kernel": "k_force (+ k_force_heavy)", "bytes_per_launch": fb,
-                "avg_launch_ms": round(force_ms / args.steps, 4)}
+                "kernel": "k_force (+ k_heavy_terms, k_force_heavy)", "bytes_per_launch": fb,
+                "avg_launch_ms": round(force_ms / args.steps, 4),
+                "limiter": "vector-memory path (divergent 16-byte node gathers): TA 81% / TD 87% "
+                           "busy, VALU 54% (profiles/fa2_force_ba1m_pmc.md)"}
 
     cpu = None
     if not args.no_cpu_baseline:
