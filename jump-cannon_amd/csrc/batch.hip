@@ -459,6 +459,7 @@ __device__ __forceinline__ void b_gravity_and_stats(const BParams &P, uint32_t b
 // for k_b_force_heavy.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_b_force(const BParams P, const GFrame *fr,
+                                                const BGlobal *gl,
                                                 const uint32_t *__restrict__ gid,
                                                 const float4 *__restrict__ pos,
                                                 const float4 *__restrict__ old_force,
@@ -482,11 +483,12 @@ __global__ __launch_bounds__(BS) void k_b_force(const BParams P, const GFrame *f
     if (N > 0) {
         // the graph's tree is nodes [start, start + N); its last rope is kEnd
         const uint32_t start = fr[g].tree_start, end = start + N;
-        if (end <= P.cap && tree_reducible(end, fr[g].root[3], P.theta2) && finite3(p)) {
+        const uint32_t total = min(gl->n_tree_total, P.cap);  // uniform bound of the buffer
+        if (end <= total && tree_reducible(total, fr[g].root[3], P.theta2) && finite3(p)) {
             if (P.fastdiv)
-                (void)walk_reduced<true>(nodes, start, end, p.x, p.y, p.z, b, krd, P.theta2, fx, fy, fz);
+                walk_reduced<true>(nodes, start, end, total, p.x, p.y, p.z, b, krd, P.theta2, fx, fy, fz);
             else
-                (void)walk_reduced<false>(nodes, start, end, p.x, p.y, p.z, b, krd, P.theta2, fx, fy, fz);
+                walk_reduced<false>(nodes, start, end, total, p.x, p.y, p.z, b, krd, P.theta2, fx, fy, fz);
         } else {
             (void)walk_exact(nodes, start, min(end, P.cap), p.x, p.y, p.z, b, krd, P.theta2, fx, fy, fz);
         }
@@ -975,12 +977,15 @@ class Fa2BhBatch {
             k_b_heavy_terms<<<cdivb(n_hent_, kBT), kBT, 0, stream_>>>(P_, n_hent_, hbody_.p, hadj_.p,
                                                                     pos, hterm_.p, hstride_);
 #define JC_BF(BS)                                                                           \
-    k_b_force<BS><<<grid, BS, 0, stream_>>>(P_, frame_.p, gid_.p, pos, old_force_.p, force_.p, \
+    k_b_force<BS><<<grid, BS, 0, stream_>>>(P_, frame_.p, global_.p, gid_.p, pos, old_force_.p, \
+                                            force_.p, \
                                             deg_.p, att_off_.p, att_adj_.p, vals_.p, nodes_.p, \
                                             stats_.p)
         switch (bs) {
             case 64: JC_BF(64); break;
             case 128: JC_BF(128); break;
+            case 512: JC_BF(512); break;
+            case 1024: JC_BF(1024); break;
             default: JC_BF(256); break;
         }
 #undef JC_BF
@@ -1011,7 +1016,7 @@ class Fa2BhBatch {
         JC_HIP(hipEventCreate(&a));
         JC_HIP(hipEventCreate(&b));
         float best = 1e30f;
-        for (uint32_t bs : {64u, 128u, 256u}) {
+        for (uint32_t bs : {64u, 128u, 256u, 512u, 1024u}) {
             JC_TRY(launch_force(pos_[cur_].p, bs));  // warm
             JC_HIP(hipEventRecord(a, stream_));
             for (int r = 0; r < 3; r++) JC_TRY(launch_force(pos_[cur_].p, bs));

@@ -202,17 +202,21 @@ __device__ __forceinline__ bool tree_reducible(uint32_t end, float root_half, fl
 // so max(r2, 1e-4) is the unsigned maximum of the bit patterns.  FASTDIV uses
 // div_in_range: the caller has checked that krd * mass / max(r2, 1e-4) stays in
 // its range.  Interactions are not counted here (walk_exact counts them).
+// buf_nodes bounds the buffer resource: it must be wave-uniform (a resource
+// lives in scalar registers; a per-lane one costs a loop over the distinct
+// values around every load), so a caller whose lanes walk different trees
+// passes the size of the whole node array.
 template <bool FASTDIV>
 __device__ __forceinline__ void walk_reduced(const Node *__restrict__ nodes, uint32_t start,
-                                             uint32_t end, float px, float py, float pz,
-                                             uint32_t b, float krd, float theta2, float &fx,
-                                             float &fy, float &fz) {
+                                             uint32_t end, uint32_t buf_nodes, float px,
+                                             float py, float pz, uint32_t b, float krd,
+                                             float theta2, float &fx, float &fy, float &fz) {
     // Both halves of a node come in as one 16-byte buffer load each (a plain
     // pointer load is split by the compiler into the fields each branch uses).
     const uint32_t end_b = end << 5;
     const __amdgpu_buffer_rsrc_t rs =
-        __builtin_amdgcn_make_buffer_rsrc(const_cast<Node *>(nodes), (short)0, (int)end_b,
-                                          kBufferRsrcWord3);
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<Node *>(nodes), (short)0,
+                                          (int)(buf_nodes << 5), kBufferRsrcWord3);
     uint32_t off = start << 5;
     for (uint32_t steps = 0; off < end_b && steps < end - start; steps++) {
         const auto av = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
