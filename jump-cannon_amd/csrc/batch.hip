@@ -390,29 +390,13 @@ __global__ __launch_bounds__(kBT) void k_b_emit(
     if (threadIdx.x == 0 && bdepth) atomicMax(&gl->max_depth, bdepth);
 }
 
+// The cells of depth d over the whole forest (com_level_pass).
 __global__ __launch_bounds__(kBT) void k_b_com_level(const BGlobal *gl, uint32_t d, uint32_t cap,
                                                      Node *__restrict__ nodes,
                                                      const uint8_t *__restrict__ node_depth) {
     if (d > gl->max_depth) return;
-    const uint32_t N = min(gl->n_tree_total, cap);
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
-        if (node_depth[p] != d) continue;
-        const uint32_t end = nodes[p].b.w;
-        float tm = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
-        for (uint32_t ch = p + 1; ch < N && ch != end; ch = nodes[ch].b.w) {
-            const float4 a = nodes[ch].a;
-            tm = tm + a.w;
-            cx = cx + a.x * a.w;
-            cy = cy + a.y * a.w;
-            cz = cz + a.z * a.w;
-        }
-        if (tm > 0.0f) {
-            cx = cx / tm;
-            cy = cy / tm;
-            cz = cz / tm;
-        }
-        nodes[p].a = make_float4(cx, cy, cz, tm);
-    }
+    __shared__ uint32_t list[kBT / 64][kComSpan];
+    com_level_pass<kBT>(nodes, node_depth, min(gl->n_tree_total, cap), d, list);
 }
 
 // One attraction term d * att (fa2_barnes_hut.wgsl:158-181), d = q - p.
@@ -819,7 +803,7 @@ class Fa2BhBatch {
         JC_TRY(hbody_.alloc(n_hent_ ? n_hent_ : 1));
         JC_TRY(hadj_.alloc(n_hent_ ? n_hent_ : 1));
         JC_TRY(hterm_.alloc(3 * (size_t)(hstride_ ? hstride_ : 4)));
-        JC_TRY(node_depth_.alloc(cap_));
+        JC_TRY(node_depth_.alloc((cap_ + 31) & ~15ull));  // whole 16-byte loads in the COM pass
         JC_TRY(frame_.alloc(G));
         JC_TRY(global_.alloc(1));
         std::vector<float4> p4(nn, make_float4(0, 0, 0, 0));
@@ -963,7 +947,7 @@ class Fa2BhBatch {
                                                      base_.p, count_.p, n, frame_.p, global_.p,
                                                      loc_.p, blk_.p, nodes_.p, node_depth_.p,
                                                      cap_);
-        const uint32_t gcom = std::min<uint32_t>(cdivb(cap_, kBT), 4096);
+        const uint32_t gcom = std::min<uint32_t>(cdivb(cap_, kComSpan * (kBT / 64)), 1024);
         for (int d = kLevels; d >= 0; d--)
             k_b_com_level<<<gcom, kBT, 0, stream_>>>(global_.p, (uint32_t)d, (uint32_t)cap_,
                                                      nodes_.p, node_depth_.p);

@@ -397,6 +397,72 @@ __device__ __forceinline__ void walk_top(const float4 *tA, const uint4 *tB, uint
     }
 }
 
+// ---- centre of mass, one depth per launch ----
+// One cell's centre of mass from its children along the rope, in octant
+// (pre-order) order: m = sum m_c; com = sum(com_c * m_c) / m (fa2_bh.rs:944-965).
+__device__ __forceinline__ void com_of_cell(Node *__restrict__ nodes, uint32_t p, uint32_t N) {
+    const uint32_t end = nodes[p].b.w;
+    float tm = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    for (uint32_t ch = p + 1; ch < N && ch != end; ch = nodes[ch].b.w) {
+        const float4 a = nodes[ch].a;
+        tm = tm + a.w;
+        cx = cx + a.x * a.w;
+        cy = cy + a.y * a.w;
+        cz = cz + a.z * a.w;
+    }
+    if (tm > 0.0f) {
+        cx = cx / tm;
+        cy = cy / tm;
+        cz = cz / tm;
+    }
+    nodes[p].a = make_float4(cx, cy, cz, tm);
+}
+
+// All cells of depth d among nodes [0, N) (node_depth: cell depth, 0x80 | depth
+// for a leaf).  Each lane reads 16 depth bytes with one load, the wave gathers
+// the matching node indices into its LDS list (an exclusive scan of the lanes'
+// match counts), and the matches are then shared out over the lanes, so a
+// lane that found several cells does not chase all their ropes alone.
+// node_depth must hold whole 16-byte groups past N (callers allocate rounded up).
+constexpr uint32_t kComSpan = 64 * 16;  // nodes scanned per wave and round
+template <uint32_t BS>
+__device__ __forceinline__ void com_level_pass(Node *__restrict__ nodes,
+                                               const uint8_t *__restrict__ node_depth,
+                                               uint32_t N, uint32_t d,
+                                               uint32_t (*list)[kComSpan]) {
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t nw = gridDim.x * (BS / 64);
+    for (uint32_t base = (blockIdx.x * (BS / 64) + w) * kComSpan; base < N; base += nw * kComSpan) {
+        const uint32_t p0 = base + lane * 16;
+        uint32_t m = 0;
+        if (p0 < N) {
+            const uint4 dv = *reinterpret_cast<const uint4 *>(node_depth + p0);
+            const uint32_t wd[4] = {dv.x, dv.y, dv.z, dv.w};
+#pragma unroll
+            for (uint32_t j = 0; j < 16; j++)
+                if (((wd[j >> 2] >> (8 * (j & 3))) & 0xFFu) == d && p0 + j < N) m |= 1u << j;
+        }
+        const uint32_t c = (uint32_t)__popc(m);
+        uint32_t incl = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o, 64);
+            if ((int)lane >= o) incl += t;
+        }
+        const uint32_t total = __shfl(incl, 63, 64);
+        uint32_t at = incl - c;
+        while (m) {
+            list[w][at++] = p0 + (uint32_t)(__ffs(m) - 1);
+            m &= m - 1u;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (uint32_t q = lane; q < total; q += 64) com_of_cell(nodes, list[w][q], N);
+        __builtin_amdgcn_wave_barrier();  // the list is read before the next round writes it
+    }
+}
+
 // Octant path of p through the reference cell chain for kLevels levels: digit
 // (x>=cx) | (y>=cy)<<1 | (z>=cz)<<2, child half = half*0.5, child centre =
 // centre +- child half (fa2_bh.rs:841-861).
