@@ -42,6 +42,7 @@ import numpy as np  # noqa: E402
 
 METRIC = "FA2+BH iters/s at 1M nodes; time-to-stress≤1.01× plain: learned vs plain"
 HBM_PEAK_GBS = 8000.0
+VALU_FP32_PEAK_TFLOPS = 157.3
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "fa2_force_ba1m_pmc.json")
 
 
@@ -59,28 +60,49 @@ def force_kernel_bytes(n: int, n_tree: int, nnz_att: int) -> int:
     return 44 * n + 24 * n_tree + 4 * (n + 1) + 16 * nnz_att
 
 
-def cpu_baseline(off, nb, xyz, theta: float, budget_s: float) -> dict:
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_rate(off, nb, xyz, s, threads: int, budget_s: float):
+    """Iterations/s of the oracle's reference-path restatement from iteration 0."""
     from oracle import oracle as O
 
-    threads = min(16, os.cpu_count() or 1)
-    s = O.Fa2Settings.make(theta=theta)
     t0 = time.perf_counter()
     x, old, se, tm = O.fa2_run(off, nb, xyz, 1, s, tree_kind=0, exact=False, threads=threads)
     first = time.perf_counter() - t0
     more = max(0, int(budget_s / max(first, 1e-3)) - 1)
-    iters = 1
+    iters, total = 1, first
     if more:
         t1 = time.perf_counter()
         O.fa2_run(off, nb, x, more, s, tree_kind=0, exact=False, threads=threads,
                   old_force=old, se=se)
         iters += more
         total = first + (time.perf_counter() - t1)
-    else:
-        total = first
-    return {"value": iters / total, "unit": "iterations/s", "cores": threads, "kind": "port",
-            "sample": f"{iters} iterations of the same 1M-node workload from iteration 0: "
+    return iters / total, iters
+
+
+def cpu_baseline(off, nb, xyz, theta: float, budget_s: float) -> dict:
+    from oracle import oracle as O
+
+    threads = min(16, os.cpu_count() or 1)
+    s = O.Fa2Settings.make(theta=theta)
+    rate, iters = _cpu_rate(off, nb, xyz, s, threads, budget_s)
+    rate1, iters1 = _cpu_rate(off, nb, xyz, s, 1, budget_s / 2)
+    return {"value": rate, "unit": "iterations/s", "cores": threads, "kind": "port",
+            "single_thread_value": rate1, "cpu_model": _cpu_model(),
+            "nproc": os.cpu_count(),
+            "sample": f"{iters} iterations ({threads} threads) and {iters1} iterations "
+                      f"(1 thread) of the same 1M-node workload from iteration 0: "
                       f"reference insertion octree built serially, walk/attraction/apply "
-                      f"under OpenMP ({threads} threads), f64 node-order speed sums"}
+                      f"under OpenMP, f64 node-order speed sums"}
 
 
 def copy_probe() -> int:
@@ -273,6 +295,15 @@ def main() -> int:
     if os.path.exists(PMC_SUMMARY):
         with open(PMC_SUMMARY) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
+    # secondary bound: FP32 vector work of the walk, about 20 FLOP per interaction
+    # (difference, squared distance, quotient, accumulation), against the vector
+    # FP32 peak; the counters' VALU busy share is in the PMC profile
+    inter_flops = 20.0 * st["interactions"]
+    tflops = inter_flops / f_avg_s / 1e12
+    roofline_valu = {"bound": "valu-fp32", "achieved": round(tflops, 3),
+                     "peak": VALU_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(tflops / VALU_FP32_PEAK_TFLOPS, 5),
+                     "flop_per_interaction": 20, "valu_busy_pmc": 0.54}
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "kernel": "k_force (+ k_heavy_terms, k_force_heavy)", "bytes_per_launch": fb,
@@ -310,6 +341,7 @@ def main() -> int:
         "walk_mode": walk,
         "phase_ms": {k: round(v, 4) for k, v in phases.items()},
         "roofline": roofline,
+        "roofline_secondary": roofline_valu,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(value / cpu["value"], 2) if cpu else None,
         "warm_start": ws,
