@@ -195,6 +195,39 @@ int jc_pagerank_plan_run(jc_pagerank_plan *plan, uint32_t max_iters, double tol_
 int jc_pagerank_plan_time(jc_pagerank_plan *plan, uint32_t n_iters, double *ms_per_iter);
 void jc_pagerank_plan_destroy(jc_pagerank_plan *plan);
 
+/* ---- graph traversal ---- */
+#define JC_UNREACHABLE 0xFFFFFFFFu
+/* Replaces gpu_bfs (crates/graph-compute/src/analytics/bfs.rs:71-300): hop
+ * distances from `source` over the CSR, in node order, JC_UNREACHABLE for nodes
+ * it does not reach (bfs.rs:20-22).  n == 0 writes nothing (bfs.rs:82-84);
+ * source >= n is JC_E_INVALID (bfs.rs:91-93); JC_E_NODEVICE without a GPU. */
+int jc_bfs(uint32_t n, const uint32_t *offsets, const uint32_t *neighbors, uint64_t nnz,
+           uint32_t source, uint32_t *out_dist);
+/* The same search with the CSR uploaded once for many sources.  out_dist
+ * (nullable) receives the distances; out_gpu_ms (nullable) the device time of
+ * the search measured with HIP events. */
+typedef struct jc_bfs_plan jc_bfs_plan;
+int jc_bfs_plan_create(uint32_t n, const uint32_t *offsets, const uint32_t *neighbors, uint64_t nnz,
+                       jc_bfs_plan **out);
+int jc_bfs_plan_run(jc_bfs_plan *plan, uint32_t source, uint32_t *out_dist, double *out_gpu_ms);
+void jc_bfs_plan_destroy(jc_bfs_plan *plan);
+/* Replaces gpu_connected_components
+ * (crates/graph-compute/src/analytics/connected_components.rs:74): out_labels[v]
+ * = smallest node index of v's component, every CSR entry taken as an
+ * undirected edge (the labels of cpu_connected_components, :45-66). */
+int jc_connected_components(uint32_t n, const uint32_t *offsets, const uint32_t *neighbors,
+                            uint64_t nnz, uint32_t *out_labels);
+/* Largest component, the pre-processing of the stress workloads (no GPU
+ * counterpart in the reference; its CPU form is the union-find above plus a
+ * size count).  Ties go to the component holding the smallest node id.
+ * keep_map[v] (nullable, n entries) = new id of v, increasing in v, or
+ * JC_UNREACHABLE; *out_size = its node count.  With sub_offsets (n+1 entries)
+ * and sub_neighbors (nnz entries) non-NULL the induced CSR with renumbered ids
+ * is written too, *out_sub_nnz its entry count (each nullable). */
+int jc_largest_component(uint32_t n, const uint32_t *offsets, const uint32_t *neighbors, uint64_t nnz,
+                         uint32_t *keep_map, uint32_t *out_size, uint32_t *sub_offsets,
+                         uint32_t *sub_neighbors, uint64_t *out_sub_nnz);
+
 /* ---- batched FA2+BH: many independent graphs per launch ----
  * No reference counterpart (the reference runs one engine per graph,
  * sim.rs:386-421); each graph follows exactly the fa2-bh-hip semantics with its

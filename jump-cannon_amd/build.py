@@ -17,9 +17,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libjc_hip.so")
-SOURCES = ["prims.hip", "fa2bh.hip", "batch.hip", "pagerank.hip", "sgd.hip", "mpnn.hip", "host.cpp"]
-CLI_SRC = os.path.join(CSRC, "graph_pagerank.cpp")
-CLI_OUT = os.path.join(HERE, "graph-pagerank")
+SOURCES = ["prims.hip", "fa2bh.hip", "batch.hip", "pagerank.hip", "sgd.hip", "traverse.hip", "mpnn.hip", "host.cpp"]
+# the reference's one-shot analytics binaries (crates/graph-compute/src/bin/)
+CLIS = {"graph-pagerank": "graph_pagerank.cpp", "graph-bfs": "graph_bfs.cpp", "graph-cc": "graph_cc.cpp"}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("JC_OFFLOAD_ARCH", "gfx950")
 
@@ -64,19 +64,23 @@ def build(verbose: bool = False) -> str:
     return OUT
 
 
-def build_cli(verbose: bool = False) -> str:
-    """graph-pagerank: the reference's PageRank CLI over the C ABI (host code
-    only, linked against libjc_hip.so next to it)."""
-    tmp = CLI_OUT + ".tmp"
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", CLI_SRC, "-o", tmp, f"-L{HERE}", "-ljc_hip",
-           "-Wl,-rpath,$ORIGIN"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"CLI build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, CLI_OUT)
-    if verbose:
-        print(f"built {CLI_OUT}")
-    return CLI_OUT
+def build_cli(verbose: bool = False) -> list:
+    """graph-pagerank, graph-bfs and graph-cc: the reference's analytics CLIs
+    over the C ABI (host code only, linked against libjc_hip.so next to them)."""
+    outs = []
+    for name, src in CLIS.items():
+        out = os.path.join(HERE, name)
+        tmp = out + ".tmp"
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, src), "-o", tmp, f"-L{HERE}",
+               "-ljc_hip", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"CLI build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(tmp, out)
+        if verbose:
+            print(f"built {out}")
+        outs.append(out)
+    return outs
 
 
 if __name__ == "__main__":

@@ -13,7 +13,6 @@
 // tabulated per sweep in LDS (values identical to the per-term expressions).
 #include <algorithm>
 #include <cmath>
-#include <deque>
 #include <vector>
 
 #include "engine.hpp"
@@ -129,25 +128,6 @@ struct Rng {
     }
 };
 
-// sgd_stress.rs:372-394
-void bfs(const CsrView &g, uint32_t src, std::vector<uint32_t> &dist, std::vector<uint32_t> &q) {
-    std::fill(dist.begin(), dist.end(), 0xFFFFFFFFu);
-    if (g.n == 0) return;
-    size_t h = 0, t = 0;
-    dist[src] = 0;
-    q[t++] = src;
-    while (h < t) {
-        const uint32_t v = q[h++];
-        for (uint64_t e = g.offsets[v]; e < g.offsets[v + 1]; e++) {
-            const uint32_t u = g.neighbors[e];
-            if (dist[u] == 0xFFFFFFFFu) {
-                dist[u] = dist[v] + 1;
-                q[t++] = u;
-            }
-        }
-    }
-}
-
 // sgd_stress.rs:324-334 anneal_eta (host f32 powf, as the reference's host code)
 float anneal_eta(uint64_t sweep, float eta_max, float eta_min, uint32_t n_anneal) {
     if (n_anneal == 0) return eta_min;
@@ -199,55 +179,31 @@ class SgdStressEngine final : public LayoutEngine {
         teardown();
         n_ = g.n;
         sweep_ = 0;
-        // sgd_stress.rs:143-175 select_pivots: first pivot rng.below(n), then max-min
-        std::vector<uint32_t> pivots;
-        std::vector<uint16_t> dist_pm;  // pivot-major [k][n]
+        // sgd_stress.rs:143-175 select_pivots: first pivot rng.below(n), then
+        // max-min, every BFS row on the device (traverse.hip)
         dmax_ = 0;
-        if (n_) {
-            const uint32_t k = std::min<uint32_t>(std::max<uint32_t>(settings_.n_pivots, 1), n_);
-            Rng rng{settings_.seed};
-            std::vector<uint32_t> mind(n_, 0xFFFFFFFFu), d(n_), q(n_);
-            dist_pm.assign((size_t)k * n_, 0);
-            uint32_t next = (uint32_t)(rng.next() % n_);
-            for (uint32_t t = 0; t < k; t++) {
-                bfs(g, next, d, q);
-                uint32_t best_node = next, best = 0;
-                for (uint32_t v = 0; v < n_; v++) {
-                    const uint32_t dv = d[v];
-                    if (dv != 0xFFFFFFFFu && dv < mind[v]) mind[v] = dv;
-                    const uint32_t cover = mind[v];
-                    if (cover != 0xFFFFFFFFu && cover > best) {
-                        best = cover;
-                        best_node = v;
-                    }
-                    if (dv != 0xFFFFFFFFu && dv > 65535u)
-                        return Status::err(JC_E_INVALID, "hop distance exceeds the u16 range");
-                    if (dv != 0xFFFFFFFFu && dv != 0) {
-                        dist_pm[(size_t)t * n_ + v] = (uint16_t)dv;
-                        dmax_ = std::max(dmax_, dv);
-                    }
-                }
-                pivots.push_back(next);
-                next = best_node;
-            }
-        }
-        k_ = (uint32_t)pivots.size();
-        eta_max_ = settings_.eta_max > 0.0f ? settings_.eta_max : (float)dmax_ * (float)dmax_;
+        k_ = 0;
+        std::vector<uint32_t> pivots;
         JC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         const uint32_t nn = n_ ? n_ : 1;
         JC_TRY(pos_[0].alloc(nn));
         JC_TRY(pos_[1].alloc(nn));
-        JC_TRY(pivots_.alloc(k_ ? k_ : 1));
-        JC_TRY(dist_.alloc(dist_pm.size()));
+        if (n_) {
+            k_ = std::min<uint32_t>(std::max<uint32_t>(settings_.n_pivots, 1), n_);
+            Rng rng{settings_.seed};
+            const uint32_t first = (uint32_t)(rng.next() % n_);
+            JC_TRY(pivots_.alloc(k_));
+            JC_TRY(dist_.alloc((size_t)k_ * n_));
+            JC_TRY(select_pivots_device(g, k_, first, stream_, pivots_.p, dist_.p, pivots, &dmax_));
+        } else {
+            JC_TRY(pivots_.alloc(1));
+            JC_TRY(dist_.alloc(1));
+        }
+        eta_max_ = settings_.eta_max > 0.0f ? settings_.eta_max : (float)dmax_ * (float)dmax_;
         std::vector<float4> p4(nn, make_float4(0, 0, 0, 0));
         for (uint32_t i = 0; i < n_; i++)
             p4[i] = make_float4(xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2], 0.0f);
         JC_HIP(hipMemcpyAsync(pos_[0].p, p4.data(), nn * sizeof(float4), hipMemcpyHostToDevice, stream_));
-        if (k_)
-            JC_HIP(hipMemcpyAsync(pivots_.p, pivots.data(), k_ * 4, hipMemcpyHostToDevice, stream_));
-        if (!dist_pm.empty())
-            JC_HIP(hipMemcpyAsync(dist_.p, dist_pm.data(), dist_pm.size() * 2, hipMemcpyHostToDevice,
-                                  stream_));
         JC_HIP(hipStreamSynchronize(stream_));
         h_pivots_ = pivots;
         cur_ = 0;

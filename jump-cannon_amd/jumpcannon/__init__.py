@@ -107,6 +107,12 @@ SIGNATURES = {
     "jc_pagerank_plan_run": (C.c_int, [_vp, C.c_uint32, C.c_double, _vp, _vp, _vp, _vp]),
     "jc_pagerank_plan_time": (C.c_int, [_vp, C.c_uint32, _vp]),
     "jc_pagerank_plan_destroy": (None, [_vp]),
+    "jc_bfs": (C.c_int, [C.c_uint32, _vp, _vp, C.c_uint64, C.c_uint32, _vp]),
+    "jc_bfs_plan_create": (C.c_int, [C.c_uint32, _vp, _vp, C.c_uint64, _vp]),
+    "jc_bfs_plan_run": (C.c_int, [_vp, C.c_uint32, _vp, _vp]),
+    "jc_bfs_plan_destroy": (None, [_vp]),
+    "jc_connected_components": (C.c_int, [C.c_uint32, _vp, _vp, C.c_uint64, _vp]),
+    "jc_largest_component": (C.c_int, [C.c_uint32, _vp, _vp, C.c_uint64, _vp, _vp, _vp, _vp, _vp]),
     "jc_fa2bh_batch_create": (C.c_int, [_vp, _vp]),
     "jc_fa2bh_batch_init": (C.c_int, [_vp, C.c_uint32, _vp, _vp, _vp, _vp, _vp]),
     "jc_fa2bh_batch_step": (C.c_int, [_vp, C.c_uint32, _vp]),
@@ -592,6 +598,82 @@ class PageRankPlan:
         if h:
             lib().jc_pagerank_plan_destroy(h)
             self._h = None
+
+
+# ------------------------------------------------------------------ traversal
+UNREACHABLE = 0xFFFFFFFF
+
+
+def bfs(offsets, neighbors, source: int):
+    """gpu_bfs (crates/graph-compute/src/analytics/bfs.rs:71-300): u32 hop
+    distances from `source`, UNREACHABLE for nodes it does not reach.  Raises
+    JcError (JC_E_INVALID) when source >= n; an empty graph gives an empty array."""
+    offsets = _u32(offsets)
+    neighbors = _u32(neighbors)
+    n = len(offsets) - 1
+    dist = np.empty(max(n, 1), np.uint32)
+    _check(lib().jc_bfs(n, _ptr(offsets), _ptr(neighbors) if len(neighbors) else None,
+                        len(neighbors), int(source), _ptr(dist)))
+    return dist[:n]
+
+
+class BfsPlan:
+    """A CSR graph resident on the device for repeated single-source BFS."""
+
+    def __init__(self, offsets, neighbors):
+        self._off = _u32(offsets)
+        self._nb = _u32(neighbors)
+        self.n = len(self._off) - 1
+        h = C.c_void_p()
+        _check(lib().jc_bfs_plan_create(self.n, _ptr(self._off), _ptr(self._nb) if len(self._nb) else None,
+                                        len(self._nb), C.addressof(h)))
+        self._h = h.value
+
+    def run(self, source: int, readback: bool = True):
+        """Returns (distances or None, device ms of the search)."""
+        dist = np.empty(max(self.n, 1), np.uint32) if readback else None
+        ms = np.zeros(1, np.float64)
+        _check(lib().jc_bfs_plan_run(self._h, int(source), _ptr(dist), _ptr(ms)))
+        return (dist[: self.n] if readback else None), float(ms[0])
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().jc_bfs_plan_destroy(h)
+            self._h = None
+
+
+def connected_components(offsets, neighbors):
+    """gpu_connected_components (crates/graph-compute/src/analytics/
+    connected_components.rs:74): label = smallest node index of the node's
+    component, CSR entries taken as undirected edges."""
+    offsets = _u32(offsets)
+    neighbors = _u32(neighbors)
+    n = len(offsets) - 1
+    lab = np.empty(max(n, 1), np.uint32)
+    _check(lib().jc_connected_components(n, _ptr(offsets), _ptr(neighbors) if len(neighbors) else None,
+                                         len(neighbors), _ptr(lab)))
+    return lab[:n]
+
+
+def largest_component(offsets, neighbors):
+    """The largest component (ties: the one holding the smallest node id) on the
+    device.  Returns (keep, sub_offsets, sub_neighbors): keep[v] = new id of v,
+    increasing in v, or UNREACHABLE for nodes outside it; the sub-CSR is the
+    induced graph with renumbered ids."""
+    offsets = _u32(offsets)
+    neighbors = _u32(neighbors)
+    n = len(offsets) - 1
+    keep = np.empty(max(n, 1), np.uint32)
+    size = np.zeros(1, np.uint32)
+    sub_off = np.empty(n + 1, np.uint32)
+    sub_nb = np.empty(max(len(neighbors), 1), np.uint32)
+    sub_nnz = np.zeros(1, np.uint64)
+    _check(lib().jc_largest_component(n, _ptr(offsets), _ptr(neighbors) if len(neighbors) else None,
+                                      len(neighbors), _ptr(keep), _ptr(size), _ptr(sub_off), _ptr(sub_nb),
+                                      _ptr(sub_nnz)))
+    k = int(size[0])
+    return keep[:n], sub_off[: k + 1].copy(), sub_nb[: int(sub_nnz[0])].copy()
 
 
 # ------------------------------------------------------------------ fixtures

@@ -35,6 +35,13 @@
  *                                   path fixed point and a brute max-min recount.
  *   metrics ...................... metrics.rs:31-164; pinned by collinear-path zero
  *                                   stress, scale invariance and hand-computed CV.
+ *   BFS distances ................ analytics/bfs.rs:44-66; pinned by the closed
+ *                                   forms of analytics tests (path |i-src|, star,
+ *                                   grid Manhattan, unreachable node) and by scipy's
+ *                                   unweighted shortest paths on random graphs.
+ *   connected components / LCC ... analytics/connected_components.rs:45-66; pinned by
+ *                                   disjoint cliques, path, isolated nodes, two cycles
+ *                                   and scipy's connected_components on random graphs.
  */
 #include "jc_oracle.h"
 
@@ -1111,7 +1118,8 @@ void jco_bfs(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, uint32_t
              uint32_t *dist) {
     uint32_t *q = (uint32_t *)malloc((size_t)(n ? n : 1) * sizeof(uint32_t));
     for (uint32_t v = 0; v < n; v++) dist[v] = 0xFFFFFFFFu;
-    if (n == 0) { free(q); return; }
+    /* analytics/bfs.rs:48-50: a source out of range reaches nothing */
+    if (n == 0 || src >= n) { free(q); return; }
     uint32_t h = 0, t = 0;
     dist[src] = 0;
     q[t++] = src;
@@ -1123,6 +1131,32 @@ void jco_bfs(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, uint32_t
         }
     }
     free(q);
+}
+
+/* analytics/connected_components.rs:45-66 cpu_connected_components: union-find
+ * over every CSR entry taken as an undirected edge, path halving, the larger
+ * root hooked under the smaller, so labels[v] = smallest node id of v's
+ * component. */
+static uint32_t jco_uf_find(uint32_t *parent, uint32_t x) {
+    while (parent[x] != x) {
+        parent[x] = parent[parent[x]];
+        x = parent[x];
+    }
+    return x;
+}
+
+void jco_connected_components(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs,
+                              uint32_t *labels) {
+    for (uint32_t v = 0; v < n; v++) labels[v] = v;
+    for (uint32_t v = 0; v < n; v++)
+        for (uint32_t e = offsets[v]; e < offsets[v + 1]; e++) {
+            uint32_t rv = jco_uf_find(labels, v), ru = jco_uf_find(labels, nbrs[e]);
+            if (rv != ru) {
+                uint32_t lo = rv < ru ? rv : ru, hi = rv < ru ? ru : rv;
+                labels[hi] = lo;
+            }
+        }
+    for (uint32_t v = 0; v < n; v++) labels[v] = jco_uf_find(labels, v);
 }
 
 /* sgd_stress.rs:143-175 select_pivots (rng = SplitMix64(seed), first pivot

@@ -107,6 +107,8 @@ void jco_pagerank_f64(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs,
 /* ---- SGD sparse stress (engines/sgd_stress.rs, sgd_stress.wgsl) ---- */
 void jco_bfs(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, uint32_t src,
              uint32_t *dist);
+void jco_connected_components(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs,
+                              uint32_t *labels);
 uint32_t jco_select_pivots(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs,
                            uint32_t n_pivots, uint64_t seed, uint32_t *pivots, uint16_t *dist);
 float jco_anneal_eta(uint64_t sweep, float eta_max, float eta_min, uint32_t n_anneal);

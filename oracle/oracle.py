@@ -100,6 +100,7 @@ def lib():
         "jco_pagerank_push": (None, [C.c_uint32, u32p, u32p, C.c_float, C.c_uint32, f32p]),
         "jco_pagerank_f64": (None, [C.c_uint32, u32p, u32p, C.c_double, C.c_uint32, f64p]),
         "jco_bfs": (None, [C.c_uint32, u32p, u32p, C.c_uint32, u32p]),
+        "jco_connected_components": (None, [C.c_uint32, u32p, u32p, u32p]),
         "jco_select_pivots": (C.c_uint32, [C.c_uint32, u32p, u32p, C.c_uint32, C.c_uint64, u32p,
                                            u16p]),
         "jco_anneal_eta": (C.c_float, [C.c_uint64, C.c_float, C.c_float, C.c_uint32]),
@@ -352,6 +353,14 @@ def bfs(offsets, nbrs, src):
     d = np.zeros(max(n, 1), np.uint32)
     lib().jco_bfs(n, _u32(offsets), _u32(nbrs) if len(nbrs) else np.zeros(1, np.uint32), src, d)
     return d[:n]
+
+
+def connected_components(offsets, nbrs):
+    n = len(offsets) - 1
+    lab = np.zeros(max(n, 1), np.uint32)
+    lib().jco_connected_components(n, _u32(offsets), _u32(nbrs) if len(nbrs) else np.zeros(1, np.uint32),
+                                   lab)
+    return lab[:n]
 
 
 def select_pivots(offsets, nbrs, k, seed=0x5EEDC0DE12345678):
