@@ -15,11 +15,13 @@
 //
 // BFS.  Hop distances are exact whatever the visiting order, so the GPU result
 // is bit-identical to the queue BFS.  Frontiers are level-synchronous:
-//   * k_bfs_block: one 1024-thread workgroup runs level after level with the
-//     frontier in LDS and, for graphs up to kBfsBmNodes nodes, the visited set as
-//     an LDS bitmap.  A level costs two dependent global loads (row offsets,
-//     neighbour ids) and two barriers, which is what long thin frontiers need
-//     (a random geometric graph has hundreds of levels of a few hundred nodes).
+//   * k_bfs_block_ell: one 1024-thread workgroup runs level after level with
+//     the frontier in LDS, the visited set as an LDS bitmap (graphs up to
+//     kBfsBmNodes nodes) and every row padded to 32 entries, so a level costs
+//     one global row load and one barrier: what long thin frontiers need (a
+//     random geometric graph has hundreds of levels of a few hundred nodes).
+//     k_bfs_block is the same loop for larger graphs, over the CSR, claiming
+//     nodes by atomicCAS on their distance.
 //   * k_bfs_level: one launch per level over the whole chip for frontiers that
 //     outgrow the LDS queue (scale-free graphs reach half the graph within a
 //     few hops); visited = atomicCAS on the distance array, appends aggregated
@@ -82,17 +84,15 @@ __global__ __launch_bounds__(kTB) void k_bfs_init(uint32_t n, uint32_t *__restri
     }
 }
 
-// Whole levels in one workgroup while the frontier fits kBfsQ.  LDS_BM keeps
-// the visited set as a bitmap in LDS (n <= kBfsBmNodes); otherwise a node is
-// claimed by atomicCAS on its distance.  On exit BfsState says whether the
-// search ended (done) or which global queue holds the next, larger frontier.
-template <bool LDS_BM>
+// Whole levels in one workgroup while the frontier fits kBfsQ, for graphs too
+// large for the LDS bitmap: a node is claimed by atomicCAS on its distance.  On
+// exit BfsState says whether the search ended (done) or which global queue
+// holds the next, larger frontier.
 __global__ __launch_bounds__(kBfsThreads) void k_bfs_block(const uint32_t *__restrict__ off,
                                                            const uint32_t *__restrict__ nbr,
-                                                           uint32_t n, uint32_t *__restrict__ dist,
+                                                           uint32_t *__restrict__ dist,
                                                            uint32_t *__restrict__ q0,
                                                            uint32_t *__restrict__ q1, BfsState *st) {
-    __shared__ uint32_t bm[LDS_BM ? kBfsBmWords : 1];
     __shared__ uint32_t lq[2][kBfsQ];
     __shared__ uint32_t heavy[kBfsQ];
     __shared__ uint32_t s_cnt[2], s_nh[2];
@@ -103,29 +103,6 @@ __global__ __launch_bounds__(kBfsThreads) void k_bfs_block(const uint32_t *__res
     const uint32_t *gin = cur ? q1 : q0;
     uint32_t *gout = cur ? q0 : q1;
     for (uint32_t i = tid; i < qn; i += kBfsThreads) lq[0][i] = gin[i];
-    if constexpr (LDS_BM) {
-        const uint32_t words = (n + 31) / 32;
-        if (lvl == 0) {
-            for (uint32_t w = tid; w < words; w += kBfsThreads) bm[w] = 0u;
-            __syncthreads();
-            for (uint32_t i = tid; i < qn; i += kBfsThreads) {
-                const uint32_t u = lq[0][i];
-                atomicOr(&bm[u >> 5], 1u << (u & 31));
-            }
-        } else {
-            // visited = every node that already has a distance (earlier levels
-            // ran in k_bfs_level): one ballot per 64 consecutive nodes
-            const uint32_t lane = tid & 63;
-            for (uint32_t base = (tid >> 6) * 64; base < words * 32; base += kBfsThreads) {
-                const uint32_t v = base + lane;
-                const unsigned long long m = __ballot(v < n && dist[v] != kUnreached);
-                if (lane == 0) {
-                    bm[base >> 5] = (uint32_t)m;
-                    if ((base >> 5) + 1 < words) bm[(base >> 5) + 1] = (uint32_t)(m >> 32);
-                }
-            }
-        }
-    }
     if (tid < 2) {
         s_cnt[tid] = 0;
         s_nh[tid] = 0;
@@ -137,15 +114,7 @@ __global__ __launch_bounds__(kBfsThreads) void k_bfs_block(const uint32_t *__res
         uint32_t *out = lq[c ^ 1];
         uint32_t *cnt = &s_cnt[c ^ 1];
         auto visit = [&](uint32_t u) {
-            bool fresh;
-            if constexpr (LDS_BM) {
-                const uint32_t bit = 1u << (u & 31);
-                fresh = !(atomicOr(&bm[u >> 5], bit) & bit);
-                if (fresh) dist[u] = lvl1;
-            } else {
-                fresh = atomicCAS(&dist[u], kUnreached, lvl1) == kUnreached;
-            }
-            if (fresh) {
+            if (atomicCAS(&dist[u], kUnreached, lvl1) == kUnreached) {
                 const uint32_t idx = atomicAdd(cnt, 1u);
                 if (idx < kBfsQ) out[idx] = u;
                 else gout[idx] = u;  // spills past the LDS queue; idx < n
@@ -202,6 +171,207 @@ __global__ __launch_bounds__(kBfsThreads) void k_bfs_block(const uint32_t *__res
         }
         qn = nq;
         c ^= 1;
+    }
+}
+
+// Padded rows for the workgroup kernel: slot 0 of row v holds v's degree, the
+// next kEllW-1 slots its neighbours; a longer row holds kHeavyRow in slot 0 and
+// is expanded from the CSR by the whole block.  One 128-byte row load per
+// frontier node replaces the dependent offsets-then-neighbours loads.
+constexpr uint32_t kEllW = 32;
+constexpr uint32_t kHeavyRow = 0xFFFFFFFEu;
+
+__global__ __launch_bounds__(kTB) void k_build_ell(uint32_t n, const uint32_t *__restrict__ off,
+                                                   const uint32_t *__restrict__ nbr,
+                                                   uint4 *__restrict__ ell) {
+    for (uint32_t v = blockIdx.x * kTB + threadIdx.x; v < n; v += gridDim.x * kTB) {
+        const uint32_t b = off[v], deg = off[v + 1] - b;
+        uint32_t r[kEllW];
+        r[0] = deg < kEllW ? deg : kHeavyRow;
+#pragma unroll
+        for (uint32_t k = 1; k < kEllW; k++) r[k] = k <= deg && deg < kEllW ? nbr[b + k - 1] : kUnreached;
+#pragma unroll
+        for (uint32_t k = 0; k < kEllW / 4; k++)
+            ell[(size_t)v * (kEllW / 4) + k] = make_uint4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
+    }
+}
+
+// Whole levels in one workgroup for graphs up to kBfsBmNodes nodes: visited
+// set as an LDS bitmap, frontier in LDS, rows from the padded copy.  Per level:
+// one row load per frontier node, the bitmap claims of all its neighbours
+// issued together (as many as the wave's longest row), one queue reservation
+// per wave (wave prefix sum of the claim counts), and one barrier (two when a
+// row longer than kEllW needs the whole block).  A node's distance is written
+// when its own row is processed, so claims touch LDS only.  Counters rotate over three slots so a single barrier orders
+// every reset before the next use.  On exit BfsState says whether the search
+// ended or which global queue holds the next, larger frontier.
+__global__ __launch_bounds__(kBfsThreads) void k_bfs_block_ell(const uint4 *__restrict__ ell,
+                                                               const uint32_t *__restrict__ off,
+                                                               const uint32_t *__restrict__ nbr, uint32_t n,
+                                                               uint32_t *__restrict__ dist,
+                                                               uint32_t *__restrict__ q0,
+                                                               uint32_t *__restrict__ q1, BfsState *st) {
+    __shared__ uint32_t bm[kBfsBmWords];
+    __shared__ uint32_t lq[2][kBfsQ];
+    __shared__ uint32_t heavy[kBfsQ];
+    __shared__ uint32_t s_cnt[3], s_nh[3];
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    uint32_t qn = st->qn, lvl = st->level;
+    const uint32_t cur = st->cur;
+    if (st->done || qn == 0 || qn > kBfsQ) return;
+    const uint32_t *gin = cur ? q1 : q0;
+    uint32_t *gout = cur ? q0 : q1;
+    for (uint32_t i = tid; i < qn; i += kBfsThreads) lq[0][i] = gin[i];
+    const uint32_t words = (n + 31) / 32;
+    if (lvl == 0) {
+        for (uint32_t w = tid; w < words; w += kBfsThreads) bm[w] = 0u;
+        __syncthreads();
+        for (uint32_t i = tid; i < qn; i += kBfsThreads) {
+            const uint32_t u = lq[0][i];
+            atomicOr(&bm[u >> 5], 1u << (u & 31));
+        }
+    } else {
+        // visited = every node that already has a distance (earlier levels ran
+        // in k_bfs_level): one ballot per 64 consecutive nodes
+        for (uint32_t base = (tid >> 6) * 64; base < words * 32; base += kBfsThreads) {
+            const uint32_t v = base + lane;
+            const unsigned long long m = __ballot(v < n && dist[v] != kUnreached);
+            if (lane == 0) {
+                bm[base >> 5] = (uint32_t)m;
+                if ((base >> 5) + 1 < words) bm[(base >> 5) + 1] = (uint32_t)(m >> 32);
+            }
+        }
+    }
+    if (tid < 3) {
+        s_cnt[tid] = 0;
+        s_nh[tid] = 0;
+    }
+    __syncthreads();
+    uint32_t c = 0, r3 = 0;
+    while (true) {
+        const uint32_t lvl1 = lvl + 1;
+        uint32_t *out = lq[c ^ 1];
+        uint32_t *cnt = &s_cnt[r3 == 2 ? 0 : r3 + 1];
+        uint32_t *nhp = &s_nh[r3];
+        if (tid == 0) {
+            // read for the last time two levels ago; used again next level
+            s_cnt[r3 == 0 ? 2 : r3 - 1] = 0;
+            s_nh[r3 == 2 ? 0 : r3 + 1] = 0;
+        }
+        for (uint32_t base = 0; base < qn; base += kBfsThreads) {
+            // whole wave idle: a scalar branch, so it issues nothing
+            if (__builtin_amdgcn_readfirstlane(base + (tid & ~63u)) >= qn) continue;
+            const uint32_t i = base + tid;
+            uint32_t u[kEllW];
+            uint32_t deg = 0;
+            if (i < qn) {
+                const uint32_t v = lq[c][i];
+                // the frontier's own distance, written by the node's processor
+                // (claims only set the bitmap)
+                dist[v] = lvl;
+                const uint4 *row = ell + (size_t)v * (kEllW / 4);
+#pragma unroll
+                for (uint32_t k = 0; k < kEllW / 4; k++) {
+                    const uint4 x = row[k];
+                    u[4 * k] = x.x;
+                    u[4 * k + 1] = x.y;
+                    u[4 * k + 2] = x.z;
+                    u[4 * k + 3] = x.w;
+                }
+                if (u[0] == kHeavyRow) heavy[atomicAdd(nhp, 1u)] = v;
+                else deg = u[0];
+            }
+            // the wave's longest row (uniform, from ballots) ends the loops
+            // below with scalar branches
+            uint32_t dmax = 0;
+#pragma unroll
+            for (int bit = 4; bit >= 0; bit--) {
+                const uint32_t cand = dmax | (1u << bit);
+                if (__ballot(deg >= cand)) dmax = cand;
+            }
+            uint32_t fresh = 0;
+#pragma unroll
+            for (uint32_t k = 1; k < kEllW; k++) {
+                if (k > dmax) break;
+                if (k <= deg) {
+                    const uint32_t bit = 1u << (u[k] & 31);
+                    if (!(atomicOr(&bm[u[k] >> 5], bit) & bit)) fresh |= 1u << k;
+                }
+            }
+            // one reservation per wave: exclusive prefix of the claim counts,
+            // bit by bit from ballots (no cross-lane shuffles)
+            const uint32_t mine = (uint32_t)__popc(fresh);
+            uint32_t pre = 0, wtot = 0;
+#pragma unroll
+            for (int bit = 0; bit < 6; bit++) {
+                const unsigned long long m = __ballot((mine >> bit) & 1u);
+                const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                pre += below << bit;
+                wtot += (uint32_t)__popcll(m) << bit;
+            }
+            uint32_t wbase = 0;
+            if (wtot) {
+                if (lane == 0) wbase = atomicAdd(cnt, wtot);
+                wbase = __builtin_amdgcn_readfirstlane(wbase);
+            }
+            uint32_t o = wbase + pre;
+#pragma unroll
+            for (uint32_t k = 1; k < kEllW; k++) {
+                if (k > dmax) break;
+                if (fresh & (1u << k)) {
+                    if (o < kBfsQ) out[o] = u[k];
+                    else gout[o] = u[k];  // spills past the LDS queue; o < n
+                    o++;
+                }
+            }
+        }
+        __syncthreads();
+        const uint32_t nh = *nhp;
+        if (nh) {
+            for (uint32_t h = 0; h < nh; h++) {
+                const uint32_t v = heavy[h];
+                const uint32_t e = off[v + 1];
+                for (uint32_t j = off[v] + tid; j < e; j += kBfsThreads) {
+                    const uint32_t u = nbr[j];
+                    const uint32_t bit = 1u << (u & 31);
+                    if (!(atomicOr(&bm[u >> 5], bit) & bit)) {
+                        const uint32_t idx = atomicAdd(cnt, 1u);
+                        if (idx < kBfsQ) out[idx] = u;
+                        else gout[idx] = u;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        const uint32_t nq = *cnt;
+        lvl = lvl1;
+        if (nq == 0) {
+            if (tid == 0) {
+                st->qn = 0;
+                st->level = lvl;
+                st->done = 1;
+            }
+            return;
+        }
+        if (nq > kBfsQ) {
+            // the chip-wide levels that take over claim nodes by their distance:
+            // the new frontier gets its distance here
+            for (uint32_t i = tid; i < nq; i += kBfsThreads) {
+                const uint32_t u = i < kBfsQ ? out[i] : gout[i];
+                dist[u] = lvl;
+                if (i < kBfsQ) gout[i] = u;
+            }
+            if (tid == 0) {
+                st->qn = nq;
+                st->level = lvl;
+                st->cur = cur ^ 1;
+            }
+            return;
+        }
+        qn = nq;
+        c ^= 1;
+        r3 = r3 == 2 ? 0 : r3 + 1;
     }
 }
 
@@ -609,6 +779,12 @@ class BfsPlan {
         JC_TRY(cnt_.alloc(1));
         if (n_) JC_HIP(hipMemcpyAsync(off_.p, g.offsets, ((size_t)n_ + 1) * 4, hipMemcpyHostToDevice, stream_));
         if (nnz_) JC_HIP(hipMemcpyAsync(nbr_.p, g.neighbors, nnz_ * 4, hipMemcpyHostToDevice, stream_));
+        if (n_ && n_ <= kBfsBmNodes) {
+            // padded rows for the one-workgroup kernel (128 B per node)
+            JC_TRY(ell_.alloc((size_t)n_ * (kEllW / 4)));
+            k_build_ell<<<grid_for(n_), kTB, 0, stream_>>>(n_, off_.p, nbr_.p, ell_.p);
+            JC_HIP(hipGetLastError());
+        }
         JC_HIP(hipStreamSynchronize(stream_));
         return Status::ok();
     }
@@ -619,10 +795,12 @@ class BfsPlan {
         if (n_ == 0) return Status::ok();
         k_bfs_init<<<grid_for(n_), kTB, 0, stream_>>>(n_, dist_.p, q_[0].p, st_.p, src_dev, src, best);
         JC_HIP(hipGetLastError());
-        const bool lds_bm = n_ <= kBfsBmNodes;
         while (true) {
-            if (lds_bm) k_bfs_block<true><<<1, kBfsThreads, 0, stream_>>>(off_.p, nbr_.p, n_, dist_.p, q_[0].p, q_[1].p, st_.p);
-            else k_bfs_block<false><<<1, kBfsThreads, 0, stream_>>>(off_.p, nbr_.p, n_, dist_.p, q_[0].p, q_[1].p, st_.p);
+            if (ell_.p)
+                k_bfs_block_ell<<<1, kBfsThreads, 0, stream_>>>(ell_.p, off_.p, nbr_.p, n_, dist_.p, q_[0].p,
+                                                                q_[1].p, st_.p);
+            else
+                k_bfs_block<<<1, kBfsThreads, 0, stream_>>>(off_.p, nbr_.p, dist_.p, q_[0].p, q_[1].p, st_.p);
             JC_HIP(hipGetLastError());
             JC_HIP(hipMemcpyAsync(&h_, st_.p, sizeof h_, hipMemcpyDeviceToHost, stream_));
             JC_HIP(hipStreamSynchronize(stream_));
@@ -689,6 +867,7 @@ class BfsPlan {
     hipStream_t stream_ = nullptr;
     bool own_stream_ = false;
     DevBuf<uint32_t> off_, nbr_, dist_, q_[2], cnt_;
+    DevBuf<uint4> ell_;
     DevBuf<BfsState> st_;
 };
 
