@@ -92,12 +92,22 @@ struct RadixScratch {
     DevBuf<uint32_t> vals_alt;
     DevBuf<uint32_t> hist;   // 256 * n_blocks, digit-major
     DevBuf<uint32_t> dbase;  // 256 digit bases
+    DevBuf<uint32_t> long_runs, long_count;  // sort_morton_pairs: runs longer than kShortRun
     uint32_t n = 0, n_blocks = 0;
     Status alloc(uint32_t n);
 };
 Status radix_sort_pairs_u64(uint64_t *keys, uint32_t *vals, RadixScratch &s, hipStream_t st,
                             uint32_t begin_bit = 0, uint32_t end_bit = 64);
+// The same result as radix_sort_pairs_u64 over all 64 bits (stable, ties in
+// input order), in four radix passes over the upper halves plus a sort of each
+// run of equal upper halves by the lower half.  Fast when such runs are short,
+// as they are for octree keys; exact for any input.
+Status sort_morton_pairs(uint64_t *keys, uint32_t *vals, RadixScratch &s, hipStream_t st);
 
+constexpr uint32_t kShortRun = 16;          // sort_morton_pairs: runs sorted by one thread
+constexpr uint32_t kLdsRun = 4096;          // longer runs up to this size sort in LDS
+constexpr uint32_t kLongRunThreads = 1024;
+constexpr uint32_t kLongRunBlocks = 32;
 constexpr uint32_t kSortThreads = 256;
 constexpr uint32_t kSortItems = 8;
 constexpr uint32_t kSortTile = kSortThreads * kSortItems;

@@ -11,11 +11,14 @@ namespace jc {
 
 namespace {
 
+// Also clears *zero_me (nullable): the long-run counter of sort_morton_pairs,
+// reset by a kernel so the sequence can be captured into a graph.
 __global__ __launch_bounds__(kSortThreads) void k_rs_hist(const uint64_t *__restrict__ keys,
                                                           uint32_t n, uint32_t shift,
                                                           uint32_t *__restrict__ hist,
-                                                          uint32_t nb) {
+                                                          uint32_t nb, uint32_t *zero_me) {
     __shared__ uint32_t h[256];
+    if (zero_me && blockIdx.x == 0 && threadIdx.x == 0) *zero_me = 0u;
     h[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * kSortTile;
@@ -178,7 +181,126 @@ Status RadixScratch::alloc(uint32_t count) {
     JC_TRY(vals_alt.alloc(count));
     JC_TRY(hist.alloc(256ull * n_blocks));
     JC_TRY(dbase.alloc(256));
+    JC_TRY(long_runs.alloc(count / (kShortRun + 1) + 1));
+    JC_TRY(long_count.alloc(1));
     return Status::ok();
+}
+
+// ---- Morton keys: the upper 32 bits by radix passes, runs by the lower ----
+//
+// After stable passes over bits [32, 64) the keys are ordered by their upper
+// halves, and keys with equal upper halves keep their input order.  Ordering
+// each such run by the lower half (ties kept in order) then gives exactly the
+// stable sort by the whole key.  Octree keys of real layouts leave short runs
+// (at most a few keys share their upper 32 bits, i.e. their top 10 octree
+// levels), so four passes and one run-sorting sweep replace eight passes.
+
+// One thread per run start: runs of up to kShortRun keys are insertion-sorted in
+// place (strict compare: stable); longer runs are listed for k_rs_long_runs.
+__global__ __launch_bounds__(256) void k_rs_runs(uint64_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                                                 uint32_t n, uint32_t *__restrict__ long_runs,
+                                                 uint32_t *long_count) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t hi = (uint32_t)(keys[i] >> 32);
+    if (i > 0 && (uint32_t)(keys[i - 1] >> 32) == hi) return;      // not a run start
+    if (i + 1 >= n || (uint32_t)(keys[i + 1] >> 32) != hi) return;  // run of one
+    uint32_t len = 2;
+    while (len <= kShortRun && i + len < n && (uint32_t)(keys[i + len] >> 32) == hi) len++;
+    if (len > kShortRun) {
+        long_runs[atomicAdd(long_count, 1u)] = i;
+        return;
+    }
+    for (uint32_t a = 1; a < len; a++) {
+        const uint64_t ka = keys[i + a];
+        const uint32_t va = vals[i + a];
+        uint32_t b = a;
+        while (b > 0 && keys[i + b - 1] > ka) {
+            keys[i + b] = keys[i + b - 1];
+            vals[i + b] = vals[i + b - 1];
+            b--;
+        }
+        keys[i + b] = ka;
+        vals[i + b] = va;
+    }
+}
+
+// Ascending bitonic sort of m unique 64-bit values (with payloads) for any m:
+// each merge stage first compares element i with its mirror i ^ (2k-1), then
+// with i ^ j; partners past m act as +infinity and never move.  The caller
+// puts a barrier-synchronised array behind `K`/`V` (LDS or, per block, global).
+template <typename KArr, typename VArr>
+__device__ void bitonic_any(KArr K, VArr V, uint32_t m) {
+    uint32_t p = 1;
+    while (p < m) p <<= 1;
+    for (uint32_t k = 2; k <= p; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+                const uint32_t partner = j == (k >> 1) ? (i ^ (k - 1)) : (i ^ j);
+                if (partner > i && partner < m) {
+                    const uint64_t a = K[i], b = K[partner];
+                    if (a > b) {
+                        K[i] = b;
+                        K[partner] = a;
+                        const uint32_t t = V[i];
+                        V[i] = V[partner];
+                        V[partner] = t;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+// One block per listed run: its end by binary search (keys are ordered by the
+// upper half), then a bitonic sort of (lower half << 32 | rank in run), which
+// makes every value unique and keeps equal lower halves in input order.  Runs
+// of up to kLdsRun keys sort in LDS; longer ones (only layouts with extreme
+// outliers) in the scratch pair, in global memory.
+__global__ __launch_bounds__(kLongRunThreads) void k_rs_long_runs(uint64_t *__restrict__ keys,
+                                                                  uint32_t *__restrict__ vals, uint32_t n,
+                                                                  const uint32_t *__restrict__ long_runs,
+                                                                  const uint32_t *long_count,
+                                                                  uint64_t *__restrict__ gk) {
+    __shared__ uint64_t sk[kLdsRun];
+    __shared__ uint32_t sv[kLdsRun];
+    __shared__ uint32_t s_end;
+    const uint32_t cnt = *long_count;
+    for (uint32_t r = blockIdx.x; r < cnt; r += gridDim.x) {
+        const uint32_t s = long_runs[r];
+        const uint32_t hi = (uint32_t)(keys[s] >> 32);
+        if (threadIdx.x == 0) {
+            uint32_t lo = s + 1, up = n;  // first index past the run
+            while (lo < up) {
+                const uint32_t mid = lo + (up - lo) / 2;
+                if ((uint32_t)(keys[mid] >> 32) == hi) lo = mid + 1;
+                else up = mid;
+            }
+            s_end = lo;
+        }
+        __syncthreads();
+        const uint32_t m = s_end - s;
+        if (m <= kLdsRun) {
+            for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
+                sk[t] = (keys[s + t] << 32) | t;
+                sv[t] = vals[s + t];
+            }
+            __syncthreads();
+            bitonic_any(sk, sv, m);
+            for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
+                keys[s + t] = ((uint64_t)hi << 32) | (sk[t] >> 32);
+                vals[s + t] = sv[t];
+            }
+        } else {
+            uint64_t *K = gk + s;
+            for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) K[t] = (keys[s + t] << 32) | t;
+            __syncthreads();
+            bitonic_any(K, vals + s, m);
+            for (uint32_t t = threadIdx.x; t < m; t += blockDim.x)
+                keys[s + t] = ((uint64_t)hi << 32) | (K[t] >> 32);
+        }
+        __syncthreads();
+    }
 }
 
 __global__ void k_copy_pairs(const uint64_t *__restrict__ ki, const uint32_t *__restrict__ vi,
@@ -196,7 +318,7 @@ Status radix_sort_pairs_u64(uint64_t *keys, uint32_t *vals, RadixScratch &s, hip
     uint64_t *ka = keys, *kb = s.keys_alt.p;
     uint32_t *va = vals, *vb = s.vals_alt.p;
     for (uint32_t shift = begin_bit; shift < end_bit; shift += 8) {
-        k_rs_hist<<<s.n_blocks, kSortThreads, 0, st>>>(ka, s.n, shift, s.hist.p, s.n_blocks);
+        k_rs_hist<<<s.n_blocks, kSortThreads, 0, st>>>(ka, s.n, shift, s.hist.p, s.n_blocks, nullptr);
         k_rs_digit_scan<<<256, 256, 0, st>>>(s.hist.p, s.n_blocks, s.dbase.p);
         k_rs_scatter<<<s.n_blocks, kSortThreads, 0, st>>>(ka, va, kb, vb, s.n, shift, s.hist.p,
                                                           s.dbase.p, s.n_blocks);
@@ -205,6 +327,27 @@ Status radix_sort_pairs_u64(uint64_t *keys, uint32_t *vals, RadixScratch &s, hip
     }
     if (ka != keys)  // odd number of passes: result is in the scratch pair
         k_copy_pairs<<<(s.n + 255) / 256, 256, 0, st>>>(ka, va, keys, vals, s.n);
+    JC_HIP(hipGetLastError());
+    return Status::ok();
+}
+
+Status sort_morton_pairs(uint64_t *keys, uint32_t *vals, RadixScratch &s, hipStream_t st) {
+    if (s.n == 0) return Status::ok();
+    uint64_t *ka = keys, *kb = s.keys_alt.p;
+    uint32_t *va = vals, *vb = s.vals_alt.p;
+    for (uint32_t shift = 32; shift < 64; shift += 8) {
+        k_rs_hist<<<s.n_blocks, kSortThreads, 0, st>>>(ka, s.n, shift, s.hist.p, s.n_blocks,
+                                                       shift == 32 ? s.long_count.p : nullptr);
+        k_rs_digit_scan<<<256, 256, 0, st>>>(s.hist.p, s.n_blocks, s.dbase.p);
+        k_rs_scatter<<<s.n_blocks, kSortThreads, 0, st>>>(ka, va, kb, vb, s.n, shift, s.hist.p,
+                                                          s.dbase.p, s.n_blocks);
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    // four passes: the result is back in keys/vals, the scratch pair is free
+    k_rs_runs<<<(s.n + 255) / 256, 256, 0, st>>>(keys, vals, s.n, s.long_runs.p, s.long_count.p);
+    k_rs_long_runs<<<kLongRunBlocks, kLongRunThreads, 0, st>>>(keys, vals, s.n, s.long_runs.p,
+                                                               s.long_count.p, s.keys_alt.p);
     JC_HIP(hipGetLastError());
     return Status::ok();
 }

@@ -338,3 +338,43 @@ def test_ba_1m_tree_and_force_parity_at_full_size():
     fc, st = O.fa2_forces(xyz, None, deg, aoff, aadj, t, _osettings(theta=0.8), threads=16)
     assert np.array_equal(f.view(np.uint32), fc.view(np.uint32))
     assert np.array_equal(sums.view(np.uint64), O.exact_sum2(st).view(np.uint64))
+
+
+def _clustered(n, n_out, spread, seed):
+    """n - n_out bodies in a cube of side `spread` near the origin and n_out
+    outliers near +-100: the cluster shares its top octree levels, so its keys
+    form long runs of equal upper halves in the sort."""
+    rng = np.random.default_rng(seed)
+    xyz = (rng.random((n, 3), dtype=np.float32) * np.float32(spread)).astype(np.float32)
+    xyz[:n_out] = (rng.choice([-100.0, 100.0], size=(n_out, 3)) + rng.random((n_out, 3))).astype(np.float32)
+    return xyz.reshape(-1)
+
+
+@pytest.mark.parametrize("n,n_out,spread,clusters", [(3000, 10, 0.01, 1), (10000, 10, 0.01, 1),
+                                                      (4000, 8, 0.003, 40)],
+                         ids=["run_lds", "run_global", "many_runs"])
+def test_sort_of_long_equal_prefix_runs(n, n_out, spread, clusters):
+    """The octree sort orders keys by their upper 32 bits with radix passes and
+    each run of equal upper halves by the lower half; runs longer than one
+    thread's share sort in LDS (up to 4096 keys) or in global memory.  Layouts
+    whose bodies share their top ten octree levels exercise both, and the tree
+    must still equal the oracle's stable sort of the full keys."""
+    off, nb = O.ba_graph(n, 2, 31)
+    xyz = _clustered(n, n_out, spread, 5).reshape(-1, 3)
+    if clusters > 1:
+        centres = np.random.default_rng(9).random((clusters, 3), dtype=np.float32) * np.float32(20.0)
+        xyz[n_out:] += centres[np.arange(n - n_out) % clusters]
+    xyz = xyz.reshape(-1).astype(np.float32)
+    keys, _ = O.morton_keys(xyz)
+    hi = np.sort(keys) >> np.uint64(32)
+    runs = np.diff(np.r_[np.nonzero(np.r_[True, hi[1:] != hi[:-1]])[0], n])
+    print(f"longest run of equal upper halves: {runs.max()}")
+    assert runs.max() > 16
+    e = _engine(off, nb, xyz, theta=0.8)
+    assert e.walk_mode(0) == 0
+    f_gpu, _ = e.forces_at(xyz)
+    t = O.octree_morton(xyz, _mass(off, nb))
+    _tree_equal(e.tree(), t)
+    deg, aoff, aadj = O.fa2_prepare(n, off, nb)
+    f_cpu, _ = O.fa2_forces(xyz, None, deg, aoff, aadj, t, _osettings(theta=0.8))
+    assert np.array_equal(f_gpu.view(np.uint32), f_cpu.view(np.uint32))
