@@ -932,7 +932,7 @@ class Fa2BhBatch {
         k_b_bbox<<<G_, kBT, 0, stream_>>>(pos, base_.p, count_.p, frame_.p);
         k_b_keys<<<cdivb(n, kBT), kBT, 0, stream_>>>(pos, gid_.p, base_.p, n, frame_.p,
                                                      keys_raw_.p, keys_.p, vals_.p);
-        JC_TRY(radix_sort_pairs_u64(keys_.p, vals_.p, sort_, stream_, 0, 64));
+        JC_TRY(sort_morton_pairs(keys_.p, vals_.p, sort_, stream_));
         if (G_ > 1) {
             k_b_gid_keys<<<cdivb(n, 256), 256, 0, stream_>>>(vals_.p, gid_.p, keys_.p, n);
             JC_TRY(radix_sort_pairs_u64(keys_.p, vals_.p, sort_, stream_, 0, gbits_));
