@@ -195,6 +195,25 @@ int jc_pagerank_plan_run(jc_pagerank_plan *plan, uint32_t max_iters, double tol_
 int jc_pagerank_plan_time(jc_pagerank_plan *plan, uint32_t n_iters, double *ms_per_iter);
 void jc_pagerank_plan_destroy(jc_pagerank_plan *plan);
 
+/* ---- multilevel wrapper ----
+ * Replaces MultilevelEngine (crates/graph-compute/src/engines/multilevel.rs:196-480),
+ * registry id "multilevel-hip": coarsen (greedy matching, coarsen.rs:53-160) ->
+ * solve the coarsest level -> prolong (coarsen.rs:166-193) -> refine, with
+ * "fa2-bh-hip" (the default) or "sgd-stress-hip" as the inner engine.  Settings
+ * through jc_engine_set_params_json, the MultilevelSettings keys
+ * (multilevel.rs:132-165): inner, inner_params (forwarded to the inner engine
+ * at init), max_levels, target_size, sweeps_per_level, sweep_schedule
+ * ("uniform" | "linear" | "geometric"), spring_len, seed.  An unknown inner id
+ * or a multilevel inner is JC_E_INVALID.  jc_engine_step(e, k, out) runs k
+ * ticks; out receives the current level's 3 * n_level coordinates (n_level
+ * <= n, from jc_multilevel_state).  get_state / set_state are JC_E_INVALID
+ * (the reference wrapper has no checkpoint state). */
+int jc_multilevel_create(jc_engine **out);
+/* The cascade position: current level (0 = the input graph), number of levels,
+ * the current level's node count and the ticks taken at it.  Each nullable. */
+int jc_multilevel_state(jc_engine *e, uint32_t *level, uint32_t *n_levels, uint32_t *n_level_nodes,
+                        uint32_t *sweeps_done);
+
 /* ---- graph traversal ---- */
 #define JC_UNREACHABLE 0xFFFFFFFFu
 /* Replaces gpu_bfs (crates/graph-compute/src/analytics/bfs.rs:71-300): hop

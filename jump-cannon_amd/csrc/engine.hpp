@@ -39,7 +39,9 @@ class LayoutEngine {
 
 // ---- minimal serde-style JSON object decoding (host.cpp) ----
 struct JsonField {
-    enum Kind { F32, BOOL, U32, U64 } kind;
+    // STR: a JSON string into std::string; RAW: any JSON value, kept as its
+    // text in std::string (settings forwarded to another decoder)
+    enum Kind { F32, BOOL, U32, U64, STR, RAW } kind;
     const char *name;
     void *dst;
 };
@@ -60,6 +62,10 @@ Status select_pivots_device(const CsrView &g, uint32_t k, uint32_t first, hipStr
 
 LayoutEngine *make_fa2bh(const jc_fa2bh_settings &s);
 LayoutEngine *make_sgd(const jc_sgd_settings &s);
+LayoutEngine *make_multilevel();
+// multilevel.cpp: the cascade position of a multilevel-hip engine.
+Status multilevel_state(LayoutEngine *e, uint32_t *level, uint32_t *n_levels, uint32_t *n_level_nodes,
+                        uint32_t *sweeps_done);
 
 // FA2+BH specific hooks (fa2bh.hip)
 Status fa2bh_forces_at(LayoutEngine *e, const float *xyz, float *out_force, double *sums2);

@@ -218,10 +218,22 @@ int decode_json_object(const char *json, size_t len, const std::vector<JsonField
                 return -1;
             }
             JVal v;
-            if (!parse_value(c, &v, 0)) {
+            c.ws();
+            const size_t v0 = c.i;
+            std::string str;
+            bool is_str = false;
+            if (c.i < c.n && c.s[c.i] == '"') {
+                is_str = true;
+                v.kind = JKind::Str;
+                if (!parse_string(c, &str)) {
+                    set_error(std::string("decode ") + what + ": malformed value for `" + key + "`");
+                    return -1;
+                }
+            } else if (!parse_value(c, &v, 0)) {
                 set_error(std::string("decode ") + what + ": malformed value for `" + key + "`");
                 return -1;
             }
+            const std::string raw(c.s + v0, c.i - v0);
             for (const JsonField &f : fields) {
                 if (key != f.name) continue;
                 bool ok = false;
@@ -241,6 +253,14 @@ int decode_json_object(const char *json, size_t len, const std::vector<JsonField
                     case JsonField::U64:
                         ok = v.kind == JKind::Num && v.integral && !v.negative;
                         if (ok) *static_cast<uint64_t *>(f.dst) = v.u;
+                        break;
+                    case JsonField::STR:
+                        ok = is_str;
+                        if (ok) *static_cast<std::string *>(f.dst) = str;
+                        break;
+                    case JsonField::RAW:
+                        ok = true;
+                        *static_cast<std::string *>(f.dst) = raw;
                         break;
                 }
                 if (!ok) {
@@ -331,6 +351,18 @@ int jc_sgd_create(const jc_sgd_settings *s, jc_engine **out) {
     if (!impl) return JC_E_OOM;
     *out = new jc_engine{impl};
     return finish(Status::ok());
+}
+
+int jc_multilevel_create(jc_engine **out) {
+    if (!out) return bad_handle();
+    *out = new jc_engine{jc::make_multilevel()};
+    return finish(Status::ok());
+}
+
+int jc_multilevel_state(jc_engine *e, uint32_t *level, uint32_t *n_levels, uint32_t *n_level_nodes,
+                        uint32_t *sweeps_done) {
+    if (!e) return bad_handle();
+    return finish(jc::multilevel_state(e->impl, level, n_levels, n_level_nodes, sweeps_done));
 }
 
 int jc_engine_descriptor(const jc_engine *e, jc_descriptor *out) {

@@ -107,6 +107,8 @@ SIGNATURES = {
     "jc_pagerank_plan_run": (C.c_int, [_vp, C.c_uint32, C.c_double, _vp, _vp, _vp, _vp]),
     "jc_pagerank_plan_time": (C.c_int, [_vp, C.c_uint32, _vp]),
     "jc_pagerank_plan_destroy": (None, [_vp]),
+    "jc_multilevel_create": (C.c_int, [_vp]),
+    "jc_multilevel_state": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "jc_bfs": (C.c_int, [C.c_uint32, _vp, _vp, C.c_uint64, C.c_uint32, _vp]),
     "jc_bfs_plan_create": (C.c_int, [C.c_uint32, _vp, _vp, C.c_uint64, _vp]),
     "jc_bfs_plan_run": (C.c_int, [_vp, C.c_uint32, _vp, _vp]),
@@ -272,6 +274,39 @@ class LayoutEngine:
 
     def is_halted(self) -> bool:
         return bool(lib().jc_engine_is_halted(self._h))
+
+
+class MultilevelEngine(LayoutEngine):
+    """Multilevel wrapper ("multilevel-hip"), the drop-in for
+    crates/graph-compute/src/engines/multilevel.rs: coarsen -> solve the
+    coarsest level -> prolong -> refine, with "fa2-bh-hip" (default) or
+    "sgd-stress-hip" inside.  Settings are MultilevelSettings' keys, given as
+    keyword arguments or through set_params."""
+
+    ID = "multilevel-hip"
+
+    def __init__(self, **settings):
+        h = C.c_void_p()
+        _check(lib().jc_multilevel_create(C.addressof(h)))
+        super().__init__(h.value)
+        if settings:
+            self.set_params(settings)
+
+    def state(self) -> dict:
+        """The cascade position: level (0 = input graph), levels, the level's
+        node count and the ticks taken at it."""
+        v = np.zeros(4, np.uint32)
+        _check(lib().jc_multilevel_state(self._h, _ptr(v[0:1]), _ptr(v[1:2]), _ptr(v[2:3]), _ptr(v[3:4])))
+        return {"level": int(v[0]), "n_levels": int(v[1]), "n_level_nodes": int(v[2]),
+                "sweeps_done": int(v[3])}
+
+    def step(self, n_iters: int = 1, readback: bool = True):
+        """n_iters ticks; returns the current level's positions (3 * n_level)."""
+        out = np.empty(3 * max(self.n, 1), np.float32) if readback else None
+        _check(lib().jc_engine_step(self._h, n_iters, _ptr(out) if readback and self.n else None))
+        if not readback:
+            return None
+        return out[: 3 * self.state()["n_level_nodes"]]
 
 
 def default_fa2bh_settings(**overrides) -> Fa2BhSettings:

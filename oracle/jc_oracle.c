@@ -1270,3 +1270,116 @@ double jco_edge_length_cv(const float *xyz, const uint32_t *ei, const uint32_t *
     var /= (double)n_edges;
     return (double)(float)(sqrt(var) / mean);
 }
+
+/* ------------------------------------------------------------------------- */
+/* Multilevel cascade: crates/graph-layouts/src/layout/coarsen.rs and          */
+/* crates/graph-compute/src/engines/multilevel.rs                            */
+/* ------------------------------------------------------------------------- */
+/* coarsen.rs:96-160 contract_one_level.  edges = n_pairs (s,t) pairs.  Writes
+ * parent[n] and the contracted pairs (first occurrences, original orientation,
+ * self loops dropped) into out_edges (capacity n_pairs pairs); returns the
+ * number of super-nodes, *out_pairs the number of contracted pairs. */
+uint32_t jco_contract_level(uint32_t n, const uint32_t *edges, uint64_t n_pairs, uint32_t *parent,
+                            uint32_t *out_edges, uint64_t *out_pairs) {
+    for (uint32_t v = 0; v < n; v++) parent[v] = 0xFFFFFFFFu;
+    uint32_t next_super = 0;
+    for (uint64_t i = 0; i < n_pairs; i++) {
+        uint32_t s = edges[2 * i], t = edges[2 * i + 1];
+        if (s == t || s >= n || t >= n) continue;
+        if (parent[s] == 0xFFFFFFFFu && parent[t] == 0xFFFFFFFFu) {
+            parent[s] = next_super;
+            parent[t] = next_super;
+            next_super++;
+        }
+    }
+    for (uint32_t v = 0; v < n; v++)
+        if (parent[v] == 0xFFFFFFFFu) parent[v] = next_super++;
+    /* seen-set: open addressing over (min << 32 | max) */
+    uint64_t cap = 16;
+    while (cap < 2 * n_pairs + 1) cap <<= 1;
+    uint64_t *slot = (uint64_t *)malloc(cap * sizeof(uint64_t));
+    for (uint64_t q = 0; q < cap; q++) slot[q] = ~0ull;
+    uint64_t m = 0;
+    for (uint64_t j = 0; j < n_pairs; j++) {
+        uint32_t es = edges[2 * j], et = edges[2 * j + 1];
+        if (es >= n || et >= n) continue;
+        uint32_t a = parent[es], b = parent[et];
+        if (a == b) continue;
+        uint64_t key = a < b ? (((uint64_t)a << 32) | b) : (((uint64_t)b << 32) | a);
+        uint64_t h = (key ^ (key >> 29)) * 0xBF58476D1CE4E5B9ull & (cap - 1);
+        int fresh = 1;
+        while (slot[h] != ~0ull) {
+            if (slot[h] == key) { fresh = 0; break; }
+            h = (h + 1) & (cap - 1);
+        }
+        if (!fresh) continue;
+        slot[h] = key;
+        out_edges[2 * m] = a;
+        out_edges[2 * m + 1] = b;
+        m++;
+    }
+    free(slot);
+    *out_pairs = m;
+    return next_super;
+}
+
+/* multilevel.rs:486-511 fold_up, one level: coarse[p] = f32 mean of its
+ * children (sums in child order, then one division by the count). */
+void jco_fold_up_level(uint32_t n_fine, const float *fine, const uint32_t *parent_map, uint32_t n_coarse,
+                       float *coarse) {
+    uint32_t *cnt = (uint32_t *)calloc(n_coarse ? n_coarse : 1, sizeof(uint32_t));
+    for (uint64_t q = 0; q < 3 * (uint64_t)n_coarse; q++) coarse[q] = 0.0f;
+    for (uint32_t c = 0; c < n_fine; c++) {
+        uint32_t p = parent_map[c];
+        if (p >= n_coarse) continue;
+        coarse[3 * (size_t)p] += fine[3 * (size_t)c];
+        coarse[3 * (size_t)p + 1] += fine[3 * (size_t)c + 1];
+        coarse[3 * (size_t)p + 2] += fine[3 * (size_t)c + 2];
+        cnt[p]++;
+    }
+    for (uint32_t p = 0; p < n_coarse; p++) {
+        float k = (float)(cnt[p] ? cnt[p] : 1u);
+        coarse[3 * (size_t)p] /= k;
+        coarse[3 * (size_t)p + 1] /= k;
+        coarse[3 * (size_t)p + 2] /= k;
+    }
+    free(cnt);
+}
+
+/* coarsen.rs:166-193 prolong: Vigna's xorshift64* (shifts 12, 25, 27,
+ * multiplier 0x2545F4914F6CDD1D) from seed + 0x9E3779B97F4A7C15; u = the top
+ * 53 bits as a double in [0,1), rounded to f32, times 2 minus 1. */
+void jco_prolong(const float *parent_pos, const uint32_t *parent_map, uint32_t n_child, float jitter,
+                 uint32_t seed, float *out) {
+    uint64_t s = (uint64_t)seed + 0x9E3779B97F4A7C15ull;
+    for (uint32_t i = 0; i < n_child; i++) {
+        size_t b = 3 * (size_t)parent_map[i];
+        for (int q = 0; q < 3; q++) {
+            s ^= s >> 12;
+            s ^= s << 25;
+            s ^= s >> 27;
+            uint64_t v = s * 0x2545F4914F6CDD1Dull;
+            float u = (float)((double)(v >> 11) / 9007199254740992.0);
+            u = u * 2.0f - 1.0f;
+            out[3 * (size_t)i + q] = parent_pos[b + q] + u * jitter;
+        }
+    }
+}
+
+/* multilevel.rs:229-272 level_csr: rows filled in edge order. */
+void jco_level_csr(uint32_t n, const uint32_t *edges, uint64_t n_pairs, uint32_t *offsets, uint32_t *nbrs) {
+    for (uint32_t v = 0; v <= n; v++) offsets[v] = 0;
+    for (uint64_t i = 0; i < n_pairs; i++) {
+        uint32_t s = edges[2 * i], t = edges[2 * i + 1];
+        if (s < n && t < n && s != t) { offsets[s + 1]++; offsets[t + 1]++; }
+    }
+    for (uint32_t v = 0; v < n; v++) offsets[v + 1] += offsets[v];
+    uint32_t *cur = (uint32_t *)malloc(((size_t)n + 1) * sizeof(uint32_t));
+    memcpy(cur, offsets, ((size_t)n + 1) * sizeof(uint32_t));
+    for (uint64_t i = 0; i < n_pairs; i++) {
+        uint32_t s = edges[2 * i], t = edges[2 * i + 1];
+        if (s < n && t < n && s != t) { nbrs[cur[s]++] = t; nbrs[cur[t]++] = s; }
+    }
+    free(cur);
+}
+

@@ -121,6 +121,15 @@ double jco_stress_normalized(const float *xyz, const uint32_t *ti, const uint32_
 double jco_edge_length_cv(const float *xyz, const uint32_t *ei, const uint32_t *ej,
                           uint64_t n_edges);
 
+/* multilevel cascade (coarsen.rs, multilevel.rs) */
+uint32_t jco_contract_level(uint32_t n, const uint32_t *edges, uint64_t n_pairs, uint32_t *parent,
+                            uint32_t *out_edges, uint64_t *out_pairs);
+void jco_fold_up_level(uint32_t n_fine, const float *fine, const uint32_t *parent_map, uint32_t n_coarse,
+                       float *coarse);
+void jco_prolong(const float *parent_pos, const uint32_t *parent_map, uint32_t n_child, float jitter,
+                 uint32_t seed, float *out);
+void jco_level_csr(uint32_t n, const uint32_t *edges, uint64_t n_pairs, uint32_t *offsets, uint32_t *nbrs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -101,6 +101,10 @@ def lib():
         "jco_pagerank_f64": (None, [C.c_uint32, u32p, u32p, C.c_double, C.c_uint32, f64p]),
         "jco_bfs": (None, [C.c_uint32, u32p, u32p, C.c_uint32, u32p]),
         "jco_connected_components": (None, [C.c_uint32, u32p, u32p, u32p]),
+        "jco_contract_level": (C.c_uint32, [C.c_uint32, u32p, C.c_uint64, u32p, u32p, u64p]),
+        "jco_fold_up_level": (None, [C.c_uint32, f32p, u32p, C.c_uint32, f32p]),
+        "jco_prolong": (None, [f32p, u32p, C.c_uint32, C.c_float, C.c_uint32, f32p]),
+        "jco_level_csr": (None, [C.c_uint32, u32p, C.c_uint64, u32p, u32p]),
         "jco_select_pivots": (C.c_uint32, [C.c_uint32, u32p, u32p, C.c_uint32, C.c_uint64, u32p,
                                            u16p]),
         "jco_anneal_eta": (C.c_float, [C.c_uint64, C.c_float, C.c_float, C.c_uint32]),
@@ -395,3 +399,155 @@ def stress_normalized(xyz, ti, tj, td):
 
 def edge_length_cv(xyz, ei, ej):
     return lib().jco_edge_length_cv(_f32(xyz), _u32(ei), _u32(ej), len(ei))
+
+
+# ---------------------------------------------------------------- multilevel
+# crates/graph-layouts/src/layout/coarsen.rs and
+# crates/graph-compute/src/engines/multilevel.rs.  The arithmetic (matching,
+# fold-up means, prolongation jitter, level CSR) is in jc_oracle.c; the cascade
+# loop and the tick state machine are restated here.
+
+def _pairs(edges):
+    e = np.asarray(edges, np.uint32).reshape(-1)
+    return e if len(e) else np.zeros(2, np.uint32), len(e) // 2
+
+
+def contract_level(n, edges):
+    """coarsen.rs:96-160.  Returns (n_super, parent_map, contracted flat edges)."""
+    e, m = _pairs(edges)
+    parent = np.zeros(max(n, 1), np.uint32)
+    out = np.zeros(max(2 * m, 2), np.uint32)
+    cnt = np.zeros(1, np.uint64)
+    ns = lib().jco_contract_level(n, e, m, parent, out, cnt)
+    return int(ns), parent[:n].copy(), out[: 2 * int(cnt[0])].copy()
+
+
+def coarsen(n, edges, max_levels, target_size):
+    """coarsen.rs:53-86: a list of levels {n, edges, parent_map} (parent_map on
+    the coarser level, indexed by the finer level's nodes)."""
+    levels = [{"n": n, "edges": np.asarray(edges, np.uint32).reshape(-1), "parent_map": None}]
+    for _ in range(1, max_levels):
+        cur = levels[-1]
+        if cur["n"] <= target_size:
+            break
+        nn, pm, ne = contract_level(cur["n"], cur["edges"])
+        ratio = np.float32(cur["n"]) / np.float32(max(nn, 1))
+        if ratio < np.float32(1.6) and nn == cur["n"]:
+            break
+        levels.append({"n": nn, "edges": ne, "parent_map": pm})
+        if nn <= target_size or ratio < np.float32(1.6):
+            break
+    return levels
+
+
+def fold_up(levels, xyz):
+    """multilevel.rs:486-511: the coarsest level's positions, children averaged."""
+    cur = _f32(xyz).copy()
+    for lv in levels[1:]:
+        out = np.zeros(max(3 * lv["n"], 3), np.float32)
+        lib().jco_fold_up_level(len(lv["parent_map"]), cur if len(cur) else np.zeros(3, np.float32),
+                                lv["parent_map"] if len(lv["parent_map"]) else np.zeros(1, np.uint32),
+                                lv["n"], out)
+        cur = out[: 3 * lv["n"]]
+    return cur
+
+
+def prolong(parent_pos, parent_map, n_child, jitter, seed):
+    """coarsen.rs:166-193."""
+    out = np.zeros(max(3 * n_child, 3), np.float32)
+    pm = np.asarray(parent_map, np.uint32)
+    lib().jco_prolong(_f32(parent_pos), pm if len(pm) else np.zeros(1, np.uint32), n_child, jitter,
+                      seed & 0xFFFFFFFF, out)
+    return out[: 3 * n_child]
+
+
+def level_csr(n, edges):
+    """multilevel.rs:229-272."""
+    e, m = _pairs(edges)
+    off = np.zeros(n + 1, np.uint32)
+    nb = np.zeros(max(2 * m, 1), np.uint32)
+    lib().jco_level_csr(n, e, m, off, nb)
+    return off, nb[: int(off[n])].copy()
+
+
+def sweeps_for_level(schedule, level, base):
+    """multilevel.rs:101-126."""
+    base = max(int(base), 1)
+    if schedule == "uniform":
+        s = base
+    elif schedule == "linear":
+        s = base * (1 + level)
+    else:
+        s = base * (1 << min(level, 16))
+    return max(min(s, 0xFFFFFFFF), 1)
+
+
+ML_DEFAULTS = {"max_levels": 6, "target_size": 500, "sweeps_per_level": 30, "sweep_schedule": "linear",
+               "spring_len": 30.0, "seed": 0x5EED}
+
+
+def multilevel_run(offsets, nbrs, xyz, ticks, settings=None, inner="fa2-bh-hip", fa2=None, sgd=None,
+                   threads=8):
+    """The multilevel wrapper's tick loop (multilevel.rs:333-468) over the
+    oracle FA2 (fa2_run, the Morton tree with exact sums, which the GPU engine
+    matches bit for bit) or the oracle SGD.  Returns (positions of the current
+    level, level, n_levels)."""
+    st = dict(ML_DEFAULTS)
+    st.update(settings or {})
+    n = len(offsets) - 1
+    off = np.asarray(offsets, np.uint32)
+    nb = np.asarray(nbrs, np.uint32)
+    rows = np.repeat(np.arange(n, dtype=np.uint32), np.diff(off.astype(np.int64)))
+    keep = rows < nb
+    edges = np.stack([rows[keep], nb[keep]], 1).reshape(-1)
+    levels = coarsen(n, edges, st["max_levels"], st["target_size"])
+    level = len(levels) - 1
+    pos = fold_up(levels, xyz)
+
+    def bind(lv, p):
+        lo, lnb = level_csr(levels[lv]["n"], levels[lv]["edges"])
+        if inner == "fa2-bh-hip":
+            return {"off": lo, "nb": lnb, "old": None, "se": None}
+        k_set = dict(n_pivots=50, eta_max=1.0, eta_min=0.01, n_anneal_steps=30, seed=0x5EEDC0DE12345678)
+        k_set.update(sgd or {})
+        piv, dist = select_pivots(lo, lnb, k_set["n_pivots"], k_set["seed"])
+        em = k_set["eta_max"]
+        if em <= 0:
+            dm = float(np.float32(dist.max())) if len(dist) else 0.0
+            em = float(np.float32(dm) * np.float32(dm))
+        return {**k_set, "piv": piv, "dist": dist, "eta_max": em, "t": 0}
+
+    def run(state, p, k):
+        if inner == "fa2-bh-hip":
+            p, old, se, _ = fa2_run(state["off"], state["nb"], p, k, fa2, tree_kind=1, exact=True,
+                                    threads=threads, old_force=state["old"], se=state["se"])
+            state["old"], state["se"] = old, se
+            return p
+        for _ in range(k):
+            eta = anneal_eta(state["t"], state["eta_max"], state["eta_min"], state["n_anneal_steps"])
+            p = sgd_sweep(state["piv"], state["dist"], p, eta, threads=threads)
+            state["t"] += 1
+        return p
+
+    state = bind(level, pos)
+    done = 0
+    left = ticks
+    while left > 0:
+        if level == 0:
+            pos = run(state, pos, left)
+            left = 0
+            break
+        budget = sweeps_for_level(st["sweep_schedule"], level, st["sweeps_per_level"])
+        k = min(left, budget - done)
+        pos = run(state, pos, k)
+        done += k
+        left -= k
+        if done >= budget:
+            child = level - 1
+            pos = prolong(pos, levels[level]["parent_map"], levels[child]["n"],
+                          np.float32(st["spring_len"]) * np.float32(0.5), st["seed"] + child + 1)
+            level = child
+            done = 0
+            state = bind(level, pos)
+    return pos, level, len(levels)
+
