@@ -129,7 +129,9 @@ def warm_start(off, nb, xyz, theta: float) -> dict | None:
             "plain_iters_to_target", "plain_ms_to_target", "learned_init_stress",
             "learned_stress_final", "learned_iters_to_target", "inference_ms",
             "learned_ms_to_target", "fallback_to_plain", "speedup",
-            "scaled_random_iters_to_target", "scaled_random_ms_to_target")
+            "scaled_random_iters_to_target", "scaled_random_ms_to_target", "multilevel_levels",
+            "multilevel_descent_ms", "multilevel_level0_start_stress", "multilevel_level0_iters_to_target",
+            "multilevel_ms_to_target")
     return {k: (round(r[k], 5) if isinstance(r[k], float) else r[k]) for k in keep}
 
 

@@ -292,6 +292,7 @@ class MultilevelEngine final : public LayoutEngine {
         level_ = (uint32_t)levels_.size() - 1;
         sweeps_done_ = 0;
         positions_ = fold_up(levels_, xyz, g.n);
+        pos_current_ = true;
         JC_TRY(build_inner());
         JC_TRY(bind_inner());
         inited_ = true;
@@ -302,13 +303,12 @@ class MultilevelEngine final : public LayoutEngine {
     Status step(uint32_t n_iters, float *out_xyz) override {
         if (!inited_) return Status::err(JC_E_STATE, "multilevel step called before successful init");
         uint32_t left = n_iters;
-        bool have_pos = true;  // positions_ holds the current level's positions
         while (left > 0) {
             if (level_ == 0) {
                 JC_TRY(inner_->step(left, nullptr));
                 sweeps_done_ += left;
                 left = 0;
-                have_pos = false;
+                pos_current_ = false;
                 break;
             }
             const uint32_t budget = sweeps_for_level(settings_.schedule, level_, settings_.sweeps_per_level);
@@ -327,10 +327,11 @@ class MultilevelEngine final : public LayoutEngine {
                 sweeps_done_ = 0;
                 JC_TRY(bind_inner());
             }
+            pos_current_ = true;
         }
         if (out_xyz) {
             const uint32_t nl = levels_[level_].n;
-            if (have_pos) std::copy(positions_.begin(), positions_.begin() + 3 * (size_t)nl, out_xyz);
+            if (pos_current_) std::copy(positions_.begin(), positions_.begin() + 3 * (size_t)nl, out_xyz);
             else if (nl) {
                 // level 0: the inner engine holds the positions
                 JC_TRY(inner_->get_state(out_xyz, nullptr, nullptr, nullptr));
@@ -396,6 +397,7 @@ class MultilevelEngine final : public LayoutEngine {
     std::vector<uint32_t> off_, nbr_;
     uint32_t level_ = 0, sweeps_done_ = 0;
     bool inited_ = false;
+    bool pos_current_ = false;  // positions_ holds the current level's positions
 };
 
 }  // namespace

@@ -27,7 +27,7 @@ import time
 
 import numpy as np
 
-from . import (Fa2BhBatch, Fa2BhEngine, Mpnn, SgdStressEngine, StressJudge, _splitmix_pair,
+from . import (Fa2BhBatch, Fa2BhEngine, Mpnn, MultilevelEngine, SgdStressEngine, StressJudge, _splitmix_pair,
                ba_graph, sbm_graph, uniform_cube)
 
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warmstart_weights.npz")
@@ -179,6 +179,39 @@ def stress_curve(off, nb, init, judge, iters: int, every: int, theta: float,
     return curve, dev_ms / max(done, 1)
 
 
+def multilevel_curve(off, nb, init, judge, iters: int, every: int, theta: float,
+                     stop_below: float | None = None, settings: dict | None = None):
+    """The classical multilevel start (multilevel-hip over fa2-bh-hip, the
+    reference's MultilevelSettings defaults unless `settings` overrides them):
+    wall time of init (host coarsening, fold-up, first bind) plus the descent
+    through every coarse level, then the stress at level 0 after every `every`
+    iterations.  Returns (curve over level-0 iterations, descent wall ms, levels)."""
+    st = dict(settings or {})
+    e = MultilevelEngine(inner="fa2-bh-hip", inner_params={"theta": theta}, **st)
+    t0 = time.perf_counter()
+    e.init(off, nb, init)
+    levels = e.state()["n_levels"]
+    base = int(st.get("sweeps_per_level", 30))
+    sched = st.get("sweep_schedule", "linear")
+    descent = 0
+    for lvl in range(1, levels):
+        descent += base if sched == "uniform" else (base * (1 + lvl) if sched == "linear" else base << min(lvl, 16))
+    pos = e.step(descent)
+    descent_ms = (time.perf_counter() - t0) * 1e3
+    assert e.state()["level"] == 0
+    curve = [(0, judge(pos))]
+    done = 0
+    while done < iters:
+        k = min(every, iters - done)
+        pos = e.step(k)
+        done += k
+        s = judge(pos)
+        curve.append((done, s))
+        if stop_below is not None and s <= stop_below:
+            break
+    return curve, descent_ms, levels
+
+
 def first_below(curve, target):
     for it, s in curve:
         if s <= target:
@@ -187,7 +220,8 @@ def first_below(curve, target):
 
 
 def evaluate(net: Mpnn, coef, off, nb, x0, horizon: int = 2000, every: int = 50,
-             theta: float = 0.8, k_pivots: int = 50, ablation: bool = True, log=None):
+             theta: float = 0.8, k_pivots: int = 50, ablation: bool = True, multilevel: bool = True,
+             log=None):
     """Time to reach stress <= 1.01 x the plain converged stress, plain vs learned.
 
     The plain run starts from x0 and runs `horizon` iterations; its stress there
@@ -225,6 +259,19 @@ def evaluate(net: Mpnn, coef, off, nb, x0, horizon: int = 2000, every: int = 50,
         "learned_curve": learned,
     }
     res["speedup"] = None if fallback else res["plain_ms_to_target"] / res["learned_ms_to_target"]
+    if multilevel:
+        # the classical counterpart: coarsen -> solve -> prolong -> refine
+        ml, ml_ms, ml_levels = multilevel_curve(off, nb, x0, judge, horizon, every, theta, stop_below=target)
+        t_ml = first_below(ml, target)
+        res["multilevel_levels"] = ml_levels
+        res["multilevel_descent_ms"] = ml_ms
+        res["multilevel_level0_start_stress"] = ml[0][1]
+        res["multilevel_level0_iters_to_target"] = t_ml
+        res["multilevel_ms_to_target"] = None if t_ml is None else ml_ms + t_ml * ms_it
+        res["multilevel_curve"] = ml
+        if log:
+            log(f"multilevel: {ml_levels} levels, descent {ml_ms:.0f} ms, level-0 start S={ml[0][1]:.4f}, "
+                f"target after {t_ml} more iterations")
     if ablation:
         R = size_of(coef, len(off) - 1, len(nb))
         p = np.asarray(x0, np.float64).reshape(-1, 3)
