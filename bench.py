@@ -4,7 +4,7 @@ workload (m=4, positions uniform in [-10,10]^3, theta 0.8).
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--no-cpu-baseline]
                     [--no-warm-start] [--copy-probe]
-                    [--workload ba_1m|batch_4096|pagerank_ba1m|sgd_rgg200k]
+                    [--workload ba_1m|batch_4096|pagerank_ba1m|sgd_rgg200k|traverse]
 
 A step is one full FA2+BH iteration (octree build, walk, attraction, gravity,
 adaptive speed, apply) with inputs already resident in HBM.  The timed region
@@ -25,7 +25,9 @@ The other workloads print one JSON line each for the sibling paths:
   batch_4096     4,096 graphs (50..20,000 nodes) in one batch, 500 iterations,
                  against the same graphs run one engine at a time (sampled);
   pagerank_ba1m  PageRank on the 1M-node BA CSR, d=0.85, L1 tol 1e-6;
-  sgd_rgg200k    SGD sparse stress on the 200k-node 2-D RGG, 200 pivots, 30 epochs.
+  sgd_rgg200k    SGD sparse stress on the 200k-node 2-D RGG, 200 pivots, 30 epochs;
+  traverse       single-source BFS on the 200k-node RGG and the 1M-node BA graph,
+                 and their connected components.
 """
 from __future__ import annotations
 
@@ -151,7 +153,7 @@ def bench_batch(args) -> int:
     import jumpcannon as jc
     from jumpcannon import workloads
 
-    iters = 500
+    iters = args.batch_iters
     t = time.perf_counter()
     graphs, pos = workloads.batch_4096()
     total = sum(len(o) - 1 for o, _ in graphs)
@@ -162,7 +164,7 @@ def bench_batch(args) -> int:
     b_ms, bs = b.step_timed(iters)
     log(f"batch: {iters} iterations in {b_ms:.1f} ms (force block {bs})")
     seq_ms = 0.0
-    sample = range(0, len(graphs), 16)
+    sample = range(0, len(graphs), 16) if not args.no_sequential else range(0)
     for g in sample:
         off, nb = graphs[g]
         e = jc.Fa2BhEngine(theta=1.0)
@@ -171,7 +173,7 @@ def bench_batch(args) -> int:
         _, it_ms = e.step_timed(20)
         seq_ms += it_ms / 20 * iters
         e.close()
-    seq_ms *= len(graphs) / len(sample)
+    seq_ms *= len(graphs) / max(len(sample), 1)
     log(f"one engine per graph (sampled): {seq_ms:.1f} ms for {iters} iterations")
     return _line("batched FA2+BH node-iterations/s (4,096 graphs)", total * iters / (b_ms / 1e3),
                  "node-iterations/s", iters, 2, b_ms / iters, "f32",
@@ -179,8 +181,8 @@ def bench_batch(args) -> int:
                   "sizes": "log-uniform 50..20000, half SBM half BA", "theta": 1.0,
                   "parallelism": "single-gpu"},
                  batch_ms=round(b_ms, 2), force_block=bs,
-                 sequential_ms_estimate=round(seq_ms, 1),
-                 speedup_vs_sequential=round(seq_ms / b_ms, 2),
+                 sequential_ms_estimate=round(seq_ms, 1) if len(sample) else None,
+                 speedup_vs_sequential=round(seq_ms / b_ms, 2) if len(sample) else None,
                  sequential_sample="every 16th graph, its own engine, 20 timed iterations, x16")
 
 
@@ -236,6 +238,48 @@ def bench_sgd(args) -> int:
                            "traffic": None, "bytes_per_epoch": algo})
 
 
+def bench_traverse(args) -> int:
+    """Single-source BFS on the 200k-node RGG (the stress workload's pivot rows:
+    hundreds of thin levels, one workgroup) and on the 1M-node BA graph (a few
+    levels of hundreds of thousands of nodes, chip-wide), plus connected
+    components.  Device time per search from HIP events, median over sources."""
+    import jumpcannon as jc
+    from jumpcannon import workloads
+
+    out = {}
+    for name, (off, nb) in (("rgg_200k", workloads.rgg_200k()[:2]), ("ba_1m", workloads.ba_1m(args.nodes)[:2])):
+        n, nnz = len(off) - 1, len(nb)
+        plan = jc.BfsPlan(off, nb)
+        d, _ = plan.run(0)
+        levels = int(d[d != 0xFFFFFFFF].max()) + 1
+        times = []
+        for src in range(0, n, max(n // 16, 1)):
+            times.append(plan.run(src, readback=False)[1])
+        ms = float(np.median(times))
+        t = time.perf_counter()
+        jc.connected_components(off, nb)
+        cc_ms = (time.perf_counter() - t) * 1e3
+        out[name] = {"n_nodes": n, "csr_nnz": nnz, "levels_from_0": levels, "searches": len(times),
+                     "bfs_ms": round(ms, 4),
+                     "teps": round((nnz / 2) / (ms / 1e3), 1), "us_per_level": round(1e3 * ms / levels, 3),
+                     "components_wall_ms_incl_upload": round(cc_ms, 2)}
+        log(f"bfs {name}: {ms:.3f} ms per search ({levels} levels from node 0), "
+            f"{(nnz / 2) / (ms / 1e3) / 1e9:.3f} G edges/s; components {cc_ms:.1f} ms")
+    r = out["rgg_200k"]
+    algo = 132 * r["n_nodes"]  # one padded 128-byte row per node, one 4-byte distance
+    achieved = algo / (r["bfs_ms"] / 1e3) / 1e9
+    return _line("BFS traversed edges/s (200k-node RGG, one search)", r["teps"], "edges/s", r["searches"],
+                 1, r["bfs_ms"], "u32",
+                 {"workload": "bfs_rgg200k", "n_nodes": r["n_nodes"], "csr_nnz": r["csr_nnz"],
+                  "parallelism": "single-gpu"},
+                 graphs=out,
+                 roofline={"bound": "latency", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                           "bytes_per_search": algo,
+                           "limiter": "level-synchronous: one dependent row load and one barrier per level "
+                                      "in one workgroup"})
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -247,8 +291,11 @@ def main() -> int:
     ap.add_argument("--profile-iters", type=int, default=4)
     ap.add_argument("--no-warm-start", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--batch-iters", type=int, default=500)
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="batch_4096: skip the one-engine-per-graph estimate")
     ap.add_argument("--workload", default="ba_1m",
-                    choices=["ba_1m", "batch_4096", "pagerank_ba1m", "sgd_rgg200k"])
+                    choices=["ba_1m", "batch_4096", "pagerank_ba1m", "sgd_rgg200k", "traverse"])
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe()
@@ -258,7 +305,7 @@ def main() -> int:
         return 2
     if args.workload != "ba_1m":
         return {"batch_4096": bench_batch, "pagerank_ba1m": bench_pagerank,
-                "sgd_rgg200k": bench_sgd}[args.workload](args)
+                "sgd_rgg200k": bench_sgd, "traverse": bench_traverse}[args.workload](args)
 
     import jumpcannon as jc
     from jumpcannon import workloads
