@@ -53,11 +53,12 @@ int decode_json_object(const char *json, size_t len, const std::vector<JsonField
                        const char *what);
 
 // traverse.hip: max-min pivot selection on the device (sgd_stress.rs:143-175).
-// dist_pm_dev[t*n + v] = hop distance of v from pivot t as u16 (0 for the pivot
-// itself and for unreachable nodes); pivots_dev[k]; *dmax = largest such
+// dist8_dev[((t/8) * n + v) * 8 + t%8] = hop distance of v from pivot t as u16
+// (0 for the pivot itself and for unreachable nodes; the caller zeroes the
+// buffer, ceil(k/8) * n * 8 entries); pivots_dev[k]; *dmax = largest such
 // distance.  JC_E_INVALID when a finite distance exceeds 65535.
 Status select_pivots_device(const CsrView &g, uint32_t k, uint32_t first, hipStream_t stream,
-                            uint32_t *pivots_dev, uint16_t *dist_pm_dev, std::vector<uint32_t> &h_pivots,
+                            uint32_t *pivots_dev, uint16_t *dist8_dev, std::vector<uint32_t> &h_pivots,
                             uint32_t *dmax);
 
 LayoutEngine *make_fa2bh(const jc_fa2bh_settings &s);

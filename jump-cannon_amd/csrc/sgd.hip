@@ -7,10 +7,12 @@
 // applied in pivot order to the node's own running position (Gauss-Seidel over
 // its pivots, Jacobi across nodes: q comes from the sweep's input positions).
 //
-// MI355X layout: hop distances are u16 (0 = skip) stored pivot-major, so lane i
-// of a wave reads dist[t][i] coalesced; the k pivot positions of the sweep are
-// staged in LDS once per block; w and mu depend only on d and eta, so they are
-// tabulated per sweep in LDS (values identical to the per-term expressions).
+// MI355X layout: hop distances are u16 (0 = skip) in chunks of eight pivots per
+// node, dist8[c][i] = the distances of node i to pivots 8c..8c+7 (16 bytes), so
+// lane i of a wave reads one 16-byte word per eight terms, coalesced across the
+// wave; the k pivot positions and ids of the sweep are staged in LDS once per
+// block; w and mu depend only on d and eta, so they are tabulated per sweep in
+// LDS (values identical to the per-term expressions).
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -25,18 +27,28 @@ constexpr uint32_t kMaxTable = 8192;  // distances above this use the inline for
 
 __device__ __forceinline__ float len3f(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
 
+// One sweep (sgd_stress.wgsl:34-62).  dist8[c * n + i] holds node i's u16 hop
+// distances to pivots 8c .. 8c+7 (0: skip; the padding past k is 0), so one
+// 16-byte load serves eight terms and the next chunk's load is in flight while
+// the current eight are applied.  LDS holds the pivot positions and ids (when
+// they fit) and the mu table; the terms run in pivot order as in the reference.
+template <bool LDS_PIV>
 __global__ __launch_bounds__(kSgdTB) void k_sgd_sweep(uint32_t n, uint32_t k,
                                                       const uint32_t *__restrict__ pivots,
-                                                      const uint16_t *__restrict__ dist,
+                                                      const uint4 *__restrict__ dist8,
                                                       const float4 *__restrict__ pos_in,
                                                       float4 *__restrict__ pos_out, float eta,
-                                                      uint32_t dmax, uint32_t lds_pivots) {
+                                                      uint32_t dmax) {
     extern __shared__ float4 smem[];
-    float4 *piv = smem;  // k pivot positions (when lds_pivots)
-    float *mu_tab = reinterpret_cast<float *>(smem + (lds_pivots ? k : 0));  // dmax+1 entries
+    float4 *piv = smem;                                                            // k positions
+    uint32_t *pid = reinterpret_cast<uint32_t *>(smem + (LDS_PIV ? k : 0));       // k ids
+    float *mu_tab = reinterpret_cast<float *>(pid + (LDS_PIV ? k : 0));            // dmax+1 entries
     const bool tab = dmax <= kMaxTable;
-    if (lds_pivots)
-        for (uint32_t t = threadIdx.x; t < k; t += blockDim.x) piv[t] = pos_in[pivots[t]];
+    if (LDS_PIV)
+        for (uint32_t t = threadIdx.x; t < k; t += blockDim.x) {
+            pid[t] = pivots[t];
+            piv[t] = pos_in[pivots[t]];
+        }
     if (tab)
         for (uint32_t d = threadIdx.x; d <= dmax; d += blockDim.x) {
             const float df = (float)d;
@@ -48,24 +60,38 @@ __global__ __launch_bounds__(kSgdTB) void k_sgd_sweep(uint32_t n, uint32_t k,
     if (i >= n) return;
     const float4 p0 = pos_in[i];
     float px = p0.x, py = p0.y, pz = p0.z;
-    for (uint32_t t = 0; t < k; t++) {
-        const uint32_t du = dist[(size_t)t * n + i];
-        if (du == 0u) continue;                  // self / unreachable sentinel
-        if (pivots[t] == i) continue;
-        const float d = (float)du;
-        const float4 q = lds_pivots ? piv[t] : pos_in[pivots[t]];
-        const float dx = px - q.x, dy = py - q.y, dz = pz - q.z;
-        const float mag = fmaxf(len3f(dx, dy, dz), 1e-6f);
-        float mu;
-        if (tab) mu = mu_tab[du];
-        else {
-            const float w = 1.0f / (d * d);
-            mu = fminf(1.0f, w * eta);
+    const uint32_t kc = (k + 7) >> 3;
+    uint4 cur = kc ? dist8[i] : make_uint4(0u, 0u, 0u, 0u);
+    for (uint32_t c = 0; c < kc; c++) {
+        const uint4 nxt = c + 1 < kc ? dist8[(size_t)(c + 1) * n + i] : make_uint4(0u, 0u, 0u, 0u);
+        const uint32_t wd[4] = {cur.x, cur.y, cur.z, cur.w};
+        // Branch-free over the chunk, so the eight terms' LDS reads issue
+        // together: a skipped term (distance 0: self, unreachable, padding; or
+        // the node itself as pivot) is computed and then not taken, by select,
+        // which leaves px/py/pz bit for bit unchanged.
+#pragma unroll
+        for (uint32_t j = 0; j < 8; j++) {
+            const uint32_t du = (wd[j >> 1] >> ((j & 1u) * 16u)) & 0xFFFFu;
+            const uint32_t t = min(c * 8 + j, k - 1);
+            const uint32_t pv = LDS_PIV ? pid[t] : pivots[t];
+            const bool take = du != 0u && pv != i;
+            const float d = (float)du;
+            const float4 q = LDS_PIV ? piv[t] : pos_in[pv];
+            const float dx = px - q.x, dy = py - q.y, dz = pz - q.z;
+            const float mag = fmaxf(len3f(dx, dy, dz), 1e-6f);
+            float mu;
+            if (tab) mu = mu_tab[du];
+            else {
+                const float w = 1.0f / (d * d);
+                mu = fminf(1.0f, w * eta);
+            }
+            const float r = mu * (mag - d) / mag;
+            const float nx = px - r * dx, ny = py - r * dy, nz = pz - r * dz;
+            px = take ? nx : px;
+            py = take ? ny : py;
+            pz = take ? nz : pz;
         }
-        const float r = mu * (mag - d) / mag;
-        px = px - r * dx;
-        py = py - r * dy;
-        pz = pz - r * dz;
+        cur = nxt;
     }
     pos_out[i] = make_float4(px, py, pz, 0.0f);
 }
@@ -193,7 +219,9 @@ class SgdStressEngine final : public LayoutEngine {
             Rng rng{settings_.seed};
             const uint32_t first = (uint32_t)(rng.next() % n_);
             JC_TRY(pivots_.alloc(k_));
-            JC_TRY(dist_.alloc((size_t)k_ * n_));
+            const size_t chunks = (k_ + 7) / 8;
+            JC_TRY(dist_.alloc(chunks * n_ * 8));
+            JC_HIP(hipMemsetAsync(dist_.p, 0, chunks * n_ * 8 * sizeof(uint16_t), stream_));
             JC_TRY(select_pivots_device(g, k_, first, stream_, pivots_.p, dist_.p, pivots, &dmax_));
         } else {
             JC_TRY(pivots_.alloc(1));
@@ -217,13 +245,19 @@ class SgdStressEngine final : public LayoutEngine {
         if (n_ == 0 || k_ == 0) return Status::ok();
         const uint32_t sweeps = std::max<uint32_t>(settings_.sweeps_per_step, 1);
         const size_t tab = dmax_ <= kMaxTable ? (size_t)(dmax_ + 1) * sizeof(float) : 0;
-        const uint32_t lds_piv = (k_ * sizeof(float4) + tab <= 60 * 1024) ? 1u : 0u;
-        const size_t shmem = (lds_piv ? k_ * sizeof(float4) : 0) + tab;
+        const size_t piv_bytes = (size_t)k_ * (sizeof(float4) + sizeof(uint32_t));
+        const bool lds_piv = piv_bytes + tab <= 60 * 1024;
+        const size_t shmem = (lds_piv ? piv_bytes : 0) + tab;
+        const uint4 *d8 = reinterpret_cast<const uint4 *>(dist_.p);
         for (uint32_t it = 0; it < n_iters; it++) {
             const float eta = anneal_eta(sweep_, eta_max_, settings_.eta_min, settings_.n_anneal_steps);
             for (uint32_t s = 0; s < sweeps; s++) {
-                k_sgd_sweep<<<cdiv(n_, kSgdTB), kSgdTB, shmem, stream_>>>(
-                    n_, k_, pivots_.p, dist_.p, pos_[cur_].p, pos_[cur_ ^ 1].p, eta, dmax_, lds_piv);
+                if (lds_piv)
+                    k_sgd_sweep<true><<<cdiv(n_, kSgdTB), kSgdTB, shmem, stream_>>>(
+                        n_, k_, pivots_.p, d8, pos_[cur_].p, pos_[cur_ ^ 1].p, eta, dmax_);
+                else
+                    k_sgd_sweep<false><<<cdiv(n_, kSgdTB), kSgdTB, shmem, stream_>>>(
+                        n_, k_, pivots_.p, d8, pos_[cur_].p, pos_[cur_ ^ 1].p, eta, dmax_);
                 cur_ ^= 1;
             }
             sweep_ += sweeps;
@@ -268,11 +302,13 @@ class SgdStressEngine final : public LayoutEngine {
     const std::vector<uint32_t> &pivots() const { return h_pivots_; }
     Status read_dist(uint16_t *out) {
         // back to the reference's node-major layout dist[i*k + t]
-        std::vector<uint16_t> pm((size_t)k_ * n_);
-        if (!pm.empty())
-            JC_HIP(hipMemcpy(pm.data(), dist_.p, pm.size() * 2, hipMemcpyDeviceToHost));
+        const size_t chunks = (k_ + 7) / 8;
+        std::vector<uint16_t> c8(chunks * n_ * 8);
+        if (!c8.empty())
+            JC_HIP(hipMemcpy(c8.data(), dist_.p, c8.size() * 2, hipMemcpyDeviceToHost));
         for (uint32_t t = 0; t < k_; t++)
-            for (uint32_t i = 0; i < n_; i++) out[(size_t)i * k_ + t] = pm[(size_t)t * n_ + i];
+            for (uint32_t i = 0; i < n_; i++)
+                out[(size_t)i * k_ + t] = c8[((size_t)(t >> 3) * n_ + i) * 8 + (t & 7)];
         return Status::ok();
     }
     float eta_max() const { return eta_max_; }

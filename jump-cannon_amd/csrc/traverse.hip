@@ -202,9 +202,10 @@ __global__ __launch_bounds__(kTB) void k_build_ell(uint32_t n, const uint32_t *_
 // issued together (as many as the wave's longest row), one queue reservation
 // per wave (wave prefix sum of the claim counts), and one barrier (two when a
 // row longer than kEllW needs the whole block).  A node's distance is written
-// when its own row is processed, so claims touch LDS only.  Counters rotate over three slots so a single barrier orders
-// every reset before the next use.  On exit BfsState says whether the search
-// ended or which global queue holds the next, larger frontier.
+// when its own row is processed, so claims touch LDS only.  Counters rotate
+// over three slots so a single barrier orders every reset before the next use.
+// On exit BfsState says whether the search ended or which global queue holds
+// the next, larger frontier.
 __global__ __launch_bounds__(kBfsThreads) void k_bfs_block_ell(const uint4 *__restrict__ ell,
                                                                const uint32_t *__restrict__ off,
                                                                const uint32_t *__restrict__ nbr, uint32_t n,
@@ -434,11 +435,12 @@ __global__ __launch_bounds__(kLevelTB) void k_bfs_level(const uint32_t *__restri
 // After the BFS of pivot t: mind[v] = min(mind[v], d[v]) over reachable v; the
 // next pivot is the node of largest finite cover mind[v] > 0, smallest index on
 // ties (the reference's strict '>' in node order), kept in *best as
-// (cover << 32) | ~v.  Row t of the pivot-major u16 matrix gets d[v], 0 for the
-// pivot itself and for unreachable nodes; *bad flags a finite d above 65535.
+// (cover << 32) | ~v.  Column t of the u16 distance matrix (chunks of eight
+// pivots per node: dist8[((t/8) * n + v) * 8 + t%8]) gets d[v], 0 for the pivot
+// itself and for unreachable nodes; *bad flags a finite d above 65535.
 __global__ __launch_bounds__(kTB) void k_pivot_select(uint32_t n, const uint32_t *__restrict__ dist,
                                                       uint32_t *__restrict__ mind,
-                                                      uint16_t *__restrict__ row,
+                                                      uint16_t *__restrict__ dist8, uint32_t t,
                                                       unsigned long long *best, uint32_t *dmax,
                                                       uint32_t *bad) {
     __shared__ unsigned long long s_key[kTB / 64];
@@ -456,7 +458,7 @@ __global__ __launch_bounds__(kTB) void k_pivot_select(uint32_t n, const uint32_t
         const bool fin = dv != kUnreached && dv != 0u;
         if (fin && dv > 65535u) b = 1u;
         if (fin) dm = max(dm, dv);
-        row[v] = fin ? (uint16_t)dv : (uint16_t)0;
+        dist8[((size_t)(t >> 3) * n + v) * 8 + (t & 7u)] = fin ? (uint16_t)dv : (uint16_t)0;
     }
     for (int o = 32; o > 0; o >>= 1) {
         key = max(key, (unsigned long long)__shfl_xor((unsigned long long)key, o));
@@ -873,10 +875,10 @@ class BfsPlan {
 
 // Max-min pivots on the device (sgd_stress.rs:143-175): the first pivot is
 // `first`, each later one the node farthest (in hops) from those chosen so far.
-// dist_pm[t*n + v] = hop distance of v from pivot t as u16 (0: the pivot itself
-// or unreachable).
+// dist8[((t/8) * n + v) * 8 + t%8] = hop distance of v from pivot t as u16 (0:
+// the pivot itself or unreachable); the caller zeroes the buffer.
 Status select_pivots_device(const CsrView &g, uint32_t k, uint32_t first, hipStream_t stream,
-                            uint32_t *pivots_dev, uint16_t *dist_pm_dev, std::vector<uint32_t> &h_pivots,
+                            uint32_t *pivots_dev, uint16_t *dist8_dev, std::vector<uint32_t> &h_pivots,
                             uint32_t *dmax_out) {
     h_pivots.clear();
     *dmax_out = 0;
@@ -894,8 +896,8 @@ Status select_pivots_device(const CsrView &g, uint32_t k, uint32_t first, hipStr
     k_fill_u32<<<1, kTB, 0, stream>>>(small.p, 1, first);
     for (uint32_t t = 0; t < k; t++) {
         JC_TRY(bfs.run(small.p, 0, best.p));
-        k_pivot_select<<<grid_for(n), kTB, 0, stream>>>(n, bfs.dist(), mind.p, dist_pm_dev + (size_t)t * n,
-                                                        best.p, small.p + 1, small.p + 2);
+        k_pivot_select<<<grid_for(n), kTB, 0, stream>>>(n, bfs.dist(), mind.p, dist8_dev, t, best.p,
+                                                        small.p + 1, small.p + 2);
         k_pivot_next<<<1, 1, 0, stream>>>(best.p, small.p, pivots_dev + t);
         JC_HIP(hipGetLastError());
     }
