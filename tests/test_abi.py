@@ -140,3 +140,30 @@ def test_csr_bin_rejects_truncated_and_length_mismatch():
     assert "length mismatch" in str(ei.value) and "expected 60" in str(ei.value)
     with pytest.raises(jc.JcError):
         jc.csr_from_bin(jc.csr_to_bin(*_path_csr(5)) + b"\0")
+
+
+def test_multilevel_settings_decode_without_a_device():
+    """multilevel.rs:307-331 (and its unit tests rejects_self_wrap,
+    rejects_unknown_inner, accepts_known_inner): settings decode and the inner
+    id checks run on the host, before any device work."""
+    e = jc.MultilevelEngine()
+    assert e.descriptor["id"] == "multilevel-hip"
+    e.set_params({"inner": "sgd-stress-hip"})
+    e.set_params({"inner": "fa2-bh-hip", "inner_params": {"theta": 0.9, "nested": [1, {"a": "}"}]},
+                  "max_levels": 3, "target_size": 100, "sweeps_per_level": 7, "sweep_schedule": "geometric",
+                  "spring_len": 2.5, "seed": 12345})
+    e.set_params(None)
+    e.set_params({"inner_params": None})  # null forwards the inner defaults
+    for bad, msg in (({"inner": "multilevel"}, "cannot wrap itself"),
+                     ({"inner": "fa2-bh"}, "unknown inner engine"),
+                     ({"inner": 3}, "invalid type for field `inner`"),
+                     ({"sweep_schedule": "Linear"}, "unknown variant"),
+                     ({"seed": -1}, "invalid type"),
+                     ({"max_levels": 2.5}, "invalid type"),
+                     ('{"inner": "fa2-bh-hip"', "expected")):
+        with pytest.raises(jc.JcError) as ei:
+            e.set_params(bad)
+        assert ei.value.code == jc.JC_E_INVALID and msg in str(ei.value), (bad, str(ei.value))
+    with pytest.raises(jc.JcError) as ei:
+        e.state()  # before init
+    assert ei.value.code == jc.JC_E_STATE
