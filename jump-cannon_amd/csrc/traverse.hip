@@ -25,7 +25,8 @@
 //   * k_bfs_level: one launch per level over the whole chip for frontiers that
 //     outgrow the LDS queue (scale-free graphs reach half the graph within a
 //     few hops); visited = atomicCAS on the distance array, appends aggregated
-//     per wave.
+//     per wave.  Frontiers above n/32 run bottom-up instead (k_bfs_level_up:
+//     each unreached node stops at its first neighbour in the frontier).
 // The host switches between the two by the frontier size the block kernel
 // leaves in BfsState.
 //
@@ -53,6 +54,7 @@ constexpr uint32_t kBfsBmNodes = kBfsBmWords * 32;
 constexpr uint32_t kBfsLight = 32;          // longer rows are expanded by the whole block
 constexpr uint32_t kLevelTB = 256;
 constexpr uint32_t kWaveHeavy = 64;         // k_bfs_level / k_cc_hook: longer rows go wave-wide
+constexpr uint32_t kBottomUpShare = 32;     // bottom-up levels for frontiers above n / 32
 constexpr uint32_t kTB = 256;
 
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
@@ -428,6 +430,36 @@ __global__ __launch_bounds__(kLevelTB) void k_bfs_level(const uint32_t *__restri
                 wave_append(fresh, u, qout, qn_out);
             }
         }
+    }
+}
+
+// One level bottom-up (direction-optimizing BFS, Beamer, Asanovic & Patterson,
+// SC 2012), for frontiers holding a large share of the graph: every node still
+// unreached scans its own row in CSR order and takes distance lvl + 1 at the
+// first neighbour whose distance is lvl.  A frontier node's distance does not
+// change during the level and a node written in it gets lvl + 1, so no claim
+// races and the distances are the queue BFS's.  The new frontier is appended to
+// qout (one reservation per wave).
+__global__ __launch_bounds__(kLevelTB) void k_bfs_level_up(const uint32_t *__restrict__ off,
+                                                           const uint32_t *__restrict__ nbr,
+                                                           uint32_t *__restrict__ dist, uint32_t n,
+                                                           uint32_t lvl, uint32_t *__restrict__ qout,
+                                                           uint32_t *qn_out) {
+    const uint32_t lane = __lane_id();
+    const uint32_t stride = gridDim.x * kLevelTB;
+    for (uint32_t base = blockIdx.x * kLevelTB + (threadIdx.x & ~63u); base < n; base += stride) {
+        const uint32_t v = base + lane;
+        bool fresh = false;
+        if (v < n && dist[v] == kUnreached) {
+            const uint32_t e = off[v + 1];
+            for (uint32_t j = off[v]; j < e; j++)
+                if (dist[nbr[j]] == lvl) {
+                    fresh = true;
+                    break;
+                }
+            if (fresh) dist[v] = lvl + 1;
+        }
+        wave_append(fresh, v, qout, qn_out);
     }
 }
 
@@ -811,9 +843,16 @@ class BfsPlan {
             // fits again (or the search ends)
             while (h_.qn > kBfsQ) {
                 k_fill_u32<<<1, kTB, 0, stream_>>>(cnt_.p, 1, 0u);
-                const uint32_t blocks = std::min<uint32_t>(cdiv(h_.qn, kLevelTB), 4096);
-                k_bfs_level<<<blocks, kLevelTB, 0, stream_>>>(off_.p, nbr_.p, dist_.p, q_[h_.cur].p, h_.qn,
-                                                              q_[h_.cur ^ 1].p, cnt_.p, h_.level + 1);
+                if ((uint64_t)h_.qn * kBottomUpShare > n_) {
+                    // large frontier: every unreached node looks for a parent
+                    const uint32_t blocks = std::min<uint32_t>(cdiv(n_, kLevelTB), 4096);
+                    k_bfs_level_up<<<blocks, kLevelTB, 0, stream_>>>(off_.p, nbr_.p, dist_.p, n_, h_.level,
+                                                                      q_[h_.cur ^ 1].p, cnt_.p);
+                } else {
+                    const uint32_t blocks = std::min<uint32_t>(cdiv(h_.qn, kLevelTB), 4096);
+                    k_bfs_level<<<blocks, kLevelTB, 0, stream_>>>(off_.p, nbr_.p, dist_.p, q_[h_.cur].p,
+                                                                  h_.qn, q_[h_.cur ^ 1].p, cnt_.p, h_.level + 1);
+                }
                 JC_HIP(hipGetLastError());
                 JC_HIP(hipMemcpyAsync(&h_qn_, cnt_.p, 4, hipMemcpyDeviceToHost, stream_));
                 JC_HIP(hipStreamSynchronize(stream_));
