@@ -162,10 +162,34 @@ __global__ __launch_bounds__(kBT) void k_b_keys(const float4 *__restrict__ pos,
 }
 
 // After the key sort: key := graph id of the body (for the graph-major passes).
-__global__ void k_b_gid_keys(const uint32_t *__restrict__ vals, const uint32_t *__restrict__ gid,
-                             uint64_t *__restrict__ keys, uint32_t n) {
+// keys[i] = graph of body vals[i].  Bodies are numbered graph after graph, so
+// the graph is the last one whose first body is <= vals[i]: a binary search
+// over the graphs' first-body table, staged in LDS (a random gather from the
+// per-body graph array misses the caches on every key).
+constexpr uint32_t kGidLds = 8192;  // graphs whose table fits LDS (32 KB)
+__global__ __launch_bounds__(256) void k_b_gid_keys(const uint32_t *__restrict__ vals,
+                                                    const uint32_t *__restrict__ gid,
+                                                    const uint32_t *__restrict__ base, uint32_t G,
+                                                    uint64_t *__restrict__ keys, uint32_t n) {
+    __shared__ uint32_t sb[kGidLds];
+    const bool lds = G <= kGidLds;
+    if (lds)
+        for (uint32_t g = threadIdx.x; g < G; g += blockDim.x) sb[g] = base[g];
+    __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) keys[i] = gid[vals[i]];
+    if (i >= n) return;
+    const uint32_t v = vals[i];
+    if (!lds) {
+        keys[i] = gid[v];
+        return;
+    }
+    uint32_t lo = 0, hi = G;  // first graph whose first body is > v
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (sb[mid] <= v) lo = mid + 1;
+        else hi = mid;
+    }
+    keys[i] = lo - 1;
 }
 
 __global__ void k_b_gather_keys(const uint32_t *__restrict__ vals,
@@ -934,7 +958,7 @@ class Fa2BhBatch {
                                                      keys_raw_.p, keys_.p, vals_.p);
         JC_TRY(sort_morton_pairs(keys_.p, vals_.p, sort_, stream_));
         if (G_ > 1) {
-            k_b_gid_keys<<<cdivb(n, 256), 256, 0, stream_>>>(vals_.p, gid_.p, keys_.p, n);
+            k_b_gid_keys<<<cdivb(n, 256), 256, 0, stream_>>>(vals_.p, gid_.p, base_.p, G_, keys_.p, n);
             JC_TRY(radix_sort_pairs_u64(keys_.p, vals_.p, sort_, stream_, 0, gbits_));
             k_b_gather_keys<<<cdivb(n, 256), 256, 0, stream_>>>(vals_.p, keys_raw_.p, keys_.p, n);
         }
