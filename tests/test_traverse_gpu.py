@@ -166,3 +166,20 @@ def test_sgd_pivot_distance_beyond_u16_is_rejected():
     with pytest.raises(jc.JcError) as err:
         e.init(off, nb, np.zeros(3 * 140_000, np.float32))
     assert "u16" in str(err.value)
+
+
+def test_bfs_large_grid_many_levels_over_the_csr():
+    """An 800 x 700 grid (560,000 nodes, above the LDS bitmap's 524,288) keeps
+    its frontier under the LDS queue for ~1,500 levels: the workgroup loop over
+    the CSR with atomicCAS claims.  From a corner the distance is Manhattan."""
+    rows, cols = 700, 800
+    v = np.arange(rows * cols, dtype=np.uint32).reshape(rows, cols)
+    pairs = np.concatenate([np.stack([v[:, :-1].ravel(), v[:, 1:].ravel()], 1),
+                            np.stack([v[:-1, :].ravel(), v[1:, :].ravel()], 1)]).astype(np.uint32)
+    off, nb = O.csr_from_pairs(rows * cols, pairs.reshape(-1))
+    d = jc.bfs(off, nb, 0)
+    r, c = np.divmod(np.arange(rows * cols), cols)
+    assert np.array_equal(d, (r + c).astype(np.uint32))
+    mid = (rows // 2) * cols + cols // 2
+    assert np.array_equal(jc.bfs(off, nb, mid), O.bfs(off, nb, mid))
+    assert (jc.connected_components(off, nb) == 0).all()
