@@ -45,6 +45,14 @@ def _cases():
     out.append(("sbm2000", off, nb, xyz))
     off, nb = O.ba_graph(20000, 4, 5)
     out.append(("ba20000", off, nb, O.uniform_cube(20000, 10.0, 6)))
+    # coincident bodies: positions on a 0.5 grid, many shared exactly (a body's
+    # own leaf and its twins at distance 0)
+    off, nb = O.ba_graph(3000, 3, 8)
+    xyz = (np.round(O.uniform_cube(3000, 2.0, 9) * 2.0) / 2.0).astype(np.float32)
+    out.append(("dup3000", off, nb, xyz))
+    # a root half-size of about 1e-32: cell sizes near the bottom of the f32 range
+    off, nb = O.ba_graph(300, 2, 10)
+    out.append(("tiny300", off, nb, (O.uniform_cube(300, 1.0, 12) * np.float32(1e-32)).astype(np.float32)))
     return out
 
 
