@@ -211,6 +211,38 @@ def bench_pagerank(args) -> int:
                            "traffic": None, "bytes_per_iteration": algo})
 
 
+def bench_spmv(args) -> int:
+    """y = A x on the 1M-node BA CSR with PageRank's per-entry weights 1/deg(u)
+    (the SpMV form of one pull step): products/s of the serial f32 product, with
+    the f16-storage and hybrid variants timed beside it."""
+    import jumpcannon as jc
+    from jumpcannon import workloads
+
+    off, nb, _ = workloads.ba_1m(args.nodes)
+    n, nnz = len(off) - 1, len(nb)
+    deg = np.diff(off).astype(np.float32)
+    w = (np.float32(1.0) / deg)[nb].astype(np.float32)
+    x = np.random.default_rng(1).uniform(0, 2.0 / n, n).astype(np.float32)
+    times = {}
+    for name, kind in (("f32", jc.SPMV_F32), ("f16", jc.SPMV_F16), ("hybrid", jc.SPMV_HYBRID)):
+        plan = jc.SpmvPlan(off, nb, w, kind)
+        plan.run(x)
+        times[name] = plan.time_products(100)
+        del plan
+    ms = times["f32"]
+    # offsets, column ids, f32 weights, x and y once each
+    algo = 4 * (n + 1) + 8 * nnz + 8 * n
+    achieved = algo / (ms / 1e3) / 1e9
+    log("spmv: " + ", ".join(f"{k} {v:.4f} ms" for k, v in times.items()))
+    return _line("SpMV products/s at 1M nodes (BA m=4, PageRank weights)", 1e3 / ms, "products/s", 100,
+                 1, ms, "f32",
+                 {"workload": "spmv_ba1m", "n_nodes": n, "csr_nnz": nnz, "parallelism": "single-gpu"},
+                 ms_per_product={k: round(v, 5) for k, v in times.items()},
+                 roofline={"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                           "traffic": None, "bytes_per_product": algo})
+
+
 def bench_sgd(args) -> int:
     """SGD sparse stress on the 200k-node RGG: 200 pivots, 30 epochs."""
     import jumpcannon as jc
@@ -297,7 +329,8 @@ def main() -> int:
     ap.add_argument("--no-sequential", action="store_true",
                     help="batch_4096: skip the one-engine-per-graph estimate")
     ap.add_argument("--workload", default="ba_1m",
-                    choices=["ba_1m", "batch_4096", "pagerank_ba1m", "sgd_rgg200k", "traverse"])
+                    choices=["ba_1m", "batch_4096", "pagerank_ba1m", "spmv_ba1m", "sgd_rgg200k",
+                             "traverse"])
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe()
@@ -306,7 +339,7 @@ def main() -> int:
               "(the layout path is single-GPU in this build).", file=sys.stderr)
         return 2
     if args.workload != "ba_1m":
-        return {"batch_4096": bench_batch, "pagerank_ba1m": bench_pagerank,
+        return {"batch_4096": bench_batch, "pagerank_ba1m": bench_pagerank, "spmv_ba1m": bench_spmv,
                 "sgd_rgg200k": bench_sgd, "traverse": bench_traverse}[args.workload](args)
 
     import jumpcannon as jc

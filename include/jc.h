@@ -195,6 +195,38 @@ int jc_pagerank_plan_run(jc_pagerank_plan *plan, uint32_t max_iters, double tol_
 int jc_pagerank_plan_time(jc_pagerank_plan *plan, uint32_t n_iters, double *ms_per_iter);
 void jc_pagerank_plan_destroy(jc_pagerank_plan *plan);
 
+/* ---- SpMV (crates/graph-compute/src/analytics/spmv.rs) ----
+ * y = A x for a weighted CSR matrix (WeightedCsr, spmv.rs:26-32: n rows, offsets
+ * [n+1], neighbors and weights [nnz]); x and y hold n floats.  n == 0 returns
+ * JC_OK and touches nothing (Ok(Vec::new())).  Rows are summed in f32:
+ *   jc_spmv         gpu_spmv (spmv.rs:103-287): each row in CSR order, as cpu_spmv
+ *                   (:88-99) does;
+ *   jc_spmv_f16     gpu_spmv_f16 (:293-469): weights and x rounded to f16
+ *                   (nearest-even, as pack_f16_pairs :68-86), products and row sums in
+ *                   f32, in CSR order;
+ *   jc_spmv_hybrid  gpu_spmv_hybrid (:491-676): lane l of 64 sums entries start+l,
+ *                   start+l+64, ...; a tree over the 64 partials (stride 32 .. 1)
+ *                   gives the row (shaders/spmv_hybrid.wgsl).
+ * JC_E_INVALID for a malformed CSR (the reference indexes out of bounds) or a NULL
+ * weights / x / y pointer. */
+int jc_spmv(uint32_t n, const uint32_t *offsets, const uint32_t *neighbors, const float *weights,
+            uint64_t nnz, const float *x, float *y);
+int jc_spmv_f16(uint32_t n, const uint32_t *offsets, const uint32_t *neighbors,
+                const float *weights, uint64_t nnz, const float *x, float *y);
+int jc_spmv_hybrid(uint32_t n, const uint32_t *offsets, const uint32_t *neighbors,
+                   const float *weights, uint64_t nnz, const float *x, float *y);
+/* The matrix uploaded once (kind: JC_SPMV_F32 / _F16 / _HYBRID, the three functions
+ * above) and products over it; out_gpu_ms (nullable) is the device time of the
+ * product.  plan_time runs n_iters products of the x of the last run and returns
+ * the mean device ms per product. */
+enum { JC_SPMV_F32 = 0, JC_SPMV_F16 = 1, JC_SPMV_HYBRID = 2 };
+typedef struct jc_spmv_plan jc_spmv_plan;
+int jc_spmv_plan_create(uint32_t n, const uint32_t *offsets, const uint32_t *neighbors,
+                        const float *weights, uint64_t nnz, int kind, jc_spmv_plan **out);
+int jc_spmv_plan_run(jc_spmv_plan *plan, const float *x, float *y, double *out_gpu_ms);
+int jc_spmv_plan_time(jc_spmv_plan *plan, uint32_t n_iters, double *ms_per_product);
+void jc_spmv_plan_destroy(jc_spmv_plan *plan);
+
 /* ---- multilevel wrapper ----
  * Replaces MultilevelEngine (crates/graph-compute/src/engines/multilevel.rs:196-480),
  * registry id "multilevel-hip": coarsen (greedy matching, coarsen.rs:53-160) ->

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libjc_hip.so")
-SOURCES = ["prims.hip", "fa2bh.hip", "batch.hip", "pagerank.hip", "sgd.hip", "traverse.hip", "mpnn.hip", "host.cpp", "multilevel.cpp"]
+SOURCES = ["prims.hip", "fa2bh.hip", "batch.hip", "pagerank.hip", "spmv.hip", "sgd.hip", "traverse.hip", "mpnn.hip", "host.cpp", "multilevel.cpp"]
 # the reference's one-shot analytics binaries (crates/graph-compute/src/bin/)
 CLIS = {"graph-pagerank": "graph_pagerank.cpp", "graph-bfs": "graph_bfs.cpp", "graph-cc": "graph_cc.cpp"}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

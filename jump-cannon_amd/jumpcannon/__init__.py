@@ -107,6 +107,13 @@ SIGNATURES = {
     "jc_pagerank_plan_run": (C.c_int, [_vp, C.c_uint32, C.c_double, _vp, _vp, _vp, _vp]),
     "jc_pagerank_plan_time": (C.c_int, [_vp, C.c_uint32, _vp]),
     "jc_pagerank_plan_destroy": (None, [_vp]),
+    "jc_spmv": (C.c_int, [C.c_uint32, _vp, _vp, _vp, C.c_uint64, _vp, _vp]),
+    "jc_spmv_f16": (C.c_int, [C.c_uint32, _vp, _vp, _vp, C.c_uint64, _vp, _vp]),
+    "jc_spmv_hybrid": (C.c_int, [C.c_uint32, _vp, _vp, _vp, C.c_uint64, _vp, _vp]),
+    "jc_spmv_plan_create": (C.c_int, [C.c_uint32, _vp, _vp, _vp, C.c_uint64, C.c_int, _vp]),
+    "jc_spmv_plan_run": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "jc_spmv_plan_time": (C.c_int, [_vp, C.c_uint32, _vp]),
+    "jc_spmv_plan_destroy": (None, [_vp]),
     "jc_multilevel_create": (C.c_int, [_vp]),
     "jc_multilevel_state": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "jc_bfs": (C.c_int, [C.c_uint32, _vp, _vp, C.c_uint64, C.c_uint32, _vp]),
@@ -633,6 +640,80 @@ class PageRankPlan:
         h = getattr(self, "_h", None)
         if h:
             lib().jc_pagerank_plan_destroy(h)
+            self._h = None
+
+
+# ------------------------------------------------------------------ SpMV
+SPMV_F32, SPMV_F16, SPMV_HYBRID = 0, 1, 2
+
+
+def _spmv(fn, offsets, neighbors, weights, x):
+    offsets, neighbors, weights, x = _u32(offsets), _u32(neighbors), _f32(weights), _f32(x)
+    n = len(offsets) - 1
+    if len(weights) != len(neighbors):
+        raise JcError(JC_E_INVALID, f"neighbors ({len(neighbors)}) and weights ({len(weights)}) "
+                                    "length mismatch")
+    if len(x) != max(n, 0):
+        raise JcError(JC_E_INVALID, f"x length {len(x)} != n ({n})")
+    y = np.empty(max(n, 1), np.float32)
+    _check(fn(n, _ptr(offsets), _ptr(neighbors) if len(neighbors) else None,
+              _ptr(weights) if len(weights) else None, len(neighbors),
+              _ptr(x) if n else None, _ptr(y)))
+    return y[:n]
+
+
+def spmv(offsets, neighbors, weights, x):
+    """gpu_spmv (crates/graph-compute/src/analytics/spmv.rs:103-287): y = A x,
+    each row summed in CSR order in f32."""
+    return _spmv(lib().jc_spmv, offsets, neighbors, weights, x)
+
+
+def spmv_f16(offsets, neighbors, weights, x):
+    """gpu_spmv_f16 (spmv.rs:293-469): weights and x stored as f16 (nearest even),
+    products and sums in f32."""
+    return _spmv(lib().jc_spmv_f16, offsets, neighbors, weights, x)
+
+
+def spmv_hybrid(offsets, neighbors, weights, x):
+    """gpu_spmv_hybrid (spmv.rs:491-676): one 64-lane group per row, lane-strided
+    partial sums and a stride-halving tree."""
+    return _spmv(lib().jc_spmv_hybrid, offsets, neighbors, weights, x)
+
+
+class SpmvPlan:
+    """A weighted CSR matrix resident on the device for repeated products."""
+
+    def __init__(self, offsets, neighbors, weights, kind: int = SPMV_F32):
+        self._off, self._nb, self._w = _u32(offsets), _u32(neighbors), _f32(weights)
+        self.n = len(self._off) - 1
+        if len(self._w) != len(self._nb):
+            raise JcError(JC_E_INVALID, "neighbors and weights length mismatch")
+        h = C.c_void_p()
+        _check(lib().jc_spmv_plan_create(self.n, _ptr(self._off),
+                                         _ptr(self._nb) if len(self._nb) else None,
+                                         _ptr(self._w) if len(self._w) else None,
+                                         len(self._nb), kind, C.addressof(h)))
+        self._h = h.value
+
+    def run(self, x):
+        """Returns (y, device ms of the product)."""
+        x = _f32(x)
+        if len(x) != self.n:
+            raise JcError(JC_E_INVALID, f"x length {len(x)} != n ({self.n})")
+        y = np.empty(max(self.n, 1), np.float32)
+        ms = np.zeros(1, np.float64)
+        _check(lib().jc_spmv_plan_run(self._h, _ptr(x) if self.n else None, _ptr(y), _ptr(ms)))
+        return y[: self.n], float(ms[0])
+
+    def time_products(self, n_iters: int) -> float:
+        ms = np.zeros(1, np.float64)
+        _check(lib().jc_spmv_plan_time(self._h, n_iters, _ptr(ms)))
+        return float(ms[0])
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().jc_spmv_plan_destroy(h)
             self._h = None
 
 

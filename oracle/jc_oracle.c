@@ -42,6 +42,12 @@
  *   connected components / LCC ... analytics/connected_components.rs:45-66; pinned by
  *                                   disjoint cliques, path, isolated nodes, two cycles
  *                                   and scipy's connected_components on random graphs.
+ *   SpMV serial / f16 / hybrid ... spmv.rs:68-99, spmv.wgsl:27-41, spmv_f16.wgsl:27-45,
+ *                                   spmv_hybrid.wgsl:44-84; pinned by the identity and
+ *                                   the [5,6,13] matrix of the reference's tests, exact
+ *                                   integer sums against scipy's sparse product, a
+ *                                   hand-worked row whose serial and tree orders round
+ *                                   differently, and numpy's float16 conversion.
  */
 #include "jc_oracle.h"
 
@@ -1011,6 +1017,105 @@ int jco_fa2_run(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, float
     free(deg); free(att_off); free(att_adj); free(mass); free(node_a); free(half); free(body);
     free(skip); free(force); free(stats); free(out);
     return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* SpMV                                                                        */
+/* ------------------------------------------------------------------------- */
+/* cpu_spmv (spmv.rs:88-99) = gpu_spmv's thread per row (spmv.wgsl:27-41): acc
+ * from +0, acc = acc + w[e] * x[nbr[e]] over the row in CSR order. */
+void jco_spmv(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, const float *w,
+              const float *x, float *y) {
+    for (uint32_t v = 0; v < n; v++) {
+        float acc = 0.0f;
+        for (uint32_t e = offsets[v]; e < offsets[v + 1]; e++) acc = acc + w[e] * x[nbrs[e]];
+        y[v] = acc;
+    }
+}
+
+/* spmv_hybrid.wgsl:44-84: 64 lanes per row; lane l from +0 adds the entries
+ * start+l, start+l+64, ...; then for stride 32, 16, ..., 1 every lane below the
+ * stride adds partials[l + stride]; y = partials[0]. */
+void jco_spmv_hybrid(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, const float *w,
+                     const float *x, float *y) {
+    float part[64];
+    for (uint32_t v = 0; v < n; v++) {
+        for (uint32_t l = 0; l < 64; l++) {
+            float acc = 0.0f;
+            for (uint32_t e = offsets[v] + l; e < offsets[v + 1]; e += 64)
+                acc = acc + w[e] * x[nbrs[e]];
+            part[l] = acc;
+        }
+        for (uint32_t stride = 32; stride > 0; stride >>= 1)
+            for (uint32_t l = 0; l < stride; l++) part[l] = part[l] + part[l + stride];
+        y[v] = part[0];
+    }
+}
+
+/* IEEE binary16 from binary32, round to nearest, ties to even: what
+ * f16::from_f32 gives pack_f16_pairs (spmv.rs:68-86).  Overflow goes to inf
+ * (the rounding carry out of the largest exponent), results below half the
+ * smallest subnormal to zero, NaN stays a quiet NaN with the top payload bits. */
+uint16_t jco_f32_to_f16(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    const uint32_t sign = (u >> 16) & 0x8000u, ex = (u >> 23) & 0xFFu, man = u & 0x7FFFFFu;
+    if (ex == 0xFFu) return (uint16_t)(sign | 0x7C00u | (man ? 0x200u | (man >> 13) : 0u));
+    const int e = (int)ex - 127 + 15;
+    if (e >= 31) return (uint16_t)(sign | 0x7C00u);
+    if (e <= 0) {
+        if (e < -10) return (uint16_t)sign;
+        const uint32_t m = man | 0x800000u;
+        const uint32_t shift = (uint32_t)(14 - e);
+        uint32_t h = m >> shift;
+        const uint32_t rem = m & ((1u << shift) - 1u), halfway = 1u << (shift - 1u);
+        if (rem > halfway || (rem == halfway && (h & 1u))) h++;
+        return (uint16_t)(sign | h);
+    }
+    uint32_t h = sign | ((uint32_t)e << 10) | (man >> 13);
+    const uint32_t rem = man & 0x1FFFu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
+    return (uint16_t)h;
+}
+
+/* unpack2x16float's decode: binary16 to binary32, exact. */
+float jco_f16_to_f32(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16, ex = (h >> 10) & 0x1Fu;
+    uint32_t man = h & 0x3FFu, u;
+    if (ex == 0) {
+        if (man == 0) {
+            u = sign;
+        } else {
+            int k = 0;
+            while (!(man & 0x400u)) {
+                man <<= 1;
+                k++;
+            }
+            u = sign | ((uint32_t)(113 - k) << 23) | ((man & 0x3FFu) << 13);
+        }
+    } else if (ex == 0x1Fu) {
+        u = sign | 0x7F800000u | (man << 13);
+    } else {
+        u = sign | ((ex + 112u) << 23) | (man << 13);
+    }
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+/* gpu_spmv_f16 (spmv.rs:293-469, spmv_f16.wgsl:27-45): weights and x stored as
+ * f16, each product f32(w) * f32(x) and the row sum in f32, in CSR order. */
+void jco_spmv_f16(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, const float *w,
+                  const float *x, float *y) {
+    for (uint32_t v = 0; v < n; v++) {
+        float acc = 0.0f;
+        for (uint32_t e = offsets[v]; e < offsets[v + 1]; e++) {
+            const float a = jco_f16_to_f32(jco_f32_to_f16(w[e]));
+            const float b = jco_f16_to_f32(jco_f32_to_f16(x[nbrs[e]]));
+            acc = acc + a * b;
+        }
+        y[v] = acc;
+    }
 }
 
 /* ------------------------------------------------------------------------- */

@@ -104,6 +104,16 @@ void jco_pagerank_push(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs
 void jco_pagerank_f64(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, double damping,
                       uint32_t iters, double *ranks);
 
+/* ---- SpMV (analytics/spmv.rs, spmv.wgsl, spmv_f16.wgsl, spmv_hybrid.wgsl) ---- */
+void jco_spmv(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, const float *w,
+              const float *x, float *y);
+void jco_spmv_hybrid(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, const float *w,
+                     const float *x, float *y);
+uint16_t jco_f32_to_f16(float f);
+float jco_f16_to_f32(uint16_t h);
+void jco_spmv_f16(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, const float *w,
+                  const float *x, float *y);
+
 /* ---- SGD sparse stress (engines/sgd_stress.rs, sgd_stress.wgsl) ---- */
 void jco_bfs(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, uint32_t src,
              uint32_t *dist);

@@ -99,6 +99,11 @@ def lib():
                                         f32p, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
         "jco_pagerank_push": (None, [C.c_uint32, u32p, u32p, C.c_float, C.c_uint32, f32p]),
         "jco_pagerank_f64": (None, [C.c_uint32, u32p, u32p, C.c_double, C.c_uint32, f64p]),
+        "jco_spmv": (None, [C.c_uint32, u32p, u32p, f32p, f32p, f32p]),
+        "jco_spmv_hybrid": (None, [C.c_uint32, u32p, u32p, f32p, f32p, f32p]),
+        "jco_spmv_f16": (None, [C.c_uint32, u32p, u32p, f32p, f32p, f32p]),
+        "jco_f32_to_f16": (C.c_uint16, [C.c_float]),
+        "jco_f16_to_f32": (C.c_float, [C.c_uint16]),
         "jco_bfs": (None, [C.c_uint32, u32p, u32p, C.c_uint32, u32p]),
         "jco_connected_components": (None, [C.c_uint32, u32p, u32p, u32p]),
         "jco_contract_level": (C.c_uint32, [C.c_uint32, u32p, C.c_uint64, u32p, u32p, u64p]),
@@ -333,6 +338,43 @@ def pagerank_pull(offsets, nbrs, damping=0.85, max_iters=100, tol=1e-6):
     if rc != 0:
         raise ValueError("dangling node")
     return ranks[:n], it.value, res.value
+
+
+def _spmv(fn, offsets, nbrs, weights, x):
+    n = len(offsets) - 1
+    y = np.zeros(max(n, 1), np.float32)
+    pad_u = np.zeros(1, np.uint32)
+    pad_f = np.zeros(1, np.float32)
+    fn(n, _u32(offsets), _u32(nbrs) if len(nbrs) else pad_u,
+       _f32(weights) if len(weights) else pad_f, _f32(x) if len(x) else pad_f, y)
+    return y[:n]
+
+
+def spmv(offsets, nbrs, weights, x):
+    """cpu_spmv (spmv.rs:88-99): each row summed in CSR order in f32."""
+    return _spmv(lib().jco_spmv, offsets, nbrs, weights, x)
+
+
+def spmv_hybrid(offsets, nbrs, weights, x):
+    """gpu_spmv_hybrid's order (spmv_hybrid.wgsl:44-84): 64 lane-strided partials
+    and a stride-halving tree."""
+    return _spmv(lib().jco_spmv_hybrid, offsets, nbrs, weights, x)
+
+
+def spmv_f16(offsets, nbrs, weights, x):
+    """gpu_spmv_f16 (spmv_f16.wgsl:27-45): f16-rounded weights and x, f32 sums."""
+    return _spmv(lib().jco_spmv_f16, offsets, nbrs, weights, x)
+
+
+def f32_to_f16_bits(values):
+    """Nearest-even binary16 bits of each f32 (pack_f16_pairs, spmv.rs:68-86)."""
+    f = lib().jco_f32_to_f16
+    return np.array([f(float(v)) for v in np.asarray(values, np.float32)], np.uint16)
+
+
+def f16_bits_to_f32(bits):
+    f = lib().jco_f16_to_f32
+    return np.array([f(int(b)) for b in np.asarray(bits, np.uint16)], np.float32)
 
 
 def pagerank_push(offsets, nbrs, damping=0.85, iters=50):
