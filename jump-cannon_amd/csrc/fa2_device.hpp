@@ -359,12 +359,18 @@ __device__ __forceinline__ void walk_top(const float4 *tA, const uint4 *tB, uint
             id = q.y;
             link = q.z;
         } else {
+            // Below the top tree only the rope and the size are read (8 bytes,
+            // not 12): the body's own leaf is then taken like any other leaf.
+            // It holds the body's own position, so its offset is zero in every
+            // component and its term dx * cf is +0; adding +0 leaves every
+            // partial sum unchanged (a sum that starts at +0 is never -0 in
+            // round-to-nearest), so the forces keep their bits.
             const auto av = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
-            const auto bv = __builtin_amdgcn_raw_buffer_load_b128(rs, off + 16u, 0, 0);
+            const auto bv = __builtin_amdgcn_raw_buffer_load_b64(rs, off + 16u, 0, 0);
             a = make_float4(__uint_as_float(av[0]), __uint_as_float(av[1]), __uint_as_float(av[2]),
                             __uint_as_float(av[3]));
             size = bv[1];
-            id = bv[2];
+            id = kCellId;
             link = bv[0];
         }
         const float dx = px - a.x, dy = py - a.y, dz = pz - a.z;
