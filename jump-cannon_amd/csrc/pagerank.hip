@@ -4,11 +4,12 @@
 // ping-pong ranks, one submit) and shaders/pagerank.wgsl:30-46 (one thread per
 // row: rank_out[v] = teleport + d * sum_{u in N(v)} rank_in[u] * inv_deg[u]).
 //
-// Layout per iteration: every kernel writes rank_out[v] and x_out[v] =
+// Layout per iteration: every row writes rank_out[v] and x_out[v] =
 // rank_out[v] * inv_deg[v], so the next iteration gathers one f32 per CSR entry
 // (the product is the one the reference forms per edge, rounded identically).
-// Rows are summed in CSR order exactly as the reference thread does:
-//   light rows (<= kHeavy entries): one thread per row;
+// Rows are summed in CSR order exactly as the reference thread does, in one
+// launch per iteration (k_pr_iter):
+//   light rows (<= kHeavy entries): one lane per row;
 //   hub rows: one wave per row gathers the x values into LDS in parallel, then
 //   one lane adds them in order -- the hub's sum stays bit-identical while its
 //   gathers run 64-wide.
@@ -32,7 +33,7 @@ namespace {
 constexpr uint32_t kPrTB = 256;
 constexpr uint32_t kHeavy = 32;     // rows with more CSR entries go to the hub kernel
 constexpr uint32_t kHubChunk = 1024;  // x values per hub-row round: 16 gathers per lane
-constexpr uint32_t kAccCols = 1024;   // residual limb accumulator columns (few atomics per address)
+constexpr uint32_t kAccCols = 256;    // residual limb accumulator columns (few atomics per address)
 constexpr uint32_t kChunkIters = 10;
 
 struct PrState {
@@ -46,6 +47,35 @@ __device__ __forceinline__ void resid_add(unsigned long long *acc, float r_new, 
     // |r_new - r_old| = max - min, both non-negative ranks: exact as two sums
     exact_acc_add(acc, fmaxf(r_new, r_old));
     exact_acc_add(acc + kLimbs, fminf(r_new, r_old));
+}
+
+// The same for one row per lane of a wave (valid: the lane has a row): each
+// lane places its two values in register limbs, the wave sums every limb that
+// some lane touched, and one lane adds the sums to acc.  The 64 lanes' values
+// share an exponent range, so adding them one by one into acc would queue 64
+// atomics on the same few LDS words.
+__device__ __forceinline__ void resid_add_wave(unsigned long long *acc, bool valid, float r_new,
+                                               float r_old, uint32_t lane) {
+    unsigned long long p[kLimbs] = {}, q[kLimbs] = {};
+    if (valid) {
+        exact_reg_add(p, fmaxf(r_new, r_old));
+        exact_reg_add(q, fminf(r_new, r_old));
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kLimbs; j++) {
+        if (__any(p[j] != 0ull)) {
+            unsigned long long t = p[j];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+            if (lane == 0 && t) atomicAdd(&acc[j], t);
+        }
+        if (__any(q[j] != 0ull)) {
+            unsigned long long t = q[j];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+            if (lane == 0 && t) atomicAdd(&acc[kLimbs + j], t);
+        }
+    }
 }
 
 // Also clears the residual accumulator (k_pr_reduce clears it again after
@@ -68,114 +98,121 @@ __global__ __launch_bounds__(kPrTB) void k_pr_init(uint32_t n, float inv_n,
     x0[v] = inv_n * inv[v];
 }
 
-// Rows of at most kHeavy entries.  Each wave takes 64 consecutive rows: their
-// entries are contiguous in the light CSR (offl/nbrl, where hub rows are empty),
-// so the wave reads the index run coalesced, gathers all of its x values at
-// once into LDS, and then every lane adds its own row from LDS in CSR order.
-__global__ __launch_bounds__(kPrTB) void k_pr_light(
-    uint32_t n, const uint32_t *__restrict__ off, const uint32_t *__restrict__ offl,
-    const uint32_t *__restrict__ nbrl, const float *__restrict__ inv,
-    const float *__restrict__ x_in, const float *__restrict__ rank_in,
-    float *__restrict__ rank_out, float *__restrict__ x_out, float teleport, float damping,
-    const PrState *st, unsigned long long *__restrict__ partial) {
+// One iteration over all rows in one launch.  The first hub_blocks blocks take
+// the hub rows (more than kHeavy entries), one row per wave, longest first; the
+// rest take the light rows, 64 consecutive rows per wave.  Both halves run side
+// by side: the hub rows' serial sums are latency chains that leave most of the
+// machine to the light rows.
+//   light: the wave's rows have contiguous entries in the light CSR (offl/nbrl,
+//     where hub rows are empty); it reads the index run coalesced, gathers all
+//     x values at once into LDS, and each lane then adds its own row in CSR
+//     order.
+//   hub: each round gathers up to kHubChunk x values into the wave's LDS
+//     segment (all sixteen index loads of a lane, then all sixteen gathers, in
+//     flight together), then lane 0 adds them in CSR order.
+__global__ __launch_bounds__(kPrTB) void k_pr_iter(
+    uint32_t n, uint32_t hub_blocks, uint32_t n_hubs, const uint32_t *__restrict__ hubs,
+    const uint32_t *__restrict__ off, const uint32_t *__restrict__ nbr,
+    const uint32_t *__restrict__ offl, const uint32_t *__restrict__ nbrl,
+    const float *__restrict__ inv, const float *__restrict__ x_in,
+    const float *__restrict__ rank_in, float *__restrict__ rank_out, float *__restrict__ x_out,
+    float teleport, float damping, const PrState *st, unsigned long long *__restrict__ partial) {
     if (st->converged) return;
     constexpr uint32_t W = kPrTB / 64, SEG = 64 * kHeavy;
-    __shared__ float xs[W][SEG];
+    static_assert(kHubChunk <= SEG, "a hub round fits the wave's segment");
+    __shared__ __align__(16) float xs[W][SEG];
     __shared__ unsigned long long acc[2 * kLimbs];
     if (threadIdx.x < 2 * kLimbs) acc[threadIdx.x] = 0ull;
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t r0 = (blockIdx.x * W + w) * 64;
-    const uint32_t v = r0 + lane;
-    uint32_t seg0 = 0, seg1 = 0;
-    if (r0 < n) {
-        seg0 = offl[r0];
-        seg1 = offl[min(r0 + 64u, n)];
-    }
-    for (uint32_t base = seg0; base < seg1; base += 8 * 64) {
-        uint32_t u[8];
-        float a[8];
-#pragma unroll
-        for (uint32_t q = 0; q < 8; q++) {
-            const uint32_t k = base + lane + 64u * q;
-            u[q] = k < seg1 ? nbrl[k] : 0u;
-        }
-#pragma unroll
-        for (uint32_t q = 0; q < 8; q++) a[q] = x_in[u[q]];
-#pragma unroll
-        for (uint32_t q = 0; q < 8; q++) {
-            const uint32_t k = base + lane + 64u * q;
-            if (k < seg1) xs[w][k - seg0] = a[q];
-        }
-    }
     __syncthreads();
-    if (v < n && off[v + 1] - off[v] <= kHeavy) {
-        const uint32_t s = offl[v] - seg0, e = offl[v + 1] - seg0;
-        float sum = 0.0f;
-        for (uint32_t k = s; k < e; k++) sum = sum + xs[w][k];
-        const float r = teleport + damping * sum;
-        rank_out[v] = r;
-        x_out[v] = r * inv[v];
-        resid_add(acc, r, rank_in[v]);
-    }
-    __syncthreads();
-    if (threadIdx.x < 2 * kLimbs && acc[threadIdx.x])
-        atomicAdd(&partial[threadIdx.x * kAccCols + blockIdx.x % kAccCols], acc[threadIdx.x]);
-}
-
-// One 64-lane block per hub row: each round gathers up to kHubChunk x values
-// into LDS (all sixteen index loads of a lane, then all sixteen gathers, in
-// flight together), then lane 0 adds them in CSR order.
-__global__ __launch_bounds__(64) void k_pr_hub(
-    const uint32_t *__restrict__ hubs, const uint32_t *__restrict__ off,
-    const uint32_t *__restrict__ nbr, const float *__restrict__ inv,
-    const float *__restrict__ x_in, const float *__restrict__ rank_in,
-    float *__restrict__ rank_out, float *__restrict__ x_out, float teleport, float damping,
-    const PrState *st, unsigned long long *__restrict__ partial) {
-    if (st->converged) return;
-    __shared__ __align__(16) float buf[kHubChunk];
-    __shared__ unsigned long long acc[2 * kLimbs];
-    const uint32_t v = hubs[blockIdx.x];
-    const uint32_t s = off[v], e = off[v + 1];
-    float sum = 0.0f;
-    for (uint32_t c = s; c < e; c += kHubChunk) {
-        const uint32_t len = min(kHubChunk, e - c);
-        constexpr uint32_t R = kHubChunk / 64;
-        uint32_t j[R];
+    if (blockIdx.x < hub_blocks) {
+        const uint32_t hi = blockIdx.x * W + w;
+        if (hi < n_hubs) {
+            const uint32_t v = hubs[hi];
+            const uint32_t s = off[v], e = off[v + 1];
+            float sum = 0.0f;
+            for (uint32_t c = s; c < e; c += kHubChunk) {
+                const uint32_t len = min(kHubChunk, e - c);
+                constexpr uint32_t R = kHubChunk / 64;
+                uint32_t j[R];
 #pragma unroll
-        for (uint32_t r = 0; r < R; r++) {
-            const uint32_t k = threadIdx.x + 64u * r;
-            j[r] = k < len ? nbr[c + k] : 0u;
-        }
-        float a[R];
+                for (uint32_t r = 0; r < R; r++) {
+                    const uint32_t k = lane + 64u * r;
+                    j[r] = k < len ? nbr[c + k] : 0u;
+                }
+                float a[R];
 #pragma unroll
-        for (uint32_t r = 0; r < R; r++) a[r] = x_in[j[r]];
+                for (uint32_t r = 0; r < R; r++) a[r] = x_in[j[r]];
 #pragma unroll
-        for (uint32_t r = 0; r < R; r++) {
-            const uint32_t k = threadIdx.x + 64u * r;
-            if (k < len) buf[k] = a[r];
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t k = 0;
-#pragma unroll 8
-            for (; k + 4 <= len; k += 4) {
-                const float4 q = *reinterpret_cast<const float4 *>(&buf[k]);
-                sum = sum + q.x;
-                sum = sum + q.y;
-                sum = sum + q.z;
-                sum = sum + q.w;
+                for (uint32_t r = 0; r < R; r++) {
+                    const uint32_t k = lane + 64u * r;
+                    if (k < len) xs[w][k] = a[r];
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (lane == 0) {
+                    uint32_t k = 0;
+                    for (; k + 4 <= len; k += 4) {
+                        const float4 q = *reinterpret_cast<const float4 *>(&xs[w][k]);
+                        sum = sum + q.x;
+                        sum = sum + q.y;
+                        sum = sum + q.z;
+                        sum = sum + q.w;
+                    }
+                    for (; k < len; k++) sum = sum + xs[w][k];
+                }
+                // lane 0 has read the round before the next one overwrites it
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
-            for (; k < len; k++) sum = sum + buf[k];
+            if (lane == 0) {
+                const float r = teleport + damping * sum;
+                rank_out[v] = r;
+                x_out[v] = r * inv[v];
+                resid_add(acc, r, rank_in[v]);
+            }
         }
-        __syncthreads();
-    }
-    if (threadIdx.x < 2 * kLimbs) acc[threadIdx.x] = 0ull;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const float r = teleport + damping * sum;
-        rank_out[v] = r;
-        x_out[v] = r * inv[v];
-        resid_add(acc, r, rank_in[v]);
+    } else {
+        const uint32_t r0 = ((blockIdx.x - hub_blocks) * W + w) * 64;
+        const uint32_t v = r0 + lane;
+        uint32_t seg0 = 0, seg1 = 0;
+        if (r0 < n) {
+            seg0 = offl[r0];
+            seg1 = offl[min(r0 + 64u, n)];
+        }
+        for (uint32_t base = seg0; base < seg1; base += 8 * 64) {
+            uint32_t u[8];
+            float a[8];
+#pragma unroll
+            for (uint32_t q = 0; q < 8; q++) {
+                const uint32_t k = base + lane + 64u * q;
+                u[q] = k < seg1 ? nbrl[k] : 0u;
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < 8; q++) a[q] = x_in[u[q]];
+#pragma unroll
+            for (uint32_t q = 0; q < 8; q++) {
+                const uint32_t k = base + lane + 64u * q;
+                if (k < seg1) xs[w][k - seg0] = a[q];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const bool mine = v < n && off[v + 1] - off[v] <= kHeavy;
+        float r = 0.0f, r_old = 0.0f;
+        if (mine) {
+            const uint32_t s = offl[v] - seg0, e = offl[v + 1] - seg0;
+            float sum = 0.0f;
+            for (uint32_t k = s; k < e; k++) sum = sum + xs[w][k];
+            r = teleport + damping * sum;
+            rank_out[v] = r;
+            x_out[v] = r * inv[v];
+            r_old = rank_in[v];
+        }
+        resid_add_wave(acc, mine, r, r_old, lane);
     }
     __syncthreads();
     if (threadIdx.x < 2 * kLimbs && acc[threadIdx.x])
@@ -353,13 +390,10 @@ class PageRankPlan {
 
   private:
     Status enqueue_iteration(uint32_t c) {
-        if (n_hubs_)
-            k_pr_hub<<<n_hubs_, 64, 0, stream_>>>(hubs_.p, off_.p, nbr_.p, inv_.p, x_[c].p,
-                                                   rank_[c].p, rank_[c ^ 1].p, x_[c ^ 1].p,
-                                                   teleport_, damping_, st_.p, partial_.p);
-        k_pr_light<<<nb_light_, kPrTB, 0, stream_>>>(n_, off_.p, offl_.p, nbrl_.p, inv_.p, x_[c].p,
-                                                     rank_[c].p, rank_[c ^ 1].p, x_[c ^ 1].p,
-                                                     teleport_, damping_, st_.p, partial_.p);
+        const uint32_t hub_blocks = cdiv32(n_hubs_, kPrTB / 64);
+        k_pr_iter<<<hub_blocks + nb_light_, kPrTB, 0, stream_>>>(
+            n_, hub_blocks, n_hubs_, hubs_.p, off_.p, nbr_.p, offl_.p, nbrl_.p, inv_.p, x_[c].p,
+            rank_[c].p, rank_[c ^ 1].p, x_[c ^ 1].p, teleport_, damping_, st_.p, partial_.p);
         k_pr_reduce<<<1, 640, 0, stream_>>>(partial_.p, st_.p);
         JC_HIP(hipGetLastError());
         return Status::ok();

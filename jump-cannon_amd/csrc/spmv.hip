@@ -13,11 +13,12 @@
 //
 // Each variant here performs the reference's f32 operations in the reference's
 // order, so its result has the reference algorithm's bits:
-//   serial (f32, f16): rows of at most kLight entries go 64 to a wave; the wave
-//     reads their index and weight runs coalesced, forms every product in
-//     parallel into LDS, then each lane adds its own row's products in CSR
-//     order.  Longer rows take one 64-lane block each: the products of 1024
-//     entries per round land in LDS and one lane adds them in order.
+//   serial (f32, f16): one launch (k_spmv_rows).  Rows of at most kLight
+//     entries go 64 to a wave; the wave reads their index and weight runs
+//     coalesced, forms every product in parallel into LDS, then each lane adds
+//     its own row's products in CSR order.  Longer rows take one wave each: the
+//     products of 1024 entries per round land in LDS and one lane adds them in
+//     order.
 //   hybrid: one wave per row with the reference's lane-strided partials; the
 //     tree is six cross-lane shuffles that pair the same partials.
 #include <algorithm>
@@ -31,7 +32,7 @@ namespace jc {
 namespace {
 
 constexpr uint32_t kSpTB = 256;
-constexpr uint32_t kLight = 32;       // rows with more entries go to the long-row kernel
+constexpr uint32_t kLight = 32;       // rows with more entries take a wave each
 constexpr uint32_t kLongChunk = 1024;  // products per long-row round: 16 per lane
 
 inline uint32_t cdiv_u32(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
@@ -57,25 +58,76 @@ __global__ __launch_bounds__(kSpTB) void k_to_f16(const float *__restrict__ src,
         dst[i] = __builtin_bit_cast(uint16_t, (_Float16)src[i]);
 }
 
-// Rows of at most kLight entries, 64 consecutive rows per wave.  Their entries
-// are contiguous in the light CSR (offl / nbrl / wl, where long rows are empty).
+// One product over all rows in one launch.  The first long_blocks blocks take
+// the long rows (more than kLight entries), one row per wave, longest first;
+// the rest take the light rows, 64 consecutive rows per wave, whose entries
+// are contiguous in the light CSR (offl / nbrl / wl, where long rows are
+// empty).  Both halves run side by side: a long row's serial sum is a latency
+// chain that leaves most of the machine to the light rows.
+//   light: the wave reads its index and weight runs coalesced, forms every
+//     product into its LDS segment, and each lane adds its own row in order;
+//   long: each round forms up to kLongChunk products into the segment (all
+//     loads of a lane in flight together), then lane 0 adds them in order.
 template <typename T>
-__global__ __launch_bounds__(kSpTB) void k_spmv_light(uint32_t n, const uint32_t *__restrict__ off,
-                                                      const uint32_t *__restrict__ offl,
-                                                      const uint32_t *__restrict__ nbrl,
-                                                      const T *__restrict__ wl,
-                                                      const T *__restrict__ x,
-                                                      float *__restrict__ y) {
+__global__ __launch_bounds__(kSpTB) void k_spmv_rows(
+    uint32_t n, uint32_t long_blocks, uint32_t n_long, const uint32_t *__restrict__ rows,
+    const uint32_t *__restrict__ off, const uint32_t *__restrict__ nbr, const T *__restrict__ wt,
+    const uint32_t *__restrict__ offl, const uint32_t *__restrict__ nbrl,
+    const T *__restrict__ wl, const T *__restrict__ x, float *__restrict__ y) {
     constexpr uint32_t W = kSpTB / 64, SEG = 64 * kLight;
-    __shared__ float prod[W][SEG];
+    static_assert(kLongChunk <= SEG, "a long-row round fits the wave's segment");
+    __shared__ __align__(16) float prod[W][SEG];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t r0 = (blockIdx.x * W + w) * 64;
-    const uint32_t v = r0 + lane;
-    uint32_t seg0 = 0, seg1 = 0;
-    if (r0 < n) {
-        seg0 = offl[r0];
-        seg1 = offl[min(r0 + 64u, n)];
+    if (blockIdx.x < long_blocks) {
+        const uint32_t li = blockIdx.x * W + w;
+        if (li >= n_long) return;
+        const uint32_t v = rows[li];
+        const uint32_t s = off[v], e = off[v + 1];
+        float acc = 0.0f;
+        for (uint32_t c = s; c < e; c += kLongChunk) {
+            const uint32_t len = min(kLongChunk, e - c);
+            constexpr uint32_t R = kLongChunk / 64;
+            uint32_t j[R];
+            float a[R], b[R];
+#pragma unroll
+            for (uint32_t r = 0; r < R; r++) {
+                const uint32_t k = lane + 64u * r;
+                j[r] = k < len ? nbr[c + k] : 0u;
+                a[r] = k < len ? load_val(wt, c + k) : 0.0f;
+            }
+#pragma unroll
+            for (uint32_t r = 0; r < R; r++) b[r] = load_val(x, j[r]);
+#pragma unroll
+            for (uint32_t r = 0; r < R; r++) {
+                const uint32_t k = lane + 64u * r;
+                if (k < len) prod[w][k] = a[r] * b[r];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane == 0) {
+                uint32_t k = 0;
+                for (; k + 4 <= len; k += 4) {
+                    const float4 q = *reinterpret_cast<const float4 *>(&prod[w][k]);
+                    acc = acc + q.x;
+                    acc = acc + q.y;
+                    acc = acc + q.z;
+                    acc = acc + q.w;
+                }
+                for (; k < len; k++) acc = acc + prod[w][k];
+            }
+            // lane 0 has read the round before the next one overwrites it
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        if (lane == 0) y[v] = acc;
+        return;
     }
+    const uint32_t r0 = ((blockIdx.x - long_blocks) * W + w) * 64;
+    if (r0 >= n) return;
+    const uint32_t v = r0 + lane;
+    const uint32_t seg0 = offl[r0], seg1 = offl[min(r0 + 64u, n)];
     for (uint32_t base = seg0; base < seg1; base += 8 * 64) {
         uint32_t u[8];
         float a[8], b[8];
@@ -93,61 +145,15 @@ __global__ __launch_bounds__(kSpTB) void k_spmv_light(uint32_t n, const uint32_t
             if (k < seg1) prod[w][k - seg0] = a[q] * b[q];
         }
     }
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (v < n && off[v + 1] - off[v] <= kLight) {
         const uint32_t s = offl[v] - seg0, e = offl[v + 1] - seg0;
         float acc = 0.0f;
         for (uint32_t k = s; k < e; k++) acc = acc + prod[w][k];
         y[v] = acc;
     }
-}
-
-// One 64-lane block per long row: each round forms up to kLongChunk products
-// into LDS (all index, weight and x loads of a lane in flight together), then
-// lane 0 adds them in CSR order.
-template <typename T>
-__global__ __launch_bounds__(64) void k_spmv_long(const uint32_t *__restrict__ rows,
-                                                  const uint32_t *__restrict__ off,
-                                                  const uint32_t *__restrict__ nbr,
-                                                  const T *__restrict__ wt,
-                                                  const T *__restrict__ x, float *__restrict__ y) {
-    __shared__ __align__(16) float buf[kLongChunk];
-    const uint32_t v = rows[blockIdx.x];
-    const uint32_t s = off[v], e = off[v + 1];
-    float acc = 0.0f;
-    for (uint32_t c = s; c < e; c += kLongChunk) {
-        const uint32_t len = min(kLongChunk, e - c);
-        constexpr uint32_t R = kLongChunk / 64;
-        uint32_t j[R];
-        float a[R], b[R];
-#pragma unroll
-        for (uint32_t r = 0; r < R; r++) {
-            const uint32_t k = threadIdx.x + 64u * r;
-            j[r] = k < len ? nbr[c + k] : 0u;
-            a[r] = k < len ? load_val(wt, c + k) : 0.0f;
-        }
-#pragma unroll
-        for (uint32_t r = 0; r < R; r++) b[r] = load_val(x, j[r]);
-#pragma unroll
-        for (uint32_t r = 0; r < R; r++) {
-            const uint32_t k = threadIdx.x + 64u * r;
-            if (k < len) buf[k] = a[r] * b[r];
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t k = 0;
-            for (; k + 4 <= len; k += 4) {
-                const float4 q = *reinterpret_cast<const float4 *>(&buf[k]);
-                acc = acc + q.x;
-                acc = acc + q.y;
-                acc = acc + q.z;
-                acc = acc + q.w;
-            }
-            for (; k < len; k++) acc = acc + buf[k];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) y[v] = acc;
 }
 
 // gpu_spmv_hybrid: one wave per row.  Lane l adds w[e] * x[nbr[e]] for
@@ -326,19 +332,19 @@ class SpmvPlan {
         if (kind_ == JC_SPMV_HYBRID) {
             k_spmv_hybrid<<<cdiv_u32(n_, kSpTB / 64), kSpTB, 0, stream_>>>(n_, off_.p, nbr_.p,
                                                                           w32_.p, x32_.p, y_.p);
-        } else if (kind_ == JC_SPMV_F16) {
-            JC_TRY(convert(x32_.p, n_, x16_.p));
-            if (n_long_)
-                k_spmv_long<uint16_t><<<n_long_, 64, 0, stream_>>>(rows_.p, off_.p, nbr_.p,
-                                                                   w16_.p, x16_.p, y_.p);
-            k_spmv_light<uint16_t><<<cdiv_u32(n_, kSpTB), kSpTB, 0, stream_>>>(
-                n_, off_.p, offl_.p, nbrl_.p, wl16_.p, x16_.p, y_.p);
         } else {
-            if (n_long_)
-                k_spmv_long<float><<<n_long_, 64, 0, stream_>>>(rows_.p, off_.p, nbr_.p, w32_.p,
-                                                                x32_.p, y_.p);
-            k_spmv_light<float><<<cdiv_u32(n_, kSpTB), kSpTB, 0, stream_>>>(
-                n_, off_.p, offl_.p, nbrl_.p, wl32_.p, x32_.p, y_.p);
+            const uint32_t lb = cdiv_u32(n_long_, kSpTB / 64);
+            const uint32_t grid = lb + cdiv_u32(n_, kSpTB);
+            if (kind_ == JC_SPMV_F16) {
+                JC_TRY(convert(x32_.p, n_, x16_.p));
+                k_spmv_rows<uint16_t><<<grid, kSpTB, 0, stream_>>>(
+                    n_, lb, n_long_, rows_.p, off_.p, nbr_.p, w16_.p, offl_.p, nbrl_.p, wl16_.p,
+                    x16_.p, y_.p);
+            } else {
+                k_spmv_rows<float><<<grid, kSpTB, 0, stream_>>>(
+                    n_, lb, n_long_, rows_.p, off_.p, nbr_.p, w32_.p, offl_.p, nbrl_.p, wl32_.p,
+                    x32_.p, y_.p);
+            }
         }
         JC_HIP(hipGetLastError());
         return Status::ok();
