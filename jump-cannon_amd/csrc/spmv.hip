@@ -19,8 +19,9 @@
 //     its own row's products in CSR order.  Longer rows take one wave each: the
 //     products of 1024 entries per round land in LDS and one lane adds them in
 //     order.
-//   hybrid: one wave per row with the reference's lane-strided partials; the
-//     tree is six cross-lane shuffles that pair the same partials.
+//   hybrid: the reference's lane-strided partials and tree as cross-lane
+//     shuffles that pair the same partials; rows of at most S entries share a
+//     wave 64 / S to a wave (S a power of two), longer rows take a wave each.
 #include <algorithm>
 #include <vector>
 
@@ -156,39 +157,69 @@ __global__ __launch_bounds__(kSpTB) void k_spmv_rows(
     }
 }
 
-// gpu_spmv_hybrid: one wave per row.  Lane l adds w[e] * x[nbr[e]] for
-// e = start + l, start + l + 64, ... from +0; then for stride 32, 16, ..., 1 the
-// lanes below the stride add the partial `stride` lanes up (spmv_hybrid.wgsl
-// :66-79: partials[l] = partials[l] + partials[l + stride]); lane 0 has y[v].
-__global__ __launch_bounds__(kSpTB) void k_spmv_hybrid(uint32_t n, const uint32_t *__restrict__ off,
+// gpu_spmv_hybrid's order (spmv_hybrid.wgsl:44-84): lane l of 64 adds
+// w[e] * x[nbr[e]] for e = start + l, start + l + 64, ... from +0; then for
+// stride 32, 16, ..., 1 the lanes below the stride add the partial `stride`
+// lanes up; lane 0 has y[v].
+// A row of L <= S entries (S a power of two <= 64) gives the same bits on S
+// lanes with the strides S/2 ... 1 only: every partial past the row is +0, no
+// partial is -0 (each starts as +0 + product), so the reference's stages with
+// stride >= S add +0 to every partial that matters and change nothing.  Rows
+// are therefore grouped by S on the host: a wave takes 64 / S rows of one
+// class, and rows longer than 64 take a whole wave.
+constexpr uint32_t kHybridClasses = 8;  // S = 1, 2, 4, ..., 64; then rows > 64
+struct HybridClasses {
+    uint32_t block_begin[kHybridClasses + 1];
+    uint32_t list_begin[kHybridClasses];
+    uint32_t count[kHybridClasses];
+};
+__device__ __forceinline__ uint32_t hybrid_width(uint32_t c) { return c < 7 ? 1u << c : 64u; }
+
+__global__ __launch_bounds__(kSpTB) void k_spmv_hybrid(const HybridClasses hc,
+                                                       const uint32_t *__restrict__ rows,
+                                                       const uint32_t *__restrict__ off,
                                                        const uint32_t *__restrict__ nbr,
                                                        const float *__restrict__ wt,
                                                        const float *__restrict__ x,
                                                        float *__restrict__ y) {
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t v = blockIdx.x * (kSpTB / 64) + (threadIdx.x >> 6);
-    if (v >= n) return;  // whole waves leave together
-    const uint32_t s = off[v], e = off[v + 1];
-    float acc = 0.0f;
-    uint32_t k = s + lane;
-    // four strided entries per round, their loads in flight together; the
-    // additions stay in the lane's order
-    for (; k + 3u * 64u < e; k += 4u * 64u) {
-        const uint32_t j0 = nbr[k], j1 = nbr[k + 64], j2 = nbr[k + 128], j3 = nbr[k + 192];
-        const float w0 = wt[k], w1 = wt[k + 64], w2 = wt[k + 128], w3 = wt[k + 192];
-        const float x0 = x[j0], x1 = x[j1], x2 = x[j2], x3 = x[j3];
-        acc = acc + w0 * x0;
-        acc = acc + w1 * x1;
-        acc = acc + w2 * x2;
-        acc = acc + w3 * x3;
-    }
-    for (; k < e; k += 64u) acc = acc + wt[k] * x[nbr[k]];
+    uint32_t c = 0;
 #pragma unroll
-    for (uint32_t stride = 32; stride > 0; stride >>= 1) {
-        const float up = __shfl_down(acc, stride, 64);
-        if (lane < stride) acc = acc + up;
+    for (uint32_t k = 1; k < kHybridClasses; k++)
+        if (blockIdx.x >= hc.block_begin[k]) c = k;
+    const uint32_t S = hybrid_width(c);
+    const uint32_t lane = threadIdx.x & 63, sub = lane & (S - 1);
+    const uint32_t wave = (blockIdx.x - hc.block_begin[c]) * (kSpTB / 64) + (threadIdx.x >> 6);
+    const uint32_t slot = wave * (64u / S) + lane / S;
+    const bool live = slot < hc.count[c];
+    uint32_t v = 0, s = 0, e = 0;
+    if (live) {
+        v = rows[hc.list_begin[c] + slot];
+        s = off[v];
+        e = off[v + 1];
     }
-    if (lane == 0) y[v] = acc;
+    float acc = 0.0f;
+    uint32_t k = s + sub;
+    if (c == kHybridClasses - 1) {
+        // four strided entries per round, their loads in flight together; the
+        // additions stay in the lane's order
+        for (; k + 3u * 64u < e; k += 4u * 64u) {
+            const uint32_t j0 = nbr[k], j1 = nbr[k + 64], j2 = nbr[k + 128], j3 = nbr[k + 192];
+            const float w0 = wt[k], w1 = wt[k + 64], w2 = wt[k + 128], w3 = wt[k + 192];
+            const float x0 = x[j0], x1 = x[j1], x2 = x[j2], x3 = x[j3];
+            acc = acc + w0 * x0;
+            acc = acc + w1 * x1;
+            acc = acc + w2 * x2;
+            acc = acc + w3 * x3;
+        }
+        for (; k < e; k += 64u) acc = acc + wt[k] * x[nbr[k]];
+    } else if (k < e) {
+        acc = acc + wt[k] * x[nbr[k]];
+    }
+    for (uint32_t stride = S >> 1; stride > 0; stride >>= 1) {
+        const float up = __shfl_down(acc, stride, (int)S);
+        if (sub < stride) acc = acc + up;
+    }
+    if (live && sub == 0) y[v] = acc;
 }
 
 }  // namespace
@@ -218,6 +249,34 @@ class SpmvPlan {
         JC_TRY(x32_.alloc(n));
         if (nnz) JC_HIP(hipMemcpyAsync(w32_.p, wt, nnz * 4, hipMemcpyHostToDevice, stream_));
         if (kind == JC_SPMV_HYBRID) {
+            // rows by class: S = the smallest power of two >= the row length
+            // (1 for empty rows) up to 64, then the longer rows, longest first
+            std::vector<uint32_t> cls[kHybridClasses];
+            for (uint32_t v = 0; v < n; v++) {
+                const uint32_t d = off[v + 1] - off[v];
+                uint32_t c = 0;
+                while (c < kHybridClasses - 1 && (1u << c) < d) c++;
+                cls[c].push_back(v);
+            }
+            std::stable_sort(cls[kHybridClasses - 1].begin(), cls[kHybridClasses - 1].end(),
+                             [&](uint32_t a, uint32_t b) {
+                                 return off[a + 1] - off[a] > off[b + 1] - off[b];
+                             });
+            std::vector<uint32_t> order;
+            order.reserve(n);
+            uint32_t blocks = 0;
+            for (uint32_t c = 0; c < kHybridClasses; c++) {
+                const uint32_t width = c < 7 ? 1u << c : 64u;
+                hc_.block_begin[c] = blocks;
+                hc_.list_begin[c] = (uint32_t)order.size();
+                hc_.count[c] = (uint32_t)cls[c].size();
+                blocks += cdiv_u32(cls[c].size(), (kSpTB / 64) * (64u / width));
+                order.insert(order.end(), cls[c].begin(), cls[c].end());
+            }
+            hc_.block_begin[kHybridClasses] = blocks;
+            JC_TRY(rows_.alloc(order.size()));
+            JC_HIP(hipMemcpyAsync(rows_.p, order.data(), order.size() * 4, hipMemcpyHostToDevice,
+                                  stream_));
             JC_HIP(hipStreamSynchronize(stream_));
             return Status::ok();
         }
@@ -330,8 +389,9 @@ class SpmvPlan {
 
     Status enqueue_product() {
         if (kind_ == JC_SPMV_HYBRID) {
-            k_spmv_hybrid<<<cdiv_u32(n_, kSpTB / 64), kSpTB, 0, stream_>>>(n_, off_.p, nbr_.p,
-                                                                          w32_.p, x32_.p, y_.p);
+            if (hc_.block_begin[kHybridClasses])
+                k_spmv_hybrid<<<hc_.block_begin[kHybridClasses], kSpTB, 0, stream_>>>(
+                    hc_, rows_.p, off_.p, nbr_.p, w32_.p, x32_.p, y_.p);
         } else {
             const uint32_t lb = cdiv_u32(n_long_, kSpTB / 64);
             const uint32_t grid = lb + cdiv_u32(n_, kSpTB);
@@ -350,6 +410,7 @@ class SpmvPlan {
         return Status::ok();
     }
 
+    HybridClasses hc_{};
     int kind_ = JC_SPMV_F32;
     uint32_t n_ = 0, n_long_ = 0;
     uint64_t nnz_ = 0;
