@@ -236,7 +236,6 @@ class SpmvPlan {
         if (nnz && !wt) return Status::err(JC_E_INVALID, "weights pointer is NULL");
         kind_ = kind;
         n_ = n;
-        nnz_ = nnz;
         if (n == 0) return Status::ok();
         JC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         JC_TRY(off_.alloc((size_t)n + 1));
@@ -338,7 +337,7 @@ class SpmvPlan {
         if (ms) *ms = 0.0;
         if (n_ == 0) return Status::ok();
         if (!x || !y) return Status::err(JC_E_INVALID, "x or y pointer is NULL");
-        JC_TRY(ensure_events());
+        JC_TRY(ensure_ready());
         JC_HIP(hipMemcpyAsync(x32_.p, x, (size_t)n_ * 4, hipMemcpyHostToDevice, stream_));
         JC_HIP(hipEventRecord(ev_[0], stream_));
         JC_TRY(enqueue_product());
@@ -356,7 +355,7 @@ class SpmvPlan {
     Status time_products(uint32_t n_iters, double *ms_per) {
         if (ms_per) *ms_per = 0.0;
         if (n_ == 0 || n_iters == 0) return Status::ok();
-        JC_TRY(ensure_events());
+        JC_TRY(ensure_ready());
         JC_TRY(enqueue_product());  // warm-up
         JC_HIP(hipEventRecord(ev_[0], stream_));
         for (uint32_t k = 0; k < n_iters; k++) JC_TRY(enqueue_product());
@@ -369,7 +368,9 @@ class SpmvPlan {
     }
 
   private:
-    Status ensure_events() {
+    // Timing events, and x zeroed before its first use (plan_time may run
+    // before any product has uploaded an x).
+    Status ensure_ready() {
         for (auto &e : ev_)
             if (!e) JC_HIP(hipEventCreate(&e));
         if (!x_ready_) {
@@ -413,7 +414,6 @@ class SpmvPlan {
     HybridClasses hc_{};
     int kind_ = JC_SPMV_F32;
     uint32_t n_ = 0, n_long_ = 0;
-    uint64_t nnz_ = 0;
     bool x_ready_ = false;
     hipStream_t stream_ = nullptr;
     DevBuf<uint32_t> off_, nbr_, offl_, nbrl_, rows_;
