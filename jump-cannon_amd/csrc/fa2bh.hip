@@ -52,8 +52,10 @@ constexpr uint32_t kHeavyStage = 1024;    // heavy-row terms staged in LDS per r
 constexpr uint32_t kEmitTasks = 1024;     // cell tasks per k_emit block held in LDS
 // Columns of the swing/traction limb accumulator.  Blocks add into column
 // blockIdx.x % kAccCols; many columns keep the device-scope atomics on any one
-// address few (they serialise at the memory side across XCDs).
-constexpr uint32_t kAccCols = 1024;
+// address few (they serialise at the memory side across XCDs), few columns keep
+// k_speed's read of them short: 256 columns give a 1M-node force launch (977
+// blocks) four blocks per address.
+constexpr uint32_t kAccCols = 256;
 constexpr uint32_t kSpeedThreads = 640;   // k_speed: 32 threads per limb
 
 struct Frame {
