@@ -321,6 +321,10 @@ int jc_mpnn_features(jc_mpnn *m, float *out /* n x 16 */);
 int jc_mpnn_activations(jc_mpnn *m, int layer, float *out /* n x 128 */);
 /* Predicted coordinates (3n) and the device time of the forward pass. */
 int jc_mpnn_forward(jc_mpnn *m, float *out_xyz, double *gpu_ms);
+/* The same prediction centred at the origin and scaled to RMS radius target_rms
+ * on the device (mean and RMS summed in f64 in a fixed order: deterministic);
+ * gpu_ms covers the forward pass and the rescale. */
+int jc_mpnn_forward_scaled(jc_mpnn *m, double target_rms, float *out_xyz, double *gpu_ms);
 /* Target layout (3n) whose pair distances the loss matches: graph edges plus
  * pairs with n_anchor seeded anchor nodes. */
 int jc_mpnn_set_target(jc_mpnn *m, const float *xyz, uint32_t n_anchor, uint64_t seed);

@@ -450,6 +450,63 @@ __global__ __launch_bounds__(256) void k_pack_w(uint32_t K, const float *__restr
 
 inline uint32_t cd(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// Centring and scaling of the predicted layout (n x 3), in f64 with a fixed
+// summation order so the result is deterministic: per-block partial sums over
+// a grid-stride of rows (kCenterBlocks blocks), then one thread adding the
+// partials in block order.  Pass 0 sums the coordinates; pass 1 sums the
+// squared distances from the mean.
+constexpr uint32_t kCenterBlocks = 256;
+__global__ __launch_bounds__(256) void k_center_part(uint32_t n, const float *__restrict__ y,
+                                                     const double *__restrict__ mean, int pass,
+                                                     double *__restrict__ part) {
+    __shared__ double red[3][256];
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const double x0 = y[3 * (size_t)i], x1 = y[3 * (size_t)i + 1], x2 = y[3 * (size_t)i + 2];
+        if (pass == 0) {
+            a += x0;
+            b += x1;
+            c += x2;
+        } else {
+            const double d0 = x0 - mean[0], d1 = x1 - mean[1], d2 = x2 - mean[2];
+            a += d0 * d0 + d1 * d1 + d2 * d2;
+        }
+    }
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = b;
+    red[2][threadIdx.x] = c;
+    __syncthreads();
+    for (uint32_t s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int k = 0; k < 3; k++) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) part[blockIdx.x * 3 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// stat[0..2] = mean after pass 0; stat[3] = target / max(rms, 1e-30) after pass 1.
+__global__ void k_center_final(uint32_t n, const double *__restrict__ part, int pass,
+                               double target, double *__restrict__ stat) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double t[3] = {0.0, 0.0, 0.0};
+    for (uint32_t b = 0; b < kCenterBlocks; b++)
+        for (int k = 0; k < 3; k++) t[k] += part[b * 3 + k];
+    if (pass == 0) {
+        for (int k = 0; k < 3; k++) stat[k] = t[k] / (double)n;
+    } else {
+        const double r = sqrt(t[0] / (double)n);
+        stat[3] = target / fmax(r, 1e-30);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_center_apply(uint32_t n, const float *__restrict__ y,
+                                                      const double *__restrict__ stat,
+                                                      float *__restrict__ out) {
+    const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= 3 * n) return;
+    out[q] = (float)(((double)y[q] - stat[q % 3]) * stat[3]);
+}
+
 }  // namespace
 
 // ====================================================================== network
@@ -597,6 +654,36 @@ class Mpnn {
         (void)hipEventDestroy(b);
         if (ms) *ms = t;
         if (out_xyz) JC_HIP(hipMemcpy(out_xyz, y_.p, (size_t)n_ * 3 * 4, hipMemcpyDeviceToHost));
+        return Status::ok();
+    }
+
+    // The forward pass, then the layout centred at the origin and scaled to
+    // RMS radius `target` on the device (f64 sums, fixed order); `ms` is the
+    // device time of both.
+    Status forward_scaled(double target, float *out_xyz, double *ms) {
+        if (n_ == 0) return Status::ok();
+        JC_TRY(cpart_.fit(kCenterBlocks * 3));
+        JC_TRY(cstat_.fit(4));
+        JC_TRY(yout_.fit((size_t)n_ * 3));
+        hipEvent_t a, b;
+        JC_HIP(hipEventCreate(&a));
+        JC_HIP(hipEventCreate(&b));
+        JC_HIP(hipEventRecord(a, stream_));
+        JC_TRY(enqueue_forward());
+        for (int pass = 0; pass < 2; pass++) {
+            k_center_part<<<kCenterBlocks, 256, 0, stream_>>>(n_, y_.p, cstat_.p, pass, cpart_.p);
+            k_center_final<<<1, 64, 0, stream_>>>(n_, cpart_.p, pass, target, cstat_.p);
+        }
+        k_center_apply<<<cd((uint64_t)n_ * 3, 256), 256, 0, stream_>>>(n_, y_.p, cstat_.p, yout_.p);
+        JC_HIP(hipGetLastError());
+        JC_HIP(hipEventRecord(b, stream_));
+        JC_HIP(hipEventSynchronize(b));
+        float t = 0;
+        JC_HIP(hipEventElapsedTime(&t, a, b));
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        if (ms) *ms = t;
+        if (out_xyz) JC_HIP(hipMemcpy(out_xyz, yout_.p, (size_t)n_ * 3 * 4, hipMemcpyDeviceToHost));
         return Status::ok();
     }
 
@@ -753,7 +840,8 @@ class Mpnn {
     DevBuf<uint16_t> wt_[kLayers], wb_[kLayers];
     DevBuf<uint32_t> off_, nbr_, anchors_;
     DevBuf<uint16_t> x_, a0_, h_[kLayers], a_[kLayers], dz_;
-    DevBuf<float> y_, dy_, dzf_, dx_, dh_, rw_, tgt_;
+    DevBuf<float> y_, dy_, dzf_, dx_, dh_, rw_, tgt_, yout_;
+    DevBuf<double> cpart_, cstat_;  // centring partial sums and {mean xyz, scale}
     DevBuf<LossAcc> lparts_, ltot_;
 };
 
@@ -805,6 +893,9 @@ int jc_mpnn_activations(jc_mpnn *m, int layer, float *out) {
 }
 int jc_mpnn_forward(jc_mpnn *m, float *out_xyz, double *gpu_ms) {
     return m ? done(m->impl.forward(out_xyz, gpu_ms)) : nullh();
+}
+int jc_mpnn_forward_scaled(jc_mpnn *m, double target_rms, float *out_xyz, double *gpu_ms) {
+    return m ? done(m->impl.forward_scaled(target_rms, out_xyz, gpu_ms)) : nullh();
 }
 int jc_mpnn_set_target(jc_mpnn *m, const float *xyz, uint32_t n_anchor, uint64_t seed) {
     return m ? done(m->impl.set_target(xyz, n_anchor, seed)) : nullh();

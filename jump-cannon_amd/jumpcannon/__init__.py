@@ -138,6 +138,7 @@ SIGNATURES = {
     "jc_mpnn_features": (C.c_int, [_vp, _vp]),
     "jc_mpnn_activations": (C.c_int, [_vp, C.c_int, _vp]),
     "jc_mpnn_forward": (C.c_int, [_vp, _vp, _vp]),
+    "jc_mpnn_forward_scaled": (C.c_int, [_vp, C.c_double, _vp, _vp]),
     "jc_mpnn_set_target": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint64]),
     "jc_mpnn_loss_grad": (C.c_int, [_vp, _vp]),
     "jc_mpnn_adam": (C.c_int, [_vp, C.c_float]),
@@ -566,6 +567,14 @@ class Mpnn:
         out = np.empty(3 * self.n, np.float32)
         ms = np.zeros(1, np.float64)
         _check(lib().jc_mpnn_forward(self._h, _ptr(out), _ptr(ms)))
+        return out, float(ms[0])
+
+    def forward_scaled(self, target_rms: float):
+        """Coordinates (3n) centred at the origin and scaled to RMS radius
+        target_rms on the device, and the device ms of forward plus rescale."""
+        out = np.empty(3 * self.n, np.float32)
+        ms = np.zeros(1, np.float64)
+        _check(lib().jc_mpnn_forward_scaled(self._h, float(target_rms), _ptr(out), _ptr(ms)))
         return out, float(ms[0])
 
     def set_target(self, xyz, n_anchor: int = 32, seed: int = 3):

@@ -139,14 +139,10 @@ def predict(net: Mpnn, coef, off, nb, x0, walks: int = 16, feat_seed: int = 7):
     """Predicted initial layout (3n) and the device time of features + forward (ms)."""
     t0 = time.perf_counter()
     net.set_graph(off, nb, feat_seed=feat_seed, walks=walks, noise=np.asarray(x0) / INIT_HALF)
-    y, fwd_ms = net.forward()
-    p = y.reshape(-1, 3).astype(np.float64)
-    p -= p.mean(0)
-    r = rms_radius(p)
-    n = len(off) - 1
-    p *= size_of(coef, n, len(nb)) / max(r, 1e-30)
+    # centred and scaled to the size law's RMS radius on the device (f64 sums)
+    p, fwd_ms = net.forward_scaled(size_of(coef, len(off) - 1, len(nb)))
     wall_ms = (time.perf_counter() - t0) * 1e3
-    return p.astype(np.float32).reshape(-1), fwd_ms, wall_ms
+    return p, fwd_ms, wall_ms
 
 
 def stress_judge(off, nb, k: int = 50, seed: int = JUDGE_PIVOT_SEED):

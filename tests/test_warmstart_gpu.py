@@ -46,3 +46,25 @@ def test_prediction_is_deterministic_and_finite(net):
     b, _, _ = W.predict(m, coef, off, nb, x0)
     assert np.isfinite(a).all()
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_device_rescale_matches_the_host_f64_rescale(net):
+    """forward_scaled centres and scales on the device in f64; the host form of
+    the same arithmetic (numpy f64 mean, RMS radius, scale, cast to f32) agrees
+    to within one f32 rounding of each coordinate, the mean is ~0 and the RMS
+    radius is the target."""
+    m, coef = net
+    off, nb = jc.ba_graph(20_000, 4, 91)
+    x0 = jc.uniform_cube(20_000, 10.0, 92)
+    m.set_graph(off, nb, feat_seed=7, walks=16, noise=np.asarray(x0) / W.INIT_HALF)
+    y, _ = m.forward()
+    target = 37.5
+    got, _ = m.forward_scaled(target)
+    p = y.reshape(-1, 3).astype(np.float64)
+    p -= p.mean(0)
+    want = (p * (target / np.sqrt((p ** 2).sum(1).mean()))).astype(np.float32).reshape(-1)
+    tol = 2.0 * np.spacing(np.abs(want).astype(np.float32)) + 1e-6
+    assert np.all(np.abs(got - want) <= tol)
+    g = got.reshape(-1, 3).astype(np.float64)
+    assert np.abs(g.mean(0)).max() < 1e-4 * target
+    assert abs(np.sqrt((g ** 2).sum(1).mean()) - target) < 1e-4 * target
