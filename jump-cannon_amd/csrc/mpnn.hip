@@ -606,9 +606,10 @@ class Mpnn {
         JC_TRY(parts_.fit((size_t)cd(nn, kXtgRows) * (2 * kHid + 1) * kHid));
         DevBuf<float> noise_d;
         JC_TRY(noise_d.alloc(nn * 3));
-        std::vector<float> nz(nn * 3, 0.0f);
-        if (noise) std::memcpy(nz.data(), noise, (size_t)n * 3 * 4);
-        else {
+        // the caller's noise goes to the device as it is; without one, the
+        // seeded stream below fills a host copy
+        std::vector<float> nz(noise ? 0 : nn * 3, 0.0f);
+        if (!noise) {
             uint64_t s = feat_seed ^ 0x6E6F697365ull;
             for (size_t q = 0; q < (size_t)n * 3; q++) {
                 s += 0x9E3779B97F4A7C15ull;
@@ -622,7 +623,8 @@ class Mpnn {
         if (n) {
             JC_HIP(hipMemcpyAsync(off_.p, off, ((size_t)n + 1) * 4, hipMemcpyHostToDevice, stream_));
             if (nnz) JC_HIP(hipMemcpyAsync(nbr_.p, nbr, nnz * 4, hipMemcpyHostToDevice, stream_));
-            JC_HIP(hipMemcpyAsync(noise_d.p, nz.data(), (size_t)n * 3 * 4, hipMemcpyHostToDevice, stream_));
+            JC_HIP(hipMemcpyAsync(noise_d.p, noise ? noise : nz.data(), (size_t)n * 3 * 4,
+                                  hipMemcpyHostToDevice, stream_));
             const float mean_deg = nnz ? (float)((double)nnz / n) : 1.0f;
             k_rwpe<<<cd(n, 256), 256, 0, stream_>>>(n, off_.p, nbr_.p, feat_seed, walks, rw_.p);
             k_features<<<cd(n, 256), 256, 0, stream_>>>(n, off_.p, rw_.p, mean_deg, noise_d.p, x_.p);
