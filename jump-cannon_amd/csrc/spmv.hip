@@ -282,14 +282,14 @@ class SpmvPlan {
         // light / long split of the rows (the long rows longest first: they
         // are the critical path of a product)
         std::vector<uint32_t> rows, offl((size_t)n + 1, 0), nbrl;
-        std::vector<uint64_t> light_src;
+        std::vector<float> wl;
         for (uint32_t v = 0; v < n; v++) {
             const uint32_t d = off[v + 1] - off[v];
             if (d > kLight) {
                 rows.push_back(v);
             } else {
                 nbrl.insert(nbrl.end(), nbr + off[v], nbr + off[v + 1]);
-                for (uint32_t e = off[v]; e < off[v + 1]; e++) light_src.push_back(e);
+                wl.insert(wl.end(), wt + off[v], wt + off[v + 1]);
             }
             offl[v + 1] = (uint32_t)nbrl.size();
         }
@@ -297,8 +297,6 @@ class SpmvPlan {
             return off[a + 1] - off[a] > off[b + 1] - off[b];
         });
         n_long_ = (uint32_t)rows.size();
-        std::vector<float> wl(light_src.size());
-        for (size_t k = 0; k < light_src.size(); k++) wl[k] = wt[light_src[k]];
         JC_TRY(offl_.alloc((size_t)n + 1));
         JC_TRY(nbrl_.alloc(nbrl.size()));
         JC_TRY(rows_.alloc(rows.size()));
