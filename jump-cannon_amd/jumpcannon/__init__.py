@@ -2,7 +2,8 @@
 
 Mirrors the reference's engine API (crates/graph-compute/src/engines/mod.rs:383-444):
 ``set_params`` -> ``init`` -> ``step`` per tick, plus the analytics free
-functions (``pagerank``).  Everything runs through the C ABI declared in
+functions (``pagerank``, ``spmv`` / ``spmv_f16`` / ``spmv_hybrid``, ``bfs``,
+``connected_components``).  Everything runs through the C ABI declared in
 include/jc.h; there is no CPU fallback -- a missing library or device raises.
 """
 from __future__ import annotations
