@@ -27,6 +27,20 @@ constexpr uint32_t kMaxTable = 8192;  // distances above this use the inline for
 
 __device__ __forceinline__ float len3f(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
 
+// Correctly rounded sqrt(s) for 2^-96 <= s < inf: the hardware square root
+// (within one ulp) and the two neighbouring candidates, chosen by the sign of
+// the residual s - c*r, which is the compiler's own IEEE sequence without the
+// operand scaling for tiny inputs and the class fix-up for 0, inf and NaN.
+__device__ __forceinline__ float sqrt_normal(float s) {
+    const float r = __builtin_amdgcn_sqrtf(s);
+    const float dn = __uint_as_float(__float_as_uint(r) - 1u);
+    const float up = __uint_as_float(__float_as_uint(r) + 1u);
+    float out = r;
+    if (__builtin_fmaf(-dn, r, s) <= 0.0f) out = dn;
+    if (__builtin_fmaf(-up, r, s) > 0.0f) out = up;
+    return out;
+}
+
 // One sweep (sgd_stress.wgsl:34-62).  dist8[c * n + i] holds node i's u16 hop
 // distances to pivots 8c .. 8c+7 (0: skip; the padding past k is 0), so one
 // 16-byte load serves eight terms and the next chunk's load is in flight while
@@ -78,7 +92,12 @@ __global__ __launch_bounds__(kSgdTB) void k_sgd_sweep(uint32_t n, uint32_t k,
             const float d = (float)du;
             const float4 q = LDS_PIV ? piv[t] : pos_in[pv];
             const float dx = px - q.x, dy = py - q.y, dz = pz - q.z;
-            const float mag = fmaxf(len3f(dx, dy, dz), 1e-6f);
+            const float ss = dx * dx + dy * dy + dz * dz;
+            // wave-uniform choice: the short sequence when every lane's sum is
+            // in its range (always, for finite layouts not collapsed to a point)
+            const bool plain = ss >= 0x1p-96f && ss < __builtin_inff();
+            const float root = __all(plain) ? sqrt_normal(ss) : sqrtf(ss);
+            const float mag = fmaxf(root, 1e-6f);
             float mu;
             if (tab) mu = mu_tab[du];
             else {
