@@ -187,27 +187,36 @@ __global__ __launch_bounds__(kTB) void k_keys(const float4 *__restrict__ pos, ui
 // Cells opened by sorted body i: c_i = max(0, l_i - l_{i-1}) with l = common
 // leading octal digits of neighbouring keys (-1 past either end); a lone body
 // opens the root.  Block-local exclusive scan of c, plus the block total.
-__device__ __forceinline__ uint32_t cells_opened(const uint64_t *keys, uint32_t i, uint32_t nv) {
-    if (i >= nv) return 0;
-    if (nv == 1) return 1;
-    const uint64_t k = keys[i];
-    const int lp = i > 0 ? lcp_digits(keys[i - 1], k) : -1;
-    const int li = i + 1 < nv ? lcp_digits(k, keys[i + 1]) : -1;
-    return li > lp ? (uint32_t)(li - lp) : 0u;
-}
-
 __global__ __launch_bounds__(kTB) void k_count_scan(const uint64_t *__restrict__ keys, uint32_t n,
                                                     const Frame *fr, uint32_t *__restrict__ loc,
                                                     uint32_t *__restrict__ blk_tot) {
     __shared__ uint32_t ws[kTB / 64];
     const uint32_t nv = n - fr->n_invalid;
     const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    // the thread's keys and one neighbour on each side, each loaded once
+    uint64_t kk[kScanItems + 2];
+#pragma unroll
+    for (uint32_t j = 0; j < kScanItems + 2; j++) {
+        const uint32_t q = base + j - 1u;  // base - 1 wraps for base 0: not loaded
+        kk[j] = (base + j >= 1u && q < nv) ? keys[q] : 0ull;
+    }
     uint32_t c[kScanItems];
     uint32_t s = 0;
 #pragma unroll
     for (uint32_t j = 0; j < kScanItems; j++) {
-        c[j] = cells_opened(keys, base + j, nv);
-        s += c[j];
+        const uint32_t i = base + j;
+        uint32_t cj = 0;
+        if (i < nv) {
+            if (nv == 1) {
+                cj = 1;
+            } else {
+                const int lp = i > 0 ? lcp_digits(kk[j], kk[j + 1]) : -1;
+                const int li = i + 1 < nv ? lcp_digits(kk[j + 1], kk[j + 2]) : -1;
+                cj = li > lp ? (uint32_t)(li - lp) : 0u;
+            }
+        }
+        c[j] = cj;
+        s += cj;
     }
     // block exclusive scan of s
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
