@@ -332,15 +332,21 @@ __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
     __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nv = n - fr->n_invalid;
+    // the neighbours' keys come from the neighbouring lanes; the wave's first
+    // and last lanes load the one past the wave's edge
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t k = i < nv ? keys[i] : 0ull;
+    uint64_t kprev = __shfl_up(k, 1, 64), knext = __shfl_down(k, 1, 64);
+    if (lane == 0 && i > 0 && i < nv) kprev = keys[i - 1];
+    if (lane == 63 && i + 1 < nv) knext = keys[i + 1];
     if (i < nv) {
-        const uint64_t k = keys[i];
         int lp, li;
         if (nv == 1) {
             lp = -1;
             li = 0;  // the lone body hangs below an internal root
         } else {
-            lp = i > 0 ? lcp_digits(keys[i - 1], k) : -1;
-            li = i + 1 < nv ? lcp_digits(k, keys[i + 1]) : -1;
+            lp = i > 0 ? lcp_digits(kprev, k) : -1;
+            li = i + 1 < nv ? lcp_digits(k, knext) : -1;
         }
         const uint32_t c = li > lp ? (uint32_t)(li - lp) : 0u;
         const int a0 = lp + 1;
