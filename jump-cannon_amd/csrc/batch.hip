@@ -363,17 +363,25 @@ __global__ __launch_bounds__(kBT) void k_b_emit(
     }
     __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    // the neighbours' keys come from the neighbouring lanes; the wave's first
+    // and last lanes load the one past the wave's edge (only read inside the
+    // body's own graph segment)
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t kown = i < n ? keys[i] : 0ull;
+    uint64_t kprev = __shfl_up(kown, 1, 64), knext = __shfl_down(kown, 1, 64);
+    if (lane == 0 && i > 0 && i < n) kprev = keys[i - 1];
+    if (lane == 63 && i + 1 < n) knext = keys[i + 1];
     if (i < n) {
         const Seg sg = seg_of(i, gid, base, count, fr);
         if (i < sg.ve) {
-            const uint64_t k = keys[i];
+            const uint64_t k = kown;
             int lp, li;
             if (sg.ve - sg.lo == 1) {
                 lp = -1;
                 li = 0;
             } else {
-                lp = i > sg.lo ? lcp_digits(keys[i - 1], k) : -1;
-                li = i + 1 < sg.ve ? lcp_digits(k, keys[i + 1]) : -1;
+                lp = i > sg.lo ? lcp_digits(kprev, k) : -1;
+                li = i + 1 < sg.ve ? lcp_digits(k, knext) : -1;
             }
             const uint32_t c = li > lp ? (uint32_t)(li - lp) : 0u;
             const int a0 = lp + 1;
