@@ -1288,19 +1288,19 @@ class Fa2BhEngine final : public LayoutEngine {
         if (n_valid) *n_valid = fr.n_valid;
         if (root4) std::memcpy(root4, fr.root, sizeof fr.root);
         if (n_ == 0) return Status::ok();
-        if (keys) JC_HIP(hipMemcpy(keys, keys_.p, n_ * sizeof(uint64_t), hipMemcpyDeviceToHost));
-        if (perm) JC_HIP(hipMemcpy(perm, vals_.p, n_ * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (keys) JC_TRY(copy_sync(keys, keys_.p, n_ * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_));
+        if (perm) JC_TRY(copy_sync(perm, vals_.p, n_ * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
         const uint32_t m = std::min<uint32_t>(cap, fr.n_tree);
         if (m) {
             std::vector<Node> h(m);
-            JC_HIP(hipMemcpy(h.data(), nodes_.p, m * sizeof(Node), hipMemcpyDeviceToHost));
+            JC_TRY(copy_sync(h.data(), nodes_.p, m * sizeof(Node), hipMemcpyDeviceToHost, stream_));
             for (uint32_t q = 0; q < m; q++) {
                 if (node_a) std::memcpy(node_a + 4 * (size_t)q, &h[q].a, sizeof(float4));
                 // exported form: leaf 0x80000000 | body, cell the bits of (2*half)^2
                 if (info) info[q] = h[q].b.z != kCellId ? (kLeaf | h[q].b.z) : h[q].b.y;
                 if (skip) skip[q] = h[q].b.w;
             }
-            if (depth) JC_HIP(hipMemcpy(depth, node_depth_.p, m, hipMemcpyDeviceToHost));
+            if (depth) JC_TRY(copy_sync(depth, node_depth_.p, m, hipMemcpyDeviceToHost, stream_));
         }
         return Status::ok();
     }

@@ -44,6 +44,16 @@ struct Status {
 // device is visible; selects device `dev` otherwise.
 Status require_device(int *dev_out);
 
+// A copy queued on `st` behind the work already there, then a wait for the
+// stream.  Engines run on non-blocking streams, which a plain hipMemcpy on the
+// null stream does not wait for.
+inline Status copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t st) {
+    if (!bytes) return Status::ok();
+    JC_HIP(hipMemcpyAsync(dst, src, bytes, kind, st));
+    JC_HIP(hipStreamSynchronize(st));
+    return Status::ok();
+}
+
 // Owning device allocation.
 template <typename T>
 struct DevBuf {

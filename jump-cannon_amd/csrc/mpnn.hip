@@ -568,17 +568,17 @@ class Mpnn {
     }
 
     Status get_params(float *out) {
-        JC_HIP(hipMemcpy(out, params_.p, param_count() * 4, hipMemcpyDeviceToHost));
+        JC_TRY(copy_sync(out, params_.p, param_count() * 4, hipMemcpyDeviceToHost, stream_));
         return Status::ok();
     }
     Status set_params(const float *in) {
-        JC_HIP(hipMemcpy(params_.p, in, param_count() * 4, hipMemcpyHostToDevice));
+        JC_TRY(copy_sync(params_.p, in, param_count() * 4, hipMemcpyHostToDevice, stream_));
         JC_TRY(pack());
         JC_HIP(hipStreamSynchronize(stream_));
         return Status::ok();
     }
     Status get_grads(float *out) {
-        JC_HIP(hipMemcpy(out, grads_.p, param_count() * 4, hipMemcpyDeviceToHost));
+        JC_TRY(copy_sync(out, grads_.p, param_count() * 4, hipMemcpyDeviceToHost, stream_));
         return Status::ok();
     }
 
@@ -636,7 +636,7 @@ class Mpnn {
 
     Status features(float *out) {
         std::vector<uint16_t> h((size_t)n_ * kIn);
-        JC_HIP(hipMemcpy(h.data(), x_.p, h.size() * 2, hipMemcpyDeviceToHost));
+        JC_TRY(copy_sync(h.data(), x_.p, h.size() * 2, hipMemcpyDeviceToHost, stream_));
         for (size_t q = 0; q < h.size(); q++) out[q] = bf2f(h[q]);
         return Status::ok();
     }
@@ -655,7 +655,7 @@ class Mpnn {
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
         if (ms) *ms = t;
-        if (out_xyz) JC_HIP(hipMemcpy(out_xyz, y_.p, (size_t)n_ * 3 * 4, hipMemcpyDeviceToHost));
+        if (out_xyz) JC_TRY(copy_sync(out_xyz, y_.p, (size_t)n_ * 3 * 4, hipMemcpyDeviceToHost, stream_));
         return Status::ok();
     }
 
@@ -685,14 +685,14 @@ class Mpnn {
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
         if (ms) *ms = t;
-        if (out_xyz) JC_HIP(hipMemcpy(out_xyz, yout_.p, (size_t)n_ * 3 * 4, hipMemcpyDeviceToHost));
+        if (out_xyz) JC_TRY(copy_sync(out_xyz, yout_.p, (size_t)n_ * 3 * 4, hipMemcpyDeviceToHost, stream_));
         return Status::ok();
     }
 
     Status activations(int layer, float *out) {  // debug: h_layer (n x 128) as f32
         if (layer < 0 || layer >= kLayers) return Status::err(JC_E_INVALID, "bad layer");
         std::vector<uint16_t> h((size_t)n_ * kHid);
-        JC_HIP(hipMemcpy(h.data(), h_[layer].p, h.size() * 2, hipMemcpyDeviceToHost));
+        JC_TRY(copy_sync(h.data(), h_[layer].p, h.size() * 2, hipMemcpyDeviceToHost, stream_));
         for (size_t q = 0; q < h.size(); q++) out[q] = bf2f(h[q]);
         return Status::ok();
     }
@@ -700,7 +700,7 @@ class Mpnn {
     Status set_target(const float *tgt, uint32_t n_anchor, uint64_t seed) {
         if (n_ == 0) return Status::ok();
         JC_TRY(tgt_.alloc((size_t)n_ * 3));
-        JC_HIP(hipMemcpy(tgt_.p, tgt, (size_t)n_ * 3 * 4, hipMemcpyHostToDevice));
+        JC_TRY(copy_sync(tgt_.p, tgt, (size_t)n_ * 3 * 4, hipMemcpyHostToDevice, stream_));
         std::vector<uint32_t> anc;
         uint64_t s = seed;
         const uint32_t na = std::min(n_anchor, n_);
@@ -715,7 +715,7 @@ class Mpnn {
         }
         na_ = na;
         JC_TRY(anchors_.alloc(na ? na : 1));
-        if (na) JC_HIP(hipMemcpy(anchors_.p, anc.data(), na * 4, hipMemcpyHostToDevice));
+        if (na) JC_TRY(copy_sync(anchors_.p, anc.data(), na * 4, hipMemcpyHostToDevice, stream_));
         JC_TRY(lparts_.alloc(cd(n_, 256)));
         JC_TRY(ltot_.alloc(1));
         has_target_ = true;

@@ -363,7 +363,7 @@ class PageRankPlan {
         if (iters) *iters = hs.iters;
         if (resid) *resid = hs.resid;
         if (out)
-            JC_HIP(hipMemcpy(out, rank_[hs.iters & 1u].p, (size_t)n_ * 4, hipMemcpyDeviceToHost));
+            JC_TRY(copy_sync(out, rank_[hs.iters & 1u].p, (size_t)n_ * 4, hipMemcpyDeviceToHost, stream_));
         return Status::ok();
     }
 

@@ -310,7 +310,7 @@ class SgdStressEngine final : public LayoutEngine {
             std::vector<float4> h(n_);
             for (uint32_t i = 0; i < n_; i++)
                 h[i] = make_float4(xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2], 0.0f);
-            JC_HIP(hipMemcpy(pos_[cur_].p, h.data(), n_ * sizeof(float4), hipMemcpyHostToDevice));
+            JC_TRY(copy_sync(pos_[cur_].p, h.data(), n_ * sizeof(float4), hipMemcpyHostToDevice, stream_));
         }
         if (se2) sweep_ = (uint64_t)se2[1];
         iteration_ = it;
@@ -324,7 +324,7 @@ class SgdStressEngine final : public LayoutEngine {
         const size_t chunks = (k_ + 7) / 8;
         std::vector<uint16_t> c8(chunks * n_ * 8);
         if (!c8.empty())
-            JC_HIP(hipMemcpy(c8.data(), dist_.p, c8.size() * 2, hipMemcpyDeviceToHost));
+            JC_TRY(copy_sync(c8.data(), dist_.p, c8.size() * 2, hipMemcpyDeviceToHost, stream_));
         for (uint32_t t = 0; t < k_; t++)
             for (uint32_t i = 0; i < n_; i++)
                 out[(size_t)i * k_ + t] = c8[((size_t)(t >> 3) * n_ + i) * 8 + (t & 7)];
@@ -407,8 +407,8 @@ class StressPlan {
         JC_TRY(dist_.alloc(pm.size()));
         JC_TRY(pos_.alloc(n ? n : 1));
         JC_TRY(part_.alloc(3 * (size_t)cdiv(n ? n : 1, 256)));
-        if (k) JC_HIP(hipMemcpy(piv_.p, pivots, k * 4, hipMemcpyHostToDevice));
-        if (!pm.empty()) JC_HIP(hipMemcpy(dist_.p, pm.data(), pm.size() * 2, hipMemcpyHostToDevice));
+        if (k) JC_TRY(copy_sync(piv_.p, pivots, k * 4, hipMemcpyHostToDevice, stream_));
+        if (!pm.empty()) JC_TRY(copy_sync(dist_.p, pm.data(), pm.size() * 2, hipMemcpyHostToDevice, stream_));
         return Status::ok();
     }
     Status eval(const float *xyz, double *out) {

@@ -25,8 +25,9 @@
 //   * k_bfs_level: one launch per level over the whole chip for frontiers that
 //     outgrow the LDS queue (scale-free graphs reach half the graph within a
 //     few hops); visited = atomicCAS on the distance array, appends aggregated
-//     per wave.  Frontiers above n/32 run bottom-up instead (k_bfs_level_up:
-//     each unreached node stops at its first neighbour in the frontier).
+//     per wave.  Frontiers above n/32 of a symmetric CSR run bottom-up instead
+//     (k_bfs_level_up: each unreached node stops at its first neighbour in the
+//     frontier); a directed CSR stays top-down (k_csr_symmetric decides).
 // The host switches between the two by the frontier size the block kernel
 // leaves in BfsState.
 //
@@ -463,6 +464,37 @@ __global__ __launch_bounds__(kLevelTB) void k_bfs_level_up(const uint32_t *__res
     }
 }
 
+// Bottom-up levels scan a node's own row for a parent, which finds every
+// parent only when each entry u -> v has its reverse v -> u.  One thread per
+// CSR entry: the row u holding entry j (the last u with off[u] <= j), then a
+// binary search for u in row v.  *not_sym is set by an entry without a
+// reverse and by a row out of ascending order (its entries cannot be searched,
+// and the CSR is then taken as directed).
+__global__ __launch_bounds__(kTB) void k_csr_symmetric(const uint32_t *__restrict__ off,
+                                                       const uint32_t *__restrict__ nbr, uint32_t n,
+                                                       uint64_t nnz, uint32_t *not_sym) {
+    for (uint64_t j = (uint64_t)blockIdx.x * kTB + threadIdx.x; j < nnz; j += (uint64_t)gridDim.x * kTB) {
+        uint32_t lo = 0, hi = n;  // last u in [0, n) with off[u] <= j
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if ((uint64_t)off[mid] <= j) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint32_t u = lo, v = nbr[j];
+        bool bad = j > off[u] && nbr[j - 1] > v;
+        if (!bad) {
+            uint32_t a = off[v], b = off[v + 1];
+            while (a < b) {
+                const uint32_t mid = a + ((b - a) >> 1);
+                if (nbr[mid] < u) a = mid + 1;
+                else b = mid;
+            }
+            bad = a == off[v + 1] || nbr[a] != u;
+        }
+        if (bad) atomicOr(not_sym, 1u);
+    }
+}
+
 // ---- max-min pivot selection (sgd_stress.rs:143-175) ----
 // After the BFS of pivot t: mind[v] = min(mind[v], d[v]) over reachable v; the
 // next pivot is the node of largest finite cover mind[v] > 0, smallest index on
@@ -819,6 +851,16 @@ class BfsPlan {
             k_build_ell<<<grid_for(n_), kTB, 0, stream_>>>(n_, off_.p, nbr_.p, ell_.p);
             JC_HIP(hipGetLastError());
         }
+        symmetric_ = true;
+        if (nnz_) {
+            k_fill_u32<<<1, kTB, 0, stream_>>>(cnt_.p, 1, 0u);
+            k_csr_symmetric<<<std::min<uint32_t>(cdiv(nnz_, kTB), 4096), kTB, 0, stream_>>>(off_.p, nbr_.p, n_,
+                                                                                          nnz_, cnt_.p);
+            JC_HIP(hipGetLastError());
+            JC_HIP(hipMemcpyAsync(&h_qn_, cnt_.p, 4, hipMemcpyDeviceToHost, stream_));
+            JC_HIP(hipStreamSynchronize(stream_));
+            symmetric_ = h_qn_ == 0;
+        }
         JC_HIP(hipStreamSynchronize(stream_));
         return Status::ok();
     }
@@ -843,8 +885,9 @@ class BfsPlan {
             // fits again (or the search ends)
             while (h_.qn > kBfsQ) {
                 k_fill_u32<<<1, kTB, 0, stream_>>>(cnt_.p, 1, 0u);
-                if ((uint64_t)h_.qn * kBottomUpShare > n_) {
+                if (symmetric_ && (uint64_t)h_.qn * kBottomUpShare > n_) {
                     // large frontier: every unreached node looks for a parent
+                    // in its own row (every in-neighbour is in it: symmetric CSR)
                     const uint32_t blocks = std::min<uint32_t>(cdiv(n_, kLevelTB), 4096);
                     k_bfs_level_up<<<blocks, kLevelTB, 0, stream_>>>(off_.p, nbr_.p, dist_.p, n_, h_.level,
                                                                       q_[h_.cur ^ 1].p, cnt_.p);
@@ -899,9 +942,11 @@ class BfsPlan {
     const uint32_t *off() const { return off_.p; }
     const uint32_t *nbr() const { return nbr_.p; }
     hipStream_t stream() const { return stream_; }
+    bool symmetric() const { return symmetric_; }
 
   private:
     uint32_t n_ = 0;
+    bool symmetric_ = true;  // every entry u -> v has v -> u: bottom-up levels allowed
     uint64_t nnz_ = 0;
     BfsState h_{};
     uint32_t h_qn_ = 0;
@@ -974,7 +1019,7 @@ Status connected_components(const CsrView &g, uint32_t *out_labels) {
     JC_TRY(plan.create(g, nullptr));
     DevBuf<uint32_t> label;
     JC_TRY(components_on(plan, label));
-    if (g.n) JC_HIP(hipMemcpy(out_labels, label.p, (size_t)g.n * 4, hipMemcpyDeviceToHost));
+    if (g.n) JC_TRY(copy_sync(out_labels, label.p, (size_t)g.n * 4, hipMemcpyDeviceToHost, plan.stream()));
     return Status::ok();
 }
 

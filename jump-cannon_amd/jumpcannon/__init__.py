@@ -200,6 +200,14 @@ def _f32(a):
     return np.ascontiguousarray(a, dtype=np.float32)
 
 
+def _need_len(a, want: int, what: str):
+    """The C entry points read exactly `want` elements from this buffer: a
+    shorter one would be read past its end, a longer one means a caller error."""
+    if a is not None and len(a) != want:
+        raise JcError(JC_E_INVALID, f"{what} length {len(a)} != {want}")
+    return a
+
+
 class LayoutEngine:
     """One engine handle (the reference LayoutEngine trait object)."""
 
@@ -253,6 +261,7 @@ class LayoutEngine:
         neighbors = _u32(neighbors)
         xyz = _f32(xyz)
         n = len(offsets) - 1
+        _need_len(xyz, 3 * n, "positions")
         _check(lib().jc_engine_reinit(self._h, n, _ptr(offsets), _ptr(neighbors) if len(neighbors) else None,
                                       len(neighbors), _ptr(xyz) if n else None))
         self.n = n
@@ -276,9 +285,9 @@ class LayoutEngine:
                 "speed_efficiency": float(se[1]), "iteration": int(it[0]), "speed_eff2": se}
 
     def set_state(self, positions=None, old_force=None, speed_eff2=None, iteration=0):
-        p = None if positions is None else _f32(positions)
-        o = None if old_force is None else _f32(old_force)
-        s = None if speed_eff2 is None else _f32(speed_eff2)
+        p = _need_len(None if positions is None else _f32(positions), 3 * self.n, "positions")
+        o = _need_len(None if old_force is None else _f32(old_force), 3 * self.n, "old_force")
+        s = _need_len(None if speed_eff2 is None else _f32(speed_eff2), 2, "speed_eff2")
         _check(lib().jc_engine_set_state(self._h, _ptr(p), _ptr(o), _ptr(s), iteration))
 
     def is_halted(self) -> bool:
@@ -341,7 +350,7 @@ class Fa2BhEngine(LayoutEngine):
 
     def forces_at(self, xyz):
         """One force pass at xyz: returns (F (3n) f32, [sum swing, sum traction] f64)."""
-        xyz = _f32(xyz)
+        xyz = _need_len(_f32(xyz), 3 * self.n, "positions")
         f = np.empty(3 * self.n, np.float32)
         sums = np.zeros(2, np.float64)
         _check(lib().jc_fa2bh_forces_at(self._h, _ptr(xyz), _ptr(f), _ptr(sums)))
@@ -501,13 +510,15 @@ class StressJudge:
     def __init__(self, n: int, pivots, dist_node_major):
         self._piv = _u32(pivots)
         self._dist = np.ascontiguousarray(dist_node_major, np.uint16)
+        self.n = int(n)
+        _need_len(self._dist, self.n * len(self._piv), "pivot distance table")
         h = C.c_void_p()
         _check(lib().jc_stress_plan_create(n, len(self._piv), _ptr(self._piv), _ptr(self._dist),
                                            C.addressof(h)))
         self._h = h.value
 
     def __call__(self, xyz) -> float:
-        x = _f32(xyz)
+        x = _need_len(_f32(xyz), 3 * self.n, "positions")
         out = np.zeros(1, np.float64)
         _check(lib().jc_stress_plan_eval(self._h, _ptr(x), _ptr(out)))
         return float(out[0])
@@ -535,8 +546,7 @@ class Mpnn:
         return out
 
     def set_params(self, p):
-        p = _f32(p)
-        assert len(p) == self.P
+        p = _need_len(_f32(p), self.P, "parameters")
         _check(lib().jc_mpnn_set_params(self._h, _ptr(p)))
 
     def grads(self):
@@ -548,8 +558,8 @@ class Mpnn:
         off = _u32(offsets)
         nb = _u32(neighbors)
         self._keep = (off, nb)
+        nz = _need_len(None if noise is None else _f32(noise), 3 * (len(off) - 1), "noise")
         self.n = len(off) - 1
-        nz = None if noise is None else _f32(noise)
         _check(lib().jc_mpnn_set_graph(self._h, self.n, _ptr(off), _ptr(nb) if len(nb) else None,
                                        len(nb), feat_seed, walks, _ptr(nz)))
 
@@ -579,7 +589,7 @@ class Mpnn:
         return out, float(ms[0])
 
     def set_target(self, xyz, n_anchor: int = 32, seed: int = 3):
-        x = _f32(xyz)
+        x = _need_len(_f32(xyz), 3 * self.n, "target positions")
         _check(lib().jc_mpnn_set_target(self._h, _ptr(x), n_anchor, seed))
 
     def loss_grad(self) -> float:

@@ -167,3 +167,33 @@ def test_multilevel_settings_decode_without_a_device():
     with pytest.raises(jc.JcError) as ei:
         e.state()  # before init
     assert ei.value.code == jc.JC_E_STATE
+
+
+def test_buffer_lengths_checked_before_the_call():
+    """Every buffer the C side reads a fixed count from is checked in Python
+    first (a short one would be read past its end)."""
+    e = jc.Fa2BhEngine()  # not initialised: n = 0
+    off2 = np.array([0, 1, 2], np.uint32)
+    nb2 = np.array([1, 0], np.uint32)
+    cases = [
+        lambda: e.reinit(off2, nb2, np.zeros(5, np.float32)),
+        lambda: e.set_state(positions=np.zeros(3, np.float32)),
+        lambda: e.set_state(old_force=np.zeros(3, np.float32)),
+        lambda: e.set_state(speed_eff2=np.zeros(1, np.float32)),
+        lambda: e.forces_at(np.zeros(3, np.float32)),
+        lambda: jc.StressJudge(10, np.array([0], np.uint32), np.zeros(9, np.uint16)),
+    ]
+    for k, call in enumerate(cases):
+        with pytest.raises(jc.JcError) as ei:
+            call()
+        assert ei.value.code == jc.JC_E_INVALID and "length" in str(ei.value), k
+    # the network's buffers, checked without creating a device handle
+    m = jc.Mpnn.__new__(jc.Mpnn)
+    m._h, m.n, m.P = None, 4, int(jc.lib().jc_mpnn_param_count())
+    for call in (lambda: m.set_params(np.zeros(m.P - 1, np.float32)),
+                 lambda: m.set_target(np.zeros(11, np.float32)),
+                 lambda: m.set_graph(np.array([0, 1, 2, 3, 4], np.uint32), np.array([1, 0, 3, 2], np.uint32),
+                                     noise=np.zeros(11, np.float32))):
+        with pytest.raises(jc.JcError) as ei:
+            call()
+        assert ei.value.code == jc.JC_E_INVALID

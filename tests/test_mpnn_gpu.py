@@ -93,3 +93,21 @@ def test_training_reduces_loss_and_is_deterministic():
     print(f"loss {L0:.4f} -> {L1:.4f}")
     assert L1 < 0.5 * L0
     assert runs[1][1] == L1 and np.array_equal(runs[1][2], p)
+
+
+def test_param_access_is_ordered_after_queued_updates():
+    """params()/set_params()/grads() right after queued work read or write the
+    values that work leaves (copies on the network's stream)."""
+    net, off, nb = _setup(2000, 8)
+    n = len(off) - 1
+    net.set_target(O.uniform_cube(n, 1.0, 9), n_anchor=32, seed=5)
+    net.loss_grad()
+    g = net.grads()
+    net.adam(1e-3)
+    p1 = net.params()
+    assert np.array_equal(p1, net.params())
+    assert np.any(p1 != 0) and np.any(g != 0)
+    net.loss_grad()
+    net.adam(1e-3)
+    net.set_params(p1)  # lands after the queued step, which it overwrites
+    assert np.array_equal(net.params(), p1)

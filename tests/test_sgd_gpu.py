@@ -105,3 +105,23 @@ def test_rgg_200k_30_epochs_bit_exact_and_stress_drops():
     for ref, x in ((s0, xyz), (s1, pos)):
         g = judge(x)
         assert abs(g - ref) <= 2e-7 * ref + 1e-12, (g, ref)
+
+
+def test_set_state_waits_for_queued_sweeps():
+    """set_state after step(k, readback=False) must land after the queued
+    sweeps (the copy is ordered on the engine stream): a resumed run equals an
+    uninterrupted one bit for bit."""
+    off, nb, _ = workloads.rgg_200k()
+    n = len(off) - 1
+    xyz = O.uniform_cube(n, 0.5, 5)
+    xyz[2::3] = 0.0
+    e = jc.SgdStressEngine(n_pivots=50, eta_max=0.0, eta_min=0.01, n_anneal_steps=30)
+    e.init(off, nb, xyz)
+    mid = e.step(3)
+    st = e.get_state()
+    ref = e.step(4)
+    e.step(20, readback=False)  # queued, not waited for
+    e.set_state(positions=mid, speed_eff2=st["speed_eff2"], iteration=st["iteration"])
+    back = e.get_state()
+    assert np.array_equal(back["positions"].view(np.uint32), mid.view(np.uint32))
+    assert np.array_equal(e.step(4).view(np.uint32), ref.view(np.uint32))

@@ -62,6 +62,18 @@ def test_bfs_frontier_spills_and_returns():
     _bfs_eq(off, nb, 777)
 
 
+def test_bfs_directed_hub_stays_top_down():
+    """A directed CSR whose 10,000-node frontier would go bottom-up on a
+    symmetric graph: distances follow out-edges only, as cpu_bfs does
+    (bfs.rs:44-66).  Also the symmetric star with its rows shuffled."""
+    off, nb = G.directed_hub(10000)
+    d = _bfs_eq(off, nb, 0)
+    assert d[10001:15001].tolist() == [2] * 5000
+    assert (d[15001:] == UNREACHABLE).all()
+    _bfs_eq(*G.shuffled_rows(*G.star(20000)), 0)
+    _bfs_eq(*G.shuffled_rows(*G.broom(6000, 100)), 1)
+
+
 def test_bfs_ba_1m():
     """The 1M-node scale-free workload graph (above the LDS bitmap size, so the
     workgroup kernel claims nodes on the distance array), several sources."""

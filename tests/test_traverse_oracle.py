@@ -112,3 +112,15 @@ def test_largest_component_matches_scipy():
     off, nb = G.cycles(8, 8)
     size, keep = O.largest_component(16, off, nb)
     assert size == 8 and keep.tolist() == list(range(8)) + [UNREACHABLE] * 8
+
+
+def test_bfs_directed_hub_follows_out_edges():
+    """The directed hub of the GPU's top-down-only case: the oracle follows
+    out-edges (bfs.rs:44-66), as scipy's directed search does."""
+    off, nb = G.directed_hub(10000)
+    d = O.bfs(off, nb, 0)
+    assert np.array_equal(d, _scipy_bfs(off, nb, 0))
+    assert d[1:10001].tolist() == [1] * 10000 and d[10001:15001].tolist() == [2] * 5000
+    assert (d[15001:] == UNREACHABLE).all()
+    so, sn = G.shuffled_rows(*G.star(300))
+    assert np.array_equal(O.bfs(so, sn, 0), _scipy_bfs(so, sn, 0))

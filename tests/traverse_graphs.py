@@ -103,3 +103,31 @@ def directed_only(n=30, seed=5):
         if a != b:
             adj[int(a)].add(int(b))
     return csr_from_adj(adj)
+
+
+def directed_hub(leaves=10000):
+    """A CSR that is not symmetric, with a frontier too large for the LDS queue:
+    0 -> 1..leaves (level 1); leaf i -> leaves+i for i <= leaves/2 (level 2,
+    reached only along out-edges); leaves/2 more nodes that point INTO the
+    leaves and are never reached from 0.  A bottom-up level that scanned a
+    node's own row would get both of the last two groups wrong."""
+    half = leaves // 2
+    n = 1 + leaves + half + half
+    adj = [set() for _ in range(n)]
+    adj[0] = set(range(1, leaves + 1))
+    for i in range(1, half + 1):
+        adj[i].add(leaves + i)
+    for k in range(half):
+        adj[1 + leaves + half + k].add(1 + k)
+    return csr_from_adj(adj)
+
+
+def shuffled_rows(off, nb, seed=1):
+    """The same graph with every row in a random order (not searchable by
+    bisection: such a CSR is treated as directed)."""
+    rng = np.random.default_rng(seed)
+    nb = nb.copy()
+    for v in range(len(off) - 1):
+        seg = nb[off[v]:off[v + 1]]
+        rng.shuffle(seg)
+    return off, nb
