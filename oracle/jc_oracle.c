@@ -498,7 +498,7 @@ static int ins_insert(ins_tree *t, uint32_t root, const float *p, uint32_t body,
 
 uint32_t jco_octree_ref(uint32_t n, const float *xyz, const float *mass, uint32_t max_nodes,
                         float *node_a, float *half, uint32_t *body, uint32_t *skip,
-                        uint32_t *depth, float *root4) {
+                        uint32_t *depth, float *root4, float *centre) {
     if (n == 0 || max_nodes == 0) return 0;
     ins_tree t;
     t.v = (ins_node *)malloc((size_t)max_nodes * sizeof(ins_node));
@@ -543,6 +543,7 @@ uint32_t jco_octree_ref(uint32_t n, const float *xyz, const float *mass, uint32_
         body[q] = nd->body;
         skip[q] = (q + size[q] < cnt) ? q + size[q] : JCO_END;
         if (depth) depth[q] = nd->depth;
+        if (centre) for (int k = 0; k < 3; k++) centre[3 * (size_t)q + k] = nd->ps[k];
     }
     canonical_com(cnt, node_a, body, skip);
     free(order); free(newid); free(stack); free(size); free(t.v);
@@ -1000,7 +1001,7 @@ int jco_fa2_run(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, float
     for (uint32_t it = 0; it < iters; it++) {
         double t0 = now_s();
         uint32_t N = tree_kind == 0
-                         ? jco_octree_ref(n, xyz, mass, cap, node_a, half, body, skip, NULL, NULL)
+                         ? jco_octree_ref(n, xyz, mass, cap, node_a, half, body, skip, NULL, NULL, NULL)
                          : jco_octree_morton(n, xyz, mass, node_a, half, body, skip, NULL, NULL,
                                              NULL, NULL, NULL);
         double t1 = now_s();

@@ -49,9 +49,12 @@ uint64_t jco_fa2_prepare(uint32_t n, const uint32_t *offsets, const uint32_t *nb
  * node_a[4*k] = com.x, com.y, com.z, mass; half[k] = cell half extent;
  * body[k] = body index or JCO_INTERNAL; skip[k] = pre-order rope or JCO_END;
  * depth[k].  For an internal node with children, next == k+1. */
+/* centre[3k] (nullable) = the cell's centre (for a leaf: the cell the body
+ * was placed in).  Insertion stops at the first body that does not fit in
+ * max_nodes and the partial tree is returned (fa2_bh.rs:813-819, 881-883). */
 uint32_t jco_octree_ref(uint32_t n, const float *xyz, const float *mass, uint32_t max_nodes,
                         float *node_a, float *half, uint32_t *body, uint32_t *skip,
-                        uint32_t *depth, float *root4);
+                        uint32_t *depth, float *root4, float *centre);
 uint32_t jco_octree_morton(uint32_t n, const float *xyz, const float *mass, float *node_a,
                            float *half, uint32_t *body, uint32_t *skip, uint32_t *depth,
                            uint64_t *keys_sorted, uint32_t *perm, uint32_t *n_valid,

@@ -81,7 +81,7 @@ def lib():
         "jco_largest_component": (C.c_uint32, [C.c_uint32, u32p, u32p, u32p]),
         "jco_fa2_prepare": (C.c_uint64, [C.c_uint32, u32p, u32p, u32p, u32p, u32p]),
         "jco_octree_ref": (C.c_uint32, [C.c_uint32, f32p, f32p, C.c_uint32, f32p, f32p, u32p, u32p,
-                                        u32p, f32p]),
+                                        u32p, f32p, f32p]),
         "jco_octree_morton": (C.c_uint32, [C.c_uint32, f32p, f32p, f32p, f32p, u32p, u32p, u32p,
                                            u64p, u32p, u32p, f32p]),
         "jco_morton_keys": (None, [C.c_uint32, f32p, u64p, f32p]),
@@ -237,8 +237,11 @@ def octree_ref(xyz, mass, max_nodes=None):
     skip = np.zeros(max(cap, 1), np.uint32)
     depth = np.zeros(max(cap, 1), np.uint32)
     root4 = np.zeros(4, np.float32)
-    N = lib().jco_octree_ref(n, xyz, _f32(mass), cap, na, half, body, skip, depth, root4)
-    return Tree(N, na, half, body, skip, depth, root4)
+    centre = np.zeros(3 * max(cap, 1), np.float32)
+    N = lib().jco_octree_ref(n, xyz, _f32(mass), cap, na, half, body, skip, depth, root4, centre)
+    t = Tree(N, na, half, body, skip, depth, root4)
+    t.centre = centre[: 3 * N].reshape(-1, 3)
+    return t
 
 
 def octree_morton(xyz, mass):
@@ -312,6 +315,17 @@ def speed_update(se, n, jt, stats, exact=False):
     sums = np.zeros(2, np.float64)
     lib().jco_speed_update(se, n, jt, _f32(stats), 1 if exact else 0, sums)
     return se, sums
+
+
+def fa2_apply(xyz, force, old_force, deg, settings: Fa2Settings, speed: float):
+    """One fa2_apply pass (fa2_barnes_hut.wgsl:203-226).  Returns (new xyz,
+    new F_{t-1} = force)."""
+    xyz = _f32(xyz)
+    n = len(xyz) // 3
+    out = np.empty_like(xyz)
+    old = _f32(old_force).copy()
+    lib().jco_fa2_apply(n, xyz, out, _f32(force), old, _u32(deg), C.byref(settings), speed, 1)
+    return out, old
 
 
 def fa2_run(offsets, nbrs, xyz, iters, settings: Fa2Settings, tree_kind=1, exact=True, threads=8,
