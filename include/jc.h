@@ -194,6 +194,23 @@ int jc_pagerank_plan_run(jc_pagerank_plan *plan, uint32_t max_iters, double tol_
                          double *out_gpu_ms);
 int jc_pagerank_plan_time(jc_pagerank_plan *plan, uint32_t n_iters, double *ms_per_iter);
 void jc_pagerank_plan_destroy(jc_pagerank_plan *plan);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): jc_pagerank over a CSR the caller already holds in device
+ * memory, with no device allocation.  d_offsets (n+1), d_neighbors (nnz) and
+ * d_out_ranks (n) are device pointers; stream is a hipStream_t (NULL: the null
+ * stream); out_iters / out_residual_l1 are host pointers (nullable).  Same
+ * iteration, stopping rule and result bits as jc_pagerank; the CSR is checked
+ * on the device (JC_E_INVALID, JC_E_DANGLING as jc_pagerank).  The workspace
+ * (1/deg, two rank and two x vectors, the residual accumulator and state) is
+ * one static device array of about 20.0 MiB per device, held until the process
+ * ends; n above JC_PAGERANK_NOALLOC_MAX_N is JC_E_INVALID.  Every call on a
+ * device shares that device's workspace: calls that overlap on one device
+ * race and must be serialised by the caller. */
+#define JC_PAGERANK_NOALLOC_MAX_N (1u << 20)
+int jc_pagerank_noalloc(uint32_t n, const uint32_t *d_offsets, const uint32_t *d_neighbors, uint64_t nnz,
+                        float damping, uint32_t max_iters, double tol_l1, float *d_out_ranks,
+                        uint32_t *out_iters, double *out_residual_l1, void *stream);
 
 /* ---- SpMV (crates/graph-compute/src/analytics/spmv.rs) ----
  * y = A x for a weighted CSR matrix (WeightedCsr, spmv.rs:26-32: n rows, offsets

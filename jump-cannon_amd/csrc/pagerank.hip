@@ -20,6 +20,8 @@
 // once, so a whole chunk of iterations runs without a host round trip.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "engine.hpp"
@@ -260,6 +262,149 @@ __global__ __launch_bounds__(640) void k_pr_reduce(unsigned long long *__restric
 
 inline uint32_t cdiv32(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// ---- the allocation-free variant (jc_pagerank_noalloc) ----
+// The CSR is the caller's, in device memory, so nothing is prepared on the
+// host: the CSR is checked and 1/deg formed on the device, and each iteration
+// runs over the plain CSR (no hub list, no light CSR).  Every row is still
+// summed in CSR order, so the ranks are bit-identical to k_pr_iter's.
+
+// Set by k_pr_check_csr: bad = 1 offsets[0] != 0, 2 offsets[n] != nnz,
+// 4 decreasing offsets, 8 a neighbour id >= n; dangling = smallest node of
+// degree 0 (0xFFFFFFFF: none).
+struct CsrCheck {
+    uint32_t bad;
+    uint32_t dangling;
+};
+
+__global__ void k_pr_check_clear(CsrCheck *c) {
+    if (threadIdx.x == 0) {
+        c->bad = 0u;
+        c->dangling = 0xFFFFFFFFu;
+    }
+}
+
+__global__ __launch_bounds__(kPrTB) void k_pr_check_csr(uint32_t n, const uint32_t *__restrict__ off,
+                                                        const uint32_t *__restrict__ nbr, uint64_t nnz,
+                                                        float *__restrict__ inv, CsrCheck *c) {
+    const uint32_t stride = gridDim.x * kPrTB;
+    for (uint32_t v = blockIdx.x * kPrTB + threadIdx.x; v <= n; v += stride) {
+        const uint32_t a = off[v];
+        if (v == 0 && a != 0u) atomicOr(&c->bad, 1u);
+        if (v == n) {
+            if ((uint64_t)a != nnz) atomicOr(&c->bad, 2u);
+            continue;
+        }
+        const uint32_t b = off[v + 1];
+        if (b < a) {
+            atomicOr(&c->bad, 4u);
+            continue;
+        }
+        if (b == a) atomicMin(&c->dangling, v);
+        else inv[v] = __fdiv_rn(1.0f, (float)(b - a));  // pagerank.rs:105, correctly rounded
+    }
+    for (uint64_t j = (uint64_t)blockIdx.x * kPrTB + threadIdx.x; j < nnz; j += stride)
+        if (nbr[j] >= n) atomicOr(&c->bad, 8u);
+}
+
+// One iteration over the plain CSR, 64 consecutive rows per wave.  A row of at
+// most kHeavy entries is summed by its own lane straight from global memory;
+// longer rows are taken one after another by the whole wave, which gathers
+// kHubChunk x values per round into LDS for lane 0 to add in order (the hub
+// path of k_pr_iter).
+__global__ __launch_bounds__(kPrTB) void k_pr_iter_csr(
+    uint32_t n, const uint32_t *__restrict__ off, const uint32_t *__restrict__ nbr,
+    const float *__restrict__ inv, const float *__restrict__ x_in, const float *__restrict__ rank_in,
+    float *__restrict__ rank_out, float *__restrict__ x_out, float teleport, float damping,
+    const PrState *st, unsigned long long *__restrict__ partial) {
+    if (st->converged) return;
+    constexpr uint32_t W = kPrTB / 64;
+    __shared__ __align__(16) float xs[W][kHubChunk];
+    __shared__ unsigned long long acc[2 * kLimbs];
+    if (threadIdx.x < 2 * kLimbs) acc[threadIdx.x] = 0ull;
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    const uint32_t v = (blockIdx.x * W + w) * 64 + lane;
+    uint32_t s = 0, e = 0;
+    if (v < n) {
+        s = off[v];
+        e = off[v + 1];
+    }
+    const bool light = v < n && e - s <= kHeavy;
+    float r = 0.0f, r_old = 0.0f;
+    if (light) {
+        float sum = 0.0f;
+        for (uint32_t j = s; j < e; j++) sum = sum + x_in[nbr[j]];
+        r = teleport + damping * sum;
+        rank_out[v] = r;
+        x_out[v] = r * inv[v];
+        r_old = rank_in[v];
+    }
+    resid_add_wave(acc, light, r, r_old, lane);
+    unsigned long long heavy = __ballot(v < n && !light);
+    while (heavy) {
+        const uint32_t L = (uint32_t)__builtin_ctzll(heavy);
+        heavy &= heavy - 1ull;
+        const uint32_t hv = __shfl(v, (int)L, 64), hs = __shfl(s, (int)L, 64), he = __shfl(e, (int)L, 64);
+        float sum = 0.0f;
+        for (uint32_t c = hs; c < he; c += kHubChunk) {
+            const uint32_t len = min(kHubChunk, he - c);
+            constexpr uint32_t R = kHubChunk / 64;
+            uint32_t j[R];
+#pragma unroll
+            for (uint32_t q = 0; q < R; q++) {
+                const uint32_t k = lane + 64u * q;
+                j[q] = k < len ? nbr[c + k] : 0u;
+            }
+            float a[R];
+#pragma unroll
+            for (uint32_t q = 0; q < R; q++) a[q] = x_in[j[q]];
+#pragma unroll
+            for (uint32_t q = 0; q < R; q++) {
+                const uint32_t k = lane + 64u * q;
+                if (k < len) xs[w][k] = a[q];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane == 0) {
+                uint32_t k = 0;
+                for (; k + 4 <= len; k += 4) {
+                    const float4 q = *reinterpret_cast<const float4 *>(&xs[w][k]);
+                    sum = sum + q.x;
+                    sum = sum + q.y;
+                    sum = sum + q.z;
+                    sum = sum + q.w;
+                }
+                for (; k < len; k++) sum = sum + xs[w][k];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        if (lane == 0) {
+            const float rh = teleport + damping * sum;
+            rank_out[hv] = rh;
+            x_out[hv] = rh * inv[hv];
+            resid_add(acc, rh, rank_in[hv]);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * kLimbs && acc[threadIdx.x])
+        atomicAdd(&partial[threadIdx.x * kAccCols + blockIdx.x % kAccCols], acc[threadIdx.x]);
+}
+
+// Workspace of jc_pagerank_noalloc, carved from one static device array:
+// residual accumulator, iteration state, CSR check, then five node vectors
+// (1/deg, two rank and two x buffers) of kNoAllocMaxN floats each.
+constexpr uint32_t kNoAllocMaxN = JC_PAGERANK_NOALLOC_MAX_N;
+constexpr size_t kNaPartial = 0;
+constexpr size_t kNaState = kNaPartial + (size_t)kAccCols * 2 * kLimbs * sizeof(unsigned long long);
+constexpr size_t kNaCheck = kNaState + 256;
+constexpr size_t kNaVec = kNaCheck + 256;
+constexpr size_t kNaVecBytes = (size_t)kNoAllocMaxN * sizeof(float);
+constexpr size_t kNaBytes = kNaVec + 5 * kNaVecBytes;
+static_assert(sizeof(PrState) <= 256 && sizeof(CsrCheck) <= 256 && kNaVec % 256 == 0, "layout");
+
 }  // namespace
 
 class PageRankPlan {
@@ -431,6 +576,101 @@ struct jc_pagerank_plan {
     jc::PageRankPlan impl;
 };
 
+// The workspace of jc_pagerank_noalloc.  Each device holds its own copy at its
+// own address: host code uses the address hipGetSymbolAddress gives for the
+// current device (noalloc_scratch), never this name.
+static __device__ __align__(256) unsigned char scratch[jc::kNaBytes];
+
+namespace jc {
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up once per device.
+Status noalloc_scratch(unsigned char **out) {
+    int dev = 0;
+    JC_TRY(require_device(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return Status::err(JC_E_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        JC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch[dev];
+    return Status::ok();
+}
+
+Status pagerank_noalloc(uint32_t n, const uint32_t *off, const uint32_t *nbr, uint64_t nnz, float damping,
+                        uint32_t max_iters, double tol, float *out, uint32_t *iters, double *resid,
+                        hipStream_t st) {
+    if (n > kNoAllocMaxN)
+        return Status::err(JC_E_INVALID, "jc_pagerank_noalloc holds at most " + std::to_string(kNoAllocMaxN) +
+                                             " nodes in its static workspace (n = " + std::to_string(n) + ")");
+    if (nnz > 0xFFFFFFFFull) return Status::err(JC_E_INVALID, "CSR nnz exceeds the u32 offset range");
+    if (n && (!off || !out || (nnz && !nbr))) return Status::err(JC_E_INVALID, "NULL CSR or output pointer");
+    unsigned char *base = nullptr;
+    JC_TRY(noalloc_scratch(&base));
+    if (n == 0) {
+        if (iters) *iters = 0;
+        if (resid) *resid = 0.0;
+        return Status::ok();
+    }
+    auto *partial = reinterpret_cast<unsigned long long *>(base + kNaPartial);
+    auto *state = reinterpret_cast<PrState *>(base + kNaState);
+    auto *check = reinterpret_cast<CsrCheck *>(base + kNaCheck);
+    float *vec[5];
+    for (int k = 0; k < 5; k++) vec[k] = reinterpret_cast<float *>(base + kNaVec + k * kNaVecBytes);
+    float *inv = vec[0], *rank[2] = {vec[1], vec[2]}, *x[2] = {vec[3], vec[4]};
+
+    k_pr_check_clear<<<1, 64, 0, st>>>(check);
+    k_pr_check_csr<<<std::min<uint32_t>(cdiv32((uint64_t)n + 1, kPrTB), 2048), kPrTB, 0, st>>>(n, off, nbr, nnz,
+                                                                                            inv, check);
+    JC_HIP(hipGetLastError());
+    CsrCheck hc{};
+    JC_HIP(hipMemcpyAsync(&hc, check, sizeof hc, hipMemcpyDeviceToHost, st));
+    JC_HIP(hipStreamSynchronize(st));
+    if (hc.bad & 1u) return Status::err(JC_E_INVALID, "CSR offsets[0] != 0");
+    if (hc.bad & 2u) return Status::err(JC_E_INVALID, "CSR offsets[n] != nnz");
+    if (hc.bad & 4u) return Status::err(JC_E_INVALID, "CSR offsets decrease");
+    if (hc.bad & 8u) return Status::err(JC_E_INVALID, "CSR neighbour id out of range");
+    if (hc.dangling != 0xFFFFFFFFu)
+        return Status::err(JC_E_DANGLING, "gpu_pagerank requires no dangling nodes (node " +
+                                              std::to_string(hc.dangling) + " has degree 0)");
+
+    const float inv_n = 1.0f / (float)n;             // pagerank.rs:108
+    const float teleport = (1.0f - damping) * inv_n;  // pagerank.rs:109
+    k_pr_init<<<cdiv32(std::max<uint32_t>(n, 2 * kLimbs * kAccCols), kPrTB), kPrTB, 0, st>>>(
+        n, inv_n, inv, rank[0], x[0], state, tol, partial);
+    JC_HIP(hipGetLastError());
+    PrState hs{};
+    uint32_t launched = 0;
+    while (launched < max_iters) {
+        const uint32_t chunk = std::min(kChunkIters, max_iters - launched);
+        for (uint32_t k = 0; k < chunk; k++) {
+            const uint32_t c = (launched + k) & 1u;
+            k_pr_iter_csr<<<cdiv32(n, kPrTB), kPrTB, 0, st>>>(n, off, nbr, inv, x[c], rank[c], rank[c ^ 1],
+                                                             x[c ^ 1], teleport, damping, state, partial);
+            k_pr_reduce<<<1, 640, 0, st>>>(partial, state);
+        }
+        JC_HIP(hipGetLastError());
+        launched += chunk;
+        JC_HIP(hipMemcpyAsync(&hs, state, sizeof hs, hipMemcpyDeviceToHost, st));
+        JC_HIP(hipStreamSynchronize(st));
+        if (hs.converged) break;
+    }
+    JC_HIP(hipMemcpyAsync(out, rank[hs.iters & 1u], (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+    JC_HIP(hipStreamSynchronize(st));
+    if (iters) *iters = hs.iters;
+    if (resid) *resid = hs.resid;
+    return Status::ok();
+}
+
+}  // namespace
+}  // namespace jc
+
 extern "C" {
 
 int jc_pagerank_plan_create(uint32_t n, const uint32_t *offsets, const uint32_t *neighbors,
@@ -482,6 +722,15 @@ int jc_pagerank(uint32_t n, const uint32_t *offsets, const uint32_t *neighbors, 
     rc = jc_pagerank_plan_run(p, max_iters, tol_l1, out_ranks, out_iters, out_residual_l1, nullptr);
     jc_pagerank_plan_destroy(p);
     return rc;
+}
+
+int jc_pagerank_noalloc(uint32_t n, const uint32_t *d_offsets, const uint32_t *d_neighbors, uint64_t nnz,
+                        float damping, uint32_t max_iters, double tol_l1, float *d_out_ranks,
+                        uint32_t *out_iters, double *out_residual_l1, void *stream) {
+    jc::Status s = jc::pagerank_noalloc(n, d_offsets, d_neighbors, nnz, damping, max_iters, tol_l1, d_out_ranks,
+                                        out_iters, out_residual_l1, static_cast<hipStream_t>(stream));
+    if (!s.failed()) jc::set_error("");
+    return s.code;
 }
 
 }  // extern "C"

@@ -108,6 +108,8 @@ SIGNATURES = {
     "jc_pagerank_plan_run": (C.c_int, [_vp, C.c_uint32, C.c_double, _vp, _vp, _vp, _vp]),
     "jc_pagerank_plan_time": (C.c_int, [_vp, C.c_uint32, _vp]),
     "jc_pagerank_plan_destroy": (None, [_vp]),
+    "jc_pagerank_noalloc": (C.c_int, [C.c_uint32, _vp, _vp, C.c_uint64, C.c_float, C.c_uint32, C.c_double,
+                                      _vp, _vp, _vp, _vp]),
     "jc_spmv": (C.c_int, [C.c_uint32, _vp, _vp, _vp, C.c_uint64, _vp, _vp]),
     "jc_spmv_f16": (C.c_int, [C.c_uint32, _vp, _vp, _vp, C.c_uint64, _vp, _vp]),
     "jc_spmv_hybrid": (C.c_int, [C.c_uint32, _vp, _vp, _vp, C.c_uint64, _vp, _vp]),
@@ -626,6 +628,21 @@ def pagerank(offsets, neighbors, damping: float = 0.85, max_iters: int = 100, to
                              len(neighbors), damping, max_iters, tol_l1, _ptr(ranks), _ptr(it),
                              _ptr(res)))
     return ranks[:n], int(it[0]), float(res[0])
+
+
+PAGERANK_NOALLOC_MAX_N = 1 << 20
+
+
+def pagerank_noalloc(n: int, d_offsets: int, d_neighbors: int, nnz: int, d_out_ranks: int,
+                     damping: float = 0.85, max_iters: int = 100, tol_l1: float = 1e-6, stream: int = 0):
+    """jc_pagerank_noalloc: PageRank over a CSR already in device memory (device
+    addresses as integers), ranks written to d_out_ranks, no device allocation.
+    Returns (iterations, final L1 residual)."""
+    it = np.zeros(1, np.uint32)
+    res = np.zeros(1, np.float64)
+    _check(lib().jc_pagerank_noalloc(n, d_offsets or None, d_neighbors or None, nnz, damping, max_iters, tol_l1,
+                                     d_out_ranks or None, _ptr(it), _ptr(res), stream or None))
+    return int(it[0]), float(res[0])
 
 
 class PageRankPlan:
