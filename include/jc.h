@@ -364,6 +364,21 @@ uint64_t jc_csr_from_pairs(uint32_t n, const uint32_t *pairs, uint64_t npairs, u
                            uint32_t *nbrs);
 /* Positions uniform in [-half, half]^3 ((2u-1)*half per coordinate). */
 void jc_gen_uniform_cube(uint32_t n, float half, uint64_t seed, float *xyz);
+/* ball_seed(n, seed) (stability.rs:30-50): the renderer's seed, a uniform random
+ * ball of radius sqrt(n)*5 by rejection sampling (xyz: 3n). */
+void jc_gen_ball_seed(uint32_t n, uint64_t seed, float *xyz);
+
+/* ---- layout telemetry: StepStats (crates/graph-compute/src/stability.rs:128-186) ---- */
+typedef struct {
+    float max_abs;      /* max |coordinate| over finite nodes: the divergence detector */
+    float p50_radius;   /* median |pos| */
+    float p99_radius;   /* 99th-percentile |pos| */
+    float mean_disp;    /* mean per-node displacement from prev (f64 sum over finite nodes / n) */
+    float p50_disp;     /* median displacement */
+    uint64_t nonfinite; /* nodes with a NaN/Inf coordinate */
+} jc_step_stats_t;
+/* StepStats::measure(prev, cur) over n nodes (interleaved xyz); host-side, no device. */
+int jc_step_stats(uint32_t n, const float *prev, const float *cur, jc_step_stats_t *out);
 
 /* ---- binary CSR wire format (CsrGraph::from_bin_bytes / to_bin,
  * crates/graph-compute/src/sim.rs:52-128; the graph-api /graph/csr.bin payload

@@ -689,6 +689,71 @@ void jc_gen_uniform_cube(uint32_t n, float half, uint64_t seed, float *xyz) {
     }
 }
 
+// ball_seed (stability.rs:30-50): for each node, rejection-sample the unit ball
+// from three next_f32 draws (2u - 1 each, accepted when x^2 + y^2 + z^2 <= 1),
+// scaled by sqrt(n) * 5.
+void jc_gen_ball_seed(uint32_t n, uint64_t seed, float *xyz) {
+    uint64_t s = seed;
+    const float radius = std::sqrt((float)n) * 5.0f;
+    for (uint32_t i = 0; i < n; i++) {
+        for (;;) {
+            const float x = 2.0f * sm_f32(s) - 1.0f;
+            const float y = 2.0f * sm_f32(s) - 1.0f;
+            const float z = 2.0f * sm_f32(s) - 1.0f;
+            if (x * x + y * y + z * z <= 1.0f) {
+                xyz[3 * (size_t)i] = x * radius;
+                xyz[3 * (size_t)i + 1] = y * radius;
+                xyz[3 * (size_t)i + 2] = z * radius;
+                break;
+            }
+        }
+    }
+}
+
+// StepStats::measure (stability.rs:128-186).  Non-finite nodes are counted
+// and left out of every statistic; radii and displacements are sorted (NaN
+// cannot occur among the finite nodes) and picked at index
+// floor((len - 1) * q); the mean displacement divides the f64 sum of the
+// finite nodes' displacements by n.
+int jc_step_stats(uint32_t n, const float *prev, const float *cur, jc_step_stats_t *out) {
+    if (!out || (n && (!prev || !cur))) {
+        jc::set_error("jc_step_stats: NULL pointer");
+        return JC_E_INVALID;
+    }
+    float max_abs = 0.0f;
+    double sum_disp = 0.0;
+    uint64_t nonfinite = 0;
+    std::vector<float> radii, disps;
+    radii.reserve(n);
+    disps.reserve(n);
+    for (uint32_t i = 0; i < n; i++) {
+        const float x = cur[3 * (size_t)i], y = cur[3 * (size_t)i + 1], z = cur[3 * (size_t)i + 2];
+        if (!(std::isfinite(x) && std::isfinite(y) && std::isfinite(z))) {
+            nonfinite++;
+            continue;
+        }
+        max_abs = std::fmax(std::fmax(std::fmax(max_abs, std::fabs(x)), std::fabs(y)), std::fabs(z));
+        radii.push_back(std::sqrt(x * x + y * y + z * z));
+        const float dx = x - prev[3 * (size_t)i], dy = y - prev[3 * (size_t)i + 1], dz = z - prev[3 * (size_t)i + 2];
+        const double d = std::sqrt((double)(dx * dx + dy * dy + dz * dz));
+        disps.push_back((float)d);
+        sum_disp += d;
+    }
+    std::sort(radii.begin(), radii.end());
+    std::sort(disps.begin(), disps.end());
+    auto pick = [](const std::vector<float> &v, double q) -> float {
+        return v.empty() ? 0.0f : v[(size_t)((double)(v.size() - 1) * q)];
+    };
+    out->max_abs = max_abs;
+    out->p50_radius = pick(radii, 0.5);
+    out->p99_radius = pick(radii, 0.99);
+    out->mean_disp = n ? (float)(sum_disp / (double)n) : 0.0f;
+    out->p50_disp = pick(disps, 0.5);
+    out->nonfinite = nonfinite;
+    jc::set_error("");
+    return JC_OK;
+}
+
 namespace {
 inline uint32_t rd_le32(const uint8_t *b) {
     return (uint32_t)b[0] | ((uint32_t)b[1] << 8) | ((uint32_t)b[2] << 16) | ((uint32_t)b[3] << 24);

@@ -152,6 +152,8 @@ SIGNATURES = {
     "jc_gen_rgg_pairs": (C.c_uint64, [C.c_uint32, C.c_double, C.c_uint64, _vp, _vp, C.c_uint64]),
     "jc_csr_from_pairs": (C.c_uint64, [C.c_uint32, _vp, C.c_uint64, _vp, _vp]),
     "jc_gen_uniform_cube": (None, [C.c_uint32, C.c_float, C.c_uint64, _vp]),
+    "jc_gen_ball_seed": (None, [C.c_uint32, C.c_uint64, _vp]),
+    "jc_step_stats": (C.c_int, [C.c_uint32, _vp, _vp, _vp]),
     "jc_csr_bin_header": (C.c_int, [_vp, C.c_size_t, _vp, _vp]),
     "jc_csr_from_bin": (C.c_int, [_vp, C.c_size_t, _vp, _vp]),
     "jc_csr_bin_size": (C.c_uint64, [C.c_uint32, C.c_uint64]),
@@ -893,6 +895,34 @@ def uniform_cube(n: int, half: float, seed: int):
     out = np.empty(3 * max(n, 1), np.float32)
     lib().jc_gen_uniform_cube(n, half, seed, _ptr(out))
     return out[: 3 * n]
+
+
+def ball_seed(n: int, seed: int):
+    """ball_seed (stability.rs:30-50): uniform random ball of radius sqrt(n)*5."""
+    out = np.empty(3 * max(n, 1), np.float32)
+    lib().jc_gen_ball_seed(n, seed, _ptr(out))
+    return out[: 3 * n]
+
+
+class StepStats(C.Structure):
+    """StepStats (stability.rs:128-186)."""
+
+    _fields_ = [("max_abs", C.c_float), ("p50_radius", C.c_float), ("p99_radius", C.c_float),
+                ("mean_disp", C.c_float), ("p50_disp", C.c_float), ("nonfinite", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def step_stats(prev, cur) -> StepStats:
+    """StepStats::measure(prev, cur): per-step stability telemetry."""
+    prev, cur = _f32(prev), _f32(cur)
+    if len(prev) != len(cur) or len(cur) % 3:
+        raise JcError(JC_E_INVALID, f"positions lengths {len(prev)} / {len(cur)} do not match 3n")
+    st = StepStats()
+    _check(lib().jc_step_stats(len(cur) // 3, _ptr(prev) if len(prev) else None,
+                               _ptr(cur) if len(cur) else None, C.addressof(st)))
+    return st
 
 
 def csr_from_bin(data) -> tuple:

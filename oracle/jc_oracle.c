@@ -1343,6 +1343,60 @@ static inline double euclid(const float *x, uint32_t i, uint32_t j) {
     return sqrt(dx * dx + dy * dy + dz * dz);
 }
 
+/* sgd_pair_update (sgd_stress.rs:340-366): w = 1/d^2, mu = min(1, w*eta),
+ * r = 0.5*mu*(|D| - d)/|D| with |D| >= 1e-6; i moves by -r*D, j by +r*D. */
+static void sgd_pair_update(float *pos, uint32_t i, uint32_t j, float d, float eta) {
+    float xi = pos[3 * (size_t)i], yi = pos[3 * (size_t)i + 1], zi = pos[3 * (size_t)i + 2];
+    float xj = pos[3 * (size_t)j], yj = pos[3 * (size_t)j + 1], zj = pos[3 * (size_t)j + 2];
+    float dx = xi - xj, dy = yi - yj, dz = zi - zj;
+    float mag = fmaxf(sqrtf(dx * dx + dy * dy + dz * dz), 1e-6f);
+    float w = 1.0f / (d * d);
+    float mu = fminf(w * eta, 1.0f);
+    float r = 0.5f * mu * (mag - d) / mag;
+    float rx = r * dx, ry = r * dy, rz = r * dz;
+    pos[3 * (size_t)i] = xi - rx; pos[3 * (size_t)i + 1] = yi - ry; pos[3 * (size_t)i + 2] = zi - rz;
+    pos[3 * (size_t)j] = xj + rx; pos[3 * (size_t)j + 1] = yj + ry; pos[3 * (size_t)j + 2] = zj + rz;
+}
+
+/* The reference CPU engine SgdStressEngine (sgd_stress.rs:251-316).  init:
+ * SplitMix64(seed) picks the pivots (select_pivots draws once, :152), and the
+ * same stream then samples the pairs.  step: sweeps_per_step (at least 1)
+ * sweeps, each with eta = anneal_eta(sweep) and n samples i = below(n),
+ * t = below(k), j = pivots[t] (:299-301); i == j and unreachable or zero
+ * distances are skipped (:302-310); sgd_pair_update moves both endpoints in
+ * place.  Runs `steps` steps on xyz (3n) in place.  Returns k, or 0xFFFFFFFF
+ * when a hop distance exceeds the u16 table. */
+uint32_t jco_sgd_cpu_run(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, float *xyz,
+                         uint32_t n_pivots, uint32_t sweeps_per_step, float eta_max, float eta_min,
+                         uint32_t n_anneal, uint64_t seed, uint32_t steps) {
+    if (n == 0) return 0;
+    uint32_t k = n_pivots < 1 ? 1 : (n_pivots > n ? n : n_pivots);
+    uint32_t *piv = (uint32_t *)malloc((size_t)k * sizeof(uint32_t));
+    uint16_t *dist = (uint16_t *)malloc((size_t)n * k * sizeof(uint16_t));
+    uint32_t got = jco_select_pivots(n, offsets, nbrs, n_pivots, seed, piv, dist);
+    if (got == 0xFFFFFFFFu) { free(piv); free(dist); return got; }
+    uint64_t s = seed;
+    (void)jco_next_u64(&s);  /* the draw select_pivots made */
+    uint64_t sweep = 0;
+    uint32_t per = sweeps_per_step ? sweeps_per_step : 1;
+    for (uint32_t st = 0; st < steps; st++) {
+        for (uint32_t q = 0; q < per; q++) {
+            float eta = jco_anneal_eta(sweep, eta_max, eta_min, n_anneal);
+            for (uint32_t c = 0; c < n; c++) {
+                uint32_t i = (uint32_t)(jco_next_u64(&s) % n);
+                uint32_t t = (uint32_t)(jco_next_u64(&s) % k);
+                uint32_t j = piv[t];
+                if (i == j) continue;
+                uint16_t d = dist[(size_t)i * k + t];
+                if (d == 0) continue;
+                sgd_pair_update(xyz, i, j, (float)d, eta);
+            }
+            sweep++;
+        }
+    }
+    free(piv); free(dist);
+    return k;
+}
 double jco_stress_normalized(const float *xyz, const uint32_t *ti, const uint32_t *tj,
                              const float *td, uint64_t n_terms) {
     double num = 0, den = 0, wd2 = 0;

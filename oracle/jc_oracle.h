@@ -129,6 +129,9 @@ void jco_sgd_sweep(uint32_t n, uint32_t k, const uint32_t *pivots, const uint16_
                    const float *pos_in, float *pos_out, float eta, int n_threads);
 
 /* ---- metrics (crates/graph-layouts/src/metrics.rs:17-164) ---- */
+uint32_t jco_sgd_cpu_run(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, float *xyz,
+                         uint32_t n_pivots, uint32_t sweeps_per_step, float eta_max, float eta_min,
+                         uint32_t n_anneal, uint64_t seed, uint32_t steps);
 double jco_stress_normalized(const float *xyz, const uint32_t *ti, const uint32_t *tj,
                              const float *td, uint64_t n_terms);
 double jco_edge_length_cv(const float *xyz, const uint32_t *ei, const uint32_t *ej,

@@ -114,6 +114,8 @@ def lib():
                                            u16p]),
         "jco_anneal_eta": (C.c_float, [C.c_uint64, C.c_float, C.c_float, C.c_uint32]),
         "jco_sgd_sweep": (None, [C.c_uint32, C.c_uint32, u32p, u16p, f32p, f32p, C.c_float, C.c_int]),
+        "jco_sgd_cpu_run": (C.c_uint32, [C.c_uint32, u32p, u32p, f32p, C.c_uint32, C.c_uint32, C.c_float,
+                                         C.c_float, C.c_uint32, C.c_uint64, C.c_uint32]),
         "jco_stress_normalized": (C.c_double, [f32p, u32p, u32p, f32p, C.c_uint64]),
         "jco_edge_length_cv": (C.c_double, [f32p, u32p, u32p, C.c_uint64]),
     }
@@ -446,6 +448,36 @@ def sgd_sweep(pivots, dist, pos, eta, threads=8):
     lib().jco_sgd_sweep(n, len(pivots), _u32(pivots), np.ascontiguousarray(dist, np.uint16), pos, out,
                         eta, threads)
     return out
+
+
+def sgd_cpu_run(offsets, nbrs, xyz, steps, n_pivots=50, sweeps_per_step=1, eta_max=1.0, eta_min=0.01,
+                n_anneal=30, seed=0x5EEDC0DE12345678):
+    """The reference CPU s_gd2 engine (SgdStressEngine, sgd_stress.rs:251-316):
+    `steps` steps from xyz.  Returns the positions (3n)."""
+    pos = _f32(xyz).copy()
+    n = len(offsets) - 1
+    k = lib().jco_sgd_cpu_run(n, _u32(offsets), _u32(nbrs) if len(nbrs) else np.zeros(1, np.uint32), pos,
+                              n_pivots, sweeps_per_step, eta_max, eta_min, n_anneal, seed, steps)
+    if k == 0xFFFFFFFF:
+        raise ValueError("hop distance exceeds u16")
+    return pos
+
+
+def all_pairs_terms(offsets, nbrs):
+    """(i, j, d) for every reachable pair i < j with hop distance d > 0: the
+    term set of solved_layouts.rs scale_normalized_stress (all_pairs BFS)."""
+    n = len(offsets) - 1
+    ti, tj, td = [], [], []
+    for s in range(n):
+        d = bfs(offsets, nbrs, s)
+        j = np.arange(s + 1, n)
+        dj = d[s + 1:]
+        ok = (dj != 0xFFFFFFFF) & (dj != 0)
+        ti.append(np.full(ok.sum(), s, np.uint32))
+        tj.append(j[ok].astype(np.uint32))
+        td.append(dj[ok].astype(np.float32))
+    cat = lambda a, t: np.concatenate(a).astype(t) if a else np.zeros(0, t)
+    return cat(ti, np.uint32), cat(tj, np.uint32), cat(td, np.float32)
 
 
 # ---------------------------------------------------------------- metrics

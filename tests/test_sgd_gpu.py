@@ -125,3 +125,23 @@ def test_set_state_waits_for_queued_sweeps():
     back = e.get_state()
     assert np.array_equal(back["positions"].view(np.uint32), mid.view(np.uint32))
     assert np.array_equal(e.step(4).view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("name", ["path24", "cycle16", "grid4x4", "complete6"])
+def test_gpu_not_worse_than_cpu_on_normalized_stress(name):
+    """solved_layouts.rs:308-354: from the same seed (spread 1), 200 steps of the
+    device engine against 200 steps of the reference CPU s_gd2 engine (its
+    restatement in the oracle), both with default settings: the all-pairs
+    normalised stress of the device layout is at most 1.5 x the CPU's + 0.05."""
+    import solved_layouts as S
+
+    off, nb = S.CASES[name]()
+    n = len(off) - 1
+    init = S.seed(n, 1.0)
+    ns_cpu = S.normalized_stress(O.sgd_cpu_run(off, nb, init, 200), off, nb)
+    e = jc.SgdStressEngine()
+    e.init(off, nb, init)
+    ns_gpu = S.normalized_stress(e.step(200), off, nb)
+    print(f"{name}: cpu {ns_cpu:.5f} gpu {ns_gpu:.5f}")
+    assert np.isfinite(ns_cpu) and np.isfinite(ns_gpu)
+    assert ns_gpu <= ns_cpu * 1.5 + 0.05
