@@ -4,7 +4,7 @@ workload (m=4, positions uniform in [-10,10]^3, theta 0.8).
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--no-cpu-baseline]
                     [--no-warm-start] [--copy-probe]
-                    [--workload ba_1m|batch_4096|pagerank_ba1m|sgd_rgg200k|traverse]
+                    [--workload ba_1m|batch_4096|pagerank_ba1m|spmv_ba1m|sgd_rgg200k|traverse|mpnn_ba1m]
 
 A step is one full FA2+BH iteration (octree build, walk, attraction, gravity,
 adaptive speed, apply) with inputs already resident in HBM.  The timed region
@@ -27,7 +27,8 @@ The other workloads print one JSON line each for the sibling paths:
   pagerank_ba1m  PageRank on the 1M-node BA CSR, d=0.85, L1 tol 1e-6;
   sgd_rgg200k    SGD sparse stress on the 200k-node 2-D RGG, 200 pivots, 30 epochs;
   traverse       single-source BFS on the 200k-node RGG and the 1M-node BA graph,
-                 and their connected components.
+                 and their connected components;
+  mpnn_ba1m      the warm-start network: inference at 1M nodes, a training step at 100k.
 """
 from __future__ import annotations
 
@@ -46,6 +47,36 @@ METRIC = "FA2+BH iters/s at 1M nodes; time-to-stress≤1.01× plain: learned vs 
 HBM_PEAK_GBS = 8000.0
 VALU_FP32_PEAK_TFLOPS = 157.3
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "fa2_force_ba1m_pmc.json")
+
+
+def pmc_traffic(name: str):
+    """Calibrated HBM bytes per launch of a workload's dominant kernel from its
+    committed counter summary (profiles/<name>_pmc.json), or None."""
+    path = os.path.join(ROOT, "profiles", f"{name}_pmc.json")
+    try:
+        with open(path) as fh:
+            return int(json.load(fh)["hbm_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def _threads() -> int:
+    return min(16, os.cpu_count() or 1)
+
+
+def _timed_rate(fn, budget_s: float):
+    """Calls fn(k) -> units done, growing k until about budget_s of work;
+    returns (units/s, units)."""
+    t0 = time.perf_counter()
+    done = fn(1)
+    first = time.perf_counter() - t0
+    more = max(0, int(budget_s / max(first, 1e-4)) - 1)
+    total = first
+    if more:
+        t1 = time.perf_counter()
+        done += fn(more)
+        total += time.perf_counter() - t1
+    return done / total, done
 
 
 def log(msg: str) -> None:
@@ -198,9 +229,26 @@ def bench_pagerank(args) -> int:
     plan.run(100, 1e-6, readback=False)
     _, iters, resid, run_ms = plan.run(100, 1e-6, readback=False)
     ms_it = plan.time_iterations(100)
-    algo = 4 * (n + 1) + 4 * nnz + 24 * n
+    # column ids and the gathered x = rank/deg per entry, offsets, the rank
+    # write, and the x write plus its compulsory read: 8 nnz + 16 n
+    algo = 8 * nnz + 16 * n
     achieved = algo / (ms_it / 1e3) / 1e9
     log(f"pagerank: {iters} iterations to L1 {resid:.3e} in {run_ms:.3f} ms; {ms_it:.4f} ms/iteration")
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        def it_rate(threads):
+            return _timed_rate(lambda k: O.pagerank_pull(off, nb, 0.85, k, -1.0, threads=threads)[1],
+                               args.cpu_seconds / 2)
+        rate, done = it_rate(_threads())
+        rate1, done1 = it_rate(1)
+        cpu = {"value": rate, "unit": "iterations/s", "cores": _threads(), "kind": "port",
+               "single_thread_value": rate1, "cpu_model": _cpu_model(), "nproc": os.cpu_count(),
+               "sample": f"{done} pull iterations ({_threads()} threads) and {done1} (1 thread) of the "
+                         f"oracle's restatement of cpu/gpu_pagerank's update (pagerank.rs:298-331, "
+                         f"pagerank.wgsl:30-46) on the same CSR, exact f64 residual, stopping rule off"}
+        log(f"pagerank cpu baseline {rate:.1f} it/s on {_threads()} threads, {rate1:.1f} on 1")
     return _line("PageRank iterations/s at 1M nodes (BA m=4)", 1e3 / ms_it, "iterations/s", 100, 100,
                  ms_it, "f32",
                  {"workload": "pagerank_ba1m", "n_nodes": n, "csr_nnz": nnz, "damping": 0.85,
@@ -208,7 +256,9 @@ def bench_pagerank(args) -> int:
                  iterations_to_tol=iters, residual_l1=resid, ms_to_tol=round(run_ms, 4),
                  roofline={"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                           "traffic": None, "bytes_per_iteration": algo})
+                           "traffic": pmc_traffic("pagerank_ba1m"), "bytes_per_iteration": algo,
+                           "kernel": "k_pr_iter + k_pr_reduce"},
+                 cpu_baseline=cpu)
 
 
 def bench_spmv(args) -> int:
@@ -262,6 +312,27 @@ def bench_sgd(args) -> int:
     algo = 2 * n * k + 32 * n + 16 * k
     achieved = algo / (ms / 30 / 1e3) / 1e9
     log(f"sgd: 30 epochs in {ms:.3f} ms (init incl. BFS {init_s:.2f} s)")
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        piv, dist, eta_max = e.pivots()
+
+        def ep_rate(threads):
+            def run(kk):
+                p = xyz
+                for t in range(kk):
+                    p = O.sgd_sweep(piv, dist, p, O.anneal_eta(t, eta_max, 0.01, 30), threads=threads)
+                return kk
+            return _timed_rate(run, args.cpu_seconds / 2)
+        rate, done = ep_rate(_threads())
+        rate1, done1 = ep_rate(1)
+        cpu = {"value": rate, "unit": "epochs/s", "cores": _threads(), "kind": "port",
+               "single_thread_value": rate1, "cpu_model": _cpu_model(), "nproc": os.cpu_count(),
+               "sample": f"{done} epochs ({_threads()} threads) and {done1} (1 thread) of the oracle's "
+                         f"Jacobi sweep (sgd_stress.wgsl:34-62) over the same 200 pivot rows, from the "
+                         f"same start"}
+        log(f"sgd cpu baseline {rate:.2f} epochs/s on {_threads()} threads, {rate1:.2f} on 1")
     return _line("SGD sparse-stress epochs/s at 200k nodes (RGG, 200 pivots)", 30 / (ms / 1e3),
                  "epochs/s", 30, warm, ms / 30, "f32",
                  {"workload": "sgd_rgg200k", "n_nodes": n, "csr_nnz": len(nb), "pivots": k,
@@ -269,7 +340,67 @@ def bench_sgd(args) -> int:
                  init_seconds_incl_bfs=round(init_s, 3),
                  roofline={"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                           "traffic": None, "bytes_per_epoch": algo})
+                           "traffic": pmc_traffic("sgd_rgg200k"), "bytes_per_epoch": algo,
+                           "kernel": "k_sgd_sweep"},
+                 cpu_baseline=cpu)
+
+
+def bench_mpnn(args) -> int:
+    """The warm-start network on the 1M-node BA graph: one inference pass
+    (forward + on-device centring and scaling, as the warm start runs it) with
+    the committed weights, and one training step (forward, loss, backward, Adam)
+    on a 100k-node BA graph against its plain FA2 layout.  Device times from HIP
+    events for inference (host clock for the training step); the roofline is
+    the pass's algorithmic bytes: the CSR and one gathered bf16 feature row per
+    entry in each layer's aggregation, the GEMMs' activations in and out, the
+    head and the centring."""
+    import jumpcannon as jc
+    from jumpcannon import warmstart as W, workloads
+
+    off, nb, xyz = workloads.ba_1m(args.nodes)
+    n, nnz = len(off) - 1, len(nb)
+    net, coef = W.load()
+    t = time.perf_counter()
+    net.set_graph(off, nb, feat_seed=7, walks=16, noise=np.asarray(xyz) / W.INIT_HALF)
+    feat_s = time.perf_counter() - t
+    net.forward_scaled(W.size_of(coef, n, nnz))  # first launch loads the kernels
+    fwd = [net.forward_scaled(W.size_of(coef, n, nnz))[1] for _ in range(5)]
+    fwd_ms = float(np.median(fwd))
+    log(f"mpnn: features {feat_s:.2f} s, inference {fwd_ms:.3f} ms at {n} nodes")
+    # one training step on 100k nodes
+    toff, tnb = jc.ba_graph(100_000, 4, 0xC0FFEE + 7)
+    tn = len(toff) - 1
+    tx0 = jc.uniform_cube(tn, W.INIT_HALF, 11)
+    tgt = jc.Fa2BhEngine(theta=1.0)
+    tgt.init(toff, tnb, tx0)
+    target = tgt.step(300)
+    tnet = jc.Mpnn(seed=3)
+    tnet.set_graph(toff, tnb, feat_seed=7, walks=16, noise=tx0 / W.INIT_HALF)
+    tnet.set_target(target, n_anchor=32, seed=5)
+    tnet.train(2, 1e-3)
+    t = time.perf_counter()
+    steps = 20
+    tnet.train(steps, 1e-3)
+    train_ms = (time.perf_counter() - t) * 1e3 / steps
+    log(f"mpnn: training step {train_ms:.3f} ms at {tn} nodes (host clock, includes the loss read-back)")
+    # per layer: the aggregation reads the CSR and gathers one bf16 feature row
+    # per entry (16 features in layer 0, 128 after), writes one mean row per
+    # node; the GEMM reads [h | mean] and writes h'; the head reads h and writes
+    # 3 f32 per node; centring reads and writes the 3 coordinates
+    csr = 4 * (n + 1) + 4 * nnz
+    agg_bytes = (csr + 32 * nnz + 32 * n) + 2 * (csr + 256 * nnz + 256 * n)
+    gemm_bytes = (64 * n + 256 * n) + 2 * (512 * n + 256 * n)
+    pass_bytes = agg_bytes + gemm_bytes + 256 * n + 12 * n + 3 * 12 * n
+    achieved = pass_bytes / (fwd_ms / 1e3) / 1e9
+    return _line("MPNN warm-start inference passes/s at 1M nodes (BA m=4)", 1e3 / fwd_ms, "passes/s", 5, 1,
+                 fwd_ms, "bf16",
+                 {"workload": "mpnn_ba1m", "n_nodes": n, "csr_nnz": nnz, "layers": 3, "hidden": 128,
+                  "train_workload": "ba_100k_m4", "parallelism": "single-gpu"},
+                 features_seconds=round(feat_s, 3), train_step_ms=round(train_ms, 3),
+                 roofline={"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                           "bytes_per_pass": pass_bytes, "aggregation_bytes": agg_bytes, "gemm_bytes": gemm_bytes,
+                           "note": "algorithmic bytes of the whole inference pass over its device time"})
 
 
 def bench_traverse(args) -> int:
@@ -330,7 +461,7 @@ def main() -> int:
                     help="batch_4096: skip the one-engine-per-graph estimate")
     ap.add_argument("--workload", default="ba_1m",
                     choices=["ba_1m", "batch_4096", "pagerank_ba1m", "spmv_ba1m", "sgd_rgg200k",
-                             "traverse"])
+                             "traverse", "mpnn_ba1m"])
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe()
@@ -340,7 +471,8 @@ def main() -> int:
         return 2
     if args.workload != "ba_1m":
         return {"batch_4096": bench_batch, "pagerank_ba1m": bench_pagerank, "spmv_ba1m": bench_spmv,
-                "sgd_rgg200k": bench_sgd, "traverse": bench_traverse}[args.workload](args)
+                "sgd_rgg200k": bench_sgd, "traverse": bench_traverse,
+                "mpnn_ba1m": bench_mpnn}[args.workload](args)
 
     import jumpcannon as jc
     from jumpcannon import workloads

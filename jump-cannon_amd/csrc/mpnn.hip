@@ -12,7 +12,7 @@
 //   layer l h' = ReLU([h | mean_{j in N(i)} h_j] W_l + b_l), 3 layers, width 128
 //   head    y_i = h_i W_h + b_h  (3 coordinates)
 //
-// Dense products run on MFMA (v_mfma_f32_16x16x16bf16_1k): activations and
+// Dense products run on MFMA (v_mfma_f32_16x16x32_bf16): activations and
 // weights in BF16, accumulation in FP32.  The loss is invariant to rotation,
 // translation and scale: over pairs (graph edges + pairs to a few anchor
 // nodes) with target distances D from a layout the plain engines produced,
@@ -38,9 +38,12 @@ constexpr int kIn = 16;
 constexpr int kRw = 8;
 constexpr int kLayers = 3;
 constexpr int kOut = 3;
-constexpr uint32_t kXtgRows = 64;  // nodes per weight-gradient block
+constexpr uint32_t kXtgRows = 64;    // fewest nodes per weight-gradient block
+constexpr uint32_t kXtgParts = 512;  // most weight-gradient blocks (partials summed in block order)
+// Nodes per weight-gradient block: a fixed function of n, so the partial sums
+// and their order are the same on every run.
+inline uint32_t xtg_rows(uint32_t n) { return std::max(kXtgRows, (n + kXtgParts - 1) / kXtgParts); }
 
-typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 __device__ __host__ inline uint16_t f2bf(float f) {
@@ -101,8 +104,40 @@ __global__ __launch_bounds__(256) void k_features(uint32_t n, const uint32_t *__
 }
 
 // ---------------------------------------------------------------- aggregation
-// A_i = mean_{j in N(i)} H_j over F bf16 features (F/2 threads per node, each
-// owning two features); f32 sums in CSR order.
+// A_i = mean_{j in N(i)} H_j over F bf16 features.  A node's TPN = F/2 lanes
+// each own two features.  Rows of at most kAggLight entries are summed by
+// their node's lanes in CSR order, agg_round() neighbours per round: the group's
+// first lanes load the round's neighbour ids, shuffle them to the group, and
+// every lane then has the round's row loads (agg_round) in flight together.  Longer rows
+// are left to k_agg_mean_hub.  f32 sums, one bf16 rounding at the end.
+constexpr uint32_t kAggLight = 64;   // longer rows: one block each (k_agg_mean_hub)
+// neighbour rows in flight per lane: 16, or the group size when smaller
+template <int TPN>
+constexpr int agg_round() { return TPN < 16 ? TPN : 16; }
+constexpr int kHubThreads = 1024;
+
+template <int TPN>
+__device__ __forceinline__ void agg_rows(const uint32_t *__restrict__ nbr, const uint16_t *__restrict__ H,
+                                         int F, uint32_t c, uint32_t j, uint32_t s, uint32_t e, float &a0,
+                                         float &a1) {
+    constexpr int R = agg_round<TPN>();
+    for (uint32_t base = s; base < e; base += R) {
+        const uint32_t id = (j < (uint32_t)R && base + j < e) ? nbr[base + j] : 0u;
+        uint32_t v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const uint32_t row = (uint32_t)__shfl((int)id, r, TPN);
+            v[r] = base + r < e ? *reinterpret_cast<const uint32_t *>(H + (size_t)row * F + c) : 0u;
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            if (base + r < e) {
+                a0 += bf2f((uint16_t)(v[r] & 0xFFFFu));
+                a1 += bf2f((uint16_t)(v[r] >> 16));
+            }
+    }
+}
+
 template <int F>
 __global__ __launch_bounds__(256) void k_agg_mean(uint32_t n, const uint32_t *__restrict__ off,
                                                   const uint32_t *__restrict__ nbr,
@@ -110,106 +145,281 @@ __global__ __launch_bounds__(256) void k_agg_mean(uint32_t n, const uint32_t *__
                                                   uint16_t *__restrict__ A) {
     constexpr int TPN = F / 2;
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t i = g / TPN, c = (g % TPN) * 2;
-    if (i >= n) return;
-    const uint32_t s = off[i], e = off[i + 1];
-    float a0 = 0.0f, a1 = 0.0f;
-    for (uint32_t k = s; k < e; k++) {
-        const uint32_t pr = *reinterpret_cast<const uint32_t *>(H + (size_t)nbr[k] * F + c);
-        a0 += bf2f((uint16_t)(pr & 0xFFFFu));
-        a1 += bf2f((uint16_t)(pr >> 16));
+    const uint32_t i = g / TPN, j = g % TPN, c = j * 2;
+    uint32_t s = 0, e = 0;
+    if (i < n) {
+        s = off[i];
+        e = off[i + 1];
     }
+    if (i >= n || e - s > kAggLight) return;  // whole groups leave together
+    float a0 = 0.0f, a1 = 0.0f;
+    agg_rows<TPN>(nbr, H, F, c, j, s, e, a0, a1);
     const float inv = e > s ? 1.0f / (float)(e - s) : 0.0f;
     const uint32_t o = (uint32_t)f2bf(a0 * inv) | ((uint32_t)f2bf(a1 * inv) << 16);
     *reinterpret_cast<uint32_t *>(A + (size_t)i * F + c) = o;
 }
 
-// Transpose of the mean (symmetric CSR): dH_j += sum_{i in N(j)} dA_i / deg_i.
+// One row longer than kAggLight per block: the block's G = kHubThreads/TPN
+// lane groups take consecutive equal chunks of the row (multiples of
+// agg_round() entries), and group 0 adds the G partial sums in group order, so
+// the result is the same on every run.
+template <int F>
+__global__ __launch_bounds__(kHubThreads) void k_agg_mean_hub(const uint32_t *__restrict__ hubs,
+                                                              const uint32_t *__restrict__ off,
+                                                              const uint32_t *__restrict__ nbr,
+                                                              const uint16_t *__restrict__ H,
+                                                              uint16_t *__restrict__ A) {
+    constexpr int TPN = F / 2, G = kHubThreads / TPN;
+    __shared__ float2 part[G][TPN];
+    const uint32_t v = hubs[blockIdx.x];
+    const uint32_t s = off[v], e = off[v + 1];
+    const uint32_t q = threadIdx.x / TPN, j = threadIdx.x % TPN, c = j * 2;
+    constexpr uint32_t R = agg_round<TPN>();
+    const uint32_t per = (((e - s + G - 1) / G) + R - 1) / R * R;
+    const uint32_t cs = min(s + q * per, e), ce = min(cs + per, e);
+    float a0 = 0.0f, a1 = 0.0f;
+    agg_rows<TPN>(nbr, H, F, c, j, cs, ce, a0, a1);
+    part[q][j] = make_float2(a0, a1);
+    __syncthreads();
+    if (q == 0) {
+        float b0 = 0.0f, b1 = 0.0f;
+        for (int k = 0; k < G; k++) {
+            b0 += part[k][j].x;
+            b1 += part[k][j].y;
+        }
+        const float inv = 1.0f / (float)(e - s);
+        const uint32_t o = (uint32_t)f2bf(b0 * inv) | ((uint32_t)f2bf(b1 * inv) << 16);
+        *reinterpret_cast<uint32_t *>(A + (size_t)v * F + c) = o;
+    }
+}
+
+// Transpose of the mean (symmetric CSR): dH_j = dX_j[:F] + sum_{i in N(j)} dX_i[F:] / deg_i,
+// with inv[i] = 1/deg_i; the same lane groups, rounds and hub split as the
+// forward aggregation, two f32 features per lane.
+template <int TPN>
+__device__ __forceinline__ void agg_t_rows(const uint32_t *__restrict__ nbr, const float *__restrict__ inv,
+                                           const float *__restrict__ dX, int ldx, int F, uint32_t c,
+                                           uint32_t j, uint32_t s, uint32_t e, float &a0, float &a1) {
+    constexpr int R = agg_round<TPN>();
+    for (uint32_t base = s; base < e; base += R) {
+        const uint32_t id = (j < (uint32_t)R && base + j < e) ? nbr[base + j] : 0u;
+        float2 v[R];
+        float w[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const uint32_t row = (uint32_t)__shfl((int)id, r, TPN);
+            const bool ok = base + r < e;
+            v[r] = ok ? *reinterpret_cast<const float2 *>(dX + (size_t)row * ldx + F + c) : make_float2(0.0f, 0.0f);
+            w[r] = ok ? inv[row] : 0.0f;
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            if (base + r < e) {
+                a0 += v[r].x * w[r];
+                a1 += v[r].y * w[r];
+            }
+    }
+}
+
 template <int F>
 __global__ __launch_bounds__(256) void k_agg_mean_t(uint32_t n, const uint32_t *__restrict__ off,
                                                     const uint32_t *__restrict__ nbr,
+                                                    const float *__restrict__ inv,
                                                     const float *__restrict__ dX, int ldx,
                                                     float *__restrict__ dH) {
+    constexpr int TPN = F / 2;
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t j = g / F, c = g % F;
-    if (j >= n) return;
-    float acc = 0.0f;
-    for (uint32_t k = off[j]; k < off[j + 1]; k++) {
-        const uint32_t i = nbr[k];
-        const float inv = 1.0f / (float)(off[i + 1] - off[i]);
-        acc += dX[(size_t)i * ldx + F + c] * inv;
+    const uint32_t i = g / TPN, j = g % TPN, c = j * 2;
+    uint32_t s = 0, e = 0;
+    if (i < n) {
+        s = off[i];
+        e = off[i + 1];
     }
-    dH[(size_t)j * F + c] = dX[(size_t)j * ldx + c] + acc;
+    if (i >= n || e - s > kAggLight) return;
+    float a0 = 0.0f, a1 = 0.0f;
+    agg_t_rows<TPN>(nbr, inv, dX, ldx, F, c, j, s, e, a0, a1);
+    const float2 self = *reinterpret_cast<const float2 *>(dX + (size_t)i * ldx + c);
+    *reinterpret_cast<float2 *>(dH + (size_t)i * F + c) = make_float2(self.x + a0, self.y + a1);
+}
+
+template <int F>
+__global__ __launch_bounds__(kHubThreads) void k_agg_mean_t_hub(const uint32_t *__restrict__ hubs,
+                                                                const uint32_t *__restrict__ off,
+                                                                const uint32_t *__restrict__ nbr,
+                                                                const float *__restrict__ inv,
+                                                                const float *__restrict__ dX, int ldx,
+                                                                float *__restrict__ dH) {
+    constexpr int TPN = F / 2, G = kHubThreads / TPN;
+    __shared__ float2 part[G][TPN];
+    const uint32_t v = hubs[blockIdx.x];
+    const uint32_t s = off[v], e = off[v + 1];
+    const uint32_t q = threadIdx.x / TPN, j = threadIdx.x % TPN, c = j * 2;
+    constexpr uint32_t R = agg_round<TPN>();
+    const uint32_t per = (((e - s + G - 1) / G) + R - 1) / R * R;
+    const uint32_t cs = min(s + q * per, e), ce = min(cs + per, e);
+    float a0 = 0.0f, a1 = 0.0f;
+    agg_t_rows<TPN>(nbr, inv, dX, ldx, F, c, j, cs, ce, a0, a1);
+    part[q][j] = make_float2(a0, a1);
+    __syncthreads();
+    if (q == 0) {
+        float b0 = 0.0f, b1 = 0.0f;
+        for (int k = 0; k < G; k++) {
+            b0 += part[k][j].x;
+            b1 += part[k][j].y;
+        }
+        const float2 self = *reinterpret_cast<const float2 *>(dX + (size_t)v * ldx + c);
+        *reinterpret_cast<float2 *>(dH + (size_t)v * F + c) = make_float2(self.x + b0, self.y + b1);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_inv_deg(uint32_t n, const uint32_t *__restrict__ off,
+                                                 float *__restrict__ inv) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t d = off[i + 1] - off[i];
+    inv[i] = d ? 1.0f / (float)d : 0.0f;
 }
 
 // ---------------------------------------------------------------- MFMA GEMM
 // C[M x N] = [A1 | A2][M x K] * Bt^T with Bt = [N x K] (both K-contiguous bf16),
-// K = K1 + K2, multiples of 16; N = 16*NT.  One wave owns 16 rows and NT
-// 16x16 accumulator tiles.  Fragment layout of v_mfma_f32_16x16x16bf16_1k:
-// lane l holds A[l%16][4(l/16)..+3], B[4(l/16)..+3][l%16], and D rows
-// 4(l/16)..+3 of column l%16.
-// MODE 0: out_bf16 = ReLU(C + bias) (relu flag) in bf16; MODE 1: out_f32 = C.
-template <int NT, int MODE>
-__global__ __launch_bounds__(256) void k_gemm(uint32_t M, const uint16_t *__restrict__ A1, int K1,
-                                              const uint16_t *__restrict__ A2, int K2,
-                                              const uint16_t *__restrict__ Bt,
-                                              const float *__restrict__ bias, int relu,
-                                              uint16_t *__restrict__ out_bf, float *__restrict__ out_f) {
+// K = K1 + K2 <= KMAX, K1 and K2 multiples of 32, N = 16*NT, on
+// v_mfma_f32_16x16x32_bf16: lane l holds A[l%16][8(l/16)..+7],
+// B[8(l/16)..+7][l%16] and D rows 4(l/16)..+3 of column l%16.
+// A persistent block keeps Bt in LDS and walks row tiles of kGemmRows rows
+// (16 per wave); each tile's rows are contiguous in A1 and in A2, so they come
+// in as coalesced 16-byte loads, and the next tile's loads are issued into
+// registers before the current tile's products.
+// MODE 0: out_bf16 = ReLU(C + bias) (relu flag), staged through the wave's
+// own LDS rows and written as whole rows; MODE 1: out_f32 = C.
+constexpr int kGemmThreads = 512;
+constexpr uint32_t kGemmRows = 16 * (kGemmThreads / 64);
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+template <int NT, int MODE, int KMAX>
+__global__ __launch_bounds__(kGemmThreads) void k_gemm(uint32_t M, const uint16_t *__restrict__ A1, int K1,
+                                                       const uint16_t *__restrict__ A2, int K2,
+                                                       const uint16_t *__restrict__ Bt,
+                                                       const float *__restrict__ bias, int relu,
+                                                       uint16_t *__restrict__ out_bf,
+                                                       float *__restrict__ out_f) {
     constexpr int N = 16 * NT;
-    const int K = K1 + K2;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t row0 = wave * 16;
-    if (row0 >= M) return;
-    const uint32_t r = lane & 15, kq = (lane >> 4) * 4;
-    const uint32_t row = row0 + r;
-    v4f acc[NT];
-#pragma unroll
-    for (int t = 0; t < NT; t++) acc[t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        v4s a = v4s{0, 0, 0, 0};
-        if (row < M) {
-            const uint16_t *src = k0 < K1 ? A1 + (size_t)row * K1 + k0 + kq
-                                          : A2 + (size_t)row * K2 + (k0 - K1) + kq;
-            a = *reinterpret_cast<const v4s *>(src);
-        }
-#pragma unroll
-        for (int t = 0; t < NT; t++) {
-            const v4s b = *reinterpret_cast<const v4s *>(Bt + (size_t)(t * 16 + r) * K + k0 + kq);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
-        }
+    constexpr int LDK = KMAX + 8;  // padded row: 16 lanes' 16-byte reads hit distinct banks
+    constexpr int PER = (int)kGemmRows * (KMAX / 8) / kGemmThreads;  // 16-byte chunks per thread
+    static_assert(MODE == 1 || LDK >= N, "bf16 output rows are staged in the wave's A rows");
+    __shared__ __align__(16) uint16_t Bs[N * LDK];
+    __shared__ __align__(16) uint16_t As[kGemmRows * LDK];
+    const int K = K1 + K2, c1 = K1 / 8, cr = K / 8;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int q = (int)tid; q < N * cr; q += kGemmThreads) {
+        const int row = q / cr, kc = (q % cr) * 8;
+        *reinterpret_cast<uint4 *>(&Bs[row * LDK + kc]) =
+            *reinterpret_cast<const uint4 *>(Bt + (size_t)row * K + kc);
     }
+    const uint32_t tiles = (M + kGemmRows - 1) / kGemmRows;
+    uint4 pre[PER];
+    auto fetch = [&](uint32_t t) {
+        const uint32_t r0 = t * kGemmRows;
 #pragma unroll
-    for (int t = 0; t < NT; t++) {
-        const uint32_t col = t * 16 + r;
+        for (int p = 0; p < PER; p++) {
+            const int q = (int)tid + p * kGemmThreads;
+            const int row = q / cr, ch = q % cr;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (t < tiles && q < (int)kGemmRows * cr && r0 + row < M)
+                v = ch < c1 ? *reinterpret_cast<const uint4 *>(A1 + (size_t)(r0 + row) * K1 + ch * 8)
+                            : *reinterpret_cast<const uint4 *>(A2 + (size_t)(r0 + row) * K2 + (ch - c1) * 8);
+            pre[p] = v;
+        }
+    };
+    fetch(blockIdx.x);
+    const uint32_t m = lane & 15, kq = (lane >> 4) * 8;
+    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        __syncthreads();  // Bs is written; every wave is done with the previous tile's rows
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const uint32_t orow = row0 + (lane >> 4) * 4 + q;
-            if (orow >= M) continue;
-            if (MODE == 0) {
-                float v = acc[t][q] + bias[col];
-                if (relu) v = fmaxf(v, 0.0f);
-                out_bf[(size_t)orow * N + col] = f2bf(v);
-            } else {
-                out_f[(size_t)orow * N + col] = acc[t][q];
+        for (int p = 0; p < PER; p++) {
+            const int q = (int)tid + p * kGemmThreads;
+            if (q < (int)kGemmRows * cr)
+                *reinterpret_cast<uint4 *>(&As[(q / cr) * LDK + (q % cr) * 8]) = pre[p];
+        }
+        __syncthreads();
+        fetch(t + gridDim.x);  // in flight while this tile's products run
+        v4f acc[NT];
+#pragma unroll
+        for (int tt = 0; tt < NT; tt++) acc[tt] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int k0 = 0; k0 < K; k0 += 32) {
+            const v8bf a = *reinterpret_cast<const v8bf *>(&As[(w * 16 + m) * LDK + k0 + kq]);
+#pragma unroll
+            for (int tt = 0; tt < NT; tt++) {
+                const v8bf b = *reinterpret_cast<const v8bf *>(&Bs[(tt * 16 + m) * LDK + k0 + kq]);
+                acc[tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[tt], 0, 0, 0);
+            }
+        }
+        const uint32_t r0 = t * kGemmRows + w * 16;
+        if constexpr (MODE == 0) {
+            // the wave's own 16 rows of As take its outputs, then go out as rows
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int tt = 0; tt < NT; tt++) {
+                const uint32_t col = tt * 16 + m;
+                const float b = bias[col];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    float v = acc[tt][q] + b;
+                    if (relu) v = fmaxf(v, 0.0f);
+                    As[(w * 16 + (lane >> 4) * 4 + q) * LDK + col] = f2bf(v);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            constexpr int CH = N / 8;  // 16-byte chunks per output row
+            for (int q = (int)lane; q < 16 * CH; q += 64) {
+                const int row = q / CH, ch = q % CH;
+                if (r0 + row < M)
+                    *reinterpret_cast<uint4 *>(out_bf + (size_t)(r0 + row) * N + ch * 8) =
+                        *reinterpret_cast<const uint4 *>(&As[(w * 16 + row) * LDK + ch * 8]);
+            }
+        } else {
+#pragma unroll
+            for (int tt = 0; tt < NT; tt++) {
+                const uint32_t col = tt * 16 + m;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t orow = r0 + (lane >> 4) * 4 + q;
+                    if (orow < M) out_f[(size_t)orow * N + col] = acc[tt][q];
+                }
             }
         }
     }
 }
 
 // ---------------------------------------------------------------- head
+// y_i = h_i W_h + b_h, k = 0..127 in order (fmaf), each row read as 16 16-byte
+// loads; the weights and bias come from LDS.
 __global__ __launch_bounds__(256) void k_head(uint32_t n, const uint16_t *__restrict__ H,
                                               const float *__restrict__ Wh,
                                               const float *__restrict__ bh, float *__restrict__ y) {
+    __shared__ float w[kHid * kOut + kOut];
+    for (uint32_t q = threadIdx.x; q < kHid * kOut + kOut; q += blockDim.x)
+        w[q] = q < kHid * kOut ? Wh[q] : bh[q - kHid * kOut];
+    __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float a0 = bh[0], a1 = bh[1], a2 = bh[2];
-    const uint16_t *h = H + (size_t)i * kHid;
-    for (int k = 0; k < kHid; k++) {
-        const float v = bf2f(h[k]);
-        a0 = fmaf(v, Wh[k * 3 + 0], a0);
-        a1 = fmaf(v, Wh[k * 3 + 1], a1);
-        a2 = fmaf(v, Wh[k * 3 + 2], a2);
+    float a0 = w[kHid * kOut], a1 = w[kHid * kOut + 1], a2 = w[kHid * kOut + 2];
+    const uint4 *h = reinterpret_cast<const uint4 *>(H + (size_t)i * kHid);
+#pragma unroll 4
+    for (int kk = 0; kk < kHid / 8; kk++) {
+        const uint4 v = h[kk];
+        const uint32_t word[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const float x = bf2f((uint16_t)(word[j >> 1] >> (16 * (j & 1))));
+            const int k = kk * 8 + j;
+            a0 = fmaf(x, w[k * 3 + 0], a0);
+            a1 = fmaf(x, w[k * 3 + 1], a1);
+            a2 = fmaf(x, w[k * 3 + 2], a2);
+        }
     }
     y[3 * (size_t)i] = a0;
     y[3 * (size_t)i + 1] = a1;
@@ -603,7 +813,15 @@ class Mpnn {
         JC_TRY(dx_.fit(nn * 2 * kHid));
         JC_TRY(dh_.fit(nn * kHid));
         JC_TRY(rw_.fit(nn * kRw));
-        JC_TRY(parts_.fit((size_t)cd(nn, kXtgRows) * (2 * kHid + 1) * kHid));
+        JC_TRY(parts_.fit((size_t)cd(nn, xtg_rows((uint32_t)nn)) * (2 * kHid + 1) * kHid));
+        JC_TRY(inv_.fit(nn));
+        std::vector<uint32_t> hubs;
+        for (uint32_t v = 0; v < n; v++)
+            if (off[v + 1] - off[v] > kAggLight) hubs.push_back(v);
+        n_hubs_ = (uint32_t)hubs.size();
+        JC_TRY(hubs_.fit(hubs.empty() ? 1 : hubs.size()));
+        if (!hubs.empty())
+            JC_HIP(hipMemcpyAsync(hubs_.p, hubs.data(), hubs.size() * 4, hipMemcpyHostToDevice, stream_));
         DevBuf<float> noise_d;
         JC_TRY(noise_d.alloc(nn * 3));
         // the caller's noise goes to the device as it is; without one, the
@@ -628,6 +846,7 @@ class Mpnn {
             const float mean_deg = nnz ? (float)((double)nnz / n) : 1.0f;
             k_rwpe<<<cd(n, 256), 256, 0, stream_>>>(n, off_.p, nbr_.p, feat_seed, walks, rw_.p);
             k_features<<<cd(n, 256), 256, 0, stream_>>>(n, off_.p, rw_.p, mean_deg, noise_d.p, x_.p);
+            k_inv_deg<<<cd(n, 256), 256, 0, stream_>>>(n, off_.p, inv_.p);
             JC_HIP(hipGetLastError());
         }
         JC_HIP(hipStreamSynchronize(stream_));
@@ -772,18 +991,23 @@ class Mpnn {
 
     Status enqueue_forward() {
         const uint32_t n = n_;
-        const uint32_t gw = cd((uint64_t)cd(n, 16) * 64, 256);  // one wave per 16 rows
+        const uint32_t gw = gemm_grid(n);
         // layer 0: [x | mean x] (32) -> 128
         k_agg_mean<kIn><<<cd((uint64_t)n * (kIn / 2), 256), 256, 0, stream_>>>(n, off_.p, nbr_.p, x_.p, a0_.p);
-        k_gemm<8, 0><<<gw, 256, 0, stream_>>>(n, x_.p, kIn, a0_.p, kIn, wt_[0].p,
-                                               params_.p + layer_offset(0) + (uint64_t)kK[0] * kHid, 1,
-                                               h_[0].p, nullptr);
+        if (n_hubs_)
+            k_agg_mean_hub<kIn><<<n_hubs_, kHubThreads, 0, stream_>>>(hubs_.p, off_.p, nbr_.p, x_.p, a0_.p);
+        k_gemm<8, 0, 2 * kHid><<<gw, kGemmThreads, 0, stream_>>>(
+            n, x_.p, kIn, a0_.p, kIn, wt_[0].p, params_.p + layer_offset(0) + (uint64_t)kK[0] * kHid, 1, h_[0].p,
+            nullptr);
         for (int l = 1; l < kLayers; l++) {
             k_agg_mean<kHid><<<cd((uint64_t)n * (kHid / 2), 256), 256, 0, stream_>>>(n, off_.p, nbr_.p,
                                                                                        h_[l - 1].p, a_[l].p);
-            k_gemm<8, 0><<<gw, 256, 0, stream_>>>(n, h_[l - 1].p, kHid, a_[l].p, kHid, wt_[l].p,
-                                                   params_.p + layer_offset(l) + (uint64_t)kK[l] * kHid,
-                                                   1, h_[l].p, nullptr);
+            if (n_hubs_)
+                k_agg_mean_hub<kHid><<<n_hubs_, kHubThreads, 0, stream_>>>(hubs_.p, off_.p, nbr_.p, h_[l - 1].p,
+                                                                           a_[l].p);
+            k_gemm<8, 0, 2 * kHid><<<gw, kGemmThreads, 0, stream_>>>(
+                n, h_[l - 1].p, kHid, a_[l].p, kHid, wt_[l].p, params_.p + layer_offset(l) + (uint64_t)kK[l] * kHid,
+                1, h_[l].p, nullptr);
         }
         k_head<<<cd(n, 256), 256, 0, stream_>>>(n, h_[kLayers - 1].p, params_.p + head_offset(),
                                                 params_.p + head_offset() + kHid * kOut, y_.p);
@@ -803,7 +1027,7 @@ class Mpnn {
 
     Status enqueue_backward() {
         const uint32_t n = n_;
-        const uint32_t per = kXtgRows;
+        const uint32_t per = xtg_rows(n);
         const uint32_t nparts = cd(n, per);
         // head: dWh = H2^T dY, dbh = sum dY ; dZ2 = (dY Wh^T) * relu'(H2)
         k_xtg<kOut, kOut><<<nparts, 1024, 0, stream_>>>(n, per, h_[kLayers - 1].p, kHid, nullptr, 0,
@@ -822,11 +1046,14 @@ class Mpnn {
                 nparts, (kK[l] + 1) * kHid, parts_.p, grads_.p + layer_offset(l));
             if (l == 0) break;
             // dX = dZ W_l^T on MFMA: A = dZ (n x 128), Bt = W_l (K_l x 128)
-            k_gemm<16, 1><<<cd((uint64_t)cd(n, 16) * 64, 256), 256, 0, stream_>>>(
+            k_gemm<16, 1, kHid><<<gemm_grid(n), kGemmThreads, 0, stream_>>>(
                 n, dz_.p, kHid, nullptr, 0, wb_[l].p, nullptr, 0, nullptr, dx_.p);
             // dH_{l-1} = dX[:, :128] + mean^T(dX[:, 128:])
-            k_agg_mean_t<kHid><<<cd((uint64_t)n * kHid, 256), 256, 0, stream_>>>(n, off_.p, nbr_.p, dx_.p,
-                                                                                  2 * kHid, dh_.p);
+            k_agg_mean_t<kHid><<<cd((uint64_t)n * (kHid / 2), 256), 256, 0, stream_>>>(
+                n, off_.p, nbr_.p, inv_.p, dx_.p, 2 * kHid, dh_.p);
+            if (n_hubs_)
+                k_agg_mean_t_hub<kHid><<<n_hubs_, kHubThreads, 0, stream_>>>(hubs_.p, off_.p, nbr_.p, inv_.p, dx_.p,
+                                                                             2 * kHid, dh_.p);
             k_relu_back<<<cd((uint64_t)n * kHid, 256), 256, 0, stream_>>>(n * kHid, dh_.p, h_[l - 1].p,
                                                                           dzf_.p, dz_.p);
         }
@@ -834,8 +1061,15 @@ class Mpnn {
         return Status::ok();
     }
 
+    // Persistent GEMM blocks: one per CU, at most one per row tile.
+    static uint32_t gemm_grid(uint32_t n) {
+        return std::max<uint32_t>(1, std::min<uint32_t>(cd(n, kGemmRows), 256));
+    }
+
     hipStream_t stream_ = nullptr;
-    uint32_t n_ = 0, na_ = 0;
+    uint32_t n_ = 0, na_ = 0, n_hubs_ = 0;
+    DevBuf<uint32_t> hubs_;  // rows longer than kAggLight, summed one block each
+    DevBuf<float> inv_;      // 1 / degree per node (the backward aggregation)
     uint64_t adam_t_ = 0;
     bool has_target_ = false;
     DevBuf<float> params_, grads_, m_, v_, parts_;

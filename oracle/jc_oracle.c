@@ -1129,7 +1129,8 @@ void jco_spmv_f16(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, con
  * (pagerank.rs:98-104). */
 int jco_pagerank_pull(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, float damping,
                       uint32_t max_iters, double tol_l1, float *ranks, uint32_t *iters_done,
-                      double *resid) {
+                      double *resid, int n_threads) {
+    const int nt = n_threads > 0 ? n_threads : 1;
     if (n == 0) { if (iters_done) *iters_done = 0; return 0; }
     float *inv = (float *)malloc((size_t)n * sizeof(float));
     for (uint32_t v = 0; v < n; v++) {
@@ -1144,7 +1145,10 @@ int jco_pagerank_pull(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs,
     uint32_t it = 0;
     double r = 0.0;
     while (it < max_iters) {
-        for (uint32_t v = 0; v < n; v++) {
+        /* rows are independent; the residual's limbs add exactly in any order */
+#pragma omp parallel for schedule(dynamic, 4096) num_threads(nt)
+        for (int64_t vv = 0; vv < (int64_t)n; vv++) {
+            uint32_t v = (uint32_t)vv;
             float acc = 0.0f;
             for (uint32_t e = offsets[v]; e < offsets[v + 1]; e++) {
                 uint32_t u = nbrs[e];
@@ -1153,9 +1157,16 @@ int jco_pagerank_pull(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs,
             b[v] = teleport + damping * acc;
         }
         uint64_t P[JCO_LIMBS + 1] = {0}, Q[JCO_LIMBS + 1] = {0};
-        for (uint32_t v = 0; v < n; v++) {
-            acc_add(P, fmaxf(a[v], b[v]));
-            acc_add(Q, fminf(a[v], b[v]));
+#pragma omp parallel num_threads(nt)
+        {
+            uint64_t p[JCO_LIMBS + 1] = {0}, q[JCO_LIMBS + 1] = {0};
+#pragma omp for schedule(static)
+            for (int64_t vv = 0; vv < (int64_t)n; vv++) {
+                acc_add(p, fmaxf(a[vv], b[vv]));
+                acc_add(q, fminf(a[vv], b[vv]));
+            }
+#pragma omp critical
+            for (int k = 0; k <= JCO_LIMBS; k++) { P[k] += p[k]; Q[k] += q[k]; }
         }
         r = acc_diff_to_double(P, Q);
         float *t = a; a = b; b = t;

@@ -101,7 +101,7 @@ int jco_fa2_run(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, float
 /* ---- PageRank (analytics/pagerank.rs:67-331, pagerank.wgsl:30-46) ---- */
 int jco_pagerank_pull(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, float damping,
                       uint32_t max_iters, double tol_l1, float *ranks, uint32_t *iters_done,
-                      double *resid);
+                      double *resid, int n_threads);
 void jco_pagerank_push(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, float damping,
                        uint32_t iters, float *ranks);
 void jco_pagerank_f64(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, double damping,

@@ -96,7 +96,7 @@ def lib():
         "jco_fa2_run": (C.c_int, [C.c_uint32, u32p, u32p, f32p, f32p, f32p, C.c_uint32, S, C.c_int,
                                   C.c_int, C.c_int, f64p]),
         "jco_pagerank_pull": (C.c_int, [C.c_uint32, u32p, u32p, C.c_float, C.c_uint32, C.c_double,
-                                        f32p, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
+                                        f32p, C.POINTER(C.c_uint32), C.POINTER(C.c_double), C.c_int]),
         "jco_pagerank_push": (None, [C.c_uint32, u32p, u32p, C.c_float, C.c_uint32, f32p]),
         "jco_pagerank_f64": (None, [C.c_uint32, u32p, u32p, C.c_double, C.c_uint32, f64p]),
         "jco_spmv": (None, [C.c_uint32, u32p, u32p, f32p, f32p, f32p]),
@@ -344,13 +344,13 @@ def fa2_run(offsets, nbrs, xyz, iters, settings: Fa2Settings, tree_kind=1, exact
 
 
 # ---------------------------------------------------------------- PageRank
-def pagerank_pull(offsets, nbrs, damping=0.85, max_iters=100, tol=1e-6):
+def pagerank_pull(offsets, nbrs, damping=0.85, max_iters=100, tol=1e-6, threads=8):
     n = len(offsets) - 1
     ranks = np.zeros(max(n, 1), np.float32)
     it = C.c_uint32(0)
     res = C.c_double(0)
     rc = lib().jco_pagerank_pull(n, _u32(offsets), _u32(nbrs) if len(nbrs) else np.zeros(1, np.uint32),
-                                 damping, max_iters, tol, ranks, C.byref(it), C.byref(res))
+                                 damping, max_iters, tol, ranks, C.byref(it), C.byref(res), threads)
     if rc != 0:
         raise ValueError("dangling node")
     return ranks[:n], it.value, res.value
