@@ -164,7 +164,11 @@ def warm_start(off, nb, xyz, theta: float) -> dict | None:
             "learned_ms_to_target", "fallback_to_plain", "speedup",
             "scaled_random_iters_to_target", "scaled_random_ms_to_target", "multilevel_levels",
             "multilevel_descent_ms", "multilevel_level0_start_stress", "multilevel_level0_iters_to_target",
-            "multilevel_ms_to_target")
+            "multilevel_ms_to_target", "plain_init_cv", "plain_cv_final", "target_cv", "learned_init_cv",
+            "learned_cv_final", "plain_iters_to_both", "plain_ms_to_both", "plain_stays_within_both",
+            "learned_iters_to_both", "learned_ms_to_both", "learned_stays_within_both",
+            "learned_max_stress_after_target", "speedup_both", "scaled_random_iters_to_both",
+            "scaled_random_ms_to_both", "multilevel_level0_iters_to_both", "multilevel_ms_to_both")
     return {k: (round(r[k], 5) if isinstance(r[k], float) else r[k]) for k in keep}
 
 

@@ -154,13 +154,36 @@ def stress_judge(off, nb, k: int = 50, seed: int = JUDGE_PIVOT_SEED):
     return StressJudge(n, piv, dist)
 
 
+class EdgeCv:
+    """edge_length_cv (crates/graph-layouts/src/metrics.rs:150-164) over the
+    graph's undirected edges (i < j), f64 sums."""
+
+    def __init__(self, off, nb):
+        off = np.asarray(off, np.int64)
+        src = np.repeat(np.arange(len(off) - 1), np.diff(off))
+        keep = src < np.asarray(nb, np.int64)
+        self.i, self.j = src[keep], np.asarray(nb, np.int64)[keep]
+
+    def __call__(self, xyz):
+        if len(self.i) < 2:
+            return 0.0
+        p = np.asarray(xyz, np.float32).reshape(-1, 3)
+        d = (p[self.i] - p[self.j]).astype(np.float64)
+        ln = np.sqrt((d * d).sum(1))
+        mean = ln.mean()
+        if mean <= 1e-12:
+            return 0.0
+        return float(np.float32(np.sqrt(((ln - mean) ** 2).mean()) / mean))
+
+
 def stress_curve(off, nb, init, judge, iters: int, every: int, theta: float,
-                 stop_below: float | None = None):
-    """Runs the plain engine from `init`; stress after every `every` iterations.
-    Returns (list of (iteration, stress), ms per iteration measured on the device)."""
+                 stop_below: float | None = None, cv=None):
+    """Runs the plain engine from `init`; stress (and, with `cv`, the edge-length
+    CV) after every `every` iterations.  Returns (list of (iteration, stress) or
+    (iteration, stress, cv), ms per iteration measured on the device)."""
     e = Fa2BhEngine(theta=theta)
     e.init(off, nb, init)
-    curve = [(0, judge(init))]
+    curve = [(0, judge(init)) if cv is None else (0, judge(init), cv(init))]
     dev_ms = 0.0
     done = 0
     while done < iters:
@@ -168,20 +191,34 @@ def stress_curve(off, nb, init, judge, iters: int, every: int, theta: float,
         _, it_ms = e.step_timed(k)
         dev_ms += it_ms
         done += k
-        s = judge(e.get_state()["positions"])
-        curve.append((done, s))
+        pos = e.get_state()["positions"]
+        s = judge(pos)
+        curve.append((done, s) if cv is None else (done, s, cv(pos)))
         if stop_below is not None and s <= stop_below:
             break
     return curve, dev_ms / max(done, 1)
 
 
+def first_within(curve, s_target, cv_target):
+    """First check at which stress <= s_target and edge-length CV <= cv_target,
+    and whether every later check stays within both."""
+    for k, (it, s, c) in enumerate(curve):
+        if s <= s_target and c <= cv_target:
+            stays = all(s2 <= s_target and c2 <= cv_target for _, s2, c2 in curve[k:])
+            return it, stays
+    return None, False
+
+
 def multilevel_curve(off, nb, init, judge, iters: int, every: int, theta: float,
-                     stop_below: float | None = None, settings: dict | None = None):
+                     stop_below: float | None = None, settings: dict | None = None, cv=None,
+                     stop_cv: float | None = None):
     """The classical multilevel start (multilevel-hip over fa2-bh-hip, the
     reference's MultilevelSettings defaults unless `settings` overrides them):
     wall time of init (host coarsening, fold-up, first bind) plus the descent
     through every coarse level, then the stress at level 0 after every `every`
-    iterations.  Returns (curve over level-0 iterations, descent wall ms, levels)."""
+    iterations (and, with `cv`, the edge-length CV; the run stops once both are
+    under stop_below / stop_cv).  Returns (curve over level-0 iterations, descent
+    wall ms, levels)."""
     st = dict(settings or {})
     e = MultilevelEngine(inner="fa2-bh-hip", inner_params={"theta": theta}, **st)
     t0 = time.perf_counter()
@@ -195,15 +232,16 @@ def multilevel_curve(off, nb, init, judge, iters: int, every: int, theta: float,
     pos = e.step(descent)
     descent_ms = (time.perf_counter() - t0) * 1e3
     assert e.state()["level"] == 0
-    curve = [(0, judge(pos))]
+    curve = [(0, judge(pos)) if cv is None else (0, judge(pos), cv(pos))]
     done = 0
     while done < iters:
         k = min(every, iters - done)
         pos = e.step(k)
         done += k
         s = judge(pos)
-        curve.append((done, s))
-        if stop_below is not None and s <= stop_below:
+        c = None if cv is None else cv(pos)
+        curve.append((done, s) if cv is None else (done, s, c))
+        if stop_below is not None and s <= stop_below and (stop_cv is None or c <= stop_cv):
             break
     return curve, descent_ms, levels
 
@@ -227,13 +265,17 @@ def evaluate(net: Mpnn, coef, off, nb, x0, horizon: int = 2000, every: int = 50,
     starts the plain engine from x0 rescaled to the size law, without the
     network, to separate what the network adds from what the scale adds."""
     judge = stress_judge(off, nb, k_pivots)
-    plain, ms_it = stress_curve(off, nb, x0, judge, horizon, every, theta)
-    s_star = plain[-1][1]
+    cv = EdgeCv(off, nb)
+    plain, ms_it = stress_curve(off, nb, x0, judge, horizon, every, theta, cv=cv)
+    s_star, cv_star = plain[-1][1], plain[-1][2]
     target = 1.01 * s_star
-    t_plain_it = first_below(plain, target)
+    cv_target = 1.01 * cv_star
+    t_plain_it = first_below([c[:2] for c in plain], target)
+    t_plain_both, plain_stays = first_within(plain, target, cv_target)
     init, fwd_ms, pred_wall_ms = predict(net, coef, off, nb, x0)
-    learned, ms_it2 = stress_curve(off, nb, init, judge, horizon, every, theta)
-    t_learn_it = first_below(learned, target)
+    learned, ms_it2 = stress_curve(off, nb, init, judge, horizon, every, theta, cv=cv)
+    t_learn_it = first_below([c[:2] for c in learned], target)
+    t_learn_both, learned_stays = first_within(learned, target, cv_target)
     fallback = t_learn_it is None
     res = {
         "horizon": horizon,
@@ -253,12 +295,32 @@ def evaluate(net: Mpnn, coef, off, nb, x0, horizon: int = 2000, every: int = 50,
         "fallback_to_plain": fallback,
         "plain_curve": plain,
         "learned_curve": learned,
+        # both judges of the final layout: stress <= 1.01 S* and edge-length CV <= 1.01 CV*
+        "plain_cv_final": cv_star,
+        "target_cv": cv_target,
+        "plain_init_cv": plain[0][2],
+        "learned_init_cv": learned[0][2],
+        "learned_cv_final": learned[-1][2],
+        "plain_iters_to_both": t_plain_both,
+        "plain_ms_to_both": None if t_plain_both is None else t_plain_both * ms_it,
+        "plain_stays_within_both": plain_stays,
+        "learned_iters_to_both": t_learn_both,
+        "learned_ms_to_both": None if t_learn_both is None else pred_wall_ms + t_learn_both * ms_it2,
+        "learned_stays_within_both": learned_stays,
+        "learned_max_stress_after_target": (None if t_learn_it is None else
+                                            max(s2 for it2, s2, _ in learned if it2 >= t_learn_it)),
     }
     res["speedup"] = None if fallback else res["plain_ms_to_target"] / res["learned_ms_to_target"]
+    res["speedup_both"] = (None if t_learn_both is None or t_plain_both is None
+                           else res["plain_ms_to_both"] / res["learned_ms_to_both"])
     if multilevel:
         # the classical counterpart: coarsen -> solve -> prolong -> refine
-        ml, ml_ms, ml_levels = multilevel_curve(off, nb, x0, judge, horizon, every, theta, stop_below=target)
-        t_ml = first_below(ml, target)
+        ml, ml_ms, ml_levels = multilevel_curve(off, nb, x0, judge, horizon, every, theta, stop_below=target,
+                                                cv=cv, stop_cv=cv_target)
+        t_ml = first_below([c[:2] for c in ml], target)
+        t_ml_both, _ = first_within(ml, target, cv_target)
+        res["multilevel_level0_iters_to_both"] = t_ml_both
+        res["multilevel_ms_to_both"] = None if t_ml_both is None else ml_ms + t_ml_both * ms_it
         res["multilevel_levels"] = ml_levels
         res["multilevel_descent_ms"] = ml_ms
         res["multilevel_level0_start_stress"] = ml[0][1]
@@ -272,13 +334,18 @@ def evaluate(net: Mpnn, coef, off, nb, x0, horizon: int = 2000, every: int = 50,
         R = size_of(coef, len(off) - 1, len(nb))
         p = np.asarray(x0, np.float64).reshape(-1, 3)
         xs = ((p - p.mean(0)) * (R / max(rms_radius(p), 1e-30))).astype(np.float32).reshape(-1)
-        scaled, ms_it3 = stress_curve(off, nb, xs, judge, horizon, every, theta)
-        t_s = first_below(scaled, target)
+        scaled, ms_it3 = stress_curve(off, nb, xs, judge, horizon, every, theta, cv=cv)
+        t_s = first_below([c[:2] for c in scaled], target)
+        t_s_both, _ = first_within(scaled, target, cv_target)
+        res["scaled_random_iters_to_both"] = t_s_both
+        res["scaled_random_ms_to_both"] = None if t_s_both is None else t_s_both * ms_it3
         res["scaled_random_iters_to_target"] = t_s
         res["scaled_random_ms_to_target"] = None if t_s is None else t_s * ms_it3
         res["scaled_random_curve"] = scaled
     if log:
-        log(f"plain: S*={s_star:.4f} (init {plain[0][1]:.4f}), target reached at it {t_plain_it}; "
-            f"learned: init S={learned[0][1]:.4f}, target at it {t_learn_it}, final "
-            f"{learned[-1][1]:.4f}, speedup {res['speedup']}")
+        log(f"plain: S*={s_star:.4f} (init {plain[0][1]:.4f}), CV*={cv_star:.4f} (init {plain[0][2]:.4f}), "
+            f"stress target at it {t_plain_it}, both at {t_plain_both}; learned: init S={learned[0][1]:.4f} "
+            f"CV={learned[0][2]:.4f}, stress target at it {t_learn_it}, both at {t_learn_both} (stays "
+            f"{learned_stays}), final {learned[-1][1]:.4f} / {learned[-1][2]:.4f}, speedup {res['speedup']}, "
+            f"both {res['speedup_both']}")
     return res
