@@ -329,6 +329,10 @@ void jc_mpnn_destroy(jc_mpnn *m);
 int jc_mpnn_get_params(jc_mpnn *m, float *out);
 int jc_mpnn_set_params(jc_mpnn *m, const float *in);
 int jc_mpnn_get_grads(jc_mpnn *m, float *out);
+/* Adam's first and second moments (param_count floats each) and its step
+ * count: a training run saves them to stop and resumes from them exactly. */
+int jc_mpnn_get_adam(jc_mpnn *m, float *m1, float *m2, uint64_t *step);
+int jc_mpnn_set_adam(jc_mpnn *m, const float *m1, const float *m2, uint64_t step);
 /* Binds a graph and computes its input features (random-walk return
  * probabilities from `walks` walks per node, degree terms, noise: 3n values
  * in [-1,1] or NULL to draw them from feat_seed). */

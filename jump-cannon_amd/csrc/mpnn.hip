@@ -791,6 +791,20 @@ class Mpnn {
         JC_TRY(copy_sync(out, grads_.p, param_count() * 4, hipMemcpyDeviceToHost, stream_));
         return Status::ok();
     }
+    // Adam's first and second moments and step count, for a training run that
+    // stops and resumes where it left off.
+    Status get_adam(float *m, float *v, uint64_t *t) {
+        JC_TRY(copy_sync(m, m_.p, param_count() * 4, hipMemcpyDeviceToHost, stream_));
+        JC_TRY(copy_sync(v, v_.p, param_count() * 4, hipMemcpyDeviceToHost, stream_));
+        *t = adam_t_;
+        return Status::ok();
+    }
+    Status set_adam(const float *m, const float *v, uint64_t t) {
+        JC_TRY(copy_sync(m_.p, m, param_count() * 4, hipMemcpyHostToDevice, stream_));
+        JC_TRY(copy_sync(v_.p, v, param_count() * 4, hipMemcpyHostToDevice, stream_));
+        adam_t_ = t;
+        return Status::ok();
+    }
 
     // Graph + features.  noise (3n, nullable -> drawn from feat_seed in [-1,1]).
     Status set_graph(uint32_t n, const uint32_t *off, const uint32_t *nbr, uint64_t nnz,
@@ -1119,6 +1133,16 @@ void jc_mpnn_destroy(jc_mpnn *m) { delete m; }
 int jc_mpnn_get_params(jc_mpnn *m, float *out) { return m ? done(m->impl.get_params(out)) : nullh(); }
 int jc_mpnn_set_params(jc_mpnn *m, const float *in) { return m ? done(m->impl.set_params(in)) : nullh(); }
 int jc_mpnn_get_grads(jc_mpnn *m, float *out) { return m ? done(m->impl.get_grads(out)) : nullh(); }
+int jc_mpnn_get_adam(jc_mpnn *m, float *m1, float *m2, uint64_t *step) {
+    if (!m) return nullh();
+    if (!m1 || !m2 || !step) { jc::set_error("jc_mpnn_get_adam: NULL output"); return JC_E_INVALID; }
+    return done(m->impl.get_adam(m1, m2, step));
+}
+int jc_mpnn_set_adam(jc_mpnn *m, const float *m1, const float *m2, uint64_t step) {
+    if (!m) return nullh();
+    if (!m1 || !m2) { jc::set_error("jc_mpnn_set_adam: NULL input"); return JC_E_INVALID; }
+    return done(m->impl.set_adam(m1, m2, step));
+}
 int jc_mpnn_set_graph(jc_mpnn *m, uint32_t n, const uint32_t *off, const uint32_t *nbr, uint64_t nnz,
                       uint64_t feat_seed, uint32_t walks, const float *noise) {
     return m ? done(m->impl.set_graph(n, off, nbr, nnz, feat_seed, walks, noise)) : nullh();

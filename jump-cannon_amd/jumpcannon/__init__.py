@@ -137,6 +137,8 @@ SIGNATURES = {
     "jc_mpnn_get_params": (C.c_int, [_vp, _vp]),
     "jc_mpnn_set_params": (C.c_int, [_vp, _vp]),
     "jc_mpnn_get_grads": (C.c_int, [_vp, _vp]),
+    "jc_mpnn_get_adam": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "jc_mpnn_set_adam": (C.c_int, [_vp, _vp, _vp, C.c_uint64]),
     "jc_mpnn_set_graph": (C.c_int, [_vp, C.c_uint32, _vp, _vp, C.c_uint64, C.c_uint64, C.c_uint32, _vp]),
     "jc_mpnn_features": (C.c_int, [_vp, _vp]),
     "jc_mpnn_activations": (C.c_int, [_vp, C.c_int, _vp]),
@@ -557,6 +559,19 @@ class Mpnn:
         out = np.empty(self.P, np.float32)
         _check(lib().jc_mpnn_get_grads(self._h, _ptr(out)))
         return out
+
+    def adam_state(self):
+        """(first moments, second moments, step count) of the Adam optimizer."""
+        m1 = np.empty(self.P, np.float32)
+        m2 = np.empty(self.P, np.float32)
+        t = np.zeros(1, np.uint64)
+        _check(lib().jc_mpnn_get_adam(self._h, _ptr(m1), _ptr(m2), _ptr(t)))
+        return m1, m2, int(t[0])
+
+    def set_adam_state(self, m1, m2, step: int):
+        m1 = _need_len(_f32(m1), self.P, "Adam first moments")
+        m2 = _need_len(_f32(m2), self.P, "Adam second moments")
+        _check(lib().jc_mpnn_set_adam(self._h, _ptr(m1), _ptr(m2), int(step)))
 
     def set_graph(self, offsets, neighbors, feat_seed: int = 7, walks: int = 16, noise=None):
         off = _u32(offsets)

@@ -92,3 +92,22 @@ def test_warm_start_on_the_held_out_1m_graph(net):
     assert not r["fallback_to_plain"] and r["speedup"] >= 3.0
     assert r["learned_stress_final"] <= r["target_stress"]
     assert abs(r["learned_cv_final"] - r["plain_cv_final"]) <= 0.01 * r["plain_cv_final"]
+
+
+def test_training_stream_resumes_exactly():
+    """Stopping the training stream after graph 3 and resuming from the saved
+    parameters and Adam state gives the same parameters, bit for bit, as one
+    uninterrupted pass over graphs 0..5 (small graphs, 600-2,000 nodes)."""
+    kw = dict(total=6, seed=777, n_min=600, n_max=2000, pivots=8, sgd_epochs=6)
+    a = jc.Mpnn(seed=5)
+    la = W.train_stream(a, 0, 6, **kw)
+    b = jc.Mpnn(seed=5)
+    lb = W.train_stream(b, 0, 3, **kw)
+    m1, m2, step = b.adam_state()
+    assert step == 6
+    c = jc.Mpnn(seed=11)
+    c.set_params(b.params())
+    c.set_adam_state(m1, m2, step)
+    lb += W.train_stream(c, 3, 6, **kw)
+    assert np.array_equal(la, lb)
+    assert np.array_equal(a.params(), c.params())
