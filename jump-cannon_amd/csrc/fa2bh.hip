@@ -50,6 +50,7 @@ constexpr uint32_t kTimedIters = 10;      // timed iterations per host read-back
 constexpr uint32_t kHeavyAtt = 32;        // attraction rows longer than this go to k_force_heavy
 constexpr uint32_t kHeavyStage = 1024;    // heavy-row terms staged in LDS per round
 constexpr uint32_t kEmitTasks = 1024;     // cell tasks per k_emit block held in LDS
+constexpr uint32_t kComGrid = 1024;       // k_com_fused workgroups (each strides over chunks)
 // Columns of the swing/traction limb accumulator.  Blocks add into column
 // blockIdx.x % kAccCols; many columns keep the device-scope atomics on any one
 // address few (they serialise at the memory side across XCDs), few columns keep
@@ -72,6 +73,7 @@ struct Frame {
     uint32_t n_top;                   // nodes in the top tree (depth <= kTopDepth)
     uint32_t n_valid;                 // finite bodies of the last tree build
     unsigned long long interactions;  // accepted cells + leaf bodies summed by the walk
+    uint32_t n_big;                   // cells below the top tree too large for k_com_fused
 };
 
 struct Params {
@@ -112,6 +114,7 @@ __global__ void k_frame_reset(Frame *fr, unsigned long long *__restrict__ acc_co
         fr->n_invalid = 0;
         fr->max_depth = 0;
         fr->interactions = 0ull;
+        fr->n_big = 0;
     }
 }
 
@@ -321,7 +324,8 @@ __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
                                               const uint32_t *__restrict__ blk,
                                               Node *__restrict__ nodes,
                                               uint8_t *__restrict__ node_depth, uint64_t cap,
-                                              uint32_t *__restrict__ top_slot) {
+                                              uint32_t *__restrict__ top_slot,
+                                              uint32_t *__restrict__ cskip) {
     __shared__ uint32_t bdepth, ntask;
     __shared__ uint32_t t_node[kEmitTasks], t_body[kEmitTasks];
     __shared__ uint8_t t_depth[kEmitTasks];
@@ -361,7 +365,9 @@ __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
                         t_body[slot + t] = i;
                         t_depth[slot + t] = (uint8_t)d;
                     } else {
-                        nodes[pos0 + t].b = cell_record(keys, nv, i, d, pos0 + t, fr, loc, blk, top_slot);
+                        const uint4 rec = cell_record(keys, nv, i, d, pos0 + t, fr, loc, blk, top_slot);
+                        nodes[pos0 + t].b = rec;
+                        cskip[pos0 + t] = rec.w;
                         node_depth[pos0 + t] = (uint8_t)d;
                     }
                 }
@@ -383,15 +389,197 @@ __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
     __syncthreads();
     const uint32_t nt = min(ntask, kEmitTasks);
     for (uint32_t q = threadIdx.x; q < nt; q += blockDim.x) {
-        nodes[t_node[q]].b = cell_record(keys, nv, t_body[q], t_depth[q], t_node[q], fr, loc, blk, top_slot);
+        const uint4 rec = cell_record(keys, nv, t_body[q], t_depth[q], t_node[q], fr, loc, blk, top_slot);
+        nodes[t_node[q]].b = rec;
+        cskip[t_node[q]] = rec.w;
         node_depth[t_node[q]] = t_depth[q];
     }
     if (threadIdx.x == 0 && bdepth) atomicMax(&fr->max_depth, bdepth);
 }
 
-// Centre of mass of the internal cells at depth d from their children, in
-// octant (pre-order) order: m = sum m_c; com = sum(com_c * m_c) / m (fa2_bh.rs:944-965).
-// The cells of depth d (com_level_pass, shared with the batched engine).
+// Centre of mass of the cells of depth >= kFuseDepth, one launch.  Such a cell
+// "fits" when its subtree, the pre-order range [x, skip), holds at most
+// kFuseCap nodes.  A fitting cell with no fitting ancestor is a root: one
+// workgroup sums the root's whole subtree in LDS, deepest level first with a
+// barrier between levels, each cell from its children in octant (pre-order)
+// order: m = sum m_c; com = sum(com_c * m_c) / m (fa2_bh.rs:944-965), the same
+// operations as com_of_cell, so the bits are those of one launch per depth.
+//
+// Ownership: the workgroup of the chunk [c0, c0 + kFuseChunk) takes the roots
+// that start in it.  A fitting ancestor a of x satisfies x - kFuseCap <= a < x
+// and skip(a) > x, so x has one iff the prefix maximum of the fitting cells'
+// skips over the window [x - kFuseCap, x) passes x.  The roots of a chunk span
+// fewer than kFuseChunk + kFuseCap nodes.  Between two roots lie only leaves,
+// shallower cells and cells that do not fit; every fitting cell in the span is
+// inside a root.  Cells that do not fit (fewer than one per kFuseCap nodes and
+// depth) go to `big`, which k_com_big sums next.  The few wide cells above
+// kFuseDepth take one k_com_level launch per depth, the top tree k_top_build.
+constexpr uint32_t kFuseDepth = 6;
+constexpr uint32_t kFuseChunk = 1024;
+constexpr uint32_t kFuseCap = 512;
+constexpr uint32_t kFuseSpan = kFuseChunk + kFuseCap;
+constexpr uint32_t kFuseThreads = 512;
+constexpr uint32_t kFusePer = kFuseSpan / kFuseThreads;
+static_assert(kFuseSpan % kFuseThreads == 0, "whole segments per thread");
+static_assert(kFuseSpan <= 65535, "16-bit ropes in LDS");
+
+__device__ __forceinline__ uint32_t fit_end(const uint32_t *__restrict__ cskip, uint32_t x,
+                                            uint32_t N) {
+    const uint32_t sk = cskip[x];
+    return sk < N ? sk : N;  // kEnd: the subtree runs to the end of the tree
+}
+
+__global__ __launch_bounds__(kFuseThreads) void k_com_fused(Frame *fr, uint32_t cap,
+                                                            Node *__restrict__ nodes,
+                                                            const uint8_t *__restrict__ node_depth,
+                                                            const uint32_t *__restrict__ cskip,
+                                                            uint32_t *__restrict__ big, uint32_t big_max) {
+    __shared__ float4 ca[kFuseSpan];  // the window's skips first, then {com, mass} of the span
+    __shared__ uint32_t meta[kFuseSpan];  // depth byte << 16 | rope relative to the span (fitting cells)
+    __shared__ uint32_t wmax[kFuseThreads / 64];
+    __shared__ uint32_t s_lo, s_hi, s_dmax;
+    uint32_t *win = reinterpret_cast<uint32_t *>(ca);
+    const uint32_t N = min(fr->n_tree, cap);
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (uint32_t c0 = blockIdx.x * kFuseChunk; c0 < N; c0 += gridDim.x * kFuseChunk) {
+        const uint32_t c1 = min(c0 + kFuseChunk, N);
+        const uint32_t lo = c0 > kFuseCap ? c0 - kFuseCap : 0u;
+        const uint32_t W = c1 - lo;
+        if (t == 0) {
+            s_lo = 0xFFFFFFFFu;
+            s_hi = 0;
+            s_dmax = 0;
+        }
+        // 1. the window's values: the skip of a fitting cell, 0 elsewhere; the
+        //    chunk's cells that do not fit join the big list
+        for (uint32_t q = t; q < W; q += kFuseThreads) {
+            const uint32_t x = lo + q;
+            const uint32_t d = node_depth[x];
+            uint32_t v = 0;
+            if (d < 0x80u && d >= kFuseDepth) {
+                const uint32_t end = fit_end(cskip, x, N);
+                if (end - x <= kFuseCap) {
+                    v = end;
+                } else if (x >= c0) {
+                    const uint32_t slot = atomicAdd(&fr->n_big, 1u);
+                    if (slot < big_max) big[slot] = x;
+                }
+            }
+            win[q] = v;
+        }
+        __syncthreads();
+        // 2. exclusive prefix maximum over the window (a segment of kFusePer
+        //    entries per thread, then across the lanes and the waves); a fitting
+        //    cell of the chunk whose prefix maximum does not pass it is a root
+        uint32_t seg = 0;
+        for (uint32_t j = 0; j < kFusePer; j++) {
+            const uint32_t q = t * kFusePer + j;
+            if (q < W) seg = max(seg, win[q]);
+        }
+        uint32_t incl = seg;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(incl, o, 64);
+            if ((int)lane >= o) incl = max(incl, u);
+        }
+        if (lane == 63) wmax[w] = incl;
+        __syncthreads();
+        uint32_t run = __shfl_up(incl, 1, 64);
+        if (lane == 0) run = 0;
+        for (uint32_t q = 0; q < w; q++) run = max(run, wmax[q]);
+        uint32_t rlo = 0xFFFFFFFFu, rhi = 0;
+        for (uint32_t j = 0; j < kFusePer; j++) {
+            const uint32_t q = t * kFusePer + j;
+            if (q < W) {
+                const uint32_t v = win[q], x = lo + q;
+                if (v && x >= c0 && run <= x) {
+                    rlo = min(rlo, x);
+                    rhi = max(rhi, v);
+                }
+                run = max(run, v);
+            }
+        }
+        if (rhi) {
+            atomicMin(&s_lo, rlo);
+            atomicMax(&s_hi, rhi);
+        }
+        __syncthreads();
+        const uint32_t r0 = s_lo, r1 = s_hi;
+        if (r1 == 0) {  // no root starts in this chunk (the same answer for every thread)
+            __syncthreads();
+            continue;
+        }
+        const uint32_t R = r1 - r0;  // < kFuseSpan
+        // 3. the span: depths, ropes of the fitting cells relative to r0, {com, mass}
+        uint32_t dmax = 0;
+        for (uint32_t q = t; q < R; q += kFuseThreads) {
+            const uint32_t y = r0 + q;
+            const uint32_t d = node_depth[y];
+            uint32_t r = 0;
+            if (d < 0x80u && d >= kFuseDepth) {
+                const uint32_t end = fit_end(cskip, y, N);
+                if (end - y <= kFuseCap) {
+                    r = end - r0;
+                    dmax = max(dmax, d);
+                }
+            }
+            meta[q] = d << 16 | r;
+            ca[q] = nodes[y].a;
+        }
+        if (dmax) atomicMax(&s_dmax, dmax);
+        __syncthreads();
+        // 4. deepest level first; a fitting cell's children are fitting cells or leaves
+        for (uint32_t d = s_dmax; d >= kFuseDepth; d--) {
+            for (uint32_t q = t; q < R; q += kFuseThreads) {
+                const uint32_t mq = meta[q];
+                if ((mq >> 16) != d || !(mq & 0xFFFFu)) continue;
+                const uint32_t end = mq & 0xFFFFu;
+                float tm = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+                for (uint32_t ch = q + 1; ch < end;) {
+                    const float4 a = ca[ch];
+                    const uint32_t m = meta[ch];
+                    tm = tm + a.w;
+                    cx = cx + a.x * a.w;
+                    cy = cy + a.y * a.w;
+                    cz = cz + a.z * a.w;
+                    const uint32_t nx = (m & 0x800000u) ? ch + 1 : (m & 0xFFFFu);
+                    ch = nx > ch ? nx : end;  // ropes only move forward
+                }
+                if (tm > 0.0f) {
+                    cx = cx / tm;
+                    cy = cy / tm;
+                    cz = cz / tm;
+                }
+                ca[q] = make_float4(cx, cy, cz, tm);
+            }
+            __syncthreads();
+        }
+        for (uint32_t q = t; q < R; q += kFuseThreads)
+            if (meta[q] & 0xFFFFu) nodes[r0 + q].a = ca[q];
+        __syncthreads();  // the LDS is reused by the next chunk
+    }
+}
+
+// The cells of depth >= kFuseDepth that k_com_fused left (subtrees larger than
+// kFuseCap), deepest first, in one workgroup; their children are final: roots
+// summed by k_com_fused, leaves, or big cells of a deeper level done here.
+__global__ __launch_bounds__(1024) void k_com_big(const Frame *fr, uint32_t cap, Node *__restrict__ nodes,
+                                                  const uint8_t *__restrict__ node_depth,
+                                                  const uint32_t *__restrict__ big, uint32_t big_max) {
+    const uint32_t nbig = min(fr->n_big, big_max);
+    if (!nbig) return;
+    const uint32_t N = min(fr->n_tree, cap);
+    for (uint32_t d = min(fr->max_depth, (uint32_t)kLevels); d >= kFuseDepth; d--) {
+        for (uint32_t q = threadIdx.x; q < nbig; q += blockDim.x) {
+            const uint32_t x = big[q];
+            if (node_depth[x] == d) com_of_cell(nodes, x, N);
+        }
+        __syncthreads();
+    }
+}
+
+// The cells of depth d < kFuseDepth, one launch per depth (com_level_pass,
+// shared with the batched engine): few cells with wide subtrees.
 __global__ __launch_bounds__(kTB) void k_com_level(const Frame *fr, uint32_t d, uint32_t cap,
                                                    Node *__restrict__ nodes,
                                                    const uint8_t *__restrict__ node_depth) {
@@ -901,6 +1089,7 @@ __global__ __launch_bounds__(kSpeedThreads) void k_speed(const Params P, Frame *
         fr->n_invalid = 0;
         fr->max_depth = 0;
         fr->interactions = 0ull;
+        fr->n_big = 0;
     }
     __syncthreads();
     if (t != 0) return;
@@ -1135,6 +1324,12 @@ class Fa2BhEngine final : public LayoutEngine {
         cap_ = nn + (uint64_t)(kLevels + 1) * (nn / 2 + 1) + 8;
         JC_TRY(nodes_.alloc(cap_));
         JC_TRY(node_depth_.alloc((cap_ + 31) & ~15ull));  // whole 16-byte loads in k_com_level
+        JC_TRY(cskip_.alloc(cap_));
+        // Cells that do not fit k_com_fused at one depth are disjoint and hold
+        // more than kFuseCap nodes each.
+        big_max_ = (uint32_t)std::min<uint64_t>((uint64_t)(kLevels + 1 - kFuseDepth) * (cap_ / kFuseCap + 1),
+                                                0xFFFFFFFFull);
+        JC_TRY(big_.alloc(big_max_));
         JC_TRY(top_slot_.alloc(kTopSlots));
         JC_TRY(top_a_.alloc(kTopSlots));
         JC_TRY(top_b_.alloc(kTopSlots));
@@ -1521,6 +1716,10 @@ class Fa2BhEngine final : public LayoutEngine {
         ready_ = true;
         return Status::ok();
     }
+    // k_com_fused strides over the chunks of the tree (at most cap_ nodes)
+    uint32_t com_grid() const {
+        return std::max<uint32_t>(1, std::min<uint64_t>(cdiv(cap_, kFuseChunk), kComGrid));
+    }
     uint32_t apply_grid() const { return std::min<uint32_t>(cdiv(n_ ? n_ : 1, kTB), 512); }
 
     // The tree build of pos; the frame holds pos's bounding box and a reset
@@ -1539,10 +1738,14 @@ class Fa2BhEngine final : public LayoutEngine {
         if (ev) JC_HIP(hipEventRecord(ev[PH_EMIT], stream_));
         k_emit<<<cdiv(n, kTB), kTB, 0, stream_>>>(keys_.p, vals_.p, pos, deg_.p, n, frame_.p,
                                                   loc_.p, blk_.p, nodes_.p, node_depth_.p, cap_,
-                                                  top_slot_.p);
+                                                  top_slot_.p, cskip_.p);
         if (ev) JC_HIP(hipEventRecord(ev[PH_COM], stream_));
+        k_com_fused<<<com_grid(), kFuseThreads, 0, stream_>>>(frame_.p, (uint32_t)cap_, nodes_.p,
+                                                              node_depth_.p, cskip_.p, big_.p, big_max_);
+        k_com_big<<<1, 1024, 0, stream_>>>(frame_.p, (uint32_t)cap_, nodes_.p, node_depth_.p, big_.p,
+                                           big_max_);
         const uint32_t gcom = std::min<uint32_t>(cdiv(cap_, kComSpan * (kTB / 64)), 1024);
-        for (int d = kLevels; d > kTopDepth; d--)
+        for (int d = (int)kFuseDepth - 1; d > kTopDepth; d--)
             k_com_level<<<gcom, kTB, 0, stream_>>>(frame_.p, (uint32_t)d, (uint32_t)cap_,
                                                    nodes_.p, node_depth_.p);
         k_top_build<<<1, 1024, 0, stream_>>>(frame_.p, top_slot_.p, nodes_.p, (uint32_t)cap_,
@@ -1708,6 +1911,9 @@ class Fa2BhEngine final : public LayoutEngine {
     RadixScratch sort_;
     DevBuf<Node> nodes_;
     DevBuf<uint8_t> node_depth_;
+    DevBuf<uint32_t> cskip_;      // skip rope of each cell (k_emit), read by k_com_fused
+    DevBuf<uint32_t> big_;        // cells too large for k_com_fused, summed by k_top_build
+    uint32_t big_max_ = 0;
     DevBuf<uint32_t> top_slot_;   // kTopSlots: node of each slot of the top-tree enumeration
     DevBuf<float4> top_a_;        // top tree in pre-order: centre of mass, mass
     DevBuf<uint4> top_b_;         // and size, id, subtree end rank | frontier, byte offset
