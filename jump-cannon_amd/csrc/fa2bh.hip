@@ -51,6 +51,7 @@ constexpr uint32_t kHeavyAtt = 32;        // attraction rows longer than this go
 constexpr uint32_t kHeavyStage = 1024;    // heavy-row terms staged in LDS per round
 constexpr uint32_t kEmitTasks = 1024;     // cell tasks per k_emit block held in LDS
 constexpr uint32_t kComGrid = 1024;       // k_com_fused workgroups (each strides over chunks)
+constexpr uint32_t kApplyBlocks = 512;    // k_apply workgroups (each strides over bodies)
 // Columns of the swing/traction limb accumulator.  Blocks add into column
 // blockIdx.x % kAccCols; many columns keep the device-scope atomics on any one
 // address few (they serialise at the memory side across XCDs), few columns keep
@@ -191,9 +192,9 @@ __global__ __launch_bounds__(kTB) void k_keys(const float4 *__restrict__ pos, ui
 // leading octal digits of neighbouring keys (-1 past either end); a lone body
 // opens the root.  Block-local exclusive scan of c, plus the block total.
 __global__ __launch_bounds__(kTB) void k_count_scan(const uint64_t *__restrict__ keys, uint32_t n,
-                                                    const Frame *fr, uint32_t *__restrict__ loc,
+                                                    Frame *fr, uint32_t *__restrict__ loc,
                                                     uint32_t *__restrict__ blk_tot) {
-    __shared__ uint32_t ws[kTB / 64];
+    __shared__ uint32_t ws[kTB / 64], wd[kTB / 64];
     const uint32_t nv = n - fr->n_invalid;
     const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     // the thread's keys and one neighbour on each side, each loaded once
@@ -205,6 +206,7 @@ __global__ __launch_bounds__(kTB) void k_count_scan(const uint64_t *__restrict__
     }
     uint32_t c[kScanItems];
     uint32_t s = 0;
+    int md = 0;  // deepest cell: the largest common prefix of two neighbouring keys
 #pragma unroll
     for (uint32_t j = 0; j < kScanItems; j++) {
         const uint32_t i = base + j;
@@ -216,6 +218,7 @@ __global__ __launch_bounds__(kTB) void k_count_scan(const uint64_t *__restrict__
                 const int lp = i > 0 ? lcp_digits(kk[j], kk[j + 1]) : -1;
                 const int li = i + 1 < nv ? lcp_digits(kk[j + 1], kk[j + 2]) : -1;
                 cj = li > lp ? (uint32_t)(li - lp) : 0u;
+                md = li > md ? li : md;
             }
         }
         c[j] = cj;
@@ -229,14 +232,23 @@ __global__ __launch_bounds__(kTB) void k_count_scan(const uint64_t *__restrict__
         uint32_t t = __shfl_up(incl, o, 64);
         if ((int)lane >= o) incl += t;
     }
-    if (lane == 63) ws[w] = incl;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) md = max(md, __shfl_xor(md, o, 64));
+    if (lane == 63) {
+        ws[w] = incl;
+        wd[w] = (uint32_t)md;
+    }
     __syncthreads();
-    uint32_t wp = 0, tot = 0;
+    uint32_t wp = 0, tot = 0, bd = 0;
 #pragma unroll
     for (uint32_t q = 0; q < kTB / 64; q++) {
         if (q < w) wp += ws[q];
         tot += ws[q];
+        bd = max(bd, wd[q]);
     }
+    // one device-scope atomic per 4096-body tile (k_emit's per-block atomic on
+    // this word cost about 10 us at 1M bodies)
+    if (threadIdx.x == 0 && bd) atomicMax(&fr->max_depth, bd);
     uint32_t run = incl - s + wp;
 #pragma unroll
     for (uint32_t j = 0; j < kScanItems; j++) {
@@ -326,11 +338,10 @@ __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
                                               uint8_t *__restrict__ node_depth, uint64_t cap,
                                               uint32_t *__restrict__ top_slot,
                                               uint32_t *__restrict__ cskip) {
-    __shared__ uint32_t bdepth, ntask;
+    __shared__ uint32_t ntask;
     __shared__ uint32_t t_node[kEmitTasks], t_body[kEmitTasks];
     __shared__ uint8_t t_depth[kEmitTasks];
     if (threadIdx.x == 0) {
-        bdepth = 0;
         ntask = 0;
     }
     __syncthreads();
@@ -371,7 +382,6 @@ __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
                         node_depth[pos0 + t] = (uint8_t)d;
                     }
                 }
-                atomicMax(&bdepth, (uint32_t)(a0 + (int)c - 1));
             }
             const uint32_t leaf = pos0 + c;
             const uint32_t b = perm[i];
@@ -394,7 +404,6 @@ __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
         cskip[t_node[q]] = rec.w;
         node_depth[t_node[q]] = t_depth[q];
     }
-    if (threadIdx.x == 0 && bdepth) atomicMax(&fr->max_depth, bdepth);
 }
 
 // Centre of mass of the cells of depth >= kFuseDepth, one launch.  Such a cell
@@ -1131,7 +1140,8 @@ __global__ __launch_bounds__(kTB) void k_apply(const Params P, Frame *fr,
                                                float4 *__restrict__ pos_out,
                                                const float4 *__restrict__ force,
                                                float4 *__restrict__ old_force,
-                                               const uint32_t *__restrict__ deg) {
+                                               const uint32_t *__restrict__ deg,
+                                               uint32_t *__restrict__ bbpart) {
     __shared__ float smn[3][kTB / 64], smx[3][kTB / 64];
     float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
     const float speed = fr->speed;
@@ -1178,8 +1188,42 @@ __global__ __launch_bounds__(kTB) void k_apply(const Params P, Frame *fr,
             a = fminf(a, smn[c][q]);
             b = fmaxf(b, smx[c][q]);
         }
-        if (finitef(a)) atomicMin(&fr->bbmin[c], f2ord(a));
-        if (finitef(b)) atomicMax(&fr->bbmax[c], f2ord(b));
+        // the block's extremes; k_bbox_final reduces them (one device-scope atomic
+        // per block and axis on one line cost about 9 us at 512 blocks)
+        bbpart[6 * blockIdx.x + c] = finitef(a) ? f2ord(a) : 0xFF800000u;      // f2ord(+inf)
+        bbpart[6 * blockIdx.x + 3 + c] = finitef(b) ? f2ord(b) : ~0xFF800000u; // f2ord(-inf)
+    }
+}
+
+// The bounding box of the positions k_apply wrote, from its blocks' extremes.
+__global__ __launch_bounds__(256) void k_bbox_final(const uint32_t *__restrict__ bbpart, uint32_t nblk,
+                                                    Frame *fr) {
+    __shared__ uint32_t red[6][4];
+    uint32_t v[6] = {0xFF800000u, 0xFF800000u, 0xFF800000u, ~0xFF800000u, ~0xFF800000u, ~0xFF800000u};
+    for (uint32_t q = threadIdx.x; q < nblk; q += blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            v[k] = min(v[k], bbpart[6 * q + k]);
+            v[3 + k] = max(v[3 + k], bbpart[6 * q + 3 + k]);
+        }
+    }
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const uint32_t u = __shfl_xor(v[k], o, 64);
+            v[k] = k < 3 ? min(v[k], u) : max(v[k], u);
+        }
+        if (lane == 0) red[k][w] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        uint32_t r = red[k][0];
+        for (uint32_t q = 1; q < 4; q++) r = k < 3 ? min(r, red[k][q]) : max(r, red[k][q]);
+        if (k < 3) fr->bbmin[k] = r;
+        else fr->bbmax[k - 3] = r;
     }
 }
 
@@ -1325,6 +1369,7 @@ class Fa2BhEngine final : public LayoutEngine {
         JC_TRY(nodes_.alloc(cap_));
         JC_TRY(node_depth_.alloc((cap_ + 31) & ~15ull));  // whole 16-byte loads in k_com_level
         JC_TRY(cskip_.alloc(cap_));
+        JC_TRY(bbpart_.alloc(6 * kApplyBlocks));
         // Cells that do not fit k_com_fused at one depth are disjoint and hold
         // more than kFuseCap nodes each.
         big_max_ = (uint32_t)std::min<uint64_t>((uint64_t)(kLevels + 1 - kFuseDepth) * (cap_ / kFuseCap + 1),
@@ -1525,9 +1570,7 @@ class Fa2BhEngine final : public LayoutEngine {
                     JC_TRY(enqueue_force(pos_[c].p, nullptr));
                     JC_HIP(hipEventRecord(tev_[2 * k + 1], stream_));
                     k_speed<<<1, kSpeedThreads, 0, stream_>>>(params_, frame_.p, partial_.p, top_slot_.p, 1u);
-                    k_apply<<<apply_grid(), kTB, 0, stream_>>>(params_, frame_.p, pos_[c].p,
-                                                                pos_[c ^ 1].p, force_.p,
-                                                                old_force_.p, deg_.p);
+                    enqueue_apply(c);
                     JC_HIP(hipGetLastError());
                     cur_ ^= 1;
                 }
@@ -1720,7 +1763,13 @@ class Fa2BhEngine final : public LayoutEngine {
     uint32_t com_grid() const {
         return std::max<uint32_t>(1, std::min<uint64_t>(cdiv(cap_, kFuseChunk), kComGrid));
     }
-    uint32_t apply_grid() const { return std::min<uint32_t>(cdiv(n_ ? n_ : 1, kTB), 512); }
+    uint32_t apply_grid() const { return std::min<uint32_t>(cdiv(n_ ? n_ : 1, kTB), kApplyBlocks); }
+    // k_apply from pos_[c] into pos_[c ^ 1], then the new positions' bounding box
+    void enqueue_apply(uint32_t c) {
+        k_apply<<<apply_grid(), kTB, 0, stream_>>>(params_, frame_.p, pos_[c].p, pos_[c ^ 1].p, force_.p,
+                                                    old_force_.p, deg_.p, bbpart_.p);
+        k_bbox_final<<<1, 256, 0, stream_>>>(bbpart_.p, apply_grid(), frame_.p);
+    }
 
     // The tree build of pos; the frame holds pos's bounding box and a reset
     // per-build state (enqueue_fresh, or k_speed + k_apply of the previous
@@ -1791,9 +1840,7 @@ class Fa2BhEngine final : public LayoutEngine {
             k_speed<<<1, kSpeedThreads, 0, stream_>>>(params_, frame_.p, partial_.p, top_slot_.p, 1u);
         if (ev) JC_HIP(hipEventRecord(ev[PH_APPLY], stream_));
         if (apply)
-            k_apply<<<apply_grid(), kTB, 0, stream_>>>(params_, frame_.p, pos_[c].p,
-                                                        pos_[c ^ 1].p, force_.p, old_force_.p,
-                                                        deg_.p);
+            enqueue_apply(c);
         if (ev) JC_HIP(hipEventRecord(ev[PH_N], stream_));
         JC_HIP(hipGetLastError());
         return Status::ok();
@@ -1911,6 +1958,7 @@ class Fa2BhEngine final : public LayoutEngine {
     RadixScratch sort_;
     DevBuf<Node> nodes_;
     DevBuf<uint8_t> node_depth_;
+    DevBuf<uint32_t> bbpart_;     // k_apply: each block's bounding-box extremes (6 words)
     DevBuf<uint32_t> cskip_;      // skip rope of each cell (k_emit), read by k_com_fused
     DevBuf<uint32_t> big_;        // cells too large for k_com_fused, summed by k_top_build
     uint32_t big_max_ = 0;
