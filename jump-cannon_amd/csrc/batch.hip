@@ -36,6 +36,13 @@ constexpr uint32_t kBScanTile = kBT * kBScanItems;
 constexpr uint32_t kBEmitTasks = 1024;    // cell tasks per k_b_emit block held in LDS
 constexpr uint32_t kBHeavyAtt = 32;       // attraction rows longer than this go to k_b_force_heavy
 constexpr uint32_t kBHeavyStage = 1024;   // heavy-row terms staged in LDS per round
+// Centre of mass: cells of depth >= kBFuseDepth in one launch (com_fused_pass),
+// the shallower ones one launch per depth.
+constexpr uint32_t kBFuseDepth = 3;
+constexpr uint32_t kBFuseChunk = 1024;
+constexpr uint32_t kBFuseCap = 512;
+constexpr uint32_t kBFuseThreads = 512;
+constexpr uint32_t kBComGrid = 2048;
 
 struct GFrame {
     uint32_t bbmin[3];
@@ -54,6 +61,7 @@ struct GFrame {
 struct BGlobal {
     uint32_t n_tree_total;
     uint32_t max_depth;
+    uint32_t n_big;  // cells too large for k_b_com_fused
 };
 
 struct BParams {
@@ -74,7 +82,10 @@ struct BParams {
 
 __global__ void k_b_reset(GFrame *fr, uint32_t G, BGlobal *gl) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g == 0) gl->max_depth = 0;
+    if (g == 0) {
+        gl->max_depth = 0;
+        gl->n_big = 0;
+    }
     if (g >= G) return;
     const uint32_t pinf = 0xFF800000u;
     for (int k = 0; k < 3; k++) {
@@ -219,8 +230,10 @@ __global__ __launch_bounds__(kBT) void k_b_count_scan(const uint64_t *__restrict
                                                       const uint32_t *__restrict__ count,
                                                       uint32_t n, const GFrame *fr,
                                                       uint32_t *__restrict__ loc,
-                                                      uint32_t *__restrict__ blk_tot) {
-    __shared__ uint32_t ws[kBT / 64];
+                                                      uint32_t *__restrict__ blk_tot,
+                                                      uint32_t *__restrict__ blk_depth) {
+    __shared__ uint32_t ws[kBT / 64], wd[kBT / 64];
+    int md = 0;  // deepest cell: the largest common prefix of two neighbours in one graph
     const uint32_t b0 = blockIdx.x * kBScanTile + threadIdx.x * kBScanItems;
     uint32_t c[kBScanItems];
     uint32_t s = 0;
@@ -250,6 +263,7 @@ __global__ __launch_bounds__(kBT) void k_b_count_scan(const uint64_t *__restrict
                     const int lp = i > sg.lo ? lcp_digits(kk[j], kk[j + 1]) : -1;
                     const int li = i + 1 < sg.ve ? lcp_digits(kk[j + 1], kk[j + 2]) : -1;
                     c[j] = (li > lp ? (uint32_t)(li - lp) : 0u) + 1u;
+                    md = li > md ? li : md;
                 }
             }
         }
@@ -262,13 +276,19 @@ __global__ __launch_bounds__(kBT) void k_b_count_scan(const uint64_t *__restrict
         uint32_t t = __shfl_up(incl, o, 64);
         if ((int)lane >= o) incl += t;
     }
-    if (lane == 63) ws[w] = incl;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) md = max(md, __shfl_xor(md, o, 64));
+    if (lane == 63) {
+        ws[w] = incl;
+        wd[w] = (uint32_t)md;
+    }
     __syncthreads();
-    uint32_t wp = 0, tot = 0;
+    uint32_t wp = 0, tot = 0, bd = 0;
 #pragma unroll
     for (uint32_t q = 0; q < kBT / 64; q++) {
         if (q < w) wp += ws[q];
         tot += ws[q];
+        bd = max(bd, wd[q]);
     }
     uint32_t run = incl - s + wp;
 #pragma unroll
@@ -276,16 +296,24 @@ __global__ __launch_bounds__(kBT) void k_b_count_scan(const uint64_t *__restrict
         if (b0 + j < n) loc[b0 + j] = run;
         run += c[j];
     }
-    if (threadIdx.x == 0) blk_tot[blockIdx.x] = tot;
+    if (threadIdx.x == 0) {
+        blk_tot[blockIdx.x] = tot;
+        blk_depth[blockIdx.x] = bd;  // k_b_scan_top takes the maximum
+    }
 }
 
 __global__ __launch_bounds__(1024) void k_b_scan_top(uint32_t *__restrict__ blk, uint32_t nb,
-                                                     BGlobal *gl) {
-    __shared__ uint32_t ws[16];
+                                                     BGlobal *gl, const uint32_t *__restrict__ blk_depth) {
+    __shared__ uint32_t ws[16], wd[16];
     const uint32_t per = (nb + 1023u) / 1024u;
     const uint32_t b0 = min(threadIdx.x * per, nb), b1 = min(b0 + per, nb);
-    uint32_t s = 0;
-    for (uint32_t i = b0; i < b1; i++) s += blk[i];
+    uint32_t s = 0, md = 0;
+    for (uint32_t i = b0; i < b1; i++) {
+        s += blk[i];
+        md = max(md, blk_depth[i]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) md = max(md, __shfl_xor(md, o, 64));
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t incl = s;
 #pragma unroll
@@ -293,12 +321,16 @@ __global__ __launch_bounds__(1024) void k_b_scan_top(uint32_t *__restrict__ blk,
         uint32_t t = __shfl_up(incl, o, 64);
         if ((int)lane >= o) incl += t;
     }
-    if (lane == 63) ws[w] = incl;
+    if (lane == 63) {
+        ws[w] = incl;
+        wd[w] = md;
+    }
     __syncthreads();
     uint32_t wp = 0, tot = 0;
     for (uint32_t q = 0; q < 16; q++) {
         if (q < w) wp += ws[q];
         tot += ws[q];
+        md = max(md, wd[q]);
     }
     uint32_t run = incl - s + wp;
     for (uint32_t i = b0; i < b1; i++) {
@@ -306,7 +338,10 @@ __global__ __launch_bounds__(1024) void k_b_scan_top(uint32_t *__restrict__ blk,
         blk[i] = run;
         run += v;
     }
-    if (threadIdx.x == 0) gl->n_tree_total = tot;
+    if (threadIdx.x == 0) {
+        gl->n_tree_total = tot;
+        gl->max_depth = md;
+    }
 }
 
 // First node written by sorted body i (= total for i == n).
@@ -335,13 +370,16 @@ __device__ __forceinline__ void b_emit_cell(const uint64_t *__restrict__ keys, c
                                             const uint32_t *__restrict__ loc,
                                             const uint32_t *__restrict__ blk,
                                             Node *__restrict__ nodes,
-                                            uint8_t *__restrict__ node_depth) {
+                                            uint8_t *__restrict__ node_depth,
+                                            uint32_t *__restrict__ cskip) {
     float h = fr[sg.g].root[3];
     for (int q = 0; q < d; q++) h = h * 0.5f;
     const uint32_t j = cell_end(keys, i, sg.ve, keys[i], d);
-    const uint32_t skip = (j + 1 < sg.ve) ? node_start(loc, blk, j + 1, n, gl) : kEnd;
+    const bool last = j + 1 >= sg.ve;  // the subtree ends with its graph: rope kEnd
+    const uint32_t end = node_start(loc, blk, last ? sg.ve : j + 1, n, gl);
     const float s = h * 2.0f;
-    nodes[node].b = cell_b(s * s, skip);
+    nodes[node].b = cell_b(s * s, last ? kEnd : end);
+    cskip[node] = end;  // where the subtree ends in the forest, for k_b_com_fused
     node_depth[node] = (uint8_t)d;
 }
 
@@ -353,14 +391,12 @@ __global__ __launch_bounds__(kBT) void k_b_emit(
     const uint32_t *__restrict__ gid, const uint32_t *__restrict__ base,
     const uint32_t *__restrict__ count, uint32_t n, const GFrame *fr, BGlobal *gl,
     const uint32_t *__restrict__ loc, const uint32_t *__restrict__ blk,
-    Node *__restrict__ nodes, uint8_t *__restrict__ node_depth, uint64_t cap) {
-    __shared__ uint32_t bdepth, ntask;
+    Node *__restrict__ nodes, uint8_t *__restrict__ node_depth, uint64_t cap,
+    uint32_t *__restrict__ cskip) {
+    __shared__ uint32_t ntask;
     __shared__ uint32_t t_node[kBEmitTasks], t_body[kBEmitTasks];
     __shared__ uint8_t t_depth[kBEmitTasks];
-    if (threadIdx.x == 0) {
-        bdepth = 0;
-        ntask = 0;
-    }
+    if (threadIdx.x == 0) ntask = 0;
     __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     // the neighbours' keys come from the neighbouring lanes; the wave's first
@@ -396,10 +432,9 @@ __global__ __launch_bounds__(kBT) void k_b_emit(
                             t_depth[slot + t] = (uint8_t)(a0 + (int)t);
                         } else {
                             b_emit_cell(keys, sg, i, a0 + (int)t, pos0 + t, n, fr, gl, loc, blk,
-                                        nodes, node_depth);
+                                        nodes, node_depth, cskip);
                         }
                     }
-                    atomicMax(&bdepth, (uint32_t)(a0 + (int)c - 1));
                 }
                 const uint32_t leaf = pos0 + c;
                 const uint32_t b = perm[i];
@@ -417,9 +452,28 @@ __global__ __launch_bounds__(kBT) void k_b_emit(
     for (uint32_t q = threadIdx.x; q < nt; q += blockDim.x) {
         const uint32_t ib = t_body[q];
         const Seg sg = seg_of(ib, gid, base, count, fr);
-        b_emit_cell(keys, sg, ib, t_depth[q], t_node[q], n, fr, gl, loc, blk, nodes, node_depth);
+        b_emit_cell(keys, sg, ib, t_depth[q], t_node[q], n, fr, gl, loc, blk, nodes, node_depth, cskip);
     }
-    if (threadIdx.x == 0 && bdepth) atomicMax(&gl->max_depth, bdepth);
+}
+
+// The cells of depth >= kBFuseDepth over the whole forest: each fitting
+// subtree summed in LDS by one workgroup (com_fused_pass), then the ones too
+// large for it (com_big_pass).
+__global__ __launch_bounds__(kBFuseThreads) void k_b_com_fused(BGlobal *gl, uint32_t cap,
+                                                               Node *__restrict__ nodes,
+                                                               const uint8_t *__restrict__ node_depth,
+                                                               const uint32_t *__restrict__ cskip,
+                                                               uint32_t *__restrict__ big, uint32_t big_max) {
+    com_fused_pass<kBFuseDepth, kBFuseChunk, kBFuseCap, kBFuseThreads>(
+        min(gl->n_tree_total, cap), nodes, node_depth, cskip, big, big_max, &gl->n_big);
+}
+
+__global__ __launch_bounds__(1024) void k_b_com_big(const BGlobal *gl, uint32_t cap, Node *__restrict__ nodes,
+                                                    const uint8_t *__restrict__ node_depth,
+                                                    const uint32_t *__restrict__ big, uint32_t big_max) {
+    const uint32_t nbig = min(gl->n_big, big_max);
+    if (!nbig) return;
+    com_big_pass<kBFuseDepth>(min(gl->n_tree_total, cap), gl->max_depth, nodes, node_depth, big, nbig);
 }
 
 // The cells of depth d over the whole forest (com_level_pass).
@@ -823,6 +877,7 @@ class Fa2BhBatch {
         nb_scan_ = cdivb(nn, kBScanTile);
         JC_TRY(loc_.alloc(nn));
         JC_TRY(blk_.alloc(nb_scan_ + 1));
+        JC_TRY(blkd_.alloc(nb_scan_ + 1));
         // per graph: n_g leaves + at most n_g/2 cells per depth over 22 depths + 1
         cap_ = 0;
         for (uint32_t g = 0; g < G; g++) cap_ += (uint64_t)count[g] + (uint64_t)(kLevels + 1) * (count[g] / 2 + 1) + 1;
@@ -836,6 +891,12 @@ class Fa2BhBatch {
         JC_TRY(hadj_.alloc(n_hent_ ? n_hent_ : 1));
         JC_TRY(hterm_.alloc(3 * (size_t)(hstride_ ? hstride_ : 4)));
         JC_TRY(node_depth_.alloc((cap_ + 31) & ~15ull));  // whole 16-byte loads in the COM pass
+        JC_TRY(cskip_.alloc(cap_));
+        // Cells that do not fit k_b_com_fused at one depth are disjoint and hold
+        // more than kBFuseCap nodes each.
+        big_max_ = (uint32_t)std::min<uint64_t>((uint64_t)(kLevels + 1 - kBFuseDepth) * (cap_ / kBFuseCap + 1),
+                                                0xFFFFFFFFull);
+        JC_TRY(big_.alloc(big_max_));
         JC_TRY(frame_.alloc(G));
         JC_TRY(global_.alloc(1));
         std::vector<float4> p4(nn, make_float4(0, 0, 0, 0));
@@ -971,16 +1032,21 @@ class Fa2BhBatch {
             k_b_gather_keys<<<cdivb(n, 256), 256, 0, stream_>>>(vals_.p, keys_raw_.p, keys_.p, n);
         }
         k_b_count_scan<<<nb_scan_, kBT, 0, stream_>>>(keys_.p, gid_.p, base_.p, count_.p, n,
-                                                      frame_.p, loc_.p, blk_.p);
-        k_b_scan_top<<<1, 1024, 0, stream_>>>(blk_.p, nb_scan_, global_.p);
+                                                      frame_.p, loc_.p, blk_.p, blkd_.p);
+        k_b_scan_top<<<1, 1024, 0, stream_>>>(blk_.p, nb_scan_, global_.p, blkd_.p);
         k_b_graph_tree<<<cdivb(G_, 256), 256, 0, stream_>>>(base_.p, count_.p, G_, n, loc_.p,
                                                             blk_.p, global_.p, frame_.p);
         k_b_emit<<<cdivb(n, kBT), kBT, 0, stream_>>>(keys_.p, vals_.p, pos, deg_.p, gid_.p,
                                                      base_.p, count_.p, n, frame_.p, global_.p,
                                                      loc_.p, blk_.p, nodes_.p, node_depth_.p,
-                                                     cap_);
+                                                     cap_, cskip_.p);
+        const uint32_t gfuse = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cdivb(cap_, kBFuseChunk), kBComGrid));
+        k_b_com_fused<<<gfuse, kBFuseThreads, 0, stream_>>>(global_.p, (uint32_t)cap_, nodes_.p, node_depth_.p,
+                                                             cskip_.p, big_.p, big_max_);
+        k_b_com_big<<<1, 1024, 0, stream_>>>(global_.p, (uint32_t)cap_, nodes_.p, node_depth_.p, big_.p,
+                                             big_max_);
         const uint32_t gcom = std::min<uint32_t>(cdivb(cap_, kComSpan * (kBT / 64)), 1024);
-        for (int d = kLevels; d >= 0; d--)
+        for (int d = (int)kBFuseDepth - 1; d >= 0; d--)
             k_b_com_level<<<gcom, kBT, 0, stream_>>>(global_.p, (uint32_t)d, (uint32_t)cap_,
                                                      nodes_.p, node_depth_.p);
         JC_HIP(hipGetLastError());
@@ -1106,6 +1172,10 @@ class Fa2BhBatch {
     DevBuf<float4> pos_[2], old_force_, force_;
     DevBuf<float2> stats_;
     DevBuf<uint32_t> deg_, gid_, base_, count_, att_off_, att_adj_, vals_, loc_, blk_;
+    DevBuf<uint32_t> blkd_;       // deepest cell of each count/scan tile
+    DevBuf<uint32_t> cskip_;      // end of each cell's subtree (k_b_emit), read by k_b_com_fused
+    DevBuf<uint32_t> big_;        // cells too large for k_b_com_fused, summed by k_b_com_big
+    uint32_t big_max_ = 0;
     DevBuf<double> est_, mjt_;
     DevBuf<uint64_t> keys_raw_, keys_;
     RadixScratch sort_;

@@ -469,6 +469,183 @@ __device__ __forceinline__ void com_level_pass(Node *__restrict__ nodes,
     }
 }
 
+// ---- centre of mass of the deep cells, one launch ----
+// The cells of depth >= Depth.  Such a cell "fits" when its subtree, the
+// pre-order range [x, end(x)), holds at most Cap nodes; cskip[x] is end(x), the
+// first node after the subtree (kEnd: the end of the node array).  A fitting
+// cell with no fitting ancestor is a root: one workgroup sums the root's whole
+// subtree in LDS, deepest level first with a barrier between levels, each cell
+// from its children in octant (pre-order) order: m = sum m_c;
+// com = sum(com_c * m_c) / m (fa2_bh.rs:944-965), the same operations as
+// com_of_cell, so the bits are those of one launch per depth.
+//
+// Ownership: the workgroup of the chunk [c0, c0 + Chunk) takes the roots that
+// start in it.  A fitting ancestor a of x satisfies x - Cap <= a < x and
+// end(a) > x, so x has one iff the prefix maximum of the fitting cells' ends
+// over the window [x - Cap, x) passes x.  The roots of a chunk span fewer than
+// Chunk + Cap nodes.  Between two roots lie only leaves, shallower cells and
+// cells that do not fit; every fitting cell in the span is inside a root.
+// Cells that do not fit (fewer than one per Cap nodes and depth) are appended
+// to `big` for com_big_pass.
+__device__ __forceinline__ uint32_t fit_end(const uint32_t *__restrict__ cskip, uint32_t x,
+                                            uint32_t N) {
+    const uint32_t sk = cskip[x];
+    return sk < N ? sk : N;
+}
+
+template <uint32_t Depth, uint32_t Chunk, uint32_t Cap, uint32_t Threads>
+__device__ __forceinline__ void com_fused_pass(uint32_t N, Node *__restrict__ nodes,
+                                               const uint8_t *__restrict__ node_depth,
+                                               const uint32_t *__restrict__ cskip,
+                                               uint32_t *__restrict__ big, uint32_t big_max,
+                                               uint32_t *n_big) {
+    constexpr uint32_t Span = Chunk + Cap;
+    constexpr uint32_t Per = Span / Threads;
+    static_assert(Depth >= 1, "the level loop counts down to Depth");
+    static_assert(Span % Threads == 0, "whole segments per thread");
+    static_assert(Span <= 65535, "16-bit ropes in LDS");
+    __shared__ float4 ca[Span];  // the window's skips first, then {com, mass} of the span
+    __shared__ uint32_t meta[Span];  // depth byte << 16 | rope relative to the span (fitting cells)
+    __shared__ uint32_t wmax[Threads / 64];
+    __shared__ uint32_t s_lo, s_hi, s_dmax;
+    uint32_t *win = reinterpret_cast<uint32_t *>(ca);
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (uint32_t c0 = blockIdx.x * Chunk; c0 < N; c0 += gridDim.x * Chunk) {
+        const uint32_t c1 = min(c0 + Chunk, N);
+        const uint32_t lo = c0 > Cap ? c0 - Cap : 0u;
+        const uint32_t W = c1 - lo;
+        if (t == 0) {
+            s_lo = 0xFFFFFFFFu;
+            s_hi = 0;
+            s_dmax = 0;
+        }
+        // 1. the window's values: the skip of a fitting cell, 0 elsewhere; the
+        //    chunk's cells that do not fit join the big list
+        for (uint32_t q = t; q < W; q += Threads) {
+            const uint32_t x = lo + q;
+            const uint32_t d = node_depth[x];
+            uint32_t v = 0;
+            if (d < 0x80u && d >= Depth) {
+                const uint32_t end = fit_end(cskip, x, N);
+                if (end - x <= Cap) {
+                    v = end;
+                } else if (x >= c0) {
+                    const uint32_t slot = atomicAdd(n_big, 1u);
+                    if (slot < big_max) big[slot] = x;
+                }
+            }
+            win[q] = v;
+        }
+        __syncthreads();
+        // 2. exclusive prefix maximum over the window (a segment of Per
+        //    entries per thread, then across the lanes and the waves); a fitting
+        //    cell of the chunk whose prefix maximum does not pass it is a root
+        uint32_t seg = 0;
+        for (uint32_t j = 0; j < Per; j++) {
+            const uint32_t q = t * Per + j;
+            if (q < W) seg = max(seg, win[q]);
+        }
+        uint32_t incl = seg;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(incl, o, 64);
+            if ((int)lane >= o) incl = max(incl, u);
+        }
+        if (lane == 63) wmax[w] = incl;
+        __syncthreads();
+        uint32_t run = __shfl_up(incl, 1, 64);
+        if (lane == 0) run = 0;
+        for (uint32_t q = 0; q < w; q++) run = max(run, wmax[q]);
+        uint32_t rlo = 0xFFFFFFFFu, rhi = 0;
+        for (uint32_t j = 0; j < Per; j++) {
+            const uint32_t q = t * Per + j;
+            if (q < W) {
+                const uint32_t v = win[q], x = lo + q;
+                if (v && x >= c0 && run <= x) {
+                    rlo = min(rlo, x);
+                    rhi = max(rhi, v);
+                }
+                run = max(run, v);
+            }
+        }
+        if (rhi) {
+            atomicMin(&s_lo, rlo);
+            atomicMax(&s_hi, rhi);
+        }
+        __syncthreads();
+        const uint32_t r0 = s_lo, r1 = s_hi;
+        if (r1 == 0) {  // no root starts in this chunk (the same answer for every thread)
+            __syncthreads();
+            continue;
+        }
+        const uint32_t R = r1 - r0;  // < Span
+        // 3. the span: depths, ropes of the fitting cells relative to r0, {com, mass}
+        uint32_t dmax = 0;
+        for (uint32_t q = t; q < R; q += Threads) {
+            const uint32_t y = r0 + q;
+            const uint32_t d = node_depth[y];
+            uint32_t r = 0;
+            if (d < 0x80u && d >= Depth) {
+                const uint32_t end = fit_end(cskip, y, N);
+                if (end - y <= Cap) {
+                    r = end - r0;
+                    dmax = max(dmax, d);
+                }
+            }
+            meta[q] = d << 16 | r;
+            ca[q] = nodes[y].a;
+        }
+        if (dmax) atomicMax(&s_dmax, dmax);
+        __syncthreads();
+        // 4. deepest level first; a fitting cell's children are fitting cells or leaves
+        for (uint32_t d = s_dmax; d >= Depth; d--) {
+            for (uint32_t q = t; q < R; q += Threads) {
+                const uint32_t mq = meta[q];
+                if ((mq >> 16) != d || !(mq & 0xFFFFu)) continue;
+                const uint32_t end = mq & 0xFFFFu;
+                float tm = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+                for (uint32_t ch = q + 1; ch < end;) {
+                    const float4 a = ca[ch];
+                    const uint32_t m = meta[ch];
+                    tm = tm + a.w;
+                    cx = cx + a.x * a.w;
+                    cy = cy + a.y * a.w;
+                    cz = cz + a.z * a.w;
+                    const uint32_t nx = (m & 0x800000u) ? ch + 1 : (m & 0xFFFFu);
+                    ch = nx > ch ? nx : end;  // ropes only move forward
+                }
+                if (tm > 0.0f) {
+                    cx = cx / tm;
+                    cy = cy / tm;
+                    cz = cz / tm;
+                }
+                ca[q] = make_float4(cx, cy, cz, tm);
+            }
+            __syncthreads();
+        }
+        for (uint32_t q = t; q < R; q += Threads)
+            if (meta[q] & 0xFFFFu) nodes[r0 + q].a = ca[q];
+        __syncthreads();  // the LDS is reused by the next chunk
+    }
+}
+
+// The cells of depth >= Depth that com_fused_pass left (subtrees larger than its
+// Cap), deepest first, in one workgroup; their children are final: roots summed
+// by com_fused_pass, leaves, or big cells of a deeper level done here.
+template <uint32_t Depth>
+__device__ __forceinline__ void com_big_pass(uint32_t N, uint32_t max_depth, Node *__restrict__ nodes,
+                                             const uint8_t *__restrict__ node_depth,
+                                             const uint32_t *__restrict__ big, uint32_t nbig) {
+    static_assert(Depth >= 1, "the level loop counts down to Depth");
+    for (uint32_t d = min(max_depth, (uint32_t)kLevels); d >= Depth; d--) {
+        for (uint32_t q = threadIdx.x; q < nbig; q += blockDim.x) {
+            const uint32_t x = big[q];
+            if (node_depth[x] == d) com_of_cell(nodes, x, N);
+        }
+        __syncthreads();
+    }
+}
+
 // Octant path of p through the reference cell chain for kLevels levels: digit
 // (x>=cx) | (y>=cy)<<1 | (z>=cz)<<2, child half = half*0.5, child centre =
 // centre +- child half (fa2_bh.rs:841-861).

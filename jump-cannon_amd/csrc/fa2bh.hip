@@ -192,8 +192,9 @@ __global__ __launch_bounds__(kTB) void k_keys(const float4 *__restrict__ pos, ui
 // leading octal digits of neighbouring keys (-1 past either end); a lone body
 // opens the root.  Block-local exclusive scan of c, plus the block total.
 __global__ __launch_bounds__(kTB) void k_count_scan(const uint64_t *__restrict__ keys, uint32_t n,
-                                                    Frame *fr, uint32_t *__restrict__ loc,
-                                                    uint32_t *__restrict__ blk_tot) {
+                                                    const Frame *fr, uint32_t *__restrict__ loc,
+                                                    uint32_t *__restrict__ blk_tot,
+                                                    uint32_t *__restrict__ blk_depth) {
     __shared__ uint32_t ws[kTB / 64], wd[kTB / 64];
     const uint32_t nv = n - fr->n_invalid;
     const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
@@ -246,9 +247,9 @@ __global__ __launch_bounds__(kTB) void k_count_scan(const uint64_t *__restrict__
         tot += ws[q];
         bd = max(bd, wd[q]);
     }
-    // one device-scope atomic per 4096-body tile (k_emit's per-block atomic on
-    // this word cost about 10 us at 1M bodies)
-    if (threadIdx.x == 0 && bd) atomicMax(&fr->max_depth, bd);
+    // the tile's deepest cell; k_scan_top takes the maximum (a per-block
+    // device-scope atomic on one word in k_emit cost about 10 us at 1M bodies)
+    if (threadIdx.x == 0) blk_depth[blockIdx.x] = bd;
     uint32_t run = incl - s + wp;
 #pragma unroll
     for (uint32_t j = 0; j < kScanItems; j++) {
@@ -259,12 +260,18 @@ __global__ __launch_bounds__(kTB) void k_count_scan(const uint64_t *__restrict__
 }
 
 __global__ __launch_bounds__(1024) void k_scan_top(uint32_t *__restrict__ blk, uint32_t nb,
-                                                   uint32_t n, Frame *fr) {
-    __shared__ uint32_t ws[16];
+                                                   uint32_t n, Frame *fr,
+                                                   const uint32_t *__restrict__ blk_depth) {
+    __shared__ uint32_t ws[16], wd[16];
     const uint32_t per = (nb + 1023u) / 1024u;
     const uint32_t b0 = min(threadIdx.x * per, nb), b1 = min(b0 + per, nb);
-    uint32_t s = 0;
-    for (uint32_t i = b0; i < b1; i++) s += blk[i];
+    uint32_t s = 0, md = 0;
+    for (uint32_t i = b0; i < b1; i++) {
+        s += blk[i];
+        md = max(md, blk_depth[i]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) md = max(md, __shfl_xor(md, o, 64));
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t incl = s;
 #pragma unroll
@@ -272,12 +279,16 @@ __global__ __launch_bounds__(1024) void k_scan_top(uint32_t *__restrict__ blk, u
         uint32_t t = __shfl_up(incl, o, 64);
         if ((int)lane >= o) incl += t;
     }
-    if (lane == 63) ws[w] = incl;
+    if (lane == 63) {
+        ws[w] = incl;
+        wd[w] = md;
+    }
     __syncthreads();
     uint32_t wp = 0, tot = 0;
     for (uint32_t q = 0; q < 16; q++) {
         if (q < w) wp += ws[q];
         tot += ws[q];
+        md = max(md, wd[q]);
     }
     uint32_t run = incl - s + wp;
     for (uint32_t i = b0; i < b1; i++) {
@@ -286,6 +297,7 @@ __global__ __launch_bounds__(1024) void k_scan_top(uint32_t *__restrict__ blk, u
         run += v;
     }
     if (threadIdx.x == 0) {
+        fr->max_depth = md;
         const uint32_t nv = n - fr->n_invalid;
         fr->n_tree = nv ? tot + nv : 0u;
         fr->n_valid = nv;
@@ -406,185 +418,30 @@ __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
     }
 }
 
-// Centre of mass of the cells of depth >= kFuseDepth, one launch.  Such a cell
-// "fits" when its subtree, the pre-order range [x, skip), holds at most
-// kFuseCap nodes.  A fitting cell with no fitting ancestor is a root: one
-// workgroup sums the root's whole subtree in LDS, deepest level first with a
-// barrier between levels, each cell from its children in octant (pre-order)
-// order: m = sum m_c; com = sum(com_c * m_c) / m (fa2_bh.rs:944-965), the same
-// operations as com_of_cell, so the bits are those of one launch per depth.
-//
-// Ownership: the workgroup of the chunk [c0, c0 + kFuseChunk) takes the roots
-// that start in it.  A fitting ancestor a of x satisfies x - kFuseCap <= a < x
-// and skip(a) > x, so x has one iff the prefix maximum of the fitting cells'
-// skips over the window [x - kFuseCap, x) passes x.  The roots of a chunk span
-// fewer than kFuseChunk + kFuseCap nodes.  Between two roots lie only leaves,
-// shallower cells and cells that do not fit; every fitting cell in the span is
-// inside a root.  Cells that do not fit (fewer than one per kFuseCap nodes and
-// depth) go to `big`, which k_com_big sums next.  The few wide cells above
+// Centre of mass of the cells of depth >= kFuseDepth in one launch
+// (com_fused_pass: each fitting subtree summed in LDS by one workgroup), the
+// cells too large for it in one more (com_big_pass); the few wide cells above
 // kFuseDepth take one k_com_level launch per depth, the top tree k_top_build.
 constexpr uint32_t kFuseDepth = 6;
 constexpr uint32_t kFuseChunk = 1024;
 constexpr uint32_t kFuseCap = 512;
-constexpr uint32_t kFuseSpan = kFuseChunk + kFuseCap;
 constexpr uint32_t kFuseThreads = 512;
-constexpr uint32_t kFusePer = kFuseSpan / kFuseThreads;
-static_assert(kFuseSpan % kFuseThreads == 0, "whole segments per thread");
-static_assert(kFuseSpan <= 65535, "16-bit ropes in LDS");
-
-__device__ __forceinline__ uint32_t fit_end(const uint32_t *__restrict__ cskip, uint32_t x,
-                                            uint32_t N) {
-    const uint32_t sk = cskip[x];
-    return sk < N ? sk : N;  // kEnd: the subtree runs to the end of the tree
-}
 
 __global__ __launch_bounds__(kFuseThreads) void k_com_fused(Frame *fr, uint32_t cap,
                                                             Node *__restrict__ nodes,
                                                             const uint8_t *__restrict__ node_depth,
                                                             const uint32_t *__restrict__ cskip,
                                                             uint32_t *__restrict__ big, uint32_t big_max) {
-    __shared__ float4 ca[kFuseSpan];  // the window's skips first, then {com, mass} of the span
-    __shared__ uint32_t meta[kFuseSpan];  // depth byte << 16 | rope relative to the span (fitting cells)
-    __shared__ uint32_t wmax[kFuseThreads / 64];
-    __shared__ uint32_t s_lo, s_hi, s_dmax;
-    uint32_t *win = reinterpret_cast<uint32_t *>(ca);
-    const uint32_t N = min(fr->n_tree, cap);
-    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
-    for (uint32_t c0 = blockIdx.x * kFuseChunk; c0 < N; c0 += gridDim.x * kFuseChunk) {
-        const uint32_t c1 = min(c0 + kFuseChunk, N);
-        const uint32_t lo = c0 > kFuseCap ? c0 - kFuseCap : 0u;
-        const uint32_t W = c1 - lo;
-        if (t == 0) {
-            s_lo = 0xFFFFFFFFu;
-            s_hi = 0;
-            s_dmax = 0;
-        }
-        // 1. the window's values: the skip of a fitting cell, 0 elsewhere; the
-        //    chunk's cells that do not fit join the big list
-        for (uint32_t q = t; q < W; q += kFuseThreads) {
-            const uint32_t x = lo + q;
-            const uint32_t d = node_depth[x];
-            uint32_t v = 0;
-            if (d < 0x80u && d >= kFuseDepth) {
-                const uint32_t end = fit_end(cskip, x, N);
-                if (end - x <= kFuseCap) {
-                    v = end;
-                } else if (x >= c0) {
-                    const uint32_t slot = atomicAdd(&fr->n_big, 1u);
-                    if (slot < big_max) big[slot] = x;
-                }
-            }
-            win[q] = v;
-        }
-        __syncthreads();
-        // 2. exclusive prefix maximum over the window (a segment of kFusePer
-        //    entries per thread, then across the lanes and the waves); a fitting
-        //    cell of the chunk whose prefix maximum does not pass it is a root
-        uint32_t seg = 0;
-        for (uint32_t j = 0; j < kFusePer; j++) {
-            const uint32_t q = t * kFusePer + j;
-            if (q < W) seg = max(seg, win[q]);
-        }
-        uint32_t incl = seg;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t u = __shfl_up(incl, o, 64);
-            if ((int)lane >= o) incl = max(incl, u);
-        }
-        if (lane == 63) wmax[w] = incl;
-        __syncthreads();
-        uint32_t run = __shfl_up(incl, 1, 64);
-        if (lane == 0) run = 0;
-        for (uint32_t q = 0; q < w; q++) run = max(run, wmax[q]);
-        uint32_t rlo = 0xFFFFFFFFu, rhi = 0;
-        for (uint32_t j = 0; j < kFusePer; j++) {
-            const uint32_t q = t * kFusePer + j;
-            if (q < W) {
-                const uint32_t v = win[q], x = lo + q;
-                if (v && x >= c0 && run <= x) {
-                    rlo = min(rlo, x);
-                    rhi = max(rhi, v);
-                }
-                run = max(run, v);
-            }
-        }
-        if (rhi) {
-            atomicMin(&s_lo, rlo);
-            atomicMax(&s_hi, rhi);
-        }
-        __syncthreads();
-        const uint32_t r0 = s_lo, r1 = s_hi;
-        if (r1 == 0) {  // no root starts in this chunk (the same answer for every thread)
-            __syncthreads();
-            continue;
-        }
-        const uint32_t R = r1 - r0;  // < kFuseSpan
-        // 3. the span: depths, ropes of the fitting cells relative to r0, {com, mass}
-        uint32_t dmax = 0;
-        for (uint32_t q = t; q < R; q += kFuseThreads) {
-            const uint32_t y = r0 + q;
-            const uint32_t d = node_depth[y];
-            uint32_t r = 0;
-            if (d < 0x80u && d >= kFuseDepth) {
-                const uint32_t end = fit_end(cskip, y, N);
-                if (end - y <= kFuseCap) {
-                    r = end - r0;
-                    dmax = max(dmax, d);
-                }
-            }
-            meta[q] = d << 16 | r;
-            ca[q] = nodes[y].a;
-        }
-        if (dmax) atomicMax(&s_dmax, dmax);
-        __syncthreads();
-        // 4. deepest level first; a fitting cell's children are fitting cells or leaves
-        for (uint32_t d = s_dmax; d >= kFuseDepth; d--) {
-            for (uint32_t q = t; q < R; q += kFuseThreads) {
-                const uint32_t mq = meta[q];
-                if ((mq >> 16) != d || !(mq & 0xFFFFu)) continue;
-                const uint32_t end = mq & 0xFFFFu;
-                float tm = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
-                for (uint32_t ch = q + 1; ch < end;) {
-                    const float4 a = ca[ch];
-                    const uint32_t m = meta[ch];
-                    tm = tm + a.w;
-                    cx = cx + a.x * a.w;
-                    cy = cy + a.y * a.w;
-                    cz = cz + a.z * a.w;
-                    const uint32_t nx = (m & 0x800000u) ? ch + 1 : (m & 0xFFFFu);
-                    ch = nx > ch ? nx : end;  // ropes only move forward
-                }
-                if (tm > 0.0f) {
-                    cx = cx / tm;
-                    cy = cy / tm;
-                    cz = cz / tm;
-                }
-                ca[q] = make_float4(cx, cy, cz, tm);
-            }
-            __syncthreads();
-        }
-        for (uint32_t q = t; q < R; q += kFuseThreads)
-            if (meta[q] & 0xFFFFu) nodes[r0 + q].a = ca[q];
-        __syncthreads();  // the LDS is reused by the next chunk
-    }
+    com_fused_pass<kFuseDepth, kFuseChunk, kFuseCap, kFuseThreads>(min(fr->n_tree, cap), nodes, node_depth,
+                                                                   cskip, big, big_max, &fr->n_big);
 }
 
-// The cells of depth >= kFuseDepth that k_com_fused left (subtrees larger than
-// kFuseCap), deepest first, in one workgroup; their children are final: roots
-// summed by k_com_fused, leaves, or big cells of a deeper level done here.
 __global__ __launch_bounds__(1024) void k_com_big(const Frame *fr, uint32_t cap, Node *__restrict__ nodes,
                                                   const uint8_t *__restrict__ node_depth,
                                                   const uint32_t *__restrict__ big, uint32_t big_max) {
     const uint32_t nbig = min(fr->n_big, big_max);
     if (!nbig) return;
-    const uint32_t N = min(fr->n_tree, cap);
-    for (uint32_t d = min(fr->max_depth, (uint32_t)kLevels); d >= kFuseDepth; d--) {
-        for (uint32_t q = threadIdx.x; q < nbig; q += blockDim.x) {
-            const uint32_t x = big[q];
-            if (node_depth[x] == d) com_of_cell(nodes, x, N);
-        }
-        __syncthreads();
-    }
+    com_big_pass<kFuseDepth>(min(fr->n_tree, cap), fr->max_depth, nodes, node_depth, big, nbig);
 }
 
 // The cells of depth d < kFuseDepth, one launch per depth (com_level_pass,
@@ -1363,6 +1220,7 @@ class Fa2BhEngine final : public LayoutEngine {
         nb_scan_ = cdiv(nn, kScanTile);
         JC_TRY(loc_.alloc(nn));
         JC_TRY(blk_.alloc(nb_scan_ + 1));
+        JC_TRY(blkd_.alloc(nb_scan_ + 1));
         // Internal cells at one depth are disjoint and hold >= 2 bodies each:
         // at most n/2 per depth over 22 depths, plus the n leaves.
         cap_ = nn + (uint64_t)(kLevels + 1) * (nn / 2 + 1) + 8;
@@ -1782,8 +1640,8 @@ class Fa2BhEngine final : public LayoutEngine {
         if (ev) JC_HIP(hipEventRecord(ev[PH_SORT], stream_));
         JC_TRY(sort_morton_pairs(keys_.p, vals_.p, sort_, stream_));
         if (ev) JC_HIP(hipEventRecord(ev[PH_SCAN], stream_));
-        k_count_scan<<<nb_scan_, kTB, 0, stream_>>>(keys_.p, n, frame_.p, loc_.p, blk_.p);
-        k_scan_top<<<1, 1024, 0, stream_>>>(blk_.p, nb_scan_, n, frame_.p);
+        k_count_scan<<<nb_scan_, kTB, 0, stream_>>>(keys_.p, n, frame_.p, loc_.p, blk_.p, blkd_.p);
+        k_scan_top<<<1, 1024, 0, stream_>>>(blk_.p, nb_scan_, n, frame_.p, blkd_.p);
         if (ev) JC_HIP(hipEventRecord(ev[PH_EMIT], stream_));
         k_emit<<<cdiv(n, kTB), kTB, 0, stream_>>>(keys_.p, vals_.p, pos, deg_.p, n, frame_.p,
                                                   loc_.p, blk_.p, nodes_.p, node_depth_.p, cap_,
@@ -1955,6 +1813,7 @@ class Fa2BhEngine final : public LayoutEngine {
     DevBuf<uint32_t> deg_, att_off_, att_adj_;
     DevBuf<uint64_t> keys_;
     DevBuf<uint32_t> vals_, loc_, blk_;
+    DevBuf<uint32_t> blkd_;       // deepest cell of each count/scan tile
     RadixScratch sort_;
     DevBuf<Node> nodes_;
     DevBuf<uint8_t> node_depth_;
