@@ -68,17 +68,20 @@ def test_batch_single_graph_and_reuse():
 
 
 def test_batch_4096_sampled_graphs_equal_standalone():
-    """The 4,096-graph workload (log-uniform 50..20,000, half SBM, half BA): run
-    the whole batch, then check a spread of graphs against standalone runs."""
+    """The 4,096-graph workload (log-uniform 50..20,000, half SBM, half BA) for
+    its configured 500 iterations: run the whole batch, then check a spread of
+    graphs (smallest, largest, both ends, the middle, six seeded picks) against
+    standalone runs of the same 500 iterations, bit for bit."""
     graphs, pos = workloads.batch_4096()
     b = jc.Fa2BhBatch(theta=1.0)
     b.init(graphs, pos)
-    out = b.step(20)
+    out = b.step(500)
     sizes = np.array([len(o) - 1 for o, _ in graphs])
     pick = [int(np.argmin(sizes)), int(np.argmax(sizes)), 0, 1, 2047, 4095]
+    pick += [int(g) for g in np.random.default_rng(4096).choice(4096, 6, replace=False)]
     for g in pick:
         off, nb = graphs[g]
         e = jc.Fa2BhEngine(theta=1.0)
         e.init(off, nb, pos[g])
-        assert np.array_equal(out[g].view(np.uint32), e.step(20).view(np.uint32)), f"graph {g}"
+        assert np.array_equal(out[g].view(np.uint32), e.step(500).view(np.uint32)), f"graph {g}"
     assert all(np.isfinite(x).all() for x in out)
