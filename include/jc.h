@@ -371,6 +371,10 @@ void jc_gen_uniform_cube(uint32_t n, float half, uint64_t seed, float *xyz);
 /* ball_seed(n, seed) (stability.rs:30-50): the renderer's seed, a uniform random
  * ball of radius sqrt(n)*5 by rejection sampling (xyz: 3n). */
 void jc_gen_ball_seed(uint32_t n, uint64_t seed, float *xyz);
+/* ring_seed(n) (sim.rs:371-384): the worker's deterministic seed for a graph
+ * loaded without positions, 3n draws of a 32-bit LCG mapped to [-1, 1] and
+ * scaled by sqrt(n)*5 (xyz: 3n). */
+void jc_gen_ring_seed(uint32_t n, float *xyz);
 
 /* ---- layout telemetry: StepStats (crates/graph-compute/src/stability.rs:128-186) ---- */
 typedef struct {

@@ -710,6 +710,20 @@ void jc_gen_ball_seed(uint32_t n, uint64_t seed, float *xyz) {
     }
 }
 
+// The gRPC worker's deterministic seed when a graph arrives without positions
+// (ring_seed, sim.rs:371-384): a 32-bit LCG (s = s * 1664525 + 1013904223 from
+// 0x9E3779B1), each draw mapped to f32 (s / 2^32) * 2 - 1 and scaled by
+// sqrt(n) * 5, filling x, y, z of each node in turn.
+void jc_gen_ring_seed(uint32_t n, float *xyz) {
+    uint32_t s = 0x9E3779B1u;
+    const float radius = std::sqrt((float)(n > 1 ? n : 1)) * 5.0f;
+    const float umax = (float)0xFFFFFFFFu;  // rounds to 2^32, as the f32 cast of u32::MAX does
+    for (uint64_t k = 0; k < 3 * (uint64_t)n; k++) {
+        s = s * 1664525u + 1013904223u;
+        xyz[k] = ((float)s / umax * 2.0f - 1.0f) * radius;
+    }
+}
+
 // StepStats::measure (stability.rs:128-186).  Non-finite nodes are counted
 // and left out of every statistic; radii and displacements are sorted (NaN
 // cannot occur among the finite nodes) and picked at index

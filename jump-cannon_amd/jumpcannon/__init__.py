@@ -137,6 +137,7 @@ SIGNATURES = {
     "jc_mpnn_get_params": (C.c_int, [_vp, _vp]),
     "jc_mpnn_set_params": (C.c_int, [_vp, _vp]),
     "jc_mpnn_get_grads": (C.c_int, [_vp, _vp]),
+    "jc_gen_ring_seed": (None, [C.c_uint32, _vp]),
     "jc_mpnn_get_adam": (C.c_int, [_vp, _vp, _vp, _vp]),
     "jc_mpnn_set_adam": (C.c_int, [_vp, _vp, _vp, C.c_uint64]),
     "jc_mpnn_set_graph": (C.c_int, [_vp, C.c_uint32, _vp, _vp, C.c_uint64, C.c_uint64, C.c_uint32, _vp]),
