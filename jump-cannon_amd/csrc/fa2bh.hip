@@ -1638,7 +1638,8 @@ class Fa2BhEngine final : public LayoutEngine {
         if (ev) JC_HIP(hipEventRecord(ev[PH_KEYS], stream_));
         k_keys<<<cdiv(n, kTB), kTB, 0, stream_>>>(pos, n, frame_.p, keys_.p, vals_.p);
         if (ev) JC_HIP(hipEventRecord(ev[PH_SORT], stream_));
-        JC_TRY(sort_morton_pairs(keys_.p, vals_.p, sort_, stream_));
+        if (n < kSort24Max) JC_TRY(sort_morton_pairs24(keys_.p, vals_.p, sort_, stream_));
+        else JC_TRY(sort_morton_pairs(keys_.p, vals_.p, sort_, stream_));
         if (ev) JC_HIP(hipEventRecord(ev[PH_SCAN], stream_));
         k_count_scan<<<nb_scan_, kTB, 0, stream_>>>(keys_.p, n, frame_.p, loc_.p, blk_.p, blkd_.p);
         k_scan_top<<<1, 1024, 0, stream_>>>(blk_.p, nb_scan_, n, frame_.p, blkd_.p);

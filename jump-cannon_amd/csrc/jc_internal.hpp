@@ -113,6 +113,12 @@ Status radix_sort_pairs_u64(uint64_t *keys, uint32_t *vals, RadixScratch &s, hip
 // run of equal upper halves by the lower half.  Fast when such runs are short,
 // as they are for octree keys; exact for any input.
 Status sort_morton_pairs(uint64_t *keys, uint32_t *vals, RadixScratch &s, hipStream_t st);
+// The same result for fewer than kSort24Max keys: three radix passes over bits
+// [40, 64), then each key ranked within its run of equal upper 24 bits.
+// Octree keys leave runs of a few keys (at most 16 in the 1M-node workload).
+Status sort_morton_pairs24(uint64_t *keys, uint32_t *vals, RadixScratch &s, hipStream_t st);
+constexpr uint32_t kSort24Max = 1u << 24;  // long-run ranks fit the 24 bits above the low 40
+constexpr uint32_t kRankRun = 64;          // sort_morton_pairs24: runs ranked in place by each key
 
 constexpr uint32_t kShortRun = 16;          // sort_morton_pairs: runs sorted by one thread
 constexpr uint32_t kLdsRun = 4096;          // longer runs up to this size sort in LDS

@@ -225,6 +225,48 @@ __global__ __launch_bounds__(256) void k_rs_runs(uint64_t *__restrict__ keys, ui
     }
 }
 
+// After three passes over bits [40, 64) (the scratch pair holds keys ordered by
+// their upper 24 bits, ties in input order), each key's final place: the start
+// of its run of equal key >> 40 plus its rank in the run, the number of the
+// run's keys smaller than it or equal and earlier.  The block's keys and
+// kRankRun on each side are staged in LDS; every thread ranks its own key over
+// its run, so no lane idles behind another's run.  A run longer than kRankRun
+// is copied as it is and listed (by its first key) for k_rs_long_runs.
+__global__ __launch_bounds__(256) void k_rs_rank40(const uint64_t *__restrict__ ki, const uint32_t *__restrict__ vi,
+                                                   uint64_t *__restrict__ ko, uint32_t *__restrict__ vo,
+                                                   uint32_t n, uint32_t *__restrict__ long_runs,
+                                                   uint32_t *long_count) {
+    constexpr uint32_t H = kRankRun;
+    __shared__ uint64_t sk[256 + 2 * H];
+    const uint32_t b0 = blockIdx.x * 256;
+    for (uint32_t q = threadIdx.x; q < 256 + 2 * H; q += 256) {
+        const uint32_t g = b0 + q - H;  // wraps below 0: not loaded
+        sk[q] = (b0 + q >= H && g < n) ? ki[g] : 0ull;
+    }
+    __syncthreads();
+    const uint32_t i = b0 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t w = threadIdx.x + H;
+    const uint64_t k = sk[w], hi = k >> 40;
+    uint32_t back = 0, fwd = 0;
+    while (back < H && i > back && (sk[w - back - 1] >> 40) == hi) back++;
+    while (fwd < H && i + fwd + 1 < n && (sk[w + fwd + 1] >> 40) == hi) fwd++;
+    const uint32_t v = vi[i];
+    if (back + fwd + 1 > H) {  // a long run: copied, and listed by its first key
+        ko[i] = k;
+        vo[i] = v;
+        if (back == 0) long_runs[atomicAdd(long_count, 1u)] = i;
+        return;
+    }
+    uint32_t rank = 0;
+    for (uint32_t j = w - back; j <= w + fwd; j++) {
+        const uint64_t kj = sk[j];
+        rank += (kj < k || (kj == k && j < w)) ? 1u : 0u;
+    }
+    ko[i - back + rank] = k;
+    vo[i - back + rank] = v;
+}
+
 // Ascending bitonic sort of m unique 64-bit values (with payloads) for any m:
 // each merge stage first compares element i with its mirror i ^ (2k-1), then
 // with i ^ j; partners past m act as +infinity and never move.  The caller
@@ -252,11 +294,13 @@ __device__ void bitonic_any(KArr K, VArr V, uint32_t m) {
         }
 }
 
-// One block per listed run: its end by binary search (keys are ordered by the
-// upper half), then a bitonic sort of (lower half << 32 | rank in run), which
-// makes every value unique and keeps equal lower halves in input order.  Runs
+// One block per listed run of equal key >> LO: its end by binary search (keys
+// are ordered by that prefix), then a bitonic sort of (low LO bits << (64 - LO)
+// | rank in run), which makes every value unique and keeps equal low parts in
+// input order (ranks need 64 - LO bits: runs shorter than 2^(64 - LO)).  Runs
 // of up to kLdsRun keys sort in LDS; longer ones (only layouts with extreme
 // outliers) in the scratch pair, in global memory.
+template <uint32_t LO>
 __global__ __launch_bounds__(kLongRunThreads) void k_rs_long_runs(uint64_t *__restrict__ keys,
                                                                   uint32_t *__restrict__ vals, uint32_t n,
                                                                   const uint32_t *__restrict__ long_runs,
@@ -265,15 +309,16 @@ __global__ __launch_bounds__(kLongRunThreads) void k_rs_long_runs(uint64_t *__re
     __shared__ uint64_t sk[kLdsRun];
     __shared__ uint32_t sv[kLdsRun];
     __shared__ uint32_t s_end;
+    constexpr uint32_t R = 64 - LO;
     const uint32_t cnt = *long_count;
     for (uint32_t r = blockIdx.x; r < cnt; r += gridDim.x) {
         const uint32_t s = long_runs[r];
-        const uint32_t hi = (uint32_t)(keys[s] >> 32);
+        const uint64_t hi = keys[s] >> LO;
         if (threadIdx.x == 0) {
             uint32_t lo = s + 1, up = n;  // first index past the run
             while (lo < up) {
                 const uint32_t mid = lo + (up - lo) / 2;
-                if ((uint32_t)(keys[mid] >> 32) == hi) lo = mid + 1;
+                if ((keys[mid] >> LO) == hi) lo = mid + 1;
                 else up = mid;
             }
             s_end = lo;
@@ -282,22 +327,22 @@ __global__ __launch_bounds__(kLongRunThreads) void k_rs_long_runs(uint64_t *__re
         const uint32_t m = s_end - s;
         if (m <= kLdsRun) {
             for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
-                sk[t] = (keys[s + t] << 32) | t;
+                sk[t] = (keys[s + t] << R) | t;
                 sv[t] = vals[s + t];
             }
             __syncthreads();
             bitonic_any(sk, sv, m);
             for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
-                keys[s + t] = ((uint64_t)hi << 32) | (sk[t] >> 32);
+                keys[s + t] = (hi << LO) | (sk[t] >> R);
                 vals[s + t] = sv[t];
             }
         } else {
             uint64_t *K = gk + s;
-            for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) K[t] = (keys[s + t] << 32) | t;
+            for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) K[t] = (keys[s + t] << R) | t;
             __syncthreads();
             bitonic_any(K, vals + s, m);
             for (uint32_t t = threadIdx.x; t < m; t += blockDim.x)
-                keys[s + t] = ((uint64_t)hi << 32) | (K[t] >> 32);
+                keys[s + t] = (hi << LO) | (K[t] >> R);
         }
         __syncthreads();
     }
@@ -346,8 +391,31 @@ Status sort_morton_pairs(uint64_t *keys, uint32_t *vals, RadixScratch &s, hipStr
     }
     // four passes: the result is back in keys/vals, the scratch pair is free
     k_rs_runs<<<(s.n + 255) / 256, 256, 0, st>>>(keys, vals, s.n, s.long_runs.p, s.long_count.p);
-    k_rs_long_runs<<<kLongRunBlocks, kLongRunThreads, 0, st>>>(keys, vals, s.n, s.long_runs.p,
-                                                               s.long_count.p, s.keys_alt.p);
+    k_rs_long_runs<32><<<kLongRunBlocks, kLongRunThreads, 0, st>>>(keys, vals, s.n, s.long_runs.p,
+                                                                   s.long_count.p, s.keys_alt.p);
+    JC_HIP(hipGetLastError());
+    return Status::ok();
+}
+
+Status sort_morton_pairs24(uint64_t *keys, uint32_t *vals, RadixScratch &s, hipStream_t st) {
+    if (s.n == 0) return Status::ok();
+    if (s.n >= kSort24Max) return Status::err(JC_E_INVALID, "sort_morton_pairs24: 2^24 keys or more");
+    uint64_t *ka = keys, *kb = s.keys_alt.p;
+    uint32_t *va = vals, *vb = s.vals_alt.p;
+    for (uint32_t shift = 40; shift < 64; shift += 8) {
+        k_rs_hist<<<s.n_blocks, kSortThreads, 0, st>>>(ka, s.n, shift, s.hist.p, s.n_blocks,
+                                                       shift == 40 ? s.long_count.p : nullptr);
+        k_rs_digit_scan<<<256, 256, 0, st>>>(s.hist.p, s.n_blocks, s.dbase.p);
+        k_rs_scatter<<<s.n_blocks, kSortThreads, 0, st>>>(ka, va, kb, vb, s.n, shift, s.hist.p,
+                                                          s.dbase.p, s.n_blocks);
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    // three passes: the keys are in the scratch pair, ordered by their upper 24
+    // bits; ranking within the runs writes the full order back to keys/vals
+    k_rs_rank40<<<(s.n + 255) / 256, 256, 0, st>>>(ka, va, keys, vals, s.n, s.long_runs.p, s.long_count.p);
+    k_rs_long_runs<40><<<kLongRunBlocks, kLongRunThreads, 0, st>>>(keys, vals, s.n, s.long_runs.p,
+                                                                   s.long_count.p, s.keys_alt.p);
     JC_HIP(hipGetLastError());
     return Status::ok();
 }
