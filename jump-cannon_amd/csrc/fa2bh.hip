@@ -398,9 +398,8 @@ __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
             const uint32_t leaf = pos0 + c;
             const uint32_t b = perm[i];
             if (b < n) {
-                const float4 p = pos[b];
-                const float m = fmaxf((float)deg[b] + 1.0f, 1e-3f);
-                nodes[leaf].a = make_float4(p.x, p.y, p.z, m);
+                const float4 p = pos[b];  // .w: the body's mass max(deg + 1, 1e-3)
+                nodes[leaf].a = p;
                 nodes[leaf].b = leaf_b(b, (i + 1 < nv) ? leaf + 1 : kEnd);
                 const int ld = (lp > li ? lp : li) + 1;
                 node_depth[leaf] = (uint8_t)(0x80 | ld);
@@ -1012,7 +1011,8 @@ __global__ __launch_bounds__(kTB) void k_apply(const Params P, Frame *fr,
         if (P.max_displacement > 0.0f && df > 1.0e-9f) factor = fminf(factor, P.max_displacement / df);
         const float k = factor * P.time_step;
         const float4 p = pos_in[i];
-        const float4 q = make_float4(p.x + f.x * k, p.y + f.y * k, p.z + f.z * k, 0.0f);
+        // .w carries the mass for k_emit (one gather there instead of two)
+        const float4 q = make_float4(p.x + f.x * k, p.y + f.y * k, p.z + f.z * k, fmaxf(deg_i, 1e-3f));
         pos_out[i] = q;
         old_force[i] = make_float4(f.x, f.y, f.z, 0.0f);
         const float v[3] = {q.x + 0.0f, q.y + 0.0f, q.z + 0.0f};  // +0: canonical zero sign
@@ -1257,10 +1257,12 @@ class Fa2BhEngine final : public LayoutEngine {
                                   hipMemcpyHostToDevice, stream_));
         }
         JC_TRY(frame_.alloc(1));
+        mass_.assign(n_, 0.0f);
+        for (uint32_t i = 0; i < n_; i++) mass_[i] = std::fmax((float)degs[i] + 1.0f, 1e-3f);
         std::vector<float4> p4(nn, make_float4(0, 0, 0, 0));
         for (uint32_t i = 0; i < n_; i++)
             p4[i] = make_float4(xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2],
-                                0.0f);
+                                mass_[i]);
         JC_HIP(hipMemcpyAsync(pos_[0].p, p4.data(), nn * sizeof(float4), hipMemcpyHostToDevice,
                               stream_));
         JC_HIP(hipMemcpyAsync(pos_[1].p, p4.data(), nn * sizeof(float4), hipMemcpyHostToDevice,
@@ -1337,7 +1339,7 @@ class Fa2BhEngine final : public LayoutEngine {
     Status set_state(const float *xyz, const float *old_force, const float *se2,
                      uint64_t it) override {
         if (!inited_) return Status::err(JC_E_STATE, "set_state before init");
-        if (xyz) JC_TRY(write_vec4(pos_[cur_].p, xyz));
+        if (xyz) JC_TRY(write_vec4(pos_[cur_].p, xyz, mass_.data()));
         if (old_force) JC_TRY(write_vec4(old_force_.p, old_force));
         ready_ = false;
         if (se2) {
@@ -1356,7 +1358,7 @@ class Fa2BhEngine final : public LayoutEngine {
         refresh_params();
         DevBuf<float4> tmp;
         JC_TRY(tmp.alloc(n_));
-        JC_TRY(write_vec4(tmp.p, xyz));
+        JC_TRY(write_vec4(tmp.p, xyz, mass_.data()));
         ready_ = false;
         JC_TRY(enqueue_fresh(tmp.p));
         JC_TRY(enqueue_tree(tmp.p, nullptr));
@@ -1593,10 +1595,13 @@ class Fa2BhEngine final : public LayoutEngine {
         }
         return Status::ok();
     }
-    Status write_vec4(float4 *dst, const float *src) {
+    // xyz (3n) into float4s; w from `w` (n, nullable: 0).  Positions carry the
+    // bodies' masses in w.
+    Status write_vec4(float4 *dst, const float *src, const float *w = nullptr) {
         std::vector<float4> h(n_);
         for (uint32_t i = 0; i < n_; i++)
-            h[i] = make_float4(src[3 * (size_t)i], src[3 * (size_t)i + 1], src[3 * (size_t)i + 2], 0.0f);
+            h[i] = make_float4(src[3 * (size_t)i], src[3 * (size_t)i + 1], src[3 * (size_t)i + 2],
+                               w ? w[i] : 0.0f);
         JC_HIP(hipMemcpyAsync(dst, h.data(), n_ * sizeof(float4), hipMemcpyHostToDevice, stream_));
         JC_HIP(hipStreamSynchronize(stream_));
         return Status::ok();
@@ -1810,7 +1815,8 @@ class Fa2BhEngine final : public LayoutEngine {
     int walk_mode_ = -1;  // -1: pick at the first step by timing both walks
     float tune_ms_[kWalkModes] = {};
     hipStream_t stream_ = nullptr;
-    DevBuf<float4> pos_[2], old_force_, force_;
+    DevBuf<float4> pos_[2], old_force_, force_;  // positions: w = mass max(deg + 1, 1e-3)
+    std::vector<float> mass_;                    // the same masses on the host, for uploads
     DevBuf<uint32_t> deg_, att_off_, att_adj_;
     DevBuf<uint64_t> keys_;
     DevBuf<uint32_t> vals_, loc_, blk_;
