@@ -883,16 +883,53 @@ __global__ __launch_bounds__(64) void k_force_heavy(const Params P,
     const float4 f0 = force[b];
     float run = lane == 0 ? f0.x : lane == 1 ? f0.y : f0.z;
     const uint32_t e0 = hoff[blockIdx.x], len = hlen[blockIdx.x];
+    // the next chunk's terms are loaded into registers while lanes 0-2 add the
+    // current one (padded entries are loaded, never added)
+    constexpr uint32_t V = kHeavyStage / 4 / 64;  // float4 per lane and component
+    typedef float v4f __attribute__((ext_vector_type(4)));  // native vectors stay in registers
+    v4f nx0[V], nx1[V], nx2[V];
+    {
+        // chunk 0; the index is clamped to the chunk, so every load is in bounds
+        const uint32_t fnv4 = len ? (min(kHeavyStage, len) + 3u) >> 2 : 1u;
+        const v4f *s0 = reinterpret_cast<const v4f *>(hterm + e0);
+        const v4f *s1 = reinterpret_cast<const v4f *>(hterm + (size_t)stride + e0);
+        const v4f *s2 = reinterpret_cast<const v4f *>(hterm + 2 * (size_t)stride + e0);
+#pragma unroll
+        for (uint32_t k = 0; k < V; k++) {
+            const uint32_t v = min(lane + 64 * k, fnv4 - 1u);
+            if (len) {
+                nx0[k] = s0[v];
+                nx1[k] = s1[v];
+                nx2[k] = s2[v];
+            }
+        }
+    }
     for (uint32_t c0 = 0; c0 < len; c0 += kHeavyStage) {
         const uint32_t cnt = min(kHeavyStage, len - c0);
-        const uint32_t nv4 = (cnt + 3u) >> 2;   // padded entries are loaded, never added
+        const uint32_t nv4 = (cnt + 3u) >> 2;
 #pragma unroll
-        for (uint32_t comp = 0; comp < 3; comp++) {
-            const float4 *src = reinterpret_cast<const float4 *>(hterm + (size_t)comp * stride + e0 + c0);
-            float4 *dst = reinterpret_cast<float4 *>(st[comp]);
-            for (uint32_t v = lane; v < nv4; v += 64) dst[v] = src[v];
+        for (uint32_t k = 0; k < V; k++) {
+            if (lane + 64 * k < nv4) {
+                reinterpret_cast<v4f *>(st[0])[lane + 64 * k] = nx0[k];
+                reinterpret_cast<v4f *>(st[1])[lane + 64 * k] = nx1[k];
+                reinterpret_cast<v4f *>(st[2])[lane + 64 * k] = nx2[k];
+            }
         }
         __syncthreads();
+        const uint32_t c1 = c0 + kHeavyStage;
+        if (c1 < len) {  // the next chunk, in flight while lanes 0-2 add this one
+            const uint32_t fnv4 = (min(kHeavyStage, len - c1) + 3u) >> 2;
+            const v4f *s0 = reinterpret_cast<const v4f *>(hterm + e0 + c1);
+            const v4f *s1 = reinterpret_cast<const v4f *>(hterm + (size_t)stride + e0 + c1);
+            const v4f *s2 = reinterpret_cast<const v4f *>(hterm + 2 * (size_t)stride + e0 + c1);
+#pragma unroll
+            for (uint32_t k = 0; k < V; k++) {
+                const uint32_t v = min(lane + 64 * k, fnv4 - 1u);
+                nx0[k] = s0[v];
+                nx1[k] = s1[v];
+                nx2[k] = s2[v];
+            }
+        }
         if (lane < 3) {
             const float4 *t4 = reinterpret_cast<const float4 *>(st[lane]);
             const uint32_t full = cnt >> 2;

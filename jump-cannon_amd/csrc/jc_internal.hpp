@@ -118,7 +118,7 @@ Status sort_morton_pairs(uint64_t *keys, uint32_t *vals, RadixScratch &s, hipStr
 // Octree keys leave runs of a few keys (at most 16 in the 1M-node workload).
 Status sort_morton_pairs24(uint64_t *keys, uint32_t *vals, RadixScratch &s, hipStream_t st);
 constexpr uint32_t kSort24Max = 1u << 24;  // long-run ranks fit the 24 bits above the low 40
-constexpr uint32_t kRankRun = 64;          // sort_morton_pairs24: runs ranked in place by each key
+constexpr uint32_t kRankRun = 32;          // sort_morton_pairs24: runs ranked in place by each key
 
 constexpr uint32_t kShortRun = 16;          // sort_morton_pairs: runs sorted by one thread
 constexpr uint32_t kLdsRun = 4096;          // longer runs up to this size sort in LDS
