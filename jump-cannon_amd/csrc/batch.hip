@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kBT) void k_b_keys(const float4 *__restrict__ pos,
                                                 const uint32_t *__restrict__ base, uint32_t n,
                                                 GFrame *fr, uint64_t *__restrict__ keys_raw,
                                                 uint64_t *__restrict__ keys,
-                                                uint32_t *__restrict__ vals) {
+                                                uint32_t *__restrict__ vals, uint32_t ck_p) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t g = gid[i];
@@ -168,46 +168,9 @@ __global__ __launch_bounds__(kBT) void k_b_keys(const float4 *__restrict__ pos,
         atomicAdd(&fr[g].n_invalid, 1u);
     }
     keys_raw[i] = key;
-    keys[i] = key;
+    // the sort key: graph id above the key's top ck_p bits (sort_segmented_pairs)
+    keys[i] = ck_p >= 64 ? key : ((uint64_t)g << ck_p) | (key >> (64 - ck_p));
     vals[i] = i;
-}
-
-// After the key sort: key := graph id of the body (for the graph-major passes).
-// keys[i] = graph of body vals[i].  Bodies are numbered graph after graph, so
-// the graph is the last one whose first body is <= vals[i]: a binary search
-// over the graphs' first-body table, staged in LDS (a random gather from the
-// per-body graph array misses the caches on every key).
-constexpr uint32_t kGidLds = 8192;  // graphs whose table fits LDS (32 KB)
-__global__ __launch_bounds__(256) void k_b_gid_keys(const uint32_t *__restrict__ vals,
-                                                    const uint32_t *__restrict__ gid,
-                                                    const uint32_t *__restrict__ base, uint32_t G,
-                                                    uint64_t *__restrict__ keys, uint32_t n) {
-    __shared__ uint32_t sb[kGidLds];
-    const bool lds = G <= kGidLds;
-    if (lds)
-        for (uint32_t g = threadIdx.x; g < G; g += blockDim.x) sb[g] = base[g];
-    __syncthreads();
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t v = vals[i];
-    if (!lds) {
-        keys[i] = gid[v];
-        return;
-    }
-    uint32_t lo = 0, hi = G;  // first graph whose first body is > v
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (sb[mid] <= v) lo = mid + 1;
-        else hi = mid;
-    }
-    keys[i] = lo - 1;
-}
-
-__global__ void k_b_gather_keys(const uint32_t *__restrict__ vals,
-                                const uint64_t *__restrict__ keys_raw, uint64_t *__restrict__ keys,
-                                uint32_t n) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) keys[i] = keys_raw[vals[i]];
 }
 
 struct Seg {
@@ -851,9 +814,17 @@ class Fa2BhBatch {
         }
         n_hent_ = (uint32_t)hbody.size();
         hstride_ = n_hent_;
-        gbits_ = 0;
-        while ((1u << gbits_) < G) gbits_++;
-        gbits_ = (gbits_ + 7) / 8 * 8;
+        // the sort's composite key: ceil(log2 G) bits of graph id above ck_p_ bits
+        // of the raw key, ck_bits_ in all (a multiple of 16); ck_p_ leaves each
+        // graph at least about 32 prefixes per body, so runs of equal prefix stay short
+        uint32_t gb = 0, cmax = 1;
+        while ((1ull << gb) < G) gb++;
+        for (uint32_t g = 0; g < G; g++) cmax = std::max(cmax, count[g]);
+        uint32_t lc = 0;
+        while ((1ull << lc) < cmax) lc++;
+        const uint32_t pmin = std::min<uint32_t>(std::max<uint32_t>(16, lc + 5), 63);
+        ck_bits_ = std::min<uint32_t>((gb + pmin + 15) / 16 * 16, 64);
+        ck_p_ = ck_bits_ - gb;
 
         JC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         const uint32_t nn = n_ ? n_ : 1;
@@ -1023,16 +994,11 @@ class Fa2BhBatch {
         const uint32_t n = n_;
         k_b_reset<<<cdivb(G_, 256), 256, 0, stream_>>>(frame_.p, G_, global_.p);
         k_b_bbox<<<G_, kBT, 0, stream_>>>(pos, base_.p, count_.p, frame_.p);
-        k_b_keys<<<cdivb(n, kBT), kBT, 0, stream_>>>(pos, gid_.p, base_.p, n, frame_.p,
-                                                     keys_raw_.p, keys_.p, vals_.p);
-        // four passes: keys of thousands of graphs share their upper 24 bits
-        // far more often than one graph's do (sort_morton_pairs24 measured slower)
-        JC_TRY(sort_morton_pairs(keys_.p, vals_.p, sort_, stream_));
-        if (G_ > 1) {
-            k_b_gid_keys<<<cdivb(n, 256), 256, 0, stream_>>>(vals_.p, gid_.p, base_.p, G_, keys_.p, n);
-            JC_TRY(radix_sort_pairs_u64(keys_.p, vals_.p, sort_, stream_, 0, gbits_));
-            k_b_gather_keys<<<cdivb(n, 256), 256, 0, stream_>>>(vals_.p, keys_raw_.p, keys_.p, n);
-        }
+        // graph-major, key-minor: radix passes over (graph id, top ck_p_ key bits),
+        // then each run of equal composite keys ranked by its raw keys
+        k_b_keys<<<cdivb(n, kBT), kBT, 0, stream_>>>(pos, gid_.p, base_.p, n, frame_.p, keys_raw_.p,
+                                                     sort_.keys_alt.p, sort_.vals_alt.p, ck_p_);
+        JC_TRY(sort_segmented_pairs(keys_.p, vals_.p, keys_raw_.p, sort_, stream_, ck_bits_));
         k_b_count_scan<<<nb_scan_, kBT, 0, stream_>>>(keys_.p, gid_.p, base_.p, count_.p, n,
                                                       frame_.p, loc_.p, blk_.p, blkd_.p);
         k_b_scan_top<<<1, 1024, 0, stream_>>>(blk_.p, nb_scan_, global_.p, blkd_.p);
@@ -1167,7 +1133,7 @@ class Fa2BhBatch {
     jc_fa2bh_settings settings_;
     BParams P_{};
     bool params_dirty_ = true, inited_ = false, tuned_ = false;
-    uint32_t G_ = 0, n_ = 0, cur_ = 0, nb_scan_ = 0, gbits_ = 0, force_bs_ = 256;
+    uint32_t G_ = 0, n_ = 0, cur_ = 0, nb_scan_ = 0, ck_bits_ = 16, ck_p_ = 16, force_bs_ = 256;
     uint64_t cap_ = 0, iteration_ = 0;
     std::vector<uint32_t> base_h_, count_h_;
     hipStream_t stream_ = nullptr;

@@ -119,6 +119,13 @@ Status sort_morton_pairs(uint64_t *keys, uint32_t *vals, RadixScratch &s, hipStr
 Status sort_morton_pairs24(uint64_t *keys, uint32_t *vals, RadixScratch &s, hipStream_t st);
 constexpr uint32_t kSort24Max = 1u << 24;  // long-run ranks fit the 24 bits above the low 40
 constexpr uint32_t kRankRun = 32;          // sort_morton_pairs24: runs ranked in place by each key
+// Keys grouped by segment (many graphs): the caller writes a composite key ck
+// (segment id above the top bits of the raw key) into s.keys_alt and the
+// element indices into s.vals_alt; radix passes over ck's low ck_bits (a
+// multiple of 16) leave them grouped, then each run of equal ck is ordered by
+// (raw[index], index).  keys/vals receive the raw keys and indices in order.
+Status sort_segmented_pairs(uint64_t *keys, uint32_t *vals, const uint64_t *raw, RadixScratch &s,
+                            hipStream_t st, uint32_t ck_bits);
 
 constexpr uint32_t kShortRun = 16;          // sort_morton_pairs: runs sorted by one thread
 constexpr uint32_t kLdsRun = 4096;          // longer runs up to this size sort in LDS

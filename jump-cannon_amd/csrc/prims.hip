@@ -267,6 +267,122 @@ __global__ __launch_bounds__(256) void k_rs_rank40(const uint64_t *__restrict__ 
     vo[i - back + rank] = v;
 }
 
+// Segmented form for many graphs at once: after radix passes over a composite
+// key ck (graph id above the top bits of the raw key), each element's final
+// place is the start of its run of equal ck plus its rank in the run by (raw
+// key, body index): raw keys are gathered through the values, and a body index
+// orders equal raw keys as a stable sort does.  Runs longer than kRankRun are
+// copied and listed for k_rs_long_runs_seg.
+__global__ __launch_bounds__(256) void k_rs_rank_seg(const uint64_t *__restrict__ ck, const uint32_t *__restrict__ vi,
+                                                     const uint64_t *__restrict__ raw, uint64_t *__restrict__ ko,
+                                                     uint32_t *__restrict__ vo, uint32_t n,
+                                                     uint32_t *__restrict__ long_runs, uint32_t *long_count) {
+    constexpr uint32_t H = kRankRun;
+    __shared__ uint64_t sc[256 + 2 * H];
+    __shared__ uint64_t sr[256 + 2 * H];
+    __shared__ uint32_t sv[256 + 2 * H];
+    const uint32_t b0 = blockIdx.x * 256;
+    for (uint32_t q = threadIdx.x; q < 256 + 2 * H; q += 256) {
+        const uint32_t g = b0 + q - H;  // wraps below 0: not loaded
+        const bool in = b0 + q >= H && g < n;
+        const uint32_t v = in ? vi[g] : 0u;
+        sc[q] = in ? ck[g] : 0ull;
+        sv[q] = v;
+        sr[q] = in ? raw[v] : 0ull;
+    }
+    __syncthreads();
+    const uint32_t i = b0 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t w = threadIdx.x + H;
+    const uint64_t c = sc[w], k = sr[w];
+    const uint32_t v = sv[w];
+    uint32_t back = 0, fwd = 0;
+    while (back < H && i > back && sc[w - back - 1] == c) back++;
+    while (fwd < H && i + fwd + 1 < n && sc[w + fwd + 1] == c) fwd++;
+    if (back + fwd + 1 > H) {
+        ko[i] = k;
+        vo[i] = v;
+        if (back == 0) long_runs[atomicAdd(long_count, 1u)] = i;
+        return;
+    }
+    uint32_t rank = 0;
+    for (uint32_t j = w - back; j <= w + fwd; j++) {
+        const uint64_t kj = sr[j];
+        rank += (kj < k || (kj == k && sv[j] < v)) ? 1u : 0u;
+    }
+    ko[i - back + rank] = k;
+    vo[i - back + rank] = v;
+}
+
+// Ascending bitonic sort of m (key, value) pairs by key, then value (every
+// pair distinct), in a barrier-synchronised array (LDS or, per block, global).
+template <typename KArr, typename VArr>
+__device__ void bitonic_pairs(KArr K, VArr V, uint32_t m) {
+    uint32_t p = 1;
+    while (p < m) p <<= 1;
+    for (uint32_t k = 2; k <= p; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+                const uint32_t partner = j == (k >> 1) ? (i ^ (k - 1)) : (i ^ j);
+                if (partner > i && partner < m) {
+                    const uint64_t a = K[i], b = K[partner];
+                    const uint32_t va = V[i], vb = V[partner];
+                    if (a > b || (a == b && va > vb)) {
+                        K[i] = b;
+                        K[partner] = a;
+                        V[i] = vb;
+                        V[partner] = va;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+// One block per listed run of equal composite key: its end by binary search
+// over ck, then the run's (raw key, body index) pairs sorted (in LDS up to
+// kLdsRun, otherwise in place in global memory).
+__global__ __launch_bounds__(kLongRunThreads) void k_rs_long_runs_seg(uint64_t *__restrict__ keys,
+                                                                      uint32_t *__restrict__ vals,
+                                                                      const uint64_t *__restrict__ ck, uint32_t n,
+                                                                      const uint32_t *__restrict__ long_runs,
+                                                                      const uint32_t *long_count) {
+    __shared__ uint64_t sk[kLdsRun];
+    __shared__ uint32_t sv[kLdsRun];
+    __shared__ uint32_t s_end;
+    const uint32_t cnt = *long_count;
+    for (uint32_t r = blockIdx.x; r < cnt; r += gridDim.x) {
+        const uint32_t s = long_runs[r];
+        const uint64_t c = ck[s];
+        if (threadIdx.x == 0) {
+            uint32_t lo = s + 1, up = n;  // first index past the run
+            while (lo < up) {
+                const uint32_t mid = lo + (up - lo) / 2;
+                if (ck[mid] == c) lo = mid + 1;
+                else up = mid;
+            }
+            s_end = lo;
+        }
+        __syncthreads();
+        const uint32_t m = s_end - s;
+        if (m <= kLdsRun) {
+            for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
+                sk[t] = keys[s + t];
+                sv[t] = vals[s + t];
+            }
+            __syncthreads();
+            bitonic_pairs(sk, sv, m);
+            for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
+                keys[s + t] = sk[t];
+                vals[s + t] = sv[t];
+            }
+        } else {
+            bitonic_pairs(keys + s, vals + s, m);
+        }
+        __syncthreads();
+    }
+}
+
 // Ascending bitonic sort of m unique 64-bit values (with payloads) for any m:
 // each merge stage first compares element i with its mirror i ^ (2k-1), then
 // with i ^ j; partners past m act as +infinity and never move.  The caller
@@ -393,6 +509,31 @@ Status sort_morton_pairs(uint64_t *keys, uint32_t *vals, RadixScratch &s, hipStr
     k_rs_runs<<<(s.n + 255) / 256, 256, 0, st>>>(keys, vals, s.n, s.long_runs.p, s.long_count.p);
     k_rs_long_runs<32><<<kLongRunBlocks, kLongRunThreads, 0, st>>>(keys, vals, s.n, s.long_runs.p,
                                                                    s.long_count.p, s.keys_alt.p);
+    JC_HIP(hipGetLastError());
+    return Status::ok();
+}
+
+Status sort_segmented_pairs(uint64_t *keys, uint32_t *vals, const uint64_t *raw, RadixScratch &s,
+                            hipStream_t st, uint32_t ck_bits) {
+    if (s.n == 0) return Status::ok();
+    if (ck_bits == 0 || ck_bits > 64 || ck_bits % 16)
+        return Status::err(JC_E_INVALID, "sort_segmented_pairs: ck_bits must be 16, 32, 48 or 64");
+    // an even number of passes from the scratch pair ends in it again
+    uint64_t *ka = s.keys_alt.p, *kb = keys;
+    uint32_t *va = s.vals_alt.p, *vb = vals;
+    for (uint32_t shift = 0; shift < ck_bits; shift += 8) {
+        k_rs_hist<<<s.n_blocks, kSortThreads, 0, st>>>(ka, s.n, shift, s.hist.p, s.n_blocks,
+                                                       shift == 0 ? s.long_count.p : nullptr);
+        k_rs_digit_scan<<<256, 256, 0, st>>>(s.hist.p, s.n_blocks, s.dbase.p);
+        k_rs_scatter<<<s.n_blocks, kSortThreads, 0, st>>>(ka, va, kb, vb, s.n, shift, s.hist.p,
+                                                          s.dbase.p, s.n_blocks);
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    k_rs_rank_seg<<<(s.n + 255) / 256, 256, 0, st>>>(s.keys_alt.p, s.vals_alt.p, raw, keys, vals, s.n,
+                                                     s.long_runs.p, s.long_count.p);
+    k_rs_long_runs_seg<<<kLongRunBlocks, kLongRunThreads, 0, st>>>(keys, vals, s.keys_alt.p, s.n, s.long_runs.p,
+                                                                   s.long_count.p);
     JC_HIP(hipGetLastError());
     return Status::ok();
 }
