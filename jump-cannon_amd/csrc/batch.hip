@@ -403,7 +403,7 @@ __global__ __launch_bounds__(kBT) void k_b_emit(
                 const uint32_t b = perm[i];
                 if (b < n) {
                     const float4 p = pos[b];
-                    nodes[leaf].a = make_float4(p.x, p.y, p.z, fmaxf((float)deg[b] + 1.0f, 1e-3f));
+                    nodes[leaf].a = p;  // .w: the body's mass max(deg + 1, 1e-3) (k_b_apply)
                     nodes[leaf].b = leaf_b(b, (i + 1 < sg.ve) ? leaf + 1 : kEnd);
                     node_depth[leaf] = (uint8_t)(0x80 | ((lp > li ? lp : li) + 1));
                 }
@@ -701,7 +701,8 @@ __global__ __launch_bounds__(kBT) void k_b_apply(const BParams P, const GFrame *
     if (P.max_displacement > 0.0f && df > 1.0e-9f) factor = fminf(factor, P.max_displacement / df);
     const float k = factor * P.time_step;
     const float4 p = pos_in[i];
-    pos_out[i] = make_float4(p.x + f.x * k, p.y + f.y * k, p.z + f.z * k, 0.0f);
+    // .w carries the mass for k_b_emit (one gather there instead of two)
+    pos_out[i] = make_float4(p.x + f.x * k, p.y + f.y * k, p.z + f.z * k, fmaxf(deg_i, 1e-3f));
     old_force[i] = make_float4(f.x, f.y, f.z, 0.0f);
 }
 
@@ -872,7 +873,8 @@ class Fa2BhBatch {
         JC_TRY(global_.alloc(1));
         std::vector<float4> p4(nn, make_float4(0, 0, 0, 0));
         for (uint32_t i = 0; i < n_; i++)
-            p4[i] = make_float4(xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2], 0.0f);
+            p4[i] = make_float4(xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2],
+                                std::fmax((float)degs[i] + 1.0f, 1e-3f));  // w: the mass
         auto up = [&](void *d, const void *h, size_t bytes) -> Status {
             JC_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream_));
             return Status::ok();
