@@ -304,9 +304,13 @@ __device__ __forceinline__ void walk_reduced_multi(const Node *__restrict__ node
 constexpr int kTopDepth = 3;
 constexpr uint32_t kTopSlots = 585;  // 1 + 8 + 64 + 512
 constexpr uint32_t kTopAbsent = 0xFFFFFFFFu;
+// Closed form (8^(kTopDepth - j + 1) - 1) / 7, not a recursion: a recursive
+// call in device code is not inlined, and the kernels calling it would get a
+// dynamic stack (scratch set up at every dispatch).
 __host__ __device__ constexpr uint32_t top_sub(int j) {
-    return j > kTopDepth ? 0u : (j == kTopDepth ? 1u : 1u + 8u * top_sub(j + 1));
+    return j > kTopDepth ? 0u : ((1u << (3 * (kTopDepth - j + 1))) - 1u) / 7u;
 }
+static_assert(top_sub(kTopDepth) == 1u && top_sub(kTopDepth - 1) == 9u, "1 + 8 top_sub(j + 1)");
 static_assert(top_sub(0) == kTopSlots, "complete octree of depth kTopDepth");
 // Slot of the depth-d (d <= kTopDepth) prefix of a key.
 __device__ __forceinline__ uint32_t top_slot_of(uint64_t key, int d) {
