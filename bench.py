@@ -196,8 +196,9 @@ def bench_batch(args) -> int:
     b = jc.Fa2BhBatch(theta=1.0)
     b.init(graphs, pos)
     b.step(2, readback=False)
+    sched = b.schedule()
     b_ms, bs = b.step_timed(iters)
-    log(f"batch: {iters} iterations in {b_ms:.1f} ms (force block {bs})")
+    log(f"batch: {iters} iterations in {b_ms:.1f} ms (schedule {sched[0]}, force block {bs})")
     seq_ms = 0.0
     stride = max(1, args.seq_stride)
     sample = range(0, len(graphs), stride) if not args.no_sequential else range(0)
@@ -218,7 +219,7 @@ def bench_batch(args) -> int:
                  {"workload": "batch_4096_fa2bh", "graphs": len(graphs), "total_nodes": total,
                   "sizes": "log-uniform 50..20000, half SBM half BA", "theta": 1.0,
                   "parallelism": "single-gpu"},
-                 batch_ms=round(b_ms, 2), force_block=bs,
+                 batch_ms=round(b_ms, 2), force_block=bs, schedule=sched[0],
                  sequential_ms_estimate=round(seq_ms, 1) if len(sample) else None,
                  speedup_vs_sequential=round(seq_ms / b_ms, 2) if len(sample) else None,
                  sequential_sample=(f"{which}, each its own engine, 20 timed iterations after 2, scaled to "

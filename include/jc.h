@@ -314,6 +314,21 @@ int jc_fa2bh_batch_speeds(jc_batch *b, float *speed_eff);
  * size the autotuner picked. */
 int jc_fa2bh_batch_step_timed(jc_batch *b, uint32_t n_iters, double *gpu_ms,
                               uint32_t *force_block);
+/* Schedule of the batch (no reference counterpart).  mode 0: picked by the
+ * autotuner at the next step (default); 1: phased, one launch per phase over the
+ * whole forest, iterations replayed from a hipGraph; 2: persistent, one launch
+ * for all iterations of a step, each resident workgroup laying out whole graphs
+ * taken largest first from a queue.  threads: the persistent workgroup size,
+ * 256, 512 or 1024, or 0 for the autotuner's pick.  Either schedule gives the
+ * same bits.  Mode 2 is JC_E_INVALID for a graph whose node region would need
+ * 2^27 nodes or more.  Mode 2 is currently switched off and returns JC_E_INVALID
+ * (its first GPU run made a memory fault whose cause has not been found); the
+ * autotuner then keeps mode 1. */
+int jc_fa2bh_batch_set_schedule(jc_batch *b, uint32_t mode, uint32_t threads);
+/* The schedule in use (mode 0 until the first step has tuned it), the
+ * persistent workgroup size and the phased force block size. */
+int jc_fa2bh_batch_schedule(jc_batch *b, uint32_t *mode, uint32_t *threads,
+                            uint32_t *force_block);
 void jc_fa2bh_batch_destroy(jc_batch *b);
 
 /* ---- learned warm start: 3-layer message-passing network, hidden 128 ----

@@ -714,9 +714,548 @@ __global__ void k_b_init_speed(GFrame *fr, uint32_t G) {
     }
 }
 
+// ---- persistent schedule: one workgroup owns a graph for every iteration ----
+//
+// The graphs of a batch are independent, so a workgroup can lay one out from
+// start to finish with nothing but its own barriers: box, keys, sort, tree,
+// centre of mass, force, speed and apply of iteration t, then iteration t + 1,
+// all in one launch.  Workgroups stay resident and take graphs from a queue in
+// decreasing size (longest first, so the big graphs do not finish last); while
+// one workgroup builds its tree, the others on the CU walk theirs.  Every phase
+// performs the operations of the phased kernels above on the same values, so
+// each graph still evolves bit-identically to its standalone run.
+//
+// Per graph g the node array holds a static region of pg[g].ncap nodes from
+// pg[g].nstart (room for any tree of the graph: n leaves and at most n/2 + 1
+// cells per depth), and ropes and byte offsets are relative to the region, so
+// the walk's buffer resource spans only the graph's own nodes.
+
+struct PGraph {
+    uint32_t base;    // first body
+    uint32_t count;   // bodies
+    uint32_t nstart;  // first node of the graph's node region
+    uint32_t ncap;    // nodes in the region
+    uint32_t hstart;  // first of the graph's long attraction rows in prow
+    uint32_t hcount;  // long rows of the graph
+};
+
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float unif(float v) {
+    return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));
+}
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+    return ((uint64_t)uni((uint32_t)(v >> 32)) << 32) | uni((uint32_t)v);
+}
+
+// Lanes of the wave holding the same 8-bit digit as this lane (valid lanes only);
+// every lane of the wave must be active.
+__device__ __forceinline__ uint64_t p_match8(uint32_t d, bool valid) {
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; bit++) {
+        const bool on = (d >> bit) & 1u;
+        const uint64_t bl = __ballot(on);
+        m &= on ? bl : ~bl;
+    }
+    return m;
+}
+
+// Exclusive prefix sum of one value per thread over the workgroup (thread
+// order); *total gets the sum.  Uses s_w[T / 64]; ends with a barrier, so s_w
+// may be reused at once.
+template <uint32_t T>
+__device__ __forceinline__ uint32_t p_excl_scan(uint32_t v, uint32_t *s_w, uint32_t *total) {
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o, 64);
+        if ((int)lane >= o) incl += t;
+    }
+    if (lane == 63) s_w[w] = incl;
+    __syncthreads();
+    uint32_t wp = 0, tot = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < T / 64; q++) {
+        const uint32_t x = s_w[q];
+        wp += q < w ? x : 0u;
+        tot += x;
+    }
+    *total = tot;
+    __syncthreads();
+    return incl - v + wp;
+}
+
+// Maximum of one value per thread over the workgroup (same barrier rules).
+template <uint32_t T>
+__device__ __forceinline__ uint32_t p_max(uint32_t v, uint32_t *s_w) {
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o, 64));
+    if (lane == 0) s_w[w] = v;
+    __syncthreads();
+    uint32_t m = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < T / 64; q++) m = max(m, s_w[q]);
+    __syncthreads();
+    return m;
+}
+
+// One stable pass of the workgroup's LSD radix sort: items [0, ng) of (ks, vs)
+// by the 8-bit digit at bit sh, into (kd, vd).  Wave w owns the contiguous tile
+// [w * tile, (w + 1) * tile) and goes through it 64 items per round, so the
+// order of (wave, round, lane) is the item order; an item's place is the count
+// of smaller digits, plus its digit in earlier waves, earlier rounds and lower
+// lanes.  s_hist holds 256 * (T / 64) counters, digit-major.
+template <uint32_t T>
+__device__ __forceinline__ void p_radix_pass(const uint64_t *__restrict__ ks,
+                                             const uint32_t *__restrict__ vs,
+                                             uint64_t *__restrict__ kd, uint32_t *__restrict__ vd,
+                                             uint32_t ng, uint32_t sh, uint32_t *s_hist,
+                                             uint32_t *s_w) {
+    constexpr uint32_t NW = T / 64;
+    static_assert(256 * NW == 4 * T, "four counters per thread in the scan");
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t tile = (ng + NW - 1) / NW;
+    const uint32_t t0 = min(w * tile, ng), t1 = min(t0 + tile, ng);
+    const uint32_t rounds = (t1 - t0 + 63) / 64;
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (uint32_t q = threadIdx.x; q < 256 * NW; q += T) s_hist[q] = 0;
+    __syncthreads();
+    for (uint32_t r = 0; r < rounds; r++) {
+        const uint32_t i = t0 + r * 64 + lane;
+        const bool valid = i < t1;
+        const uint32_t d = valid ? (uint32_t)(ks[i] >> sh) & 0xFFu : 0u;
+        const uint64_t peers = p_match8(d, valid);
+        if (valid && (peers & below) == 0) s_hist[d * NW + w] += (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    uint32_t e[4], s = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < 4; j++) {
+        e[j] = s_hist[4 * threadIdx.x + j];
+        s += e[j];
+    }
+    uint32_t tot;
+    uint32_t run = p_excl_scan<T>(s, s_w, &tot);
+#pragma unroll
+    for (uint32_t j = 0; j < 4; j++) {
+        s_hist[4 * threadIdx.x + j] = run;
+        run += e[j];
+    }
+    __syncthreads();
+    for (uint32_t r = 0; r < rounds; r++) {
+        const uint32_t i = t0 + r * 64 + lane;
+        const bool valid = i < t1;
+        uint64_t k = 0;
+        uint32_t v = 0, d = 0;
+        if (valid) {
+            k = ks[i];
+            v = vs[i];
+            d = (uint32_t)(k >> sh) & 0xFFu;
+        }
+        const uint64_t peers = p_match8(d, valid);
+        uint32_t at = 0;
+        if (valid) at = s_hist[d * NW + w] + (uint32_t)__popcll(peers & below);
+        // every lane has read the cursor before the group's lowest lane moves it
+        // past the group
+        __builtin_amdgcn_wave_barrier();
+        if (valid) {
+            kd[at] = k;
+            vd[at] = v;
+            if ((peers & below) == 0) s_hist[d * NW + w] = at + (uint32_t)__popcll(peers);
+        }
+    }
+    __syncthreads();
+}
+
+// LDS of the persistent kernel.
+template <uint32_t T>
+struct PShared {
+    static constexpr uint32_t NW = T / 64;
+    uint32_t hist[256 * NW];
+    float st[NW][3][64];                    // long-row terms, one wave's chunk
+    unsigned long long lim[NW][2 * kLimbs];  // swing / traction limbs per wave
+    unsigned long long tot[2 * kLimbs];
+    uint32_t w[NW];
+    float box[NW][6];
+    uint64_t kbits[NW][2];
+    float speed, eff;
+    uint32_t g;
+};
+
+template <uint32_t T>
+__global__ __launch_bounds__(T) void k_p_run(
+    const BParams P, uint32_t n_iters, uint32_t cur0, const PGraph *__restrict__ pg,
+    const uint32_t *__restrict__ order, uint32_t *__restrict__ queue, GFrame *fr,
+    const double *__restrict__ est_jt, const double *__restrict__ min_jt, float4 *pos0,
+    float4 *pos1, float4 *__restrict__ old_force, float4 *__restrict__ force,
+    float2 *__restrict__ stats, const uint32_t *__restrict__ deg,
+    const uint32_t *__restrict__ att_off, const uint32_t *__restrict__ att_adj,
+    const uint32_t *__restrict__ prow, uint64_t *__restrict__ keys_a,
+    uint64_t *__restrict__ keys_b, uint32_t *__restrict__ vals_a, uint32_t *__restrict__ vals_b,
+    uint32_t *__restrict__ loc, uint32_t *__restrict__ cinfo, Node *__restrict__ nodes_all,
+    uint8_t *__restrict__ depth_all) {
+    constexpr uint32_t NW = T / 64;
+    __shared__ PShared<T> S;
+    // Values the whole workgroup shares are taken through readfirstlane
+    // (uni/unif), so they live in scalar registers: this kernel holds every
+    // phase at once, and vector registers decide its occupancy.
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = uni(tid >> 6);
+    for (;;) {
+        if (tid == 0) S.g = atomicAdd(queue, 1u);
+        __syncthreads();
+        const uint32_t q = uni(S.g);
+        __syncthreads();
+        if (q >= P.n_graphs) break;
+        const uint32_t g = order[q];
+        const PGraph pgr = pg[g];
+        const uint32_t b0 = pgr.base, ng = pgr.count;
+        Node *__restrict__ nodes = nodes_all + pgr.nstart;
+        uint8_t *__restrict__ ndep = depth_all + pgr.nstart;
+        if (tid == 0) {
+            S.speed = fr[g].speed;
+            S.eff = fr[g].speed_eff;
+        }
+        __syncthreads();
+        for (uint32_t it = 0; it < n_iters; it++) {
+            float4 *pin = ((cur0 + it) & 1u) ? pos1 : pos0;
+            float4 *pout = ((cur0 + it) & 1u) ? pos0 : pos1;
+            // 1. box over finite coordinates (k_b_bbox), root cube
+            float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (uint32_t i = tid; i < ng; i += T) {
+                const float4 p = pin[b0 + i];
+                const float v[3] = {p.x + 0.0f, p.y + 0.0f, p.z + 0.0f};
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+                    if (finitef(v[k])) {
+                        mn[k] = fminf(mn[k], v[k]);
+                        mx[k] = fmaxf(mx[k], v[k]);
+                    }
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) {
+                    mn[k] = fminf(mn[k], __shfl_xor(mn[k], o, 64));
+                    mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], o, 64));
+                }
+            if (lane == 0)
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    S.box[w][k] = mn[k];
+                    S.box[w][3 + k] = mx[k];
+                }
+            __syncthreads();
+            uint32_t bbmin[3], bbmax[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                float a = S.box[0][k], c = S.box[0][3 + k];
+                for (uint32_t u = 1; u < NW; u++) {
+                    a = fminf(a, S.box[u][k]);
+                    c = fmaxf(c, S.box[u][3 + k]);
+                }
+                bbmin[k] = uni(finitef(a) ? f2ord(a) : 0xFF800000u);
+                bbmax[k] = uni(finitef(c) ? f2ord(c) : ~0xFF800000u);
+            }
+            float r4[4];
+            compute_root(bbmin, bbmax, r4);
+#pragma unroll
+            for (int k = 0; k < 4; k++) r4[k] = unif(r4[k]);
+            // 2. keys (k_b_keys): invalid bodies get ~0 and sort last
+            uint32_t inval = 0;
+            uint64_t kand = ~0ull, kor = 0ull;
+            for (uint32_t i = tid; i < ng; i += T) {
+                const float4 p = pin[b0 + i];
+                uint64_t key;
+                if (finitef(p.x) && finitef(p.y) && finitef(p.z)) {
+                    key = octant_key(r4, p.x, p.y, p.z);
+                } else {
+                    key = ~0ull;
+                    inval++;
+                }
+                keys_a[b0 + i] = key;
+                vals_a[b0 + i] = b0 + i;
+                kand &= key;
+                kor |= key;
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                kand &= __shfl_xor(kand, o, 64);
+                kor |= __shfl_xor(kor, o, 64);
+                inval += __shfl_xor(inval, o, 64);
+            }
+            if (lane == 0) {
+                S.kbits[w][0] = kand;
+                S.kbits[w][1] = kor;
+                S.w[w] = inval;
+            }
+            __syncthreads();
+            uint64_t vary = 0;
+            uint32_t nvalid = ng;
+            for (uint32_t u = 0; u < NW; u++) {
+                vary |= S.kbits[u][0] ^ S.kbits[u][1];
+                nvalid -= S.w[u];
+            }
+            vary = uni64(vary);
+            nvalid = uni(nvalid);
+            __syncthreads();
+            // 3. stable sort by key (ties keep body order): the digits that vary
+            uint64_t *ks = keys_a + b0, *kt = keys_b + b0;
+            uint32_t *vs = vals_a + b0, *vt = vals_b + b0;
+            for (uint32_t sh = 0; sh < 64; sh += 8) {
+                if (((vary >> sh) & 0xFFull) == 0) continue;
+                p_radix_pass<T>(ks, vs, kt, vt, ng, sh, S.hist, S.w);
+                uint64_t *tk = ks;
+                ks = kt;
+                kt = tk;
+                uint32_t *tv = vs;
+                vs = vt;
+                vt = tv;
+            }
+            // 4. nodes per sorted body (k_b_count_scan): opened cells + its leaf,
+            //    the graph's lone body opens the root; exclusive scan in sorted order
+            const uint32_t nv = nvalid;
+            const uint32_t chunk = (nv + T - 1) / T;
+            const uint32_t c0 = min(tid * chunk, nv), c1 = min(c0 + chunk, nv);
+            uint32_t s = 0, md = 0;
+            for (uint32_t i = c0; i < c1; i++) {
+                int lp, li;
+                if (nv == 1) {
+                    lp = -1;
+                    li = 0;
+                } else {
+                    const uint64_t k = ks[i];
+                    lp = i > 0 ? lcp_digits(ks[i - 1], k) : -1;
+                    li = i + 1 < nv ? lcp_digits(k, ks[i + 1]) : -1;
+                }
+                cinfo[b0 + i] = (uint32_t)(lp + 1) | ((uint32_t)(li + 1) << 8);
+                s += (li > lp ? (uint32_t)(li - lp) : 0u) + 1u;
+                md = max(md, (uint32_t)(li + 1));
+            }
+            uint32_t ntree;
+            uint32_t run = p_excl_scan<T>(s, S.w, &ntree);
+            ntree = uni(ntree);
+            const uint32_t dmax1 = uni(p_max<T>(md, S.w));  // deepest cell depth + 1 (0: no cell)
+            for (uint32_t i = c0; i < c1; i++) {
+                loc[b0 + i] = run;
+                const uint32_t ci = cinfo[b0 + i];
+                const int lp = (int)(ci & 0xFFu) - 1, li = (int)(ci >> 8) - 1;
+                run += (li > lp ? (uint32_t)(li - lp) : 0u) + 1u;
+            }
+            __syncthreads();
+            const uint32_t N = min(ntree, pgr.ncap);
+            // 5. pre-order emission (k_b_emit): cells opened by body i at depths
+            //    lp+1 .. li, then its leaf
+            for (uint32_t i = tid; i < nv; i += T) {
+                const uint32_t ci = cinfo[b0 + i];
+                const int lp = (int)(ci & 0xFFu) - 1, li = (int)(ci >> 8) - 1;
+                const uint32_t c = li > lp ? (uint32_t)(li - lp) : 0u;
+                const uint32_t pos0 = loc[b0 + i];
+                if ((uint64_t)pos0 + c >= pgr.ncap) continue;
+                const uint64_t k = ks[i];
+                for (uint32_t t = 0; t < c; t++) {
+                    const int d = lp + 1 + (int)t;
+                    float h = r4[3];
+                    for (int u = 0; u < d; u++) h = h * 0.5f;
+                    const uint32_t j = cell_end(ks, i, nv, k, d);
+                    const bool last = j + 1 >= nv;
+                    const uint32_t end = last ? ntree : loc[b0 + j + 1];
+                    const float sz = h * 2.0f;
+                    nodes[pos0 + t].b = cell_b(sz * sz, last ? kEnd : end);
+                    ndep[pos0 + t] = (uint8_t)d;
+                }
+                const uint32_t leaf = pos0 + c;
+                const uint32_t b = vs[i];
+                nodes[leaf].a = pin[b];  // .w: the body's mass
+                nodes[leaf].b = leaf_b(b, (i + 1 < nv) ? leaf + 1 : kEnd);
+                ndep[leaf] = (uint8_t)(0x80 | ((lp > li ? lp : li) + 1));
+            }
+            __syncthreads();
+            // 6. centre of mass, deepest cells first, children in octant order
+            for (int d = (int)dmax1 - 1; d >= 0; d--) {
+                for (uint32_t i = tid; i < nv; i += T) {
+                    const uint32_t ci = cinfo[b0 + i];
+                    const int lp = (int)(ci & 0xFFu) - 1, li = (int)(ci >> 8) - 1;
+                    if (d > lp && d <= li) {
+                        const uint32_t node = loc[b0 + i] + (uint32_t)(d - lp - 1);
+                        if (node < N) com_of_cell(nodes, node, N);
+                    }
+                }
+                __syncthreads();
+            }
+            // 7. force (k_b_force): walk, attraction of the short rows, gravity,
+            //    stats; long rows leave the repulsion sum for step 8
+            const bool reducible = tree_reducible(N, r4[3], P.theta2);
+            for (uint32_t i = tid; i < ng; i += T) {
+                const uint32_t b = vs[i];
+                const float4 p = pin[b];
+                const float deg_i = (float)deg[b] + 1.0f;
+                const float krd = P.scaling_ratio * deg_i;
+                float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+                if (N > 0) {
+                    if (reducible && finite3(p)) {
+                        if (P.fastdiv)
+                            walk_reduced<true>(nodes, 0, N, N, p.x, p.y, p.z, b, krd, P.theta2, fx, fy, fz);
+                        else
+                            walk_reduced<false>(nodes, 0, N, N, p.x, p.y, p.z, b, krd, P.theta2, fx, fy, fz);
+                    } else {
+                        (void)walk_exact(nodes, 0, N, p.x, p.y, p.z, b, krd, P.theta2, fx, fy, fz);
+                    }
+                }
+                const uint32_t e0 = att_off[b], e1 = att_off[b + 1];
+                if (e1 - e0 > kBHeavyAtt) {
+                    force[b] = make_float4(fx, fy, fz, 0.0f);
+                    continue;
+                }
+                uint32_t e = e0;
+                for (; e + 4 <= e1; e += 4) {
+                    const uint32_t j0 = att_adj[e], j1 = att_adj[e + 1], j2 = att_adj[e + 2],
+                                   j3 = att_adj[e + 3];
+                    const float4 qv[4] = {pin[j0], pin[j1], pin[j2], pin[j3]};
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        float tx, ty, tz;
+                        b_attraction_term(P, p, qv[u], tx, ty, tz);
+                        fx = fx + tx;
+                        fy = fy + ty;
+                        fz = fz + tz;
+                    }
+                }
+                for (; e < e1; e++) {
+                    float tx, ty, tz;
+                    b_attraction_term(P, p, pin[att_adj[e]], tx, ty, tz);
+                    fx = fx + tx;
+                    fy = fy + ty;
+                    fz = fz + tz;
+                }
+                b_gravity_and_stats(P, b, p, deg_i, fx, fy, fz, old_force, force, stats);
+            }
+            __syncthreads();
+            // 8. long rows (k_b_heavy_terms + k_b_force_heavy): one wave per row,
+            //    64 terms per round into LDS, lanes 0, 1, 2 add x, y, z in list order
+            for (uint32_t r = w; r < pgr.hcount; r += NW) {
+                const uint32_t b = prow[pgr.hstart + r];
+                const float4 p = pin[b];
+                const float4 f0 = force[b];
+                float acc = lane == 0 ? f0.x : lane == 1 ? f0.y : f0.z;
+                const uint32_t e0 = att_off[b], len = att_off[b + 1] - e0;
+                for (uint32_t c0r = 0; c0r < len; c0r += 64) {
+                    const uint32_t cnt = min(64u, len - c0r);
+                    if (lane < cnt) {
+                        float tx, ty, tz;
+                        b_attraction_term(P, p, pin[att_adj[e0 + c0r + lane]], tx, ty, tz);
+                        S.st[w][0][lane] = tx;
+                        S.st[w][1][lane] = ty;
+                        S.st[w][2][lane] = tz;
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    if (lane < 3)
+                        for (uint32_t k = 0; k < cnt; k++) acc = acc + S.st[w][lane][k];
+                    __builtin_amdgcn_wave_barrier();  // read before the next round writes
+                }
+                const float fx = __shfl(acc, 0, 64), fy = __shfl(acc, 1, 64), fz = __shfl(acc, 2, 64);
+                if (lane == 0)
+                    b_gravity_and_stats(P, b, p, (float)deg[b] + 1.0f, fx, fy, fz, old_force, force, stats);
+            }
+            __syncthreads();
+            // 9. exact swing / traction sums and AdaptiveSpeed (k_b_speed)
+            {
+                // each wave adds its bodies' values into its own LDS limbs
+                // (integer adds: the order does not matter)
+                if (lane < 2 * kLimbs) S.lim[w][lane] = 0ull;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                for (uint32_t i = tid; i < ng; i += T) {
+                    const float2 st = stats[b0 + i];
+                    if (finitef(st.x) && finitef(st.y)) {
+                        exact_acc_add(S.lim[w], st.x);
+                        exact_acc_add(S.lim[w] + kLimbs, st.y);
+                    }
+                }
+                __syncthreads();
+                if (tid < 2 * kLimbs) {
+                    unsigned long long a = 0;
+                    for (uint32_t u = 0; u < NW; u++) a += S.lim[u][tid];
+                    S.tot[tid] = a;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (tid == 0) {
+                    // the limbs stay in LDS: this one thread's arithmetic does not
+                    // claim registers from every other phase of the kernel
+                    exact_normalise(S.tot);
+                    exact_normalise(S.tot + kLimbs);
+                    const double ts = exact_to_double(S.tot), tt = exact_to_double(S.tot + kLimbs);
+                    fr[g].sum_swing = ts;
+                    fr[g].sum_traction = tt;
+                    float speed = S.speed, eff = S.eff;
+                    if (ts > 0.0 && tt > 0.0 && ng != 0) {
+                        const double nn = (double)ng, jt0 = (double)P.jitter_tolerance;
+                        double jt = jt0 * fmax(min_jt[g], fmin(10.0, est_jt[g] * tt / (nn * nn)));
+                        const float min_eff = 0.05f;
+                        if (ts / tt > 2.0) {
+                            if (eff > min_eff) eff = eff * 0.5f;
+                            jt = fmax(jt, jt0);
+                        }
+                        const double target = jt * (double)eff * tt / ts;
+                        if (ts > jt * tt) {
+                            if (eff > min_eff) eff = eff * 0.7f;
+                        } else if (speed < 1000.0f) {
+                            eff = eff * 1.3f;
+                        }
+                        const double sp = (double)speed;
+                        S.speed = (float)(sp + fmin(target - sp, 0.5 * sp));
+                        S.eff = eff;
+                    }
+                }
+                __syncthreads();
+            }
+            // 10. apply (k_b_apply)
+            const float speed = unif(S.speed);
+            for (uint32_t i = tid; i < ng; i += T) {
+                const uint32_t b = b0 + i;
+                const float4 f = force[b];
+                const float4 o = old_force[b];
+                const float deg_i = (float)deg[b] + 1.0f;
+                const float swinging = deg_i * len3(o.x - f.x, o.y - f.y, o.z - f.z);
+                float factor = speed / (1.0f + sqrtf(speed * swinging));
+                const float df = len3(f.x, f.y, f.z);
+                if (P.max_displacement > 0.0f && df > 1.0e-9f)
+                    factor = fminf(factor, P.max_displacement / df);
+                const float k = factor * P.time_step;
+                const float4 p = pin[b];
+                pout[b] = make_float4(p.x + f.x * k, p.y + f.y * k, p.z + f.z * k, fmaxf(deg_i, 1e-3f));
+                old_force[b] = make_float4(f.x, f.y, f.z, 0.0f);
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            fr[g].speed = S.speed;
+            fr[g].speed_eff = S.eff;
+        }
+    }
+}
+
+__global__ void k_p_queue_reset(uint32_t *queue) { *queue = 0; }
+
 inline uint32_t cdivb(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 }  // namespace
+
+constexpr uint32_t kSchedAuto = 0, kSchedPhased = 1, kSchedPersistent = 2;
+// The persistent schedule (k_p_run) is switched off.  Its first GPU test (24
+// mixed graphs, 256-thread workgroups) ended in a memory access fault.  The
+// kernel source gives the oracle's bits when its workgroups are run as threads
+// on the CPU, under address and undefined-behaviour checks, and no reading of
+// the code or of its ISA has shown an out-of-bounds access or a missing wait,
+// so the fault's cause is unknown and the kernel is not launched.  The
+// autotuner then keeps the phased schedule.
+constexpr bool kPersistentEnabled = false;
 
 class Fa2BhBatch {
   public:
@@ -815,6 +1354,30 @@ class Fa2BhBatch {
         }
         n_hent_ = (uint32_t)hbody.size();
         hstride_ = n_hent_;
+        // persistent schedule: per-graph node regions, long rows graph-major,
+        // graphs in decreasing size (stable)
+        std::vector<PGraph> pgh(G);
+        std::vector<uint32_t> prow;
+        uint64_t nacc = 0, ncap_max = 0;
+        for (uint32_t g = 0; g < G; g++) {
+            const uint64_t nc = (uint64_t)count[g] + (uint64_t)(kLevels + 1) * (count[g] / 2 + 1) + 1;
+            pgh[g].base = base[g];
+            pgh[g].count = count[g];
+            pgh[g].nstart = (uint32_t)std::min<uint64_t>(nacc, 0xFFFFFFFFull);
+            pgh[g].ncap = (uint32_t)std::min<uint64_t>(nc, 0xFFFFFFFFull);
+            pgh[g].hstart = (uint32_t)prow.size();
+            for (uint32_t v2 = base[g]; v2 < base[g] + count[g]; v2++)
+                if (aoff[v2 + 1] - aoff[v2] > kBHeavyAtt) prow.push_back(v2);
+            pgh[g].hcount = (uint32_t)prow.size() - pgh[g].hstart;
+            nacc += nc;
+            ncap_max = std::max(ncap_max, nc);
+        }
+        std::vector<uint32_t> order(G);
+        for (uint32_t g = 0; g < G; g++) order[g] = g;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint32_t a, uint32_t b) { return count[a] > count[b]; });
+        // the walk addresses a graph's region with 32-bit byte offsets
+        p_ok_ = ncap_max < kMaxNodes32;
         // the sort's composite key: ceil(log2 G) bits of graph id above ck_p_ bits
         // of the raw key, ck_bits_ in all (a multiple of 16); ck_p_ leaves each
         // graph at least about 32 prefixes per body, so runs of equal prefix stay short
@@ -897,6 +1460,13 @@ class Fa2BhBatch {
             JC_TRY(up(hbody_.p, hbody.data(), n_hent_ * 4));
             JC_TRY(up(hadj_.p, hadj.data(), n_hent_ * 4));
         }
+        JC_TRY(pg_.alloc(G));
+        JC_TRY(order_.alloc(G));
+        JC_TRY(prow_.alloc(prow.size()));
+        JC_TRY(queue_.alloc(1));
+        JC_TRY(up(pg_.p, pgh.data(), G * sizeof(PGraph)));
+        JC_TRY(up(order_.p, order.data(), G * 4));
+        if (!prow.empty()) JC_TRY(up(prow_.p, prow.data(), prow.size() * 4));
         JC_HIP(hipMemsetAsync(frame_.p, 0, G * sizeof(GFrame), stream_));
         JC_HIP(hipMemsetAsync(global_.p, 0, sizeof(BGlobal), stream_));
         k_b_init_speed<<<cdivb(G, 256), 256, 0, stream_>>>(frame_.p, G);
@@ -916,6 +1486,13 @@ class Fa2BhBatch {
         refresh_params();
         if (n_ && n_iters) {
             if (!tuned_) JC_TRY(autotune());
+            if (sched_ == kSchedPersistent) {
+                JC_TRY(launch_persistent(n_iters, p_threads_));
+                cur_ ^= n_iters & 1u;
+                iteration_ += n_iters;
+                if (out_xyz) JC_TRY(read_positions(out_xyz));
+                return Status::ok();
+            }
             uint32_t left = n_iters;
             if (left >= 2) {
                 JC_TRY(ensure_graph());
@@ -969,6 +1546,31 @@ class Fa2BhBatch {
     }
 
     uint32_t force_block() const { return force_bs_; }
+
+    // mode: kSchedAuto (tuned at the next step), kSchedPhased or
+    // kSchedPersistent; threads: the persistent workgroup size (0: tuned).
+    Status set_schedule(uint32_t mode, uint32_t threads) {
+        if (mode > kSchedPersistent) return Status::err(JC_E_INVALID, "unknown batch schedule");
+        if (threads != 0 && threads != 256 && threads != 512 && threads != 1024)
+            return Status::err(JC_E_INVALID, "persistent workgroup size must be 0, 256, 512 or 1024");
+        if (mode == kSchedPersistent && !kPersistentEnabled)
+            return Status::err(JC_E_INVALID,
+                               "the persistent batch schedule is switched off: its first GPU run "
+                               "made a memory fault whose cause has not been found");
+        if (mode == kSchedPersistent && inited_ && !p_ok_)
+            return Status::err(JC_E_INVALID, "a graph is too large for the persistent schedule");
+        want_sched_ = mode;
+        want_threads_ = threads;
+        tuned_ = false;
+        drop_graph();
+        return Status::ok();
+    }
+
+    void schedule(uint32_t *mode, uint32_t *threads, uint32_t *fbs) const {
+        if (mode) *mode = tuned_ ? sched_ : kSchedAuto;
+        if (threads) *threads = p_threads_;
+        if (fbs) *fbs = force_bs_;
+    }
 
   private:
     void refresh_params() {
@@ -1060,26 +1662,136 @@ class Fa2BhBatch {
         return Status::ok();
     }
 
-    // Times the force kernel for each candidate block size on the current
-    // positions (tree built once; no state is advanced) and keeps the fastest.
+    Status launch_persistent(uint32_t n_iters, uint32_t threads) {
+        JC_TRY(ensure_persistent_scratch());
+        k_p_queue_reset<<<1, 1, 0, stream_>>>(queue_.p);
+#define JC_PR(T)                                                                                 \
+    k_p_run<T><<<p_grid(T), T, 0, stream_>>>(                                                    \
+        P_, n_iters, cur_, pg_.p, order_.p, queue_.p, frame_.p, est_.p, mjt_.p, pos_[0].p,       \
+        pos_[1].p, old_force_.p, force_.p, stats_.p, deg_.p, att_off_.p, att_adj_.p, prow_.p,    \
+        p_keys_[0].p, p_keys_[1].p, p_vals_[0].p, p_vals_[1].p, loc_.p, p_cinfo_.p, nodes_.p,    \
+        node_depth_.p)
+        switch (threads) {
+            case 256: JC_PR(256); break;
+            case 512: JC_PR(512); break;
+            default: JC_PR(1024); break;
+        }
+#undef JC_PR
+        JC_HIP(hipGetLastError());
+        return Status::ok();
+    }
+
+    // Resident workgroups of k_p_run<T>: every CU full, no more than graphs.
+    uint32_t p_grid(uint32_t T) {
+        int dev = 0, cus = 0, per = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        hipError_t e = hipSuccess;
+        switch (T) {
+            case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_p_run<256>, 256, 0); break;
+            case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_p_run<512>, 512, 0); break;
+            default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_p_run<1024>, 1024, 0); break;
+        }
+        if (e != hipSuccess || per < 1) per = 1;
+        if (cus < 1) cus = 1;
+        return std::max<uint32_t>(1, std::min<uint32_t>(G_, (uint32_t)cus * (uint32_t)per));
+    }
+
+    Status ensure_persistent_scratch() {
+        const size_t nn = n_ ? n_ : 1;
+        if (p_cinfo_.p && p_cinfo_.n == nn) return Status::ok();
+        JC_TRY(p_keys_[0].alloc(nn));
+        JC_TRY(p_keys_[1].alloc(nn));
+        JC_TRY(p_vals_[0].alloc(nn));
+        JC_TRY(p_vals_[1].alloc(nn));
+        JC_TRY(p_cinfo_.alloc(nn));
+        return Status::ok();
+    }
+
+    // Picks the schedule.  Phased: the force kernel's block size is timed on the
+    // current positions (tree built once).  Then, unless a schedule was asked
+    // for, two whole iterations of the phased schedule and of the persistent
+    // one at each workgroup size are timed from the same state (saved before
+    // and restored after each candidate, so no state is advanced) and the
+    // fastest is kept.
     Status autotune() {
-        JC_TRY(enqueue_tree(pos_[cur_].p));
         hipEvent_t a, b;
         JC_HIP(hipEventCreate(&a));
         JC_HIP(hipEventCreate(&b));
-        float best = 1e30f;
-        for (uint32_t bs : {64u, 128u, 256u, 512u, 1024u}) {
-            JC_TRY(launch_force(pos_[cur_].p, bs));  // warm
-            JC_HIP(hipEventRecord(a, stream_));
-            for (int r = 0; r < 3; r++) JC_TRY(launch_force(pos_[cur_].p, bs));
-            JC_HIP(hipEventRecord(b, stream_));
-            JC_HIP(hipEventSynchronize(b));
+        auto elapsed = [&]() -> float {
             float t = 0;
-            JC_HIP(hipEventElapsedTime(&t, a, b));
-            if (t < best) {
-                best = t;
-                force_bs_ = bs;
+            (void)hipEventSynchronize(b);
+            (void)hipEventElapsedTime(&t, a, b);
+            return t;
+        };
+        if (want_sched_ != kSchedPersistent) {
+            JC_TRY(enqueue_tree(pos_[cur_].p));
+            float best = 1e30f;
+            for (uint32_t bs : {64u, 128u, 256u, 512u, 1024u}) {
+                JC_TRY(launch_force(pos_[cur_].p, bs));  // warm
+                JC_HIP(hipEventRecord(a, stream_));
+                for (int r = 0; r < 3; r++) JC_TRY(launch_force(pos_[cur_].p, bs));
+                JC_HIP(hipEventRecord(b, stream_));
+                const float t = elapsed();
+                if (t < best) {
+                    best = t;
+                    force_bs_ = bs;
+                }
             }
+        }
+        if (want_sched_ == kSchedPersistent && !p_ok_)
+            return Status::err(JC_E_INVALID, "a graph is too large for the persistent schedule");
+        sched_ = want_sched_ == kSchedAuto ? kSchedPhased : want_sched_;
+        if (!p_ok_) sched_ = kSchedPhased;
+        p_threads_ = want_threads_ ? want_threads_ : 1024u;
+        const bool try_p = kPersistentEnabled && p_ok_ && want_sched_ != kSchedPhased;
+        const bool pick_t = want_threads_ == 0;
+        if (try_p && (want_sched_ == kSchedAuto || pick_t)) {
+            DevBuf<float4> sp, so;
+            DevBuf<GFrame> sf;
+            JC_TRY(sp.alloc(n_));
+            JC_TRY(so.alloc(n_));
+            JC_TRY(sf.alloc(G_));
+            const size_t pb = (size_t)n_ * sizeof(float4), fb = (size_t)G_ * sizeof(GFrame);
+            JC_HIP(hipMemcpyAsync(sp.p, pos_[cur_].p, pb, hipMemcpyDeviceToDevice, stream_));
+            JC_HIP(hipMemcpyAsync(so.p, old_force_.p, pb, hipMemcpyDeviceToDevice, stream_));
+            JC_HIP(hipMemcpyAsync(sf.p, frame_.p, fb, hipMemcpyDeviceToDevice, stream_));
+            auto restore = [&]() -> Status {
+                JC_HIP(hipMemcpyAsync(pos_[cur_].p, sp.p, pb, hipMemcpyDeviceToDevice, stream_));
+                JC_HIP(hipMemcpyAsync(old_force_.p, so.p, pb, hipMemcpyDeviceToDevice, stream_));
+                JC_HIP(hipMemcpyAsync(frame_.p, sf.p, fb, hipMemcpyDeviceToDevice, stream_));
+                return Status::ok();
+            };
+            // two iterations from the saved state, run twice (the first warms up)
+            auto time_one = [&](uint32_t mode, uint32_t T, float *ms) -> Status {
+                for (int r = 0; r < 2; r++) {
+                    JC_TRY(restore());
+                    JC_HIP(hipEventRecord(a, stream_));
+                    if (mode == kSchedPersistent) {
+                        JC_TRY(launch_persistent(2, T));
+                    } else {
+                        JC_TRY(enqueue_iteration(cur_));
+                        JC_TRY(enqueue_iteration(cur_ ^ 1u));
+                    }
+                    JC_HIP(hipEventRecord(b, stream_));
+                    *ms = elapsed();
+                }
+                return Status::ok();
+            };
+            float best = 1e30f;
+            if (want_sched_ == kSchedAuto) JC_TRY(time_one(kSchedPhased, 0, &best));
+            for (uint32_t T : {256u, 512u, 1024u}) {
+                if (!pick_t && T != want_threads_) continue;
+                float t = 0;
+                JC_TRY(time_one(kSchedPersistent, T, &t));
+                if (t < best) {
+                    best = t;
+                    sched_ = kSchedPersistent;
+                    p_threads_ = T;
+                }
+            }
+            JC_TRY(restore());
+            JC_HIP(hipStreamSynchronize(stream_));
         }
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
@@ -1134,6 +1846,12 @@ class Fa2BhBatch {
 
     jc_fa2bh_settings settings_;
     BParams P_{};
+    // schedule: the autotuner's pick (sched_, p_threads_) or the caller's
+    uint32_t want_sched_ = kSchedAuto, want_threads_ = 0, sched_ = kSchedPhased, p_threads_ = 1024;
+    bool p_ok_ = false;
+    DevBuf<PGraph> pg_;
+    DevBuf<uint32_t> order_, prow_, queue_, p_vals_[2], p_cinfo_;
+    DevBuf<uint64_t> p_keys_[2];
     bool params_dirty_ = true, inited_ = false, tuned_ = false;
     uint32_t G_ = 0, n_ = 0, cur_ = 0, nb_scan_ = 0, ck_bits_ = 16, ck_p_ = 16, force_bs_ = 256;
     uint64_t cap_ = 0, iteration_ = 0;
@@ -1219,6 +1937,25 @@ int jc_fa2bh_batch_step_timed(jc_batch *b, uint32_t n_iters, double *gpu_ms, uin
     if (force_block) *force_block = b->impl.force_block();
     if (!s.failed()) jc::set_error("");
     return s.code;
+}
+
+int jc_fa2bh_batch_set_schedule(jc_batch *b, uint32_t mode, uint32_t threads) {
+    if (!b) {
+        jc::set_error("NULL batch");
+        return JC_E_INVALID;
+    }
+    jc::Status s = b->impl.set_schedule(mode, threads);
+    if (!s.failed()) jc::set_error("");
+    return s.code;
+}
+
+int jc_fa2bh_batch_schedule(jc_batch *b, uint32_t *mode, uint32_t *threads, uint32_t *force_block) {
+    if (!b) {
+        jc::set_error("NULL batch");
+        return JC_E_INVALID;
+    }
+    b->impl.schedule(mode, threads, force_block);
+    return JC_OK;
 }
 
 void jc_fa2bh_batch_destroy(jc_batch *b) { delete b; }

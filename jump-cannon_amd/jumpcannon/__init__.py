@@ -130,6 +130,8 @@ SIGNATURES = {
     "jc_fa2bh_batch_step": (C.c_int, [_vp, C.c_uint32, _vp]),
     "jc_fa2bh_batch_speeds": (C.c_int, [_vp, _vp]),
     "jc_fa2bh_batch_step_timed": (C.c_int, [_vp, C.c_uint32, _vp, _vp]),
+    "jc_fa2bh_batch_set_schedule": (C.c_int, [_vp, C.c_uint32, C.c_uint32]),
+    "jc_fa2bh_batch_schedule": (C.c_int, [_vp, _vp, _vp, _vp]),
     "jc_fa2bh_batch_destroy": (None, [_vp]),
     "jc_mpnn_param_count": (C.c_uint64, []),
     "jc_mpnn_create": (C.c_int, [C.c_uint64, _vp]),
@@ -503,6 +505,23 @@ class Fa2BhBatch:
         bs = np.zeros(1, np.uint32)
         _check(lib().jc_fa2bh_batch_step_timed(self._h, n_iters, _ptr(ms), _ptr(bs)))
         return float(ms[0]), int(bs[0])
+
+    SCHEDULES = {"auto": 0, "phased": 1, "persistent": 2}
+
+    def set_schedule(self, mode: str = "auto", threads: int = 0):
+        """'phased' (one launch per phase, hipGraph replay), 'persistent' (one
+        launch per step, a workgroup per graph; threads 256/512/1024 or 0 for
+        the tuner's pick) or 'auto' (the autotuner times both at the next step)."""
+        if mode not in self.SCHEDULES:
+            raise JcError(JC_E_INVALID, f"unknown schedule {mode!r}")
+        _check(lib().jc_fa2bh_batch_set_schedule(self._h, self.SCHEDULES[mode], int(threads)))
+
+    def schedule(self):
+        """(mode, persistent workgroup size, phased force block size)."""
+        v = np.zeros(3, np.uint32)
+        _check(lib().jc_fa2bh_batch_schedule(self._h, _ptr(v[0:1]), _ptr(v[1:2]), _ptr(v[2:3])))
+        names = {v2: k for k, v2 in self.SCHEDULES.items()}
+        return names[int(v[0])], int(v[1]), int(v[2])
 
     def __del__(self):
         h = getattr(self, "_h", None)

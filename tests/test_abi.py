@@ -197,3 +197,19 @@ def test_buffer_lengths_checked_before_the_call():
         with pytest.raises(jc.JcError) as ei:
             call()
         assert ei.value.code == jc.JC_E_INVALID
+
+
+def test_batch_schedule_arguments_checked_without_a_device():
+    """The batch schedule API validates its arguments on the host: unknown modes
+    and workgroup sizes are JC_E_INVALID, the persistent schedule is switched
+    off, and the phased / autotuned ones are accepted before init."""
+    b = jc.Fa2BhBatch(theta=1.0)
+    b.set_schedule("phased")
+    b.set_schedule("auto")
+    with pytest.raises(jc.JcError):
+        b.set_schedule("persistent", 256)
+    with pytest.raises(jc.JcError):
+        b.set_schedule("nonsense")
+    rc = jc.lib().jc_fa2bh_batch_set_schedule(b._h, 1, 300)
+    assert rc == jc.JC_E_INVALID
+    assert b.schedule()[0] == "auto"

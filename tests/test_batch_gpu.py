@@ -27,14 +27,27 @@ def _mixed(n_graphs, seed, max_n=3000):
     return graphs, pos
 
 
+# (schedule, persistent workgroup size): both schedules must give the same bits.
+# The persistent schedule is switched off in the library (its first GPU run made
+# a memory fault whose cause has not been found), so its cases are skipped.
+PERSISTENT_OFF = "persistent schedule switched off: its first GPU run faulted, cause not found"
+_P = pytest.mark.skip(reason=PERSISTENT_OFF)
+SCHEDULES = [("phased", 0), pytest.param(("persistent", 256), marks=_P),
+             pytest.param(("persistent", 512), marks=_P), pytest.param(("persistent", 1024), marks=_P)]
+SCHED_NAMES = ["phased", pytest.param("persistent", marks=_P)]
+
+
+@pytest.mark.parametrize("sched", SCHEDULES, ids=lambda s: f"{s[0]}{s[1] or ''}")
 @pytest.mark.parametrize("seed", [1, 2])
-def test_batch_equals_standalone_runs(seed):
+def test_batch_equals_standalone_runs(seed, sched):
     graphs, pos = _mixed(24, seed)
     pos[5] = pos[5].copy()
     pos[5][4] = np.nan  # a non-finite body inside one graph
     b = jc.Fa2BhBatch(theta=1.0)
+    b.set_schedule(*sched)
     b.init(graphs, pos)
     out = b.step(25)
+    assert b.schedule()[:2] == sched if sched[0] == "persistent" else b.schedule()[0] == "phased"
     se = b.speeds()
     for g, ((off, nb), p0) in enumerate(zip(graphs, pos)):
         e = jc.Fa2BhEngine(theta=1.0)
@@ -45,9 +58,11 @@ def test_batch_equals_standalone_runs(seed):
         assert np.array_equal(se[g].view(np.uint32), st["speed_eff2"].view(np.uint32))
 
 
-def test_batch_matches_oracle_trajectories():
+@pytest.mark.parametrize("sched", SCHED_NAMES)
+def test_batch_matches_oracle_trajectories(sched):
     graphs, pos = _mixed(8, 7, max_n=800)
     b = jc.Fa2BhBatch(theta=1.0)
+    b.set_schedule(sched)
     b.init(graphs, pos)
     out = b.step(12)
     for g, ((off, nb), p0) in enumerate(zip(graphs, pos)):
@@ -56,15 +71,70 @@ def test_batch_matches_oracle_trajectories():
         assert np.array_equal(out[g].view(np.uint32), want.view(np.uint32)), f"graph {g}"
 
 
-def test_batch_single_graph_and_reuse():
+@pytest.mark.parametrize("sched", SCHED_NAMES)
+def test_batch_single_graph_and_reuse(sched):
     off, nb = O.ba_graph(5000, 3, 4)
     p0 = O.uniform_cube(5000, 2.0, 5)
     b = jc.Fa2BhBatch(theta=0.9)
+    b.set_schedule(sched)
     b.init([(off, nb)], [p0])
     a = b.step(7)[0]
     e = jc.Fa2BhEngine(theta=0.9)
     e.init(off, nb, p0)
     assert np.array_equal(a.view(np.uint32), e.step(7).view(np.uint32))
+    # the same batch continues: 3 single steps then 4 in one call
+    for _ in range(3):
+        a = b.step(1)[0]
+    a = b.step(4)[0]
+    assert np.array_equal(a.view(np.uint32), e.step(7).view(np.uint32))
+
+
+@pytest.mark.skip(reason=PERSISTENT_OFF)
+def test_batch_schedules_switch_mid_run():
+    """Phased and persistent iterations interleaved on one batch give the bits of
+    one standalone run (the schedules share the positions, forces and speeds)."""
+    graphs, pos = _mixed(10, 11, max_n=1500)
+    b = jc.Fa2BhBatch(theta=1.0)
+    b.init(graphs, pos)
+    for i, sched in enumerate(["persistent", "phased", "persistent", "auto"]):
+        b.set_schedule(sched)
+        out = b.step(3 + i)
+    for g, ((off, nb), p0) in enumerate(zip(graphs, pos)):
+        e = jc.Fa2BhEngine(theta=1.0)
+        e.init(off, nb, p0)
+        assert np.array_equal(out[g].view(np.uint32), e.step(18).view(np.uint32)), f"graph {g}"
+
+
+@pytest.mark.skip(reason=PERSISTENT_OFF)
+def test_batch_persistent_edge_shapes():
+    """Single-body graphs, an empty-edge graph, coincident bodies (equal keys),
+    a hub with rows longer than the short-row limit, and all bodies non-finite."""
+    graphs, pos = [], []
+    off, nb = O.csr_from_pairs(1, np.zeros(0, np.uint32))
+    graphs.append((off, nb))
+    pos.append(np.array([0.5, -0.25, 2.0], np.float32))
+    hub = np.array([[0, j] for j in range(1, 700)] + [[j, j + 1] for j in range(1, 699)], np.uint32)
+    graphs.append(O.csr_from_pairs(700, hub.reshape(-1)))
+    pos.append(O.uniform_cube(700, 3.0, 21))
+    dup = O.uniform_cube(300, 1.0, 22).reshape(-1, 3)
+    dup[100:200] = dup[100]  # 100 coincident bodies
+    graphs.append(O.ba_graph(300, 2, 23))
+    pos.append(dup.reshape(-1))
+    graphs.append(O.csr_from_pairs(5, np.zeros(0, np.uint32)))
+    pos.append(np.full(15, np.nan, np.float32))
+    graphs.append(O.sbm_graph(2500, 4, 0.01, 0.001, 24))
+    pos.append(O.uniform_cube(2500, 10.0, 25))
+    b = jc.Fa2BhBatch(theta=1.0)
+    b.set_schedule("persistent", 256)
+    b.init(graphs, pos)
+    out = b.step(9)
+    se = b.speeds()
+    for g, ((off, nb), p0) in enumerate(zip(graphs, pos)):
+        e = jc.Fa2BhEngine(theta=1.0)
+        e.init(off, nb, p0)
+        want = e.step(9)
+        assert np.array_equal(out[g].view(np.uint32), want.view(np.uint32)), f"graph {g}"
+        assert np.array_equal(se[g].view(np.uint32), e.get_state()["speed_eff2"].view(np.uint32))
 
 
 def test_batch_4096_sampled_graphs_equal_standalone():
@@ -76,6 +146,7 @@ def test_batch_4096_sampled_graphs_equal_standalone():
     b = jc.Fa2BhBatch(theta=1.0)
     b.init(graphs, pos)
     out = b.step(500)
+    print("schedule", b.schedule())
     sizes = np.array([len(o) - 1 for o, _ in graphs])
     pick = [int(np.argmin(sizes)), int(np.argmax(sizes)), 0, 1, 2047, 4095]
     pick += [int(g) for g in np.random.default_rng(4096).choice(4096, 6, replace=False)]
