@@ -198,7 +198,8 @@ def bench_batch(args) -> int:
     b.step(2, readback=False)
     sched = b.schedule()
     b_ms, bs = b.step_timed(iters)
-    log(f"batch: {iters} iterations in {b_ms:.1f} ms (schedule {sched[0]}, force block {bs})")
+    log(f"batch: {iters} iterations in {b_ms:.1f} ms (schedule {sched[0]}, force block {bs}, "
+        f"centre of mass from depth {sched[3]} in chunks of {sched[4]}, sort key {sched[5]} bits)")
     seq_ms = 0.0
     stride = max(1, args.seq_stride)
     sample = range(0, len(graphs), stride) if not args.no_sequential else range(0)
@@ -220,6 +221,8 @@ def bench_batch(args) -> int:
                   "sizes": "log-uniform 50..20000, half SBM half BA", "theta": 1.0,
                   "parallelism": "single-gpu"},
                  batch_ms=round(b_ms, 2), force_block=bs, schedule=sched[0],
+                 tuned={"force_block": sched[2], "com_fuse_depth": sched[3], "com_chunk": sched[4],
+                        "sort_key_bits": sched[5]},
                  sequential_ms_estimate=round(seq_ms, 1) if len(sample) else None,
                  speedup_vs_sequential=round(seq_ms / b_ms, 2) if len(sample) else None,
                  sequential_sample=(f"{which}, each its own engine, 20 timed iterations after 2, scaled to "

@@ -325,10 +325,20 @@ int jc_fa2bh_batch_step_timed(jc_batch *b, uint32_t n_iters, double *gpu_ms,
  * (its first GPU run made a memory fault whose cause has not been found); the
  * autotuner then keeps mode 1. */
 int jc_fa2bh_batch_set_schedule(jc_batch *b, uint32_t mode, uint32_t threads);
-/* The schedule in use (mode 0 until the first step has tuned it), the
- * persistent workgroup size and the phased force block size. */
-int jc_fa2bh_batch_schedule(jc_batch *b, uint32_t *mode, uint32_t *threads,
-                            uint32_t *force_block);
+/* Knobs of the phased schedule (no reference counterpart), each 0 to leave it
+ * to the autotuner, which times the candidates on the live positions at the
+ * next step: force block size (64, 128, 256, 512, 1024), centre-of-mass fusion
+ * depth (3, 4, 5: cells at least that deep summed in one launch) and chunk
+ * (1024 or 2048 nodes per workgroup), composite sort key width (16 to 64 bits,
+ * a multiple of 16, no narrower than the width the batch needs for its graph
+ * ids and key prefixes).  Every setting gives the same bits; invalid values are
+ * JC_E_INVALID. */
+int jc_fa2bh_batch_set_tuning(jc_batch *b, uint32_t force_block, uint32_t com_depth,
+                              uint32_t com_chunk, uint32_t sort_key_bits);
+/* The schedule in use, out[6]: mode (0 until the first step has tuned it),
+ * persistent workgroup size, force block size, centre-of-mass fusion depth and
+ * chunk, sort key width. */
+int jc_fa2bh_batch_schedule(jc_batch *b, uint32_t *out);
 void jc_fa2bh_batch_destroy(jc_batch *b);
 
 /* ---- learned warm start: 3-layer message-passing network, hidden 128 ----

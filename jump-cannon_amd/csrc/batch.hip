@@ -36,10 +36,10 @@ constexpr uint32_t kBScanTile = kBT * kBScanItems;
 constexpr uint32_t kBEmitTasks = 1024;    // cell tasks per k_b_emit block held in LDS
 constexpr uint32_t kBHeavyAtt = 32;       // attraction rows longer than this go to k_b_force_heavy
 constexpr uint32_t kBHeavyStage = 1024;   // heavy-row terms staged in LDS per round
-// Centre of mass: cells of depth >= kBFuseDepth in one launch (com_fused_pass),
-// the shallower ones one launch per depth.
-constexpr uint32_t kBFuseDepth = 3;
-constexpr uint32_t kBFuseChunk = 1024;
+// Centre of mass: cells of depth >= the fusion depth (3, 4 or 5, picked by the
+// autotuner) in one launch (com_fused_pass, chunks of 1024 or 2048 nodes), the
+// shallower ones one launch per depth.
+constexpr uint32_t kBFuseDepthMin = 3;
 constexpr uint32_t kBFuseCap = 512;
 constexpr uint32_t kBFuseThreads = 512;
 constexpr uint32_t kBComGrid = 2048;
@@ -419,24 +419,27 @@ __global__ __launch_bounds__(kBT) void k_b_emit(
     }
 }
 
-// The cells of depth >= kBFuseDepth over the whole forest: each fitting
-// subtree summed in LDS by one workgroup (com_fused_pass), then the ones too
-// large for it (com_big_pass).
+// The cells of depth >= Depth over the whole forest: each fitting subtree
+// summed in LDS by one workgroup (com_fused_pass), then the ones too large for
+// it (com_big_pass).  Every (Depth, Chunk) gives the same bits: each cell is
+// summed from its children in octant order by the same operations.
+template <uint32_t Depth, uint32_t Chunk>
 __global__ __launch_bounds__(kBFuseThreads) void k_b_com_fused(BGlobal *gl, uint32_t cap,
                                                                Node *__restrict__ nodes,
                                                                const uint8_t *__restrict__ node_depth,
                                                                const uint32_t *__restrict__ cskip,
                                                                uint32_t *__restrict__ big, uint32_t big_max) {
-    com_fused_pass<kBFuseDepth, kBFuseChunk, kBFuseCap, kBFuseThreads>(
+    com_fused_pass<Depth, Chunk, kBFuseCap, kBFuseThreads>(
         min(gl->n_tree_total, cap), nodes, node_depth, cskip, big, big_max, &gl->n_big);
 }
 
+template <uint32_t Depth>
 __global__ __launch_bounds__(1024) void k_b_com_big(const BGlobal *gl, uint32_t cap, Node *__restrict__ nodes,
                                                     const uint8_t *__restrict__ node_depth,
                                                     const uint32_t *__restrict__ big, uint32_t big_max) {
     const uint32_t nbig = min(gl->n_big, big_max);
     if (!nbig) return;
-    com_big_pass<kBFuseDepth>(min(gl->n_tree_total, cap), gl->max_depth, nodes, node_depth, big, nbig);
+    com_big_pass<Depth>(min(gl->n_tree_total, cap), gl->max_depth, nodes, node_depth, big, nbig);
 }
 
 // The cells of depth d over the whole forest (com_level_pass).
@@ -1387,8 +1390,11 @@ class Fa2BhBatch {
         uint32_t lc = 0;
         while ((1ull << lc) < cmax) lc++;
         const uint32_t pmin = std::min<uint32_t>(std::max<uint32_t>(16, lc + 5), 63);
-        ck_bits_ = std::min<uint32_t>((gb + pmin + 15) / 16 * 16, 64);
-        ck_p_ = ck_bits_ - gb;
+        gb_ = gb;
+        ck_min_ = std::min<uint32_t>((gb + pmin + 15) / 16 * 16, 64);
+        if (want_ck_ && want_ck_ < ck_min_)
+            return Status::err(JC_E_INVALID, "sort key width narrower than the batch needs");
+        set_ck_bits(want_ck_ ? want_ck_ : ck_min_);
 
         JC_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         const uint32_t nn = n_ ? n_ : 1;
@@ -1429,7 +1435,7 @@ class Fa2BhBatch {
         JC_TRY(cskip_.alloc(cap_));
         // Cells that do not fit k_b_com_fused at one depth are disjoint and hold
         // more than kBFuseCap nodes each.
-        big_max_ = (uint32_t)std::min<uint64_t>((uint64_t)(kLevels + 1 - kBFuseDepth) * (cap_ / kBFuseCap + 1),
+        big_max_ = (uint32_t)std::min<uint64_t>((uint64_t)(kLevels + 1 - kBFuseDepthMin) * (cap_ / kBFuseCap + 1),
                                                 0xFFFFFFFFull);
         JC_TRY(big_.alloc(big_max_));
         JC_TRY(frame_.alloc(G));
@@ -1566,13 +1572,47 @@ class Fa2BhBatch {
         return Status::ok();
     }
 
-    void schedule(uint32_t *mode, uint32_t *threads, uint32_t *fbs) const {
-        if (mode) *mode = tuned_ ? sched_ : kSchedAuto;
-        if (threads) *threads = p_threads_;
-        if (fbs) *fbs = force_bs_;
+    // The phased schedule's knobs, each 0 to leave it to the autotuner: force
+    // block size (64 to 1024), centre-of-mass fusion depth (3 to 5) and chunk
+    // (1024 or 2048 nodes), composite sort key width in bits (16 to 64, a
+    // multiple of 16, no narrower than the one that holds the graph id and a
+    // key prefix).  Every setting gives the same bits.
+    Status set_tuning(uint32_t fbs, uint32_t depth, uint32_t chunk, uint32_t ck) {
+        if (fbs != 0 && fbs != 64 && fbs != 128 && fbs != 256 && fbs != 512 && fbs != 1024)
+            return Status::err(JC_E_INVALID, "force block size must be 0, 64, 128, 256, 512 or 1024");
+        if (depth != 0 && (depth < 3 || depth > 5))
+            return Status::err(JC_E_INVALID, "centre-of-mass fusion depth must be 0, 3, 4 or 5");
+        if (chunk != 0 && chunk != 1024 && chunk != 2048)
+            return Status::err(JC_E_INVALID, "centre-of-mass chunk must be 0, 1024 or 2048");
+        if (ck != 0 && (ck % 16 || ck > 64 || (inited_ && ck < ck_min_)))
+            return Status::err(JC_E_INVALID, "sort key width must be 0 or a multiple of 16 up to 64, "
+                                             "no narrower than the batch needs");
+        want_fbs_ = fbs;
+        want_com_depth_ = depth;
+        want_com_chunk_ = chunk;
+        want_ck_ = ck;
+        tuned_ = false;
+        drop_graph();
+        return Status::ok();
+    }
+
+    // mode, persistent workgroup size, force block size, centre-of-mass fusion
+    // depth and chunk, sort key width (6 values; mode 0 until the first step)
+    void schedule(uint32_t *out) const {
+        out[0] = tuned_ ? sched_ : kSchedAuto;
+        out[1] = p_threads_;
+        out[2] = force_bs_;
+        out[3] = com_depth_;
+        out[4] = com_chunk_;
+        out[5] = ck_bits_;
     }
 
   private:
+    void set_ck_bits(uint32_t bits) {
+        ck_bits_ = bits;
+        ck_p_ = bits - gb_;
+    }
+
     void refresh_params() {
         if (!params_dirty_) return;
         const jc_fa2bh_settings &s = settings_;
@@ -1612,13 +1652,25 @@ class Fa2BhBatch {
                                                      base_.p, count_.p, n, frame_.p, global_.p,
                                                      loc_.p, blk_.p, nodes_.p, node_depth_.p,
                                                      cap_, cskip_.p);
-        const uint32_t gfuse = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cdivb(cap_, kBFuseChunk), kBComGrid));
-        k_b_com_fused<<<gfuse, kBFuseThreads, 0, stream_>>>(global_.p, (uint32_t)cap_, nodes_.p, node_depth_.p,
-                                                             cskip_.p, big_.p, big_max_);
-        k_b_com_big<<<1, 1024, 0, stream_>>>(global_.p, (uint32_t)cap_, nodes_.p, node_depth_.p, big_.p,
-                                             big_max_);
+        const uint32_t gfuse = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cdivb(cap_, com_chunk_), kBComGrid));
+#define JC_BCOM(D, C)                                                                              \
+    k_b_com_fused<D, C><<<gfuse, kBFuseThreads, 0, stream_>>>(global_.p, (uint32_t)cap_, nodes_.p, \
+                                                              node_depth_.p, cskip_.p, big_.p,    \
+                                                              big_max_);                          \
+    k_b_com_big<D><<<1, 1024, 0, stream_>>>(global_.p, (uint32_t)cap_, nodes_.p, node_depth_.p,   \
+                                            big_.p, big_max_)
+        switch (com_depth_ * 10000 + com_chunk_) {
+            case 31024: JC_BCOM(3, 1024); break;
+            case 32048: JC_BCOM(3, 2048); break;
+            case 41024: JC_BCOM(4, 1024); break;
+            case 42048: JC_BCOM(4, 2048); break;
+            case 51024: JC_BCOM(5, 1024); break;
+            case 52048: JC_BCOM(5, 2048); break;
+            default: return Status::err(JC_E_STATE, "unknown centre-of-mass variant");
+        }
+#undef JC_BCOM
         const uint32_t gcom = std::min<uint32_t>(cdivb(cap_, kComSpan * (kBT / 64)), 1024);
-        for (int d = (int)kBFuseDepth - 1; d >= 0; d--)
+        for (int d = (int)com_depth_ - 1; d >= 0; d--)
             k_b_com_level<<<gcom, kBT, 0, stream_>>>(global_.p, (uint32_t)d, (uint32_t)cap_,
                                                      nodes_.p, node_depth_.p);
         JC_HIP(hipGetLastError());
@@ -1725,9 +1777,60 @@ class Fa2BhBatch {
             return t;
         };
         if (want_sched_ != kSchedPersistent) {
-            JC_TRY(enqueue_tree(pos_[cur_].p));
+            // tree builds from the current positions (no state is advanced):
+            // one to warm up, three timed
+            auto time_tree = [&](float *ms) -> Status {
+                JC_TRY(enqueue_tree(pos_[cur_].p));
+                JC_HIP(hipEventRecord(a, stream_));
+                for (int r = 0; r < 3; r++) JC_TRY(enqueue_tree(pos_[cur_].p));
+                JC_HIP(hipEventRecord(b, stream_));
+                *ms = elapsed();
+                return Status::ok();
+            };
+            // 1. centre of mass: fusion depth and chunk
             float best = 1e30f;
+            uint32_t bd = want_com_depth_ ? want_com_depth_ : 3, bc = want_com_chunk_ ? want_com_chunk_ : 1024;
+            if (!want_com_depth_ || !want_com_chunk_) {
+                for (uint32_t d : {3u, 4u, 5u}) {
+                    if (want_com_depth_ && d != want_com_depth_) continue;
+                    for (uint32_t c : {1024u, 2048u}) {
+                        if (want_com_chunk_ && c != want_com_chunk_) continue;
+                        com_depth_ = d;
+                        com_chunk_ = c;
+                        float t = 0;
+                        JC_TRY(time_tree(&t));
+                        if (t < best) {
+                            best = t;
+                            bd = d;
+                            bc = c;
+                        }
+                    }
+                }
+            }
+            com_depth_ = bd;
+            com_chunk_ = bc;
+            // 2. sort: the narrowest composite key (more equal-key runs to rank)
+            //    or 16 bits wider (two more radix passes, shorter runs)
+            if (!want_ck_) {
+                uint32_t bk = ck_min_;
+                best = 1e30f;
+                for (uint32_t ck = ck_min_; ck <= std::min<uint32_t>(ck_min_ + 16, 64); ck += 16) {
+                    set_ck_bits(ck);
+                    float t = 0;
+                    JC_TRY(time_tree(&t));
+                    if (t < best) {
+                        best = t;
+                        bk = ck;
+                    }
+                }
+                set_ck_bits(bk);
+            }
+            // 3. force block size, over one tree
+            JC_TRY(enqueue_tree(pos_[cur_].p));
+            force_bs_ = want_fbs_ ? want_fbs_ : force_bs_;
+            best = 1e30f;
             for (uint32_t bs : {64u, 128u, 256u, 512u, 1024u}) {
+                if (want_fbs_) break;
                 JC_TRY(launch_force(pos_[cur_].p, bs));  // warm
                 JC_HIP(hipEventRecord(a, stream_));
                 for (int r = 0; r < 3; r++) JC_TRY(launch_force(pos_[cur_].p, bs));
@@ -1854,6 +1957,10 @@ class Fa2BhBatch {
     DevBuf<uint64_t> p_keys_[2];
     bool params_dirty_ = true, inited_ = false, tuned_ = false;
     uint32_t G_ = 0, n_ = 0, cur_ = 0, nb_scan_ = 0, ck_bits_ = 16, ck_p_ = 16, force_bs_ = 256;
+    // tuned knobs (want_*: the caller's choice, 0 = tuned) and the composite
+    // key's graph-id bits and narrowest width
+    uint32_t com_depth_ = 3, com_chunk_ = 1024, gb_ = 0, ck_min_ = 16;
+    uint32_t want_fbs_ = 0, want_com_depth_ = 0, want_com_chunk_ = 0, want_ck_ = 0;
     uint64_t cap_ = 0, iteration_ = 0;
     std::vector<uint32_t> base_h_, count_h_;
     hipStream_t stream_ = nullptr;
@@ -1949,12 +2056,23 @@ int jc_fa2bh_batch_set_schedule(jc_batch *b, uint32_t mode, uint32_t threads) {
     return s.code;
 }
 
-int jc_fa2bh_batch_schedule(jc_batch *b, uint32_t *mode, uint32_t *threads, uint32_t *force_block) {
+int jc_fa2bh_batch_set_tuning(jc_batch *b, uint32_t force_block, uint32_t com_depth,
+                              uint32_t com_chunk, uint32_t sort_key_bits) {
     if (!b) {
         jc::set_error("NULL batch");
         return JC_E_INVALID;
     }
-    b->impl.schedule(mode, threads, force_block);
+    jc::Status s = b->impl.set_tuning(force_block, com_depth, com_chunk, sort_key_bits);
+    if (!s.failed()) jc::set_error("");
+    return s.code;
+}
+
+int jc_fa2bh_batch_schedule(jc_batch *b, uint32_t *out) {
+    if (!b || !out) {
+        jc::set_error(!b ? "NULL batch" : "NULL output");
+        return JC_E_INVALID;
+    }
+    b->impl.schedule(out);
     return JC_OK;
 }
 

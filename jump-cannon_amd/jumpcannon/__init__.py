@@ -131,7 +131,8 @@ SIGNATURES = {
     "jc_fa2bh_batch_speeds": (C.c_int, [_vp, _vp]),
     "jc_fa2bh_batch_step_timed": (C.c_int, [_vp, C.c_uint32, _vp, _vp]),
     "jc_fa2bh_batch_set_schedule": (C.c_int, [_vp, C.c_uint32, C.c_uint32]),
-    "jc_fa2bh_batch_schedule": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "jc_fa2bh_batch_schedule": (C.c_int, [_vp, _vp]),
+    "jc_fa2bh_batch_set_tuning": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     "jc_fa2bh_batch_destroy": (None, [_vp]),
     "jc_mpnn_param_count": (C.c_uint64, []),
     "jc_mpnn_create": (C.c_int, [C.c_uint64, _vp]),
@@ -516,12 +517,20 @@ class Fa2BhBatch:
             raise JcError(JC_E_INVALID, f"unknown schedule {mode!r}")
         _check(lib().jc_fa2bh_batch_set_schedule(self._h, self.SCHEDULES[mode], int(threads)))
 
+    def set_tuning(self, force_block: int = 0, com_depth: int = 0, com_chunk: int = 0,
+                   sort_key_bits: int = 0):
+        """Fixes knobs of the phased schedule (0: the autotuner picks): force block
+        size, centre-of-mass fusion depth and chunk, composite sort key width."""
+        _check(lib().jc_fa2bh_batch_set_tuning(self._h, int(force_block), int(com_depth),
+                                               int(com_chunk), int(sort_key_bits)))
+
     def schedule(self):
-        """(mode, persistent workgroup size, phased force block size)."""
-        v = np.zeros(3, np.uint32)
-        _check(lib().jc_fa2bh_batch_schedule(self._h, _ptr(v[0:1]), _ptr(v[1:2]), _ptr(v[2:3])))
+        """(mode, persistent workgroup size, force block size, centre-of-mass
+        fusion depth, centre-of-mass chunk, sort key width)."""
+        v = np.zeros(6, np.uint32)
+        _check(lib().jc_fa2bh_batch_schedule(self._h, _ptr(v)))
         names = {v2: k for k, v2 in self.SCHEDULES.items()}
-        return names[int(v[0])], int(v[1]), int(v[2])
+        return (names[int(v[0])],) + tuple(int(x) for x in v[1:])
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -213,3 +213,15 @@ def test_batch_schedule_arguments_checked_without_a_device():
     rc = jc.lib().jc_fa2bh_batch_set_schedule(b._h, 1, 300)
     assert rc == jc.JC_E_INVALID
     assert b.schedule()[0] == "auto"
+
+
+def test_batch_tuning_knobs_checked_without_a_device():
+    """Knobs of the phased batch schedule are validated on the host."""
+    b = jc.Fa2BhBatch(theta=1.0)
+    b.set_tuning(force_block=512, com_depth=4, com_chunk=2048, sort_key_bits=48)
+    b.set_tuning()
+    for bad in (dict(force_block=100), dict(com_depth=2), dict(com_depth=6), dict(com_chunk=512),
+                dict(sort_key_bits=40), dict(sort_key_bits=80)):
+        with pytest.raises(jc.JcError):
+            b.set_tuning(**bad)
+    assert b.schedule()[0] == "auto" and len(b.schedule()) == 6

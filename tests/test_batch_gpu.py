@@ -137,6 +137,43 @@ def test_batch_persistent_edge_shapes():
         assert np.array_equal(se[g].view(np.uint32), e.get_state()["speed_eff2"].view(np.uint32))
 
 
+@pytest.fixture(scope="module")
+def knob_case():
+    """A mixed batch and its graphs' standalone trajectories (10 iterations)."""
+    graphs, pos = _mixed(14, 31, max_n=2500)
+    want = []
+    for (off, nb), p0 in zip(graphs, pos):
+        e = jc.Fa2BhEngine(theta=1.0)
+        e.init(off, nb, p0)
+        want.append(e.step(10))
+    return graphs, pos, want
+
+
+# (force block, centre-of-mass fusion depth, chunk, sort key width 0 = narrowest / 1 = +16)
+KNOBS = [(64, 3, 1024, 0), (1024, 3, 2048, 1), (256, 4, 1024, 1), (512, 4, 2048, 0),
+         (128, 5, 1024, 0), (1024, 5, 2048, 1)]
+
+
+@pytest.mark.parametrize("knobs", KNOBS, ids=lambda k: "fb{}-d{}-c{}-w{}".format(*k))
+def test_batch_every_tuning_knob_gives_the_same_bits(knob_case, knobs):
+    """Each setting the autotuner can pick is bit-identical to the standalone runs."""
+    graphs, pos, want = knob_case
+    fb, depth, chunk, wide = knobs
+    b = jc.Fa2BhBatch(theta=1.0)
+    b.init(graphs, pos)
+    bits = b.schedule()[5]  # the narrowest key width this batch allows
+    b2 = jc.Fa2BhBatch(theta=1.0)
+    b2.set_tuning(force_block=fb, com_depth=depth, com_chunk=chunk,
+                  sort_key_bits=min(bits + 16 * wide, 64))
+    b2.init(graphs, pos)
+    out = b2.step(10)
+    sched = b2.schedule()
+    assert sched[2:5] == (fb, depth, chunk)
+    assert sched[5] == min(bits + 16 * wide, 64)
+    for g in range(len(graphs)):
+        assert np.array_equal(out[g].view(np.uint32), want[g].view(np.uint32)), f"graph {g}"
+
+
 def test_batch_4096_sampled_graphs_equal_standalone():
     """The 4,096-graph workload (log-uniform 50..20,000, half SBM, half BA) for
     its configured 500 iterations: run the whole batch, then check a spread of
