@@ -309,24 +309,63 @@ __device__ __forceinline__ uint32_t first_node_of(const uint32_t *loc, const uin
     return loc[i] + blk[i / kScanTile] + i;
 }
 
-// One internal cell: depth d, opened by sorted body i at node index `node`.
-// Its half-size is the root's halved d times (fa2_bh.rs:855-861); its skip rope
-// is the first node of the body after the last one sharing its d-digit prefix
-// (for a cell of depth <= kTopDepth k_top_build sets it from the slot table).
-// Returns the cell's Node::b.
-__device__ __forceinline__ uint4 cell_record(const uint64_t *__restrict__ keys, uint32_t nv,
-                                             uint32_t i, int d, uint32_t node, const Frame *fr,
-                                             const uint32_t *__restrict__ loc,
-                                             const uint32_t *__restrict__ blk,
-                                             uint32_t *__restrict__ top_slot) {
+// k_emit stages its block's sorted keys and the next kEmitWin ones in LDS, with
+// each body's first node.  cell_end_win is cell_end with its probes read from
+// that window [w0, w0 + wn) when they fall in it and from global memory
+// otherwise: the same probes in the same order, so the same answer.  Most cells
+// end within the window, so their random global reads become LDS reads.
+constexpr uint32_t kEmitWin = 256;
+__device__ __forceinline__ uint32_t cell_end_win(const uint64_t *__restrict__ keys,
+                                                 const uint64_t *wk, uint32_t w0, uint32_t wn,
+                                                 uint32_t i, uint32_t nv, uint64_t k, int d) {
+    if (d == 0) return nv - 1;
+    const int sh = 3 * (kLevels - d);
+    const uint64_t pre = k >> sh;
+    uint32_t lo = i, step = 1, hi;
+    for (;;) {
+        const uint32_t probe = lo + step;
+        if (probe >= nv) {
+            hi = nv;
+            break;
+        }
+        const uint64_t kp = probe - w0 < wn ? wk[probe - w0] : keys[probe];
+        if ((kp >> sh) != pre) {
+            hi = probe;
+            break;
+        }
+        lo = probe;
+        step <<= 1;
+    }
+    while (hi - lo > 1) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        const uint64_t km = mid - w0 < wn ? wk[mid - w0] : keys[mid];
+        if ((km >> sh) == pre) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// One internal cell: depth d, opened by sorted body i (one of the block's) at
+// node index `node`.  Its half-size is the root's halved d times
+// (fa2_bh.rs:855-861); its skip rope is the first node of the body after the
+// last one sharing its d-digit prefix (for a cell of depth <= kTopDepth
+// k_top_build sets it from the slot table).  Returns the cell's Node::b.
+__device__ __forceinline__ uint4 cell_record_win(const uint64_t *__restrict__ keys, uint32_t nv,
+                                                 uint32_t i, int d, uint32_t node, const Frame *fr,
+                                                 const uint32_t *__restrict__ loc,
+                                                 const uint32_t *__restrict__ blk,
+                                                 uint32_t *__restrict__ top_slot,
+                                                 const uint64_t *wk, const uint32_t *wf,
+                                                 uint32_t w0, uint32_t wn) {
     float h = fr->root[3];
     for (int q = 0; q < d; q++) h = h * 0.5f;
     uint32_t skip = kEnd;
+    const uint64_t ki = wk[i - w0];  // i is one of the block's bodies
     if (d <= kTopDepth) {
-        top_slot[top_slot_of(keys[i], d)] = node;
+        top_slot[top_slot_of(ki, d)] = node;
     } else {
-        const uint32_t j = cell_end(keys, i, nv, keys[i], d);
-        skip = (j + 1 < nv) ? first_node_of(loc, blk, j + 1) : kEnd;
+        const uint32_t j = cell_end_win(keys, wk, w0, wn, i, nv, ki, d);
+        if (j + 1 < nv) skip = j + 1 - w0 < wn ? wf[j + 1 - w0] : first_node_of(loc, blk, j + 1);
     }
     const float s = h * 2.0f;
     const float s2 = s * s;
@@ -353,19 +392,23 @@ __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
     __shared__ uint32_t ntask;
     __shared__ uint32_t t_node[kEmitTasks], t_body[kEmitTasks];
     __shared__ uint8_t t_depth[kEmitTasks];
+    __shared__ uint64_t wk[kTB + kEmitWin];  // sorted keys [w0, w0 + wn)
+    __shared__ uint32_t wf[kTB + kEmitWin];  // first node of each of those bodies
+    const uint32_t nv = n - fr->n_invalid;
+    const uint32_t w0 = blockIdx.x * kTB;
+    const uint32_t wn = w0 < nv ? min(kTB + kEmitWin, nv - w0) : 0u;
     if (threadIdx.x == 0) {
         ntask = 0;
     }
+    for (uint32_t q = threadIdx.x; q < wn; q += kTB) {
+        wk[q] = keys[w0 + q];
+        wf[q] = first_node_of(loc, blk, w0 + q);
+    }
     __syncthreads();
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nv = n - fr->n_invalid;
-    // the neighbours' keys come from the neighbouring lanes; the wave's first
-    // and last lanes load the one past the wave's edge
-    const uint32_t lane = threadIdx.x & 63;
-    const uint64_t k = i < nv ? keys[i] : 0ull;
-    uint64_t kprev = __shfl_up(k, 1, 64), knext = __shfl_down(k, 1, 64);
-    if (lane == 0 && i > 0 && i < nv) kprev = keys[i - 1];
-    if (lane == 63 && i + 1 < nv) knext = keys[i + 1];
+    const uint32_t i = w0 + threadIdx.x;
+    const uint64_t k = i < nv ? wk[threadIdx.x] : 0ull;
+    const uint64_t kprev = (i > 0 && i < nv) ? (threadIdx.x > 0 ? wk[threadIdx.x - 1] : keys[i - 1]) : 0ull;
+    const uint64_t knext = i + 1 < nv ? wk[threadIdx.x + 1] : 0ull;
     if (i < nv) {
         int lp, li;
         if (nv == 1) {
@@ -377,7 +420,7 @@ __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
         }
         const uint32_t c = li > lp ? (uint32_t)(li - lp) : 0u;
         const int a0 = lp + 1;
-        const uint32_t pos0 = first_node_of(loc, blk, i);
+        const uint32_t pos0 = wf[threadIdx.x];
         if ((uint64_t)pos0 + c < cap) {  // always, for a consistent scan
             if (c) {
                 const uint32_t slot = atomicAdd(&ntask, c);
@@ -388,7 +431,8 @@ __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
                         t_body[slot + t] = i;
                         t_depth[slot + t] = (uint8_t)d;
                     } else {
-                        const uint4 rec = cell_record(keys, nv, i, d, pos0 + t, fr, loc, blk, top_slot);
+                        const uint4 rec = cell_record_win(keys, nv, i, d, pos0 + t, fr, loc, blk, top_slot,
+                                                          wk, wf, w0, wn);
                         nodes[pos0 + t].b = rec;
                         cskip[pos0 + t] = rec.w;
                         node_depth[pos0 + t] = (uint8_t)d;
@@ -410,7 +454,8 @@ __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
     __syncthreads();
     const uint32_t nt = min(ntask, kEmitTasks);
     for (uint32_t q = threadIdx.x; q < nt; q += blockDim.x) {
-        const uint4 rec = cell_record(keys, nv, t_body[q], t_depth[q], t_node[q], fr, loc, blk, top_slot);
+        const uint4 rec = cell_record_win(keys, nv, t_body[q], t_depth[q], t_node[q], fr, loc, blk,
+                                          top_slot, wk, wf, w0, wn);
         nodes[t_node[q]].b = rec;
         cskip[t_node[q]] = rec.w;
         node_depth[t_node[q]] = t_depth[q];
