@@ -324,9 +324,11 @@ __global__ void k_b_graph_tree(const uint32_t *__restrict__ base, const uint32_t
     fr[g].n_tree = e - s;
 }
 
-// One internal cell of graph sg.g at depth d, opened by sorted body i (its
-// half-size: the graph's root halved d times; its skip: the first node of the
-// body after the last one sharing the d-digit prefix, inside the graph).
+// One internal cell of graph sg.g at depth d, opened by sorted body i, one of
+// the block's (its half-size: the graph's root halved d times; its skip: the
+// first node of the body after the last one sharing the d-digit prefix, inside
+// the graph).  The search reads the block's LDS window (cell_end_win): keys wk
+// and first nodes wf of the sorted bodies [w0, w0 + wn).
 __device__ __forceinline__ void b_emit_cell(const uint64_t *__restrict__ keys, const Seg &sg,
                                             uint32_t i, int d, uint32_t node, uint32_t n,
                                             const GFrame *fr, const BGlobal *gl,
@@ -334,12 +336,14 @@ __device__ __forceinline__ void b_emit_cell(const uint64_t *__restrict__ keys, c
                                             const uint32_t *__restrict__ blk,
                                             Node *__restrict__ nodes,
                                             uint8_t *__restrict__ node_depth,
-                                            uint32_t *__restrict__ cskip) {
+                                            uint32_t *__restrict__ cskip, const uint64_t *wk,
+                                            const uint32_t *wf, uint32_t w0, uint32_t wn) {
     float h = fr[sg.g].root[3];
     for (int q = 0; q < d; q++) h = h * 0.5f;
-    const uint32_t j = cell_end(keys, i, sg.ve, keys[i], d);
+    const uint32_t j = cell_end_win(keys, wk, w0, wn, i, sg.ve, wk[i - w0], d);
     const bool last = j + 1 >= sg.ve;  // the subtree ends with its graph: rope kEnd
-    const uint32_t end = node_start(loc, blk, last ? sg.ve : j + 1, n, gl);
+    const uint32_t e = last ? sg.ve : j + 1;
+    const uint32_t end = e - w0 < wn ? wf[e - w0] : node_start(loc, blk, e, n, gl);
     const float s = h * 2.0f;
     nodes[node].b = cell_b(s * s, last ? kEnd : end);
     cskip[node] = end;  // where the subtree ends in the forest, for k_b_com_fused
@@ -359,17 +363,22 @@ __global__ __launch_bounds__(kBT) void k_b_emit(
     __shared__ uint32_t ntask;
     __shared__ uint32_t t_node[kBEmitTasks], t_body[kBEmitTasks];
     __shared__ uint8_t t_depth[kBEmitTasks];
+    // the block's sorted keys and the next kEmitWin, each with its first node
+    __shared__ uint64_t wk[kBT + kEmitWin];
+    __shared__ uint32_t wf[kBT + kEmitWin];
+    const uint32_t w0 = blockIdx.x * kBT;
+    const uint32_t wn = w0 < n ? min(kBT + kEmitWin, n - w0) : 0u;
     if (threadIdx.x == 0) ntask = 0;
+    for (uint32_t q = threadIdx.x; q < wn; q += kBT) {
+        wk[q] = keys[w0 + q];
+        wf[q] = node_start(loc, blk, w0 + q, n, gl);
+    }
     __syncthreads();
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    // the neighbours' keys come from the neighbouring lanes; the wave's first
-    // and last lanes load the one past the wave's edge (only read inside the
-    // body's own graph segment)
-    const uint32_t lane = threadIdx.x & 63;
-    const uint64_t kown = i < n ? keys[i] : 0ull;
-    uint64_t kprev = __shfl_up(kown, 1, 64), knext = __shfl_down(kown, 1, 64);
-    if (lane == 0 && i > 0 && i < n) kprev = keys[i - 1];
-    if (lane == 63 && i + 1 < n) knext = keys[i + 1];
+    const uint32_t i = w0 + threadIdx.x;
+    // the neighbours' keys (only read inside the body's own graph segment)
+    const uint64_t kown = i < n ? wk[threadIdx.x] : 0ull;
+    const uint64_t kprev = (i > 0 && i < n) ? (threadIdx.x > 0 ? wk[threadIdx.x - 1] : keys[i - 1]) : 0ull;
+    const uint64_t knext = i + 1 < n ? wk[threadIdx.x + 1] : 0ull;
     if (i < n) {
         const Seg sg = seg_of(i, gid, base, count, fr);
         if (i < sg.ve) {
@@ -384,7 +393,7 @@ __global__ __launch_bounds__(kBT) void k_b_emit(
             }
             const uint32_t c = li > lp ? (uint32_t)(li - lp) : 0u;
             const int a0 = lp + 1;
-            const uint32_t pos0 = node_start(loc, blk, i, n, gl);
+            const uint32_t pos0 = wf[threadIdx.x];
             if ((uint64_t)pos0 + c < cap) {
                 if (c) {
                     const uint32_t slot = atomicAdd(&ntask, c);
@@ -395,7 +404,7 @@ __global__ __launch_bounds__(kBT) void k_b_emit(
                             t_depth[slot + t] = (uint8_t)(a0 + (int)t);
                         } else {
                             b_emit_cell(keys, sg, i, a0 + (int)t, pos0 + t, n, fr, gl, loc, blk,
-                                        nodes, node_depth, cskip);
+                                        nodes, node_depth, cskip, wk, wf, w0, wn);
                         }
                     }
                 }
@@ -415,7 +424,8 @@ __global__ __launch_bounds__(kBT) void k_b_emit(
     for (uint32_t q = threadIdx.x; q < nt; q += blockDim.x) {
         const uint32_t ib = t_body[q];
         const Seg sg = seg_of(ib, gid, base, count, fr);
-        b_emit_cell(keys, sg, ib, t_depth[q], t_node[q], n, fr, gl, loc, blk, nodes, node_depth, cskip);
+        b_emit_cell(keys, sg, ib, t_depth[q], t_node[q], n, fr, gl, loc, blk, nodes, node_depth, cskip,
+                    wk, wf, w0, wn);
     }
 }
 

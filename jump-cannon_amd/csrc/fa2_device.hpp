@@ -123,6 +123,41 @@ __device__ __forceinline__ uint32_t cell_end(const uint64_t *keys, uint32_t i, u
     return lo;
 }
 
+// cell_end with the sorted keys [w0, w0 + wn) staged in LDS (wk): a probe that
+// falls in that window reads LDS, any other reads global memory.  The probes
+// and their order are cell_end's, so the answer is the same.  An emit block
+// stages its own keys and the next kEmitWin ones, which hold the ends of most
+// of its cells, so their searches leave global memory.
+constexpr uint32_t kEmitWin = 256;
+__device__ __forceinline__ uint32_t cell_end_win(const uint64_t *__restrict__ keys,
+                                                 const uint64_t *wk, uint32_t w0, uint32_t wn,
+                                                 uint32_t i, uint32_t nv, uint64_t k, int d) {
+    if (d == 0) return nv - 1;
+    const int sh = 3 * (kLevels - d);
+    const uint64_t pre = k >> sh;
+    uint32_t lo = i, step = 1, hi;
+    for (;;) {
+        const uint32_t probe = lo + step;
+        if (probe >= nv) {
+            hi = nv;
+            break;
+        }
+        const uint64_t kp = probe - w0 < wn ? wk[probe - w0] : keys[probe];
+        if ((kp >> sh) != pre) {
+            hi = probe;
+            break;
+        }
+        lo = probe;
+        step <<= 1;
+    }
+    while (hi - lo > 1) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        const uint64_t km = mid - w0 < wn ? wk[mid - w0] : keys[mid];
+        if ((km >> sh) == pre) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
 
 // Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8),
 // and each XCD has its own L2.  This bijection gives XCD x the contiguous run

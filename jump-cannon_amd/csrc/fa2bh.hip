@@ -309,42 +309,6 @@ __device__ __forceinline__ uint32_t first_node_of(const uint32_t *loc, const uin
     return loc[i] + blk[i / kScanTile] + i;
 }
 
-// k_emit stages its block's sorted keys and the next kEmitWin ones in LDS, with
-// each body's first node.  cell_end_win is cell_end with its probes read from
-// that window [w0, w0 + wn) when they fall in it and from global memory
-// otherwise: the same probes in the same order, so the same answer.  Most cells
-// end within the window, so their random global reads become LDS reads.
-constexpr uint32_t kEmitWin = 256;
-__device__ __forceinline__ uint32_t cell_end_win(const uint64_t *__restrict__ keys,
-                                                 const uint64_t *wk, uint32_t w0, uint32_t wn,
-                                                 uint32_t i, uint32_t nv, uint64_t k, int d) {
-    if (d == 0) return nv - 1;
-    const int sh = 3 * (kLevels - d);
-    const uint64_t pre = k >> sh;
-    uint32_t lo = i, step = 1, hi;
-    for (;;) {
-        const uint32_t probe = lo + step;
-        if (probe >= nv) {
-            hi = nv;
-            break;
-        }
-        const uint64_t kp = probe - w0 < wn ? wk[probe - w0] : keys[probe];
-        if ((kp >> sh) != pre) {
-            hi = probe;
-            break;
-        }
-        lo = probe;
-        step <<= 1;
-    }
-    while (hi - lo > 1) {
-        const uint32_t mid = lo + ((hi - lo) >> 1);
-        const uint64_t km = mid - w0 < wn ? wk[mid - w0] : keys[mid];
-        if ((km >> sh) == pre) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 // One internal cell: depth d, opened by sorted body i (one of the block's) at
 // node index `node`.  Its half-size is the root's halved d times
 // (fa2_bh.rs:855-861); its skip rope is the first node of the body after the
@@ -392,8 +356,10 @@ __global__ __launch_bounds__(kTB) void k_emit(const uint64_t *__restrict__ keys,
     __shared__ uint32_t ntask;
     __shared__ uint32_t t_node[kEmitTasks], t_body[kEmitTasks];
     __shared__ uint8_t t_depth[kEmitTasks];
-    __shared__ uint64_t wk[kTB + kEmitWin];  // sorted keys [w0, w0 + wn)
-    __shared__ uint32_t wf[kTB + kEmitWin];  // first node of each of those bodies
+    // cell searches read the block's keys and the next kEmitWin from LDS
+    // (cell_end_win): the sorted keys [w0, w0 + wn) and each one's first node
+    __shared__ uint64_t wk[kTB + kEmitWin];
+    __shared__ uint32_t wf[kTB + kEmitWin];
     const uint32_t nv = n - fr->n_invalid;
     const uint32_t w0 = blockIdx.x * kTB;
     const uint32_t wn = w0 < nv ? min(kTB + kEmitWin, nv - w0) : 0u;
