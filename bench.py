@@ -296,13 +296,29 @@ def bench_spmv(args) -> int:
     algo = 4 * (n + 1) + 8 * nnz + 8 * n
     achieved = algo / (ms / 1e3) / 1e9
     log("spmv: " + ", ".join(f"{k} {v:.4f} ms" for k, v in times.items()))
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        def products(k):
+            for _ in range(k):
+                O.spmv(off, nb, w, x)
+            return k
+        rate, done = _timed_rate(products, args.cpu_seconds / 2)
+        cpu = {"value": rate, "unit": "products/s", "cores": 1, "kind": "port", "cpu_model": _cpu_model(),
+               "nproc": os.cpu_count(),
+               "sample": f"{done} products of the oracle's restatement of cpu_spmv (spmv.rs:88-99): rows "
+                         f"in order, each summed in CSR order in f32, one thread as in the reference"}
+        log(f"spmv cpu baseline {rate:.1f} products/s on 1 thread")
     return _line("SpMV products/s at 1M nodes (BA m=4, PageRank weights)", 1e3 / ms, "products/s", 100,
                  1, ms, "f32",
                  {"workload": "spmv_ba1m", "n_nodes": n, "csr_nnz": nnz, "parallelism": "single-gpu"},
                  ms_per_product={k: round(v, 5) for k, v in times.items()},
                  roofline={"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                           "traffic": None, "bytes_per_product": algo})
+                           "traffic": pmc_traffic("spmv_ba1m"), "bytes_per_product": algo,
+                           "kernel": "k_spmv_rows<float>"},
+                 cpu_baseline=cpu)
 
 
 def bench_sgd(args) -> int:
@@ -445,6 +461,24 @@ def bench_traverse(args) -> int:
     r = out["rgg_200k"]
     algo = 132 * r["n_nodes"]  # one padded 128-byte row per node, one 4-byte distance
     achieved = algo / (r["bfs_ms"] / 1e3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        off, nb = workloads.rgg_200k()[:2]
+        srcs = list(range(0, r["n_nodes"], max(r["n_nodes"] // 16, 1)))
+
+        def searches(k):
+            for q in range(k):
+                O.bfs(off, nb, srcs[q % len(srcs)])
+            return k
+        rate, done = _timed_rate(searches, args.cpu_seconds / 2)
+        cpu = {"value": rate * (r["csr_nnz"] / 2), "unit": "edges/s", "cores": 1, "kind": "port",
+               "searches_per_s": rate, "cpu_model": _cpu_model(), "nproc": os.cpu_count(),
+               "sample": f"{done} single-source searches of the oracle's restatement of cpu_bfs "
+                         f"(bfs.rs:44-66, a FIFO queue over the out-rows) on the same RGG and sources, "
+                         f"one thread as in the reference"}
+        log(f"bfs cpu baseline {rate:.1f} searches/s on 1 thread")
     return _line("BFS traversed edges/s (200k-node RGG, one search)", r["teps"], "edges/s", r["searches"],
                  1, r["bfs_ms"], "u32",
                  {"workload": "bfs_rgg200k", "n_nodes": r["n_nodes"], "csr_nnz": r["csr_nnz"],
@@ -454,7 +488,8 @@ def bench_traverse(args) -> int:
                            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
                            "bytes_per_search": algo,
                            "limiter": "level-synchronous: one dependent row load and one barrier per level "
-                                      "in one workgroup"})
+                                      "in one workgroup"},
+                 cpu_baseline=cpu)
 
 
 def main() -> int:
