@@ -49,6 +49,20 @@ typedef struct {
     float max_displacement;
 } jc_fa2bh_settings;
 
+/* Field-for-field Fa2Settings of the brute-force engine
+ * (crates/graph-compute/src/engines/fa2_brute.rs:63-93); bools as u32. */
+typedef struct {
+    float gravity;
+    uint32_t strong_gravity;
+    float scaling_ratio;
+    float edge_weight_influence;
+    float jitter_tolerance;
+    uint32_t lin_log_mode;
+    uint32_t prevent_overlap;
+    float time_step;
+    float max_displacement;
+} jc_fa2_settings;
+
 /* Field-for-field SgdStressSettings (crates/graph-compute/src/engines/sgd_stress.rs:59-98).
  * eta_max <= 0 selects the s_gd2 start 1/w_min = (max pivot distance)^2. */
 typedef struct {
@@ -83,6 +97,14 @@ int jc_device_count(void);
 void jc_fa2bh_default_settings(jc_fa2bh_settings *out);
 /* Fa2BhEngine::new (fa2_bh.rs:179-186), registry id "fa2-bh-hip". */
 int jc_fa2bh_create(const jc_fa2bh_settings *settings, jc_engine **out);
+/* Fa2Settings::default() (fa2_brute.rs:80-93). */
+void jc_fa2_default_settings(jc_fa2_settings *out);
+/* Fa2BruteEngine::new (fa2_brute.rs:135-141), registry id "fa2-brute-hip": O(n^2)
+ * repulsion in index order (force_atlas2.wgsl:69-91), then the attraction,
+ * gravity, AdaptiveSpeed and apply of fa2-bh-hip.  The fa2bh parity and
+ * measurement hooks below accept it (it has no octree: jc_fa2bh_tree reports
+ * none, and the walk mode is ignored). */
+int jc_fa2brute_create(const jc_fa2_settings *settings, jc_engine **out);
 /* SgdStressSettings::default() (sgd_stress.rs:84-98). */
 void jc_sgd_default_settings(jc_sgd_settings *out);
 /* SgdStressGpuEngine::new (sgd_stress_gpu.rs:96-120), registry id "sgd-stress-hip". */

@@ -62,6 +62,8 @@ Status select_pivots_device(const CsrView &g, uint32_t k, uint32_t first, hipStr
                             uint32_t *dmax);
 
 LayoutEngine *make_fa2bh(const jc_fa2bh_settings &s);
+// the brute-force engine (fa2_brute.rs); theta is not used
+LayoutEngine *make_fa2brute(const jc_fa2bh_settings &s);
 LayoutEngine *make_sgd(const jc_sgd_settings &s);
 LayoutEngine *make_multilevel();
 // multilevel.cpp: the cascade position of a multilevel-hip engine.

@@ -848,6 +848,84 @@ __global__ __launch_bounds__(kTB) void k_force_multi(const Params P, Frame *fr,
         atomicAdd(&partial[threadIdx.x * kAccCols + blockIdx.x % kAccCols], acc[threadIdx.x]);
 }
 
+// ---- brute-force repulsion (the fa2-brute engine) ----
+
+// Coordinates inside (-2^39, 2^39): every squared distance between two such
+// bodies is below 2^82, which with kr in kr_in_fastdiv_range keeps
+// krd * mass / max(r2, 1e-4) inside div_in_range's range.
+constexpr float kBruteBound = 549755813888.0f;  // 2^39
+__device__ __forceinline__ bool brute_bounded(float4 q) {
+    return fabsf(q.x) < kBruteBound && fabsf(q.y) < kBruteBound && fabsf(q.z) < kBruteBound;
+}
+
+// One repulsion term of force_atlas2.wgsl:82-91 with d = p - q, deg_j = q.w.
+template <bool FASTDIV>
+__device__ __forceinline__ void brute_term(float4 p, float4 q, float krd, float &fx, float &fy,
+                                           float &fz) {
+    const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    const float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1.0e-4f);
+    const float cf = FASTDIV ? div_in_range(krd * q.w, r2) : krd * q.w / r2;
+    fx = fx + dx * cf;
+    fy = fy + dy * cf;
+    fz = fz + dz * cf;
+}
+
+// Repulsion of the brute-force engine (force_atlas2.wgsl:69-91): body b sums
+// every other body j in index order, so the f32 sum is the reference's.
+// Positions (w = deg + 1, the reference's deg_j) pass through an LDS tile of
+// BS bodies that every lane reads in order (broadcast reads).  Fast form: when
+// the tile, the body and the scaling ratio keep every quotient in range
+// (brute_bounded, Params::fastdiv) the division is div_in_range and the self
+// term is not skipped: for a finite body it is (+0) * cf with cf finite, and
+// adding +0 leaves a sum that started at +0 unchanged.  Otherwise each term is
+// the IEEE quotient and j == b is skipped, as in the reference.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_force_brute(const Params P,
+                                                     const float4 *__restrict__ pos,
+                                                     const float4 *__restrict__ old_force,
+                                                     float4 *__restrict__ force,
+                                                     const uint32_t *__restrict__ deg,
+                                                     const uint32_t *__restrict__ att_off,
+                                                     const uint32_t *__restrict__ att_adj,
+                                                     unsigned long long *__restrict__ partial) {
+    __shared__ unsigned long long acc[2 * kLimbs];
+    __shared__ float4 tile[BS];
+    if (threadIdx.x < 2 * kLimbs) acc[threadIdx.x] = 0ull;
+    const uint32_t b = blockIdx.x * BS + threadIdx.x;
+    const bool live = b < P.n;
+    const float4 p = live ? pos[b] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    const float deg_i = live ? (float)deg[b] + 1.0f : 1.0f;
+    const float krd = P.scaling_ratio * deg_i;
+    const bool own_fast = P.fastdiv != 0u && brute_bounded(p);
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    for (uint32_t j0 = 0; j0 < P.n; j0 += BS) {
+        const uint32_t j = j0 + threadIdx.x;
+        const float4 q = j < P.n ? pos[j] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        __syncthreads();  // the previous tile has been read
+        tile[threadIdx.x] = q;
+        const bool tile_fast = __syncthreads_and(brute_bounded(q)) != 0;
+        const uint32_t m = min((uint32_t)BS, P.n - j0);
+        if (tile_fast && own_fast) {
+            uint32_t k = 0;
+            for (; k + 4 <= m; k += 4) {
+                const float4 q0 = tile[k], q1 = tile[k + 1], q2 = tile[k + 2], q3 = tile[k + 3];
+                brute_term<true>(p, q0, krd, fx, fy, fz);
+                brute_term<true>(p, q1, krd, fx, fy, fz);
+                brute_term<true>(p, q2, krd, fx, fy, fz);
+                brute_term<true>(p, q3, krd, fx, fy, fz);
+            }
+            for (; k < m; k++) brute_term<true>(p, tile[k], krd, fx, fy, fz);
+        } else {
+            for (uint32_t k = 0; k < m; k++)
+                if (j0 + k != b) brute_term<false>(p, tile[k], krd, fx, fy, fz);
+        }
+    }
+    if (live) finish_body(P, b, p, deg_i, fx, fy, fz, pos, old_force, force, att_off, att_adj, acc);
+    __syncthreads();
+    if (threadIdx.x < 2 * kLimbs && acc[threadIdx.x])
+        atomicAdd(&partial[threadIdx.x * kAccCols + blockIdx.x % kAccCols], acc[threadIdx.x]);
+}
+
 // Attraction rows longer than kHeavyAtt.  Their terms d*att are independent
 // of the walk, so k_heavy_terms computes all of them before the force kernel
 // (one thread per entry, every gather in flight at once) into hterm, three
@@ -1146,15 +1224,27 @@ enum Phase { PH_BBOX, PH_KEYS, PH_SORT, PH_SCAN, PH_EMIT, PH_COM, PH_FORCE, PH_S
 const char *kPhaseNames[PH_N] = {"bbox", "keys", "sort", "count_scan", "emit",
                                  "com", "force", "speed", "apply"};
 
+// Both ForceAtlas2 engines: Barnes-Hut ("fa2-bh-hip", fa2_bh.rs) and, with
+// brute set, the brute-force engine ("fa2-brute-hip", fa2_brute.rs), whose
+// iteration is the same minus the tree: k_force_brute for the walk, then the
+// same attraction, gravity, stats, speed and apply.
 class Fa2BhEngine final : public LayoutEngine {
   public:
-    explicit Fa2BhEngine(const jc_fa2bh_settings &s) : settings_(s) {
-        desc_.id = "fa2-bh-hip";
+    Fa2BhEngine(const jc_fa2bh_settings &s, bool brute) : settings_(s), brute_(brute) {
         desc_.kind = "physics";
-        desc_.display_name = "ForceAtlas2 (Barnes-Hut, MI355X)";
-        desc_.description =
-            "O(n log n) ForceAtlas2 with the octree built, walked and integrated on the GPU: "
-            "same attraction, gravity, repulsion law and adaptive speed as fa2-bh.";
+        if (brute_) {
+            desc_.id = "fa2-brute-hip";
+            desc_.display_name = "ForceAtlas2 (brute force, MI355X)";
+            desc_.description =
+                "O(n^2) repulsion ForceAtlas2 on the GPU: every pair in index order, with the "
+                "attraction, gravity and adaptive speed of fa2-brute.";
+        } else {
+            desc_.id = "fa2-bh-hip";
+            desc_.display_name = "ForceAtlas2 (Barnes-Hut, MI355X)";
+            desc_.description =
+                "O(n log n) ForceAtlas2 with the octree built, walked and integrated on the GPU: "
+                "same attraction, gravity, repulsion law and adaptive speed as fa2-bh.";
+        }
         desc_.needs_edges = 1;
         desc_.needs_cpu_positions = 1;
         desc_.needs_gpu_positions_buffer = 0;
@@ -1176,10 +1266,11 @@ class Fa2BhEngine final : public LayoutEngine {
             {JsonField::BOOL, "lin_log_mode", &linlog},
             {JsonField::BOOL, "prevent_overlap", &overlap},
             {JsonField::F32, "time_step", &s.time_step},
-            {JsonField::F32, "theta", &s.theta},
             {JsonField::F32, "max_displacement", &s.max_displacement},
         };
-        int rc = decode_json_object(json, len, f, "fa2-bh settings");
+        // Fa2Settings (fa2_brute.rs:63-78) has no theta: the key is ignored there
+        if (!brute_) f.push_back({JsonField::F32, "theta", &s.theta});
+        int rc = decode_json_object(json, len, f, brute_ ? "fa2-brute settings" : "fa2-bh settings");
         if (rc < 0) return Status{JC_E_INVALID};
         if (rc == 1) return Status::ok();
         s.strong_gravity = strong;
@@ -1336,7 +1427,7 @@ class Fa2BhEngine final : public LayoutEngine {
     }
 
     Status step(uint32_t n_iters, float *out_xyz) override {
-        if (!inited_) return Status::err(JC_E_STATE, "fa2-bh-hip step called before successful init");
+        if (!inited_) return Status::err(JC_E_STATE, std::string(desc_.id) + " step called before successful init");
         if (n_ == 0) return Status::ok();
         refresh_params();
         JC_TRY(ensure_tuned());
@@ -1409,10 +1500,11 @@ class Fa2BhEngine final : public LayoutEngine {
         JC_TRY(write_vec4(tmp.p, xyz, mass_.data()));
         ready_ = false;
         JC_TRY(enqueue_fresh(tmp.p));
-        JC_TRY(enqueue_tree(tmp.p, nullptr));
+        if (!brute_) JC_TRY(enqueue_tree(tmp.p, nullptr));
         JC_TRY(enqueue_force(tmp.p, nullptr));
-        k_count_interactions<<<cdiv(n_, kTB), kTB, 0, stream_>>>(params_, frame_.p, tmp.p, deg_.p,
-                                                                 vals_.p, nodes_.p);
+        if (!brute_)
+            k_count_interactions<<<cdiv(n_, kTB), kTB, 0, stream_>>>(params_, frame_.p, tmp.p,
+                                                                     deg_.p, vals_.p, nodes_.p);
         k_speed<<<1, kSpeedThreads, 0, stream_>>>(params_, frame_.p, partial_.p, top_slot_.p, 0u);
         JC_HIP(hipGetLastError());
         if (out_force) JC_TRY(read_vec4(force_.p, out_force));
@@ -1464,7 +1556,7 @@ class Fa2BhEngine final : public LayoutEngine {
         double fsum = 0, isum = 0;
         if (n_ && n_iters) {
             JC_TRY(ensure_ready());
-            JC_TRY(ensure_tree_graphs());
+            if (!brute_) JC_TRY(ensure_tree_graphs());
             for (auto &e : tev_)
                 if (!e) JC_HIP(hipEventCreate(&e));
             uint32_t done = 0;
@@ -1473,7 +1565,7 @@ class Fa2BhEngine final : public LayoutEngine {
                 JC_HIP(hipEventRecord(tev_[2 * kTimedIters], stream_));
                 for (uint32_t k = 0; k < chunk; k++) {
                     const uint32_t c = cur_;
-                    JC_HIP(hipGraphLaunch(tree_exec_[c], stream_));
+                    if (!brute_) JC_HIP(hipGraphLaunch(tree_exec_[c], stream_));
                     JC_HIP(hipEventRecord(tev_[2 * k], stream_));
                     JC_TRY(enqueue_force(pos_[c].p, nullptr));
                     JC_HIP(hipEventRecord(tev_[2 * k + 1], stream_));
@@ -1535,7 +1627,7 @@ class Fa2BhEngine final : public LayoutEngine {
             }
         }
         for (auto &e : ev) (void)hipEventDestroy(e);
-        if (n_iters && n_) {
+        if (n_iters && n_ && !brute_) {
             // interactions of the last profiled iteration: its tree is still in
             // nodes_, built from the positions that iteration read
             k_count_interactions<<<cdiv(n_, kTB), kTB, 0, stream_>>>(params_, frame_.p,
@@ -1577,7 +1669,7 @@ class Fa2BhEngine final : public LayoutEngine {
     // Picks the fastest walk on the current positions (all give bit-identical
     // forces).  Engine state is not advanced.
     Status ensure_tuned() {
-        if (walk_mode_ >= 0) return Status::ok();
+        if (brute_ || walk_mode_ >= 0) return Status::ok();
         hipEvent_t a, b;
         JC_HIP(hipEventCreate(&a));
         JC_HIP(hipEventCreate(&b));
@@ -1722,11 +1814,16 @@ class Fa2BhEngine final : public LayoutEngine {
             k_heavy_terms<<<cdiv(n_hent_, kTB), kTB, 0, stream_>>>(params_, n_hent_, hbody_.p,
                                                                   hadj_.p, pos, hterm_.p, hstride_);
         const uint32_t bs = block_of(walk_mode_);
+        const int mode = brute_ ? -2 : walk_mode_;
 #define JC_FORCE(KERNEL, ...)                                                              \
     KERNEL<<<grid, bs, 0, stream_>>>(params_, frame_.p, pos, old_force_.p, force_.p, deg_.p, \
                                      att_off_.p, att_adj_.p, vals_.p, nodes_.p, partial_.p   \
                                      __VA_ARGS__)
-        switch (walk_mode_) {
+        switch (mode) {
+            case -2:
+                k_force_brute<kTB><<<cdiv(n_, kTB), kTB, 0, stream_>>>(
+                    params_, pos, old_force_.p, force_.p, deg_.p, att_off_.p, att_adj_.p, partial_.p);
+                break;
             case 0: JC_FORCE(k_force<0>, , top_a_.p, top_b_.p); break;
             case 2: JC_FORCE(k_force_multi<2>); break;
             case 3: JC_FORCE(k_force_multi<4>); break;
@@ -1745,7 +1842,11 @@ class Fa2BhEngine final : public LayoutEngine {
 
     // One full iteration reading pos_[c], writing pos_[c^1].
     Status enqueue_iteration(uint32_t c, bool speed, bool apply, hipEvent_t *ev) {
-        JC_TRY(enqueue_tree(pos_[c].p, ev));
+        if (!brute_) {
+            JC_TRY(enqueue_tree(pos_[c].p, ev));
+        } else if (ev) {  // no tree: its phases take no time
+            for (int k = PH_BBOX; k < PH_FORCE; k++) JC_HIP(hipEventRecord(ev[k], stream_));
+        }
         JC_TRY(enqueue_force(pos_[c].p, ev));
         if (ev) JC_HIP(hipEventRecord(ev[PH_SPEED], stream_));
         if (speed)
@@ -1844,6 +1945,7 @@ class Fa2BhEngine final : public LayoutEngine {
 
     jc_descriptor desc_{};
     jc_fa2bh_settings settings_;
+    bool brute_ = false;  // fa2-brute-hip: no tree, k_force_brute for the repulsion
     Params params_{};
     bool params_dirty_ = true;
     bool inited_ = false;
@@ -1888,7 +1990,8 @@ class Fa2BhEngine final : public LayoutEngine {
 
 }  // namespace
 
-LayoutEngine *make_fa2bh(const jc_fa2bh_settings &s) { return new Fa2BhEngine(s); }
+LayoutEngine *make_fa2bh(const jc_fa2bh_settings &s) { return new Fa2BhEngine(s, false); }
+LayoutEngine *make_fa2brute(const jc_fa2bh_settings &s) { return new Fa2BhEngine(s, true); }
 bool is_fa2bh(const LayoutEngine *e) { return dynamic_cast<const Fa2BhEngine *>(e) != nullptr; }
 
 Status fa2bh_forces_at(LayoutEngine *e, const float *xyz, float *out_force, double *sums2) {

@@ -322,6 +322,19 @@ void jc_fa2bh_default_settings(jc_fa2bh_settings *o) {
     o->max_displacement = 10.0f;
 }
 
+void jc_fa2_default_settings(jc_fa2_settings *o) {
+    if (!o) return;
+    o->gravity = 1.0f;
+    o->strong_gravity = 0;
+    o->scaling_ratio = 2.0f;
+    o->edge_weight_influence = 1.0f;
+    o->jitter_tolerance = 1.0f;
+    o->lin_log_mode = 0;
+    o->prevent_overlap = 0;
+    o->time_step = 1.0f;
+    o->max_displacement = 10.0f;
+}
+
 void jc_sgd_default_settings(jc_sgd_settings *o) {
     if (!o) return;
     o->n_pivots = 50;
@@ -338,6 +351,28 @@ int jc_fa2bh_create(const jc_fa2bh_settings *s, jc_engine **out) {
     jc_fa2bh_settings d;
     jc_fa2bh_default_settings(&d);
     auto *h = new (std::nothrow) jc_engine{jc::make_fa2bh(s ? *s : d)};
+    if (!h) return JC_E_OOM;
+    *out = h;
+    return finish(Status::ok());
+}
+
+int jc_fa2brute_create(const jc_fa2_settings *s, jc_engine **out) {
+    if (!out) return bad_handle();
+    jc_fa2_settings d;
+    jc_fa2_default_settings(&d);
+    const jc_fa2_settings &v = s ? *s : d;
+    jc_fa2bh_settings b;
+    jc_fa2bh_default_settings(&b);
+    b.gravity = v.gravity;
+    b.strong_gravity = v.strong_gravity;
+    b.scaling_ratio = v.scaling_ratio;
+    b.edge_weight_influence = v.edge_weight_influence;
+    b.jitter_tolerance = v.jitter_tolerance;
+    b.lin_log_mode = v.lin_log_mode;
+    b.prevent_overlap = v.prevent_overlap;
+    b.time_step = v.time_step;
+    b.max_displacement = v.max_displacement;
+    auto *h = new (std::nothrow) jc_engine{jc::make_fa2brute(b)};
     if (!h) return JC_E_OOM;
     *out = h;
     return finish(Status::ok());
@@ -422,7 +457,7 @@ void jc_engine_destroy(jc_engine *e) {
 static int need_fa2(jc_engine *e) {
     if (!e) return bad_handle();
     if (!jc::is_fa2bh(e->impl)) {
-        jc::set_error("engine is not fa2-bh-hip");
+        jc::set_error("engine is not fa2-bh-hip or fa2-brute-hip");
         return JC_E_INVALID;
     }
     return JC_OK;

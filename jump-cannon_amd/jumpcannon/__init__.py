@@ -49,6 +49,16 @@ class Fa2BhSettings(C.Structure):
                 ("theta", C.c_float), ("max_displacement", C.c_float)]
 
 
+class Fa2Settings(C.Structure):
+    """Field-for-field Fa2Settings of the brute-force engine (fa2_brute.rs:63-93)."""
+
+    _fields_ = [("gravity", C.c_float), ("strong_gravity", C.c_uint32),
+                ("scaling_ratio", C.c_float), ("edge_weight_influence", C.c_float),
+                ("jitter_tolerance", C.c_float), ("lin_log_mode", C.c_uint32),
+                ("prevent_overlap", C.c_uint32), ("time_step", C.c_float),
+                ("max_displacement", C.c_float)]
+
+
 class SgdSettings(C.Structure):
     """Field-for-field SgdStressSettings (crates/graph-compute/src/engines/sgd_stress.rs:59-98)."""
 
@@ -77,6 +87,8 @@ SIGNATURES = {
     "jc_device_count": (C.c_int, []),
     "jc_fa2bh_default_settings": (None, [_vp]),
     "jc_fa2bh_create": (C.c_int, [_vp, _vp]),
+    "jc_fa2_default_settings": (None, [_vp]),
+    "jc_fa2brute_create": (C.c_int, [_vp, _vp]),
     "jc_sgd_default_settings": (None, [_vp]),
     "jc_sgd_create": (C.c_int, [_vp, _vp]),
     "jc_engine_descriptor": (C.c_int, [_vp, _vp]),
@@ -415,6 +427,32 @@ class Fa2BhEngine(LayoutEngine):
         out = np.zeros(k, np.float64)
         _check(lib().jc_fa2bh_profile(self._h, n_iters, _ptr(out), k))
         return {lib().jc_fa2bh_phase_name(i).decode(): float(out[i]) for i in range(k)}
+
+
+def default_fa2_settings(**overrides) -> Fa2Settings:
+    s = Fa2Settings()
+    lib().jc_fa2_default_settings(C.addressof(s))
+    for k, v in overrides.items():
+        setattr(s, k, v)
+    return s
+
+
+class Fa2BruteEngine(Fa2BhEngine):
+    """Brute-force ForceAtlas2 ("fa2-brute-hip"), the drop-in for fa2_brute.rs:
+    O(n^2) repulsion over every pair in index order.  The parity hooks are
+    Fa2BhEngine's (forces_at, step_timed, last_stats, profile); it builds no
+    octree and has one repulsion kernel, so tree() is empty and walk_mode() has
+    no effect."""
+
+    ID = "fa2-brute-hip"
+
+    def __init__(self, settings: Fa2Settings | None = None, **overrides):
+        s = settings if settings is not None else default_fa2_settings()
+        for k, v in overrides.items():
+            setattr(s, k, v)
+        h = C.c_void_p()
+        _check(lib().jc_fa2brute_create(C.addressof(s), C.addressof(h)))
+        LayoutEngine.__init__(self, h.value)
 
 
 def default_sgd_settings(**overrides) -> SgdSettings:

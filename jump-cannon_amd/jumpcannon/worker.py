@@ -5,10 +5,11 @@
 
     python -m jumpcannon.worker     # GRAPH_COMPUTE_GRAPH_PATH, _LAYOUT_ID, _ADDR, _TICK_HZ
 
-Engines: "fa2-bh-hip" (the default), "sgd-stress-hip" and "multilevel-hip".
-A Subscribe may also name the reference engine each one replaces ("fa2-bh",
-"sgd-stress", "sgd-stress-gpu", "multilevel"); other ids are unknown, as in
-the reference.  Where the reference falls back to its CPU spring engine when
+Engines: "fa2-brute-hip" (the default, as "fa2-brute" is the reference
+registry's), "fa2-bh-hip", "sgd-stress-hip" and "multilevel-hip".  A Subscribe
+may also name the reference engine each one replaces ("fa2-brute", "fa2-bh",
+"sgd-stress", "sgd-stress-gpu", "multilevel"); other ids (the CPU spring and
+geometric engines) are unknown.  Where the reference falls back to its CPU spring engine when
 a GPU engine fails to initialise (sim.rs:607-642), this worker fails the
 Subscribe with INTERNAL and reports the failure in Health: there is no CPU
 path.  TopoFisheye and ExchangeHalo answer UNIMPLEMENTED (the reference's
@@ -29,14 +30,16 @@ import grpc
 import numpy as np
 from google.protobuf import json_format
 
-from . import (Fa2BhEngine, JcError, LayoutEngine, MultilevelEngine, SgdStressEngine, _ptr, csr_from_bin,
+from . import (Fa2BhEngine, Fa2BruteEngine, JcError, LayoutEngine, MultilevelEngine, SgdStressEngine, _ptr, csr_from_bin,
                csr_from_pairs, lib, load_csr_bin)
 from . import compute_proto as P
 
-DEFAULT_ID = "fa2-bh-hip"
-ENGINES = {"fa2-bh-hip": Fa2BhEngine, "sgd-stress-hip": SgdStressEngine, "multilevel-hip": MultilevelEngine}
-ALIASES = {"fa2-bh": "fa2-bh-hip", "sgd-stress": "sgd-stress-hip", "sgd-stress-gpu": "sgd-stress-hip",
-           "multilevel": "multilevel-hip"}
+# the registry default is the brute-force engine (engines/mod.rs:579)
+DEFAULT_ID = "fa2-brute-hip"
+ENGINES = {"fa2-bh-hip": Fa2BhEngine, "fa2-brute-hip": Fa2BruteEngine, "sgd-stress-hip": SgdStressEngine,
+           "multilevel-hip": MultilevelEngine}
+ALIASES = {"fa2-bh": "fa2-bh-hip", "fa2-brute": "fa2-brute-hip", "sgd-stress": "sgd-stress-hip",
+           "sgd-stress-gpu": "sgd-stress-hip", "multilevel": "multilevel-hip"}
 BROADCAST_DEPTH = 32  # frames a subscriber may lag before it drops the oldest (sim.rs:425-427)
 
 

@@ -643,6 +643,41 @@ uint32_t jco_octree_morton(uint32_t n, const float *xyz, const float *mass, floa
 /* ------------------------------------------------------------------------- */
 static inline float len3(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
 
+/* Attraction (fa2_barnes_hut.wgsl:158-181), gravity (:183-189) and the force
+ * and stats writes (:191-197) of body i after its repulsion sum (fx, fy, fz).
+ * force_atlas2.wgsl:93-135 is the same tail for the brute-force engine. */
+static void finish_one(uint32_t i, float deg_i, float fx, float fy, float fz, const float *xyz,
+                       const float *old_force, const uint32_t *att_off, const uint32_t *att_adj,
+                       const jco_fa2_settings *s, float *force, float *stats) {
+    const float *p = xyz + 3 * (size_t)i;
+    /* pow(max(w,1e-6), ewi) with every edge weight 1.0 (fa2_bh.rs:281, fa2_brute.rs:239). */
+    float w = powf(fmaxf(1.0f, 1.0e-6f), s->edge_weight_influence);
+    for (uint32_t e = att_off[i]; e < att_off[i + 1]; e++) {
+        const float *q = xyz + 3 * (size_t)att_adj[e];
+        float dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
+        float att = w;
+        if (s->lin_log_mode) {
+            float r = fmaxf(len3(dx, dy, dz), 1.0e-3f);
+            att = w * logf(1.0f + r) / r;
+        }
+        fx = fx + dx * att; fy = fy + dy * att; fz = fz + dz * att;
+    }
+    float r0 = fmaxf(len3(p[0], p[1], p[2]), 1.0e-3f);
+    float gd = s->gravity * deg_i;
+    if (s->strong_gravity) {
+        fx = fx - p[0] * gd; fy = fy - p[1] * gd; fz = fz - p[2] * gd;
+    } else {
+        fx = fx - (p[0] / r0) * gd; fy = fy - (p[1] / r0) * gd; fz = fz - (p[2] / r0) * gd;
+    }
+    const float *o = old_force ? old_force + 3 * (size_t)i : NULL;
+    float ox = o ? o[0] : 0.0f, oy = o ? o[1] : 0.0f, oz = o ? o[2] : 0.0f;
+    force[3 * (size_t)i] = fx; force[3 * (size_t)i + 1] = fy; force[3 * (size_t)i + 2] = fz;
+    if (stats) {
+        stats[2 * (size_t)i] = deg_i * len3(ox - fx, oy - fy, oz - fz);
+        stats[2 * (size_t)i + 1] = deg_i * 0.5f * len3(ox + fx, oy + fy, oz + fz);
+    }
+}
+
 static void force_one(uint32_t i, const float *xyz, const float *old_force, const uint32_t *degrees,
                       const uint32_t *att_off, const uint32_t *att_adj, uint32_t N,
                       const float *node_a, const float *half, const uint32_t *body,
@@ -680,32 +715,39 @@ static void force_one(uint32_t i, const float *xyz, const float *old_force, cons
             }
         }
     }
-    /* pow(max(w,1e-6), ewi) with every fa2-bh edge weight 1.0 (fa2_bh.rs:281). */
-    float w = powf(fmaxf(1.0f, 1.0e-6f), s->edge_weight_influence);
-    for (uint32_t e = att_off[i]; e < att_off[i + 1]; e++) {
-        const float *q = xyz + 3 * (size_t)att_adj[e];
-        float dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
-        float att = w;
-        if (s->lin_log_mode) {
-            float r = fmaxf(len3(dx, dy, dz), 1.0e-3f);
-            att = w * logf(1.0f + r) / r;
-        }
-        fx = fx + dx * att; fy = fy + dy * att; fz = fz + dz * att;
+    finish_one(i, deg_i, fx, fy, fz, xyz, old_force, att_off, att_adj, s, force, stats);
+}
+
+/* fa2_force of the brute-force engine (force_atlas2.wgsl:69-135): repulsion
+ * from every other body j in index order, coefficient
+ * scaling_ratio * deg_i * deg_j / max(|d|^2, 1e-4) along d = p_i - p_j
+ * (:82-91), then the shared attraction / gravity / stats tail. */
+static void brute_one(uint32_t i, uint32_t n, const float *xyz, const float *old_force,
+                      const uint32_t *degrees, const uint32_t *att_off, const uint32_t *att_adj,
+                      const jco_fa2_settings *s, float *force, float *stats) {
+    const float *p = xyz + 3 * (size_t)i;
+    float deg_i = (float)degrees[i] + 1.0f;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    for (uint32_t j = 0; j < n; j++) {
+        if (j == i) continue;
+        const float *q = xyz + 3 * (size_t)j;
+        float dx = p[0] - q[0], dy = p[1] - q[1], dz = p[2] - q[2];
+        float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1.0e-4f);
+        float deg_j = (float)degrees[j] + 1.0f;
+        float cf = s->scaling_ratio * deg_i * deg_j / r2;
+        fx = fx + dx * cf; fy = fy + dy * cf; fz = fz + dz * cf;
     }
-    float r0 = fmaxf(len3(p[0], p[1], p[2]), 1.0e-3f);
-    float gd = s->gravity * deg_i;
-    if (s->strong_gravity) {
-        fx = fx - p[0] * gd; fy = fy - p[1] * gd; fz = fz - p[2] * gd;
-    } else {
-        fx = fx - (p[0] / r0) * gd; fy = fy - (p[1] / r0) * gd; fz = fz - (p[2] / r0) * gd;
-    }
-    const float *o = old_force ? old_force + 3 * (size_t)i : NULL;
-    float ox = o ? o[0] : 0.0f, oy = o ? o[1] : 0.0f, oz = o ? o[2] : 0.0f;
-    force[3 * (size_t)i] = fx; force[3 * (size_t)i + 1] = fy; force[3 * (size_t)i + 2] = fz;
-    if (stats) {
-        stats[2 * (size_t)i] = deg_i * len3(ox - fx, oy - fy, oz - fz);
-        stats[2 * (size_t)i + 1] = deg_i * 0.5f * len3(ox + fx, oy + fy, oz + fz);
-    }
+    finish_one(i, deg_i, fx, fy, fz, xyz, old_force, att_off, att_adj, s, force, stats);
+}
+
+void jco_fa2_brute_forces(uint32_t n, const float *xyz, const float *old_force,
+                          const uint32_t *degrees, const uint32_t *att_off,
+                          const uint32_t *att_adj, const jco_fa2_settings *s, float *force,
+                          float *stats, int n_threads) {
+    (void)n_threads;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(n_threads > 0 ? n_threads : 1)
+    for (int64_t i = 0; i < (int64_t)n; i++)
+        brute_one((uint32_t)i, n, xyz, old_force, degrees, att_off, att_adj, s, force, stats);
 }
 
 void jco_fa2_forces(uint32_t n, const float *xyz, const float *old_force, const uint32_t *degrees,
@@ -976,7 +1018,8 @@ void jco_fa2_apply(uint32_t n, const float *xyz_in, float *xyz_out, const float 
 
 /* Whole iterations (fa2_bh.rs:546-692): tree from the step-start positions,
  * force pass, speed update, apply.  tree_kind 0 = reference insertion octree
- * (capacity 2n+8, serial as in fa2_bh.rs:556), 1 = Morton pre-order octree.
+ * (capacity 2n+8, serial as in fa2_bh.rs:556), 1 = Morton pre-order octree,
+ * 2 = no tree: the brute-force engine's iteration (fa2_brute.rs:454-565).
  * timings (optional, 3 doubles): seconds in tree build, force pass, rest. */
 int jco_fa2_run(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, float *xyz,
                 float *old_force, float *se, uint32_t iters, const jco_fa2_settings *s,
@@ -989,7 +1032,7 @@ int jco_fa2_run(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, float
     jco_fa2_prepare(n, offsets, nbrs, deg, att_off, att_adj);
     float *mass = (float *)malloc((size_t)n * sizeof(float));
     for (uint32_t i = 0; i < n; i++) mass[i] = (float)deg[i] + 1.0f;
-    uint32_t cap = tree_kind == 0 ? 2 * n + 8 : (JCO_LEVELS + 2) * n + 8;
+    uint32_t cap = tree_kind == 0 ? 2 * n + 8 : tree_kind == 1 ? (JCO_LEVELS + 2) * n + 8 : 1;
     float *node_a = (float *)malloc((size_t)cap * 4 * sizeof(float));
     float *half = (float *)malloc((size_t)cap * sizeof(float));
     uint32_t *body = (uint32_t *)malloc((size_t)cap * sizeof(uint32_t));
@@ -1002,11 +1045,17 @@ int jco_fa2_run(uint32_t n, const uint32_t *offsets, const uint32_t *nbrs, float
         double t0 = now_s();
         uint32_t N = tree_kind == 0
                          ? jco_octree_ref(n, xyz, mass, cap, node_a, half, body, skip, NULL, NULL, NULL)
-                         : jco_octree_morton(n, xyz, mass, node_a, half, body, skip, NULL, NULL,
-                                             NULL, NULL, NULL);
+                     : tree_kind == 1
+                         ? jco_octree_morton(n, xyz, mass, node_a, half, body, skip, NULL, NULL,
+                                             NULL, NULL, NULL)
+                         : 0u;
         double t1 = now_s();
-        jco_fa2_forces(n, xyz, old_force, deg, att_off, att_adj, N, node_a, half, body, skip, s,
-                       force, stats, n_threads);
+        if (tree_kind == 2)
+            jco_fa2_brute_forces(n, xyz, old_force, deg, att_off, att_adj, s, force, stats,
+                                 n_threads);
+        else
+            jco_fa2_forces(n, xyz, old_force, deg, att_off, att_adj, N, node_a, half, body, skip,
+                           s, force, stats, n_threads);
         double t2 = now_s();
         jco_speed_update(se, n, s->jitter_tolerance, stats, exact_sum, NULL);
         jco_fa2_apply(n, xyz, out, force, old_force, deg, s, se[0], n_threads);

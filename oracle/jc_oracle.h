@@ -85,6 +85,12 @@ void jco_fa2_forces_f64(uint32_t n, const float *xyz, const uint32_t *degrees,
                         const float *node_a, const float *half, const uint32_t *body,
                         const uint32_t *skip, const jco_fa2_settings *s, double *force,
                         uint64_t *flips, int n_threads);
+/* f32 brute-force force pass (force_atlas2.wgsl:69-135): repulsion from every
+ * other body in index order, then the same attraction, gravity and stats. */
+void jco_fa2_brute_forces(uint32_t n, const float *xyz, const float *old_force,
+                          const uint32_t *degrees, const uint32_t *att_off,
+                          const uint32_t *att_adj, const jco_fa2_settings *s, float *force,
+                          float *stats, int n_threads);
 void jco_fa2_brute_f64(uint32_t n, const float *xyz, const uint32_t *degrees,
                        const uint32_t *att_off, const uint32_t *att_adj,
                        const jco_fa2_settings *s, double *force);

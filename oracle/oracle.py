@@ -89,6 +89,8 @@ def lib():
                                   u32p, u32p, S, f32p, f32p, C.c_int]),
         "jco_fa2_forces_f64": (None, [C.c_uint32, f32p, u32p, u32p, u32p, C.c_uint32, f32p, f32p,
                                       u32p, u32p, S, f64p, u64p, C.c_int]),
+        "jco_fa2_brute_forces": (None, [C.c_uint32, f32p, f32p, u32p, u32p, u32p, S, f32p, f32p,
+                                        C.c_int]),
         "jco_fa2_brute_f64": (None, [C.c_uint32, f32p, u32p, u32p, u32p, S, f64p]),
         "jco_exact_sum2": (None, [C.c_uint32, f32p, f64p]),
         "jco_speed_update": (C.c_float, [f32p, C.c_uint32, C.c_float, f32p, C.c_int, f64p]),
@@ -297,6 +299,19 @@ def fa2_forces_f64(xyz, deg, att_off, att_adj, tree: Tree, settings: Fa2Settings
     return force, int(flips[0])
 
 
+def fa2_brute_forces(xyz, old_force, deg, att_off, att_adj, settings: Fa2Settings, threads=8):
+    """The brute-force engine's f32 force pass (force_atlas2.wgsl:69-135):
+    returns (force 3n, stats 2n)."""
+    xyz = _f32(xyz)
+    n = len(xyz) // 3
+    force = np.zeros(3 * n, np.float32)
+    stats = np.zeros(2 * max(n, 1), np.float32)
+    old = _f32(old_force) if old_force is not None else np.zeros(3 * n, np.float32)
+    lib().jco_fa2_brute_forces(n, xyz, old, _u32(deg), _u32(att_off), _u32(att_adj),
+                               C.byref(settings), force, stats, threads)
+    return force, stats[: 2 * n]
+
+
 def fa2_brute_f64(xyz, deg, att_off, att_adj, settings: Fa2Settings):
     xyz = _f32(xyz)
     n = len(xyz) // 3
@@ -332,7 +347,8 @@ def fa2_apply(xyz, force, old_force, deg, settings: Fa2Settings, speed: float):
 
 def fa2_run(offsets, nbrs, xyz, iters, settings: Fa2Settings, tree_kind=1, exact=True, threads=8,
             old_force=None, se=None):
-    """Whole FA2+BH iterations (fa2_bh.rs:546-692).  Returns (xyz, old_force, se, timings)."""
+    """Whole FA2 iterations (fa2_bh.rs:546-692; tree_kind 2: the brute-force
+    engine's, fa2_brute.rs:454-565).  Returns (xyz, old_force, se, timings)."""
     xyz = _f32(xyz).copy()
     n = len(xyz) // 3
     old = np.zeros(3 * n, np.float32) if old_force is None else _f32(old_force).copy()

@@ -75,6 +75,29 @@ def test_default_settings_match_reference_defaults():
     assert abs(g.eta_max - 1.0) < 1e-7 and abs(g.eta_min - 0.01) < 1e-7
 
 
+def test_brute_engine_descriptor_settings_and_params():
+    """fa2_brute.rs:63-93 (Fa2Settings and its defaults), :143-158 (descriptor),
+    :170-178 (set_params: null keeps, decode errors are errors, unknown keys
+    such as fa2-bh's theta are ignored)."""
+    s = jc.default_fa2_settings()
+    assert (s.gravity, s.strong_gravity, s.scaling_ratio, s.edge_weight_influence,
+            s.jitter_tolerance, s.lin_log_mode, s.prevent_overlap, s.time_step,
+            s.max_displacement) == (1.0, 0, 2.0, 1.0, 1.0, 0, 0, 1.0, 10.0)
+    e = jc.Fa2BruteEngine()
+    d = e.descriptor
+    assert d["id"] == "fa2-brute-hip" and d["kind"] == "physics"
+    assert d["needs_edges"] and d["needs_cpu_positions"] and not d["needs_gpu_positions_buffer"]
+    e.set_params(None)
+    e.set_params({"theta": "not a number here", "gravity": 3, "lin_log_mode": True})
+    for bad in ({"gravity": "1.0"}, {"strong_gravity": 1}, {"max_displacement": None}, "[1]"):
+        with pytest.raises(jc.JcError) as ei:
+            e.set_params(bad)
+        assert ei.value.code == jc.JC_E_INVALID
+    if jc.device_count() == 0:
+        with pytest.raises(jc.NoDeviceError):
+            e.init(np.array([0, 1, 2], np.uint32), np.array([1, 0], np.uint32), np.zeros(6, np.float32))
+
+
 @pytest.mark.parametrize("n,m,seed", [(2, 1, 0), (50, 3, 1), (5000, 4, 0xC0FFEE)])
 def test_ba_generator_matches_oracle(n, m, seed):
     a = jc.ba_graph(n, m, seed)

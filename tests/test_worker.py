@@ -130,9 +130,9 @@ def test_list_engines(served):
     and the frozen kind string; the default id round-trips."""
     _, cl = served
     r = cl.ListEngines(P.ListEnginesRequest())
-    assert r.default_id == "fa2-bh-hip"
+    assert r.default_id == "fa2-brute-hip"
     by_id = {e.id: e for e in r.engines}
-    assert set(by_id) == {"fa2-bh-hip", "sgd-stress-hip", "multilevel-hip"}
+    assert set(by_id) == {"fa2-bh-hip", "fa2-brute-hip", "sgd-stress-hip", "multilevel-hip"}
     for e in r.engines:
         assert e.display_name and e.description and e.kind in ("Physics", "Static")
 
@@ -150,7 +150,7 @@ def test_subscribe_rejections_answer_before_any_engine(served):
     cl.LoadGraph(P.LoadGraphRequest(csr=jc.csr_to_bin(*_cycle4()), graph_revision=42))
     code, msg = _code(cl.Subscribe(P.SubscribeRequest(graph_id="g", graph_revision=41)))
     assert code == grpc.StatusCode.FAILED_PRECONDITION and "graph revision mismatch" in msg
-    code, msg = _code(cl.Subscribe(P.SubscribeRequest(graph_id="g", graph_revision=42, layout_id="fa2-brute")))
+    code, msg = _code(cl.Subscribe(P.SubscribeRequest(graph_id="g", graph_revision=42, layout_id="cpu-spring")))
     assert code == grpc.StatusCode.INVALID_ARGUMENT and "unknown layout_id" in msg
     bad = P.SubscribeRequest(graph_id="g", layout_id="fa2-bh")
     bad.attributes.node_mass = b"\x00" * 6
@@ -164,10 +164,10 @@ def test_subscribe_rejections_answer_before_any_engine(served):
 
 
 def test_reference_engine_ids_resolve_to_the_hip_engines():
-    assert W.resolve("") == "fa2-bh-hip"
+    assert W.resolve("") == W.resolve("fa2-brute") == "fa2-brute-hip"
     assert W.resolve("fa2-bh") == "fa2-bh-hip" and W.resolve("multilevel") == "multilevel-hip"
     assert W.resolve("sgd-stress") == W.resolve("sgd-stress-gpu") == "sgd-stress-hip"
-    assert W.resolve("cpu-spring") is None and W.resolve("fa2-brute") is None
+    assert W.resolve("cpu-spring") is None
 
 
 def test_unimplemented_streams(served):

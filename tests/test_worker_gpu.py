@@ -42,7 +42,7 @@ def _frames(stream, k, timeout=5.0):
 def test_delivers_frames_that_advance_and_equal_the_standalone_engine():
     off, nb = W.path_graph(64)
     state = W.SimState(off, nb)
-    assert state.init_engine("") == "fa2-bh-hip"
+    assert state.init_engine("") == "fa2-brute-hip"
     server, stop, ch, cl = _serve(state)
     try:
         frames = _frames(cl.Subscribe(P.SubscribeRequest(graph_id="test")), 30)
@@ -56,7 +56,7 @@ def test_delivers_frames_that_advance_and_equal_the_standalone_engine():
     b = np.frombuffer(last.positions, "<f4")
     assert np.sqrt(((a - b) ** 2).sum()) > 0.0
     # frame k is the engine's k-th step from the ring seed
-    e = jc.Fa2BhEngine()
+    e = jc.Fa2BruteEngine()
     e.init(off, nb, W.ring_seed(64))
     pos = e.step(int(first.frame))
     assert np.array_equal(pos, a)
