@@ -4,7 +4,8 @@ workload (m=4, positions uniform in [-10,10]^3, theta 0.8).
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--no-cpu-baseline]
                     [--no-warm-start] [--copy-probe]
-                    [--workload ba_1m|batch_4096|pagerank_ba1m|spmv_ba1m|sgd_rgg200k|traverse|mpnn_ba1m]
+                    [--workload ba_1m|batch_4096|brute_sbm2000|pagerank_ba1m|spmv_ba1m|sgd_rgg200k|
+                                traverse|mpnn_ba1m]
 
 A step is one full FA2+BH iteration (octree build, walk, attraction, gravity,
 adaptive speed, apply) with inputs already resident in HBM.  The timed region
@@ -24,6 +25,8 @@ calibration run (FETCH_SIZE / WRITE_SIZE of a copy of known size).
 The other workloads print one JSON line each for the sibling paths:
   batch_4096     4,096 graphs (50..20,000 nodes) in one batch, 500 iterations,
                  against the same graphs run one engine at a time (sampled);
+  brute_sbm2000  the brute-force engine (fa2-brute-hip) on the 2,000-node SBM, 500
+                 iterations;
   pagerank_ba1m  PageRank on the 1M-node BA CSR, d=0.85, L1 tol 1e-6;
   sgd_rgg200k    SGD sparse stress on the 200k-node 2-D RGG, 200 pivots, 30 epochs;
   traverse       single-source BFS on the 200k-node RGG and the 1M-node BA graph,
@@ -227,6 +230,51 @@ def bench_batch(args) -> int:
                  speedup_vs_sequential=round(seq_ms / b_ms, 2) if len(sample) else None,
                  sequential_sample=(f"{which}, each its own engine, 20 timed iterations after 2, scaled to "
                                     f"{iters} iterations" + ("" if stride == 1 else f" and x{stride}")))
+
+
+def bench_brute(args) -> int:
+    """fa2-brute-hip (fa2_brute.rs) on the 2,000-node SBM: 500 iterations of the
+    O(n^2) engine, iterations/s, with the repulsion kernel's flop rate."""
+    import jumpcannon as jc
+    from jumpcannon import workloads
+
+    iters = 500
+    off, nb, xyz = workloads.sbm_2000()
+    n = len(off) - 1
+    e = jc.Fa2BruteEngine()
+    e.init(off, nb, xyz)
+    e.step(5, readback=False)
+    mode = e.walk_mode()
+    f_ms, it_ms = e.step_timed(iters)
+    # per pair (force_atlas2.wgsl:82-91): d (3), |d|^2 (5), max (1), coefficient (3), F += d*c (6)
+    flops = 18.0 * n * (n - 1)
+    achieved = flops / (f_ms / iters / 1e3) / 1e12
+    log(f"brute: {iters} iterations in {it_ms:.1f} ms, force kernels {f_ms:.1f} ms, repulsion kernel "
+        f"{['one lane', '4 lanes', '16 lanes'][mode]} per body")
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        def run(k):
+            O.fa2_run(off, nb, xyz, k, O.Fa2Settings.make(), tree_kind=2, exact=False, threads=1)
+            return k
+        rate, done = _timed_rate(run, args.cpu_seconds / 2)
+        cpu = {"value": rate, "unit": "iterations/s", "cores": 1, "kind": "port", "cpu_model": _cpu_model(),
+               "nproc": os.cpu_count(),
+               "sample": f"{done} iterations of the oracle's restatement of the brute-force iteration "
+                         f"(force_atlas2.wgsl:69-165, fa2_speed.rs) on one thread"}
+        log(f"brute cpu baseline {rate:.1f} iterations/s on 1 thread")
+    return _line("FA2 brute-force iterations/s (2,000-node SBM)", iters / (it_ms / 1e3), "iterations/s",
+                 iters, 5, it_ms / iters, "f32",
+                 {"workload": "brute_sbm2000", "n_nodes": n, "csr_nnz": len(nb), "engine": "fa2-brute-hip",
+                  "parallelism": "single-gpu"},
+                 roofline={"bound": "valu", "achieved": round(achieved, 3), "peak": VALU_FP32_PEAK_TFLOPS,
+                           "unit": "TFLOP/s", "frac": round(achieved / VALU_FP32_PEAK_TFLOPS, 5),
+                           "traffic": None, "flops_per_launch": flops,
+                           "avg_launch_ms": round(f_ms / iters, 5),
+                           "kernel": ["k_force_brute<256>", "k_force_brute_grp<256, 4>",
+                                      "k_force_brute_grp<256, 16>"][mode]},
+                 cpu_baseline=cpu)
 
 
 def bench_pagerank(args) -> int:
@@ -509,8 +557,8 @@ def main() -> int:
     ap.add_argument("--no-sequential", action="store_true",
                     help="batch_4096: skip the one-engine-per-graph estimate")
     ap.add_argument("--workload", default="ba_1m",
-                    choices=["ba_1m", "batch_4096", "pagerank_ba1m", "spmv_ba1m", "sgd_rgg200k",
-                             "traverse", "mpnn_ba1m"])
+                    choices=["ba_1m", "batch_4096", "brute_sbm2000", "pagerank_ba1m", "spmv_ba1m",
+                             "sgd_rgg200k", "traverse", "mpnn_ba1m"])
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe()
@@ -519,7 +567,8 @@ def main() -> int:
               "(the layout path is single-GPU in this build).", file=sys.stderr)
         return 2
     if args.workload != "ba_1m":
-        return {"batch_4096": bench_batch, "pagerank_ba1m": bench_pagerank, "spmv_ba1m": bench_spmv,
+        return {"batch_4096": bench_batch, "brute_sbm2000": bench_brute, "pagerank_ba1m": bench_pagerank,
+                "spmv_ba1m": bench_spmv,
                 "sgd_rgg200k": bench_sgd, "traverse": bench_traverse,
                 "mpnn_ba1m": bench_mpnn}[args.workload](args)
 

@@ -169,7 +169,10 @@ int jc_fa2bh_last_stats(jc_engine *e, uint64_t *n_tree, uint64_t *interactions, 
  * loads), 2 / 3 = each lane interleaves the walks of 2 / 4 Morton-adjacent
  * bodies, 4 / 5 = mode 0 in 512- / 1024-thread blocks.  All give bit-identical
  * forces; by default the engine times them at its first step and keeps the
- * fastest.  mode 0..5 forces one, -1 re-times; *current receives the mode in use. */
+ * fastest.  mode 0..5 forces one, -1 re-times; *current receives the mode in use.
+ * fa2-brute-hip has three repulsion kernels, timed the same way: 0 = one lane
+ * per body, 1 / 2 = 4 / 16 lanes per body (for graphs too small to fill the
+ * chip one lane per body). */
 int jc_fa2bh_walk_mode(jc_engine *e, int mode, int *current);
 /* Eager (uncaptured) iterations with events around every phase; phase_ms[k]
  * receives the mean ms of phase k over n_iters.  Phase names: jc_fa2bh_phase_name. */

@@ -46,6 +46,8 @@ constexpr uint32_t kScanItems = 16;       // count/scan tile = 4096 bodies
 constexpr uint32_t kScanTile = kTB * kScanItems;
 // lane walk, wave-union, 2 and 4 chains per lane, lane walk in 512- and 1024-thread blocks
 constexpr int kWalkModes = 6;
+// fa2-brute-hip: one lane per body, 4 lanes per body, 16 lanes per body
+constexpr int kBruteModes = 3;
 constexpr uint32_t kTimedIters = 10;      // timed iterations per host read-back of events
 constexpr uint32_t kHeavyAtt = 32;        // attraction rows longer than this go to k_force_heavy
 constexpr uint32_t kHeavyStage = 1024;    // heavy-row terms staged in LDS per round
@@ -858,16 +860,51 @@ __device__ __forceinline__ bool brute_bounded(float4 q) {
     return fabsf(q.x) < kBruteBound && fabsf(q.y) < kBruteBound && fabsf(q.z) < kBruteBound;
 }
 
-// One repulsion term of force_atlas2.wgsl:82-91 with d = p - q, deg_j = q.w.
+// One repulsion term of force_atlas2.wgsl:82-91, d * cf with d = p - q and
+// cf = krd * deg_j / max(|d|^2, 1e-4), deg_j = q.w.
 template <bool FASTDIV>
-__device__ __forceinline__ void brute_term(float4 p, float4 q, float krd, float &fx, float &fy,
-                                           float &fz) {
+__device__ __forceinline__ void brute_term_of(float4 p, float4 q, float krd, float &tx, float &ty,
+                                              float &tz) {
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1.0e-4f);
     const float cf = FASTDIV ? div_in_range(krd * q.w, r2) : krd * q.w / r2;
-    fx = fx + dx * cf;
-    fy = fy + dy * cf;
-    fz = fz + dz * cf;
+    tx = dx * cf;
+    ty = dy * cf;
+    tz = dz * cf;
+}
+template <bool FASTDIV>
+__device__ __forceinline__ void brute_term(float4 p, float4 q, float krd, float &fx, float &fy,
+                                           float &fz) {
+    float tx, ty, tz;
+    brute_term_of<FASTDIV>(p, q, krd, tx, ty, tz);
+    fx = fx + tx;
+    fy = fy + ty;
+    fz = fz + tz;
+}
+
+// Lane K of a group of L lanes, in every lane of the group: DPP quad_perm
+// [K, K, K, K] for L = 4, row_newbcast:K (one row of 16 lanes) for L = 16.
+template <int L, int K>
+__device__ __forceinline__ float grp_bcast(float v) {
+    static_assert(L == 4 || L == 16, "quads or rows");
+    constexpr int ctrl = L == 4 ? K * 0x55 : 0x150 + K;
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xF, 0xF, true));
+}
+// s += t of group lane 0, then 1, ..., L-1, per component: the group's L
+// terms in lane order.
+template <int L, int K = 0>
+__device__ __forceinline__ void grp_add(float &fx, float &fy, float &fz, float tx, float ty,
+                                        float tz) {
+    fx = fx + grp_bcast<L, K>(tx);
+    fy = fy + grp_bcast<L, K>(ty);
+    fz = fz + grp_bcast<L, K>(tz);
+    if constexpr (K + 1 < L) grp_add<L, K + 1>(fx, fy, fz, tx, ty, tz);
+}
+
+// Body j of a tile (w = deg + 1), or a finite pad past the last body.
+__device__ __forceinline__ float4 brute_tile_load(const float4 *__restrict__ pos, uint32_t n,
+                                                  uint32_t j) {
+    return j < n ? pos[j] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
 }
 
 // Repulsion of the brute-force engine (force_atlas2.wgsl:69-91): body b sums
@@ -898,12 +935,12 @@ __global__ __launch_bounds__(BS) void k_force_brute(const Params P,
     const float krd = P.scaling_ratio * deg_i;
     const bool own_fast = P.fastdiv != 0u && brute_bounded(p);
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    float4 q = brute_tile_load(pos, P.n, threadIdx.x);
     for (uint32_t j0 = 0; j0 < P.n; j0 += BS) {
-        const uint32_t j = j0 + threadIdx.x;
-        const float4 q = j < P.n ? pos[j] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
         __syncthreads();  // the previous tile has been read
         tile[threadIdx.x] = q;
         const bool tile_fast = __syncthreads_and(brute_bounded(q)) != 0;
+        q = brute_tile_load(pos, P.n, j0 + BS + threadIdx.x);  // the next tile, in flight meanwhile
         const uint32_t m = min((uint32_t)BS, P.n - j0);
         if (tile_fast && own_fast) {
             uint32_t k = 0;
@@ -921,6 +958,77 @@ __global__ __launch_bounds__(BS) void k_force_brute(const Params P,
         }
     }
     if (live) finish_body(P, b, p, deg_i, fx, fy, fz, pos, old_force, force, att_off, att_adj, acc);
+    __syncthreads();
+    if (threadIdx.x < 2 * kLimbs && acc[threadIdx.x])
+        atomicAdd(&partial[threadIdx.x * kAccCols + blockIdx.x % kAccCols], acc[threadIdx.x]);
+}
+
+// The same sums with L = 4 or 16 lanes per body, for graphs too small to fill
+// the chip one lane per body: in each step over bodies j0+k .. j0+k+L-1, lane r
+// of the group computes the term of body j0+k+r, and all L lanes add the L
+// terms to their (identical) running sums in index order through group
+// broadcasts -- the f32 additions of the one-lane loop, in its order.  A term
+// that is not there (past the tile, or j == b in the IEEE form) is +0, and
+// adding +0 leaves a sum that started at +0 unchanged.  Lane 0 of the group
+// then finishes the body.  Groups are whole within a wave and every branch
+// around a broadcast is uniform over a group (it depends on b, the tile or the
+// launch), so the broadcasts read active lanes.
+template <int BS, int L>
+__global__ __launch_bounds__(BS) void k_force_brute_grp(const Params P,
+                                                         const float4 *__restrict__ pos,
+                                                         const float4 *__restrict__ old_force,
+                                                         float4 *__restrict__ force,
+                                                         const uint32_t *__restrict__ deg,
+                                                         const uint32_t *__restrict__ att_off,
+                                                         const uint32_t *__restrict__ att_adj,
+                                                         unsigned long long *__restrict__ partial) {
+    static_assert(BS % 64 == 0 && 64 % L == 0, "whole waves, whole groups");
+    __shared__ unsigned long long acc[2 * kLimbs];
+    __shared__ float4 tile[BS];
+    if (threadIdx.x < 2 * kLimbs) acc[threadIdx.x] = 0ull;
+    const uint32_t r = threadIdx.x % L;
+    const uint32_t b = blockIdx.x * (BS / L) + threadIdx.x / L;
+    const bool live = b < P.n;
+    const float4 p = live ? pos[b] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    const float deg_i = live ? (float)deg[b] + 1.0f : 1.0f;
+    const float krd = P.scaling_ratio * deg_i;
+    const bool own_fast = P.fastdiv != 0u && brute_bounded(p);
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    float4 q = brute_tile_load(pos, P.n, threadIdx.x);
+    for (uint32_t j0 = 0; j0 < P.n; j0 += BS) {
+        __syncthreads();  // the previous tile has been read
+        tile[threadIdx.x] = q;
+        const bool fast = (__syncthreads_and(brute_bounded(q)) != 0) && own_fast;
+        q = brute_tile_load(pos, P.n, j0 + BS + threadIdx.x);  // the next tile, in flight meanwhile
+        const uint32_t m = min((uint32_t)BS, P.n - j0);
+        // k + r < BS below: k < m <= BS, and k and BS are multiples of L
+        if (fast) {
+            uint32_t k = 0;
+            for (; k + 2 * L <= m; k += 2 * L) {
+                const float4 qa = tile[k + r], qb = tile[k + L + r];
+                float ax, ay, az, bx, by, bz;
+                brute_term_of<true>(p, qa, krd, ax, ay, az);
+                brute_term_of<true>(p, qb, krd, bx, by, bz);
+                grp_add<L>(fx, fy, fz, ax, ay, az);
+                grp_add<L>(fx, fy, fz, bx, by, bz);
+            }
+            for (; k < m; k += L) {
+                float tx, ty, tz;
+                brute_term_of<true>(p, tile[k + r], krd, tx, ty, tz);
+                if (k + r >= m) tx = ty = tz = 0.0f;
+                grp_add<L>(fx, fy, fz, tx, ty, tz);
+            }
+        } else {
+            for (uint32_t k = 0; k < m; k += L) {
+                float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+                if (k + r < m && j0 + k + r != b)
+                    brute_term_of<false>(p, tile[k + r], krd, tx, ty, tz);
+                grp_add<L>(fx, fy, fz, tx, ty, tz);
+            }
+        }
+    }
+    if (live && r == 0)
+        finish_body(P, b, p, deg_i, fx, fy, fz, pos, old_force, force, att_off, att_adj, acc);
     __syncthreads();
     if (threadIdx.x < 2 * kLimbs && acc[threadIdx.x])
         atomicAdd(&partial[threadIdx.x * kAccCols + blockIdx.x % kAccCols], acc[threadIdx.x]);
@@ -1117,6 +1225,8 @@ __global__ __launch_bounds__(kSpeedThreads) void k_speed(const Params P, Frame *
 // fa2_barnes_hut.wgsl:203-226, and the bounding box of the positions it writes
 // (finite coordinates, as k_bbox) for the next tree build: per thread over a
 // grid-stride loop, then the block, then one atomic per block and axis.
+// BBOX = false (the brute-force engine, which builds no tree) skips the box.
+template <bool BBOX>
 __global__ __launch_bounds__(kTB) void k_apply(const Params P, Frame *fr,
                                                const float4 *__restrict__ pos_in,
                                                float4 *__restrict__ pos_out,
@@ -1141,6 +1251,7 @@ __global__ __launch_bounds__(kTB) void k_apply(const Params P, Frame *fr,
         const float4 q = make_float4(p.x + f.x * k, p.y + f.y * k, p.z + f.z * k, fmaxf(deg_i, 1e-3f));
         pos_out[i] = q;
         old_force[i] = make_float4(f.x, f.y, f.z, 0.0f);
+        if constexpr (!BBOX) continue;
         const float v[3] = {q.x + 0.0f, q.y + 0.0f, q.z + 0.0f};  // +0: canonical zero sign
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -1150,6 +1261,7 @@ __global__ __launch_bounds__(kTB) void k_apply(const Params P, Frame *fr,
             }
         }
     }
+    if constexpr (!BBOX) return;
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -1641,9 +1753,10 @@ class Fa2BhEngine final : public LayoutEngine {
         return Status::ok();
     }
 
-    // mode 0/1 forces a walk; -1 re-runs the timing; *current gets the mode in use.
+    // mode >= 0 forces a walk (brute: a repulsion kernel); -1 re-runs the timing;
+    // *current gets the mode in use.
     Status set_walk_mode(int mode, int *current) {
-        if (mode >= 0 && mode < kWalkModes) {
+        if (mode >= 0 && mode < (brute_ ? kBruteModes : kWalkModes)) {
             if (walk_mode_ != mode) drop_graphs();
             walk_mode_ = mode;
         } else if (mode == -1 && inited_ && n_) {
@@ -1669,7 +1782,8 @@ class Fa2BhEngine final : public LayoutEngine {
     // Picks the fastest walk on the current positions (all give bit-identical
     // forces).  Engine state is not advanced.
     Status ensure_tuned() {
-        if (brute_ || walk_mode_ >= 0) return Status::ok();
+        if (walk_mode_ >= 0) return Status::ok();
+        const int modes = brute_ ? kBruteModes : kWalkModes;
         hipEvent_t a, b;
         JC_HIP(hipEventCreate(&a));
         JC_HIP(hipEventCreate(&b));
@@ -1677,13 +1791,13 @@ class Fa2BhEngine final : public LayoutEngine {
         // build leaves the caches the way the walk finds them then; back-to-back
         // walks would favour the variants that reuse a warm L2 most).  Four
         // interleaved rounds over the modes, best round per mode.
-        for (int mode = 0; mode < kWalkModes; mode++) tune_ms_[mode] = 1e30f;
+        for (int mode = 0; mode < modes; mode++) tune_ms_[mode] = 1e30f;
         for (int round = 0; round < 4; round++)
-            for (int mode = 0; mode < kWalkModes; mode++) {
+            for (int mode = 0; mode < modes; mode++) {
                 walk_mode_ = mode;
                 ready_ = false;
                 JC_TRY(enqueue_fresh(pos_[cur_].p));
-                JC_TRY(enqueue_tree(pos_[cur_].p, nullptr));
+                if (!brute_) JC_TRY(enqueue_tree(pos_[cur_].p, nullptr));
                 JC_HIP(hipEventRecord(a, stream_));
                 JC_TRY(enqueue_force(pos_[cur_].p, nullptr));
                 JC_HIP(hipEventRecord(b, stream_));
@@ -1693,7 +1807,7 @@ class Fa2BhEngine final : public LayoutEngine {
                 tune_ms_[mode] = std::min(tune_ms_[mode], ms);
             }
         int pick = 0;
-        for (int mode = 1; mode < kWalkModes; mode++)
+        for (int mode = 1; mode < modes; mode++)
             if (tune_ms_[mode] < tune_ms_[pick]) pick = mode;
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
@@ -1768,9 +1882,15 @@ class Fa2BhEngine final : public LayoutEngine {
     }
     uint32_t apply_grid() const { return std::min<uint32_t>(cdiv(n_ ? n_ : 1, kTB), kApplyBlocks); }
     // k_apply from pos_[c] into pos_[c ^ 1], then the new positions' bounding box
+    // (for the next tree build; the brute-force engine has none)
     void enqueue_apply(uint32_t c) {
-        k_apply<<<apply_grid(), kTB, 0, stream_>>>(params_, frame_.p, pos_[c].p, pos_[c ^ 1].p, force_.p,
-                                                    old_force_.p, deg_.p, bbpart_.p);
+        if (brute_) {
+            k_apply<false><<<apply_grid(), kTB, 0, stream_>>>(params_, frame_.p, pos_[c].p, pos_[c ^ 1].p,
+                                                              force_.p, old_force_.p, deg_.p, bbpart_.p);
+            return;
+        }
+        k_apply<true><<<apply_grid(), kTB, 0, stream_>>>(params_, frame_.p, pos_[c].p, pos_[c ^ 1].p,
+                                                         force_.p, old_force_.p, deg_.p, bbpart_.p);
         k_bbox_final<<<1, 256, 0, stream_>>>(bbpart_.p, apply_grid(), frame_.p);
     }
 
@@ -1821,8 +1941,20 @@ class Fa2BhEngine final : public LayoutEngine {
                                      __VA_ARGS__)
         switch (mode) {
             case -2:
-                k_force_brute<kTB><<<cdiv(n_, kTB), kTB, 0, stream_>>>(
-                    params_, pos, old_force_.p, force_.p, deg_.p, att_off_.p, att_adj_.p, partial_.p);
+                if (walk_mode_ == 1)
+                    k_force_brute_grp<kTB, 4><<<(uint32_t)(((uint64_t)n_ * 4 + kTB - 1) / kTB), kTB,
+                                                0, stream_>>>(params_, pos, old_force_.p, force_.p,
+                                                              deg_.p, att_off_.p, att_adj_.p,
+                                                              partial_.p);
+                else if (walk_mode_ == 2)
+                    k_force_brute_grp<kTB, 16><<<(uint32_t)(((uint64_t)n_ * 16 + kTB - 1) / kTB),
+                                                 kTB, 0, stream_>>>(params_, pos, old_force_.p,
+                                                                    force_.p, deg_.p, att_off_.p,
+                                                                    att_adj_.p, partial_.p);
+                else
+                    k_force_brute<kTB><<<cdiv(n_, kTB), kTB, 0, stream_>>>(
+                        params_, pos, old_force_.p, force_.p, deg_.p, att_off_.p, att_adj_.p,
+                        partial_.p);
                 break;
             case 0: JC_FORCE(k_force<0>, , top_a_.p, top_b_.p); break;
             case 2: JC_FORCE(k_force_multi<2>); break;

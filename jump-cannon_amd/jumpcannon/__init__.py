@@ -441,8 +441,9 @@ class Fa2BruteEngine(Fa2BhEngine):
     """Brute-force ForceAtlas2 ("fa2-brute-hip"), the drop-in for fa2_brute.rs:
     O(n^2) repulsion over every pair in index order.  The parity hooks are
     Fa2BhEngine's (forces_at, step_timed, last_stats, profile); it builds no
-    octree and has one repulsion kernel, so tree() is empty and walk_mode() has
-    no effect."""
+    octree, so tree() is empty.  walk_mode() picks its repulsion kernel: 0 one
+    lane per body, 1 four, 2 sixteen; -1 (the default) times them at the first
+    step and keeps the fastest; all give bit-identical forces."""
 
     ID = "fa2-brute-hip"
 

@@ -68,11 +68,17 @@ def _osettings(kw):
     return O.Fa2Settings.make(**kw)
 
 
+MODES = [0, 1, 2]
+MODE_IDS = ["lane", "quad", "row"]
+
+
+@pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
 @pytest.mark.parametrize("name,off,nb,xyz,kw", CASES, ids=[c[0] for c in CASES])
-def test_forces_bit_exact(name, off, nb, xyz, kw):
+def test_forces_bit_exact(name, off, nb, xyz, kw, mode):
     n = len(off) - 1
-    print(f"brute forces: case {name}, n {n}", flush=True)
+    print(f"brute forces: case {name}, n {n}, kernel {MODE_IDS[mode]}", flush=True)
     e = _engine(off, nb, xyz, **kw)
+    assert e.walk_mode(mode) == mode
     f_gpu, sums_gpu = e.forces_at(xyz)
     deg, aoff, aadj = O.fa2_prepare(n, off, nb)
     f_cpu, stats = O.fa2_brute_forces(xyz, None, deg, aoff, aadj, _osettings(kw))
@@ -93,7 +99,8 @@ def test_linlog_within_1e5():
     assert err.max() <= 1e-5, err.max()
 
 
-def test_nan_body_bit_exact():
+@pytest.mark.parametrize("mode", MODES, ids=MODE_IDS)
+def test_nan_body_bit_exact(mode):
     """One NaN coordinate: every other body's force is NaN, as in the oracle and
     the reference (no finiteness filter in the brute pass); the NaN body takes
     the IEEE form with its own term skipped."""
@@ -102,6 +109,7 @@ def test_nan_body_bit_exact():
     xyz = xyz.copy()
     xyz[3 * 5 + 2] = np.nan
     e = _engine(off, nb, xyz)
+    e.walk_mode(mode)
     f_gpu, _ = e.forces_at(xyz)
     deg, aoff, aadj = O.fa2_prepare(n, off, nb)
     f_cpu, _ = O.fa2_brute_forces(xyz, None, deg, aoff, aadj, _osettings({}))
@@ -110,13 +118,16 @@ def test_nan_body_bit_exact():
     assert np.array_equal(f_gpu[ok].view(np.uint32), f_cpu[ok].view(np.uint32))
 
 
+@pytest.mark.parametrize("mode", MODES + [-1], ids=MODE_IDS + ["tuned"])
 @pytest.mark.parametrize("iters", [1, 2, 3, 20])
-def test_trajectory_bit_exact_sbm2000(iters):
+def test_trajectory_bit_exact_sbm2000(iters, mode):
     """Whole iterations (fa2_brute.rs:454-565): positions, F_{t-1} and the speed
     state after 1, 2, 3 (single launches and the 2-iteration graph) and 20
     iterations equal the oracle's run with exact sums."""
     off, nb, xyz = workloads.sbm_2000()
     e = _engine(off, nb, xyz)
+    if mode >= 0:
+        e.walk_mode(mode)
     e.step(iters, readback=False)
     st = e.get_state()
     x, old, se, _ = O.fa2_run(off, nb, xyz, iters, O.Fa2Settings.make(), tree_kind=2, exact=True)

@@ -87,6 +87,11 @@ def test_brute_two_body_and_one_body_closed_forms():
     # F_{t-1} = 0: swing = deg*|F|, traction = deg*|F|/2 (force_atlas2.wgsl:129-133)
     assert np.allclose(st[0::2], [3 * want, 6 * want], rtol=1e-6)
     assert np.allclose(st[1::2], [1.5 * want, 3 * want], rtol=1e-6)
+    # closer than 0.01: |d|^2 clamps to 1e-4, so |F| = k_r (d1+1)(d2+1) |d| / 1e-4
+    near = np.array([0, 0, 0, 0.003, 0.004, 0], np.float32)
+    f, _ = O.fa2_brute_forces(near, None, deg, aoff, np.zeros(1, np.uint32), s)
+    want = 2.0 * 3 * 6 * 0.005 / 1e-4
+    assert abs(np.linalg.norm(f[:3].astype(np.float64)) - want) < 1e-4 * want
     # one body: no repulsion (j == i is skipped), gravity -p/|p| * g * (deg+1)
     f, _ = O.fa2_brute_forces(np.array([3, 0, 4], np.float32), None, np.zeros(1, np.uint32),
                               np.zeros(2, np.uint32), np.zeros(1, np.uint32),
