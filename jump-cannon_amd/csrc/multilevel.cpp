@@ -6,8 +6,8 @@
 // crates/graph-layouts/src/layout/coarsen.rs (greedy maximal-matching
 // contraction, child-average fold-up, prolongation with xorshift64* jitter).
 //
-// The inner solvers are this library's HIP engines ("fa2-bh-hip",
-// "sgd-stress-hip"), rebound to every level through reinit.  The cascade itself
+// The inner solvers are this library's HIP engines ("fa2-brute-hip", the
+// default, "fa2-bh-hip", "sgd-stress-hip"), rebound to every level through reinit.  The cascade itself
 // is host code, as in the reference: the matching is greedy in edge-list order
 // (first touch wins), a sequential rule whose result fixes every coarse graph,
 // so it stays on the CPU where it is computed exactly as the reference does;
@@ -33,9 +33,9 @@ namespace {
 enum class Schedule { Uniform, Linear, Geometric };
 
 // MultilevelSettings::default() (multilevel.rs:167-180); the inner default is
-// this library's FA2+BH engine.
+// the brute-force FA2 engine, as there (Fa2BruteEngine::ID, :176).
 struct MlSettings {
-    std::string inner = "fa2-bh-hip";
+    std::string inner = "fa2-brute-hip";
     std::string inner_params = "null";
     uint64_t max_levels = 6;
     uint64_t target_size = 500;
@@ -224,7 +224,9 @@ void level_csr(uint32_t n, const std::vector<uint32_t> &edges, std::vector<uint3
     }
 }
 
-bool is_inner_id(const std::string &id) { return id == "fa2-bh-hip" || id == "sgd-stress-hip"; }
+bool is_inner_id(const std::string &id) {
+    return id == "fa2-brute-hip" || id == "fa2-bh-hip" || id == "sgd-stress-hip";
+}
 
 class MultilevelEngine final : public LayoutEngine {
   public:
@@ -233,8 +235,8 @@ class MultilevelEngine final : public LayoutEngine {
         desc_.kind = "physics";
         desc_.display_name = "Multilevel (multiscale wrapper, MI355X)";
         desc_.description =
-            "Coarsen -> solve coarsest -> prolong -> refine, wrapping fa2-bh-hip or sgd-stress-hip "
-            "(Walshaw/FM3/sfdp). Select the inner solver via `inner`.";
+            "Coarsen -> solve coarsest -> prolong -> refine, wrapping fa2-brute-hip, fa2-bh-hip or "
+            "sgd-stress-hip (Walshaw/FM3/sfdp). Select the inner solver via `inner`.";
         desc_.needs_edges = 1;
         desc_.needs_cpu_positions = 1;
         desc_.needs_gpu_positions_buffer = 0;
@@ -364,10 +366,10 @@ class MultilevelEngine final : public LayoutEngine {
   private:
     // multilevel.rs:274-283: one inner instance per init, settings forwarded.
     Status build_inner() {
-        if (settings_.inner == "fa2-bh-hip") {
+        if (settings_.inner == "fa2-bh-hip" || settings_.inner == "fa2-brute-hip") {
             jc_fa2bh_settings d;
             jc_fa2bh_default_settings(&d);
-            inner_.reset(make_fa2bh(d));
+            inner_.reset(settings_.inner == "fa2-bh-hip" ? make_fa2bh(d) : make_fa2brute(d));
         } else {
             jc_sgd_settings d;
             jc_sgd_default_settings(&d);

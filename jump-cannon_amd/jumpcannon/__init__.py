@@ -319,8 +319,8 @@ class LayoutEngine:
 class MultilevelEngine(LayoutEngine):
     """Multilevel wrapper ("multilevel-hip"), the drop-in for
     crates/graph-compute/src/engines/multilevel.rs: coarsen -> solve the
-    coarsest level -> prolong -> refine, with "fa2-bh-hip" (default) or
-    "sgd-stress-hip" inside.  Settings are MultilevelSettings' keys, given as
+    coarsest level -> prolong -> refine, with "fa2-brute-hip" (the default, as
+    in multilevel.rs:176), "fa2-bh-hip" or "sgd-stress-hip" inside.  Settings are MultilevelSettings' keys, given as
     keyword arguments or through set_params."""
 
     ID = "multilevel-hip"

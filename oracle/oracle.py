@@ -590,12 +590,12 @@ ML_DEFAULTS = {"max_levels": 6, "target_size": 500, "sweeps_per_level": 30, "swe
                "spring_len": 30.0, "seed": 0x5EED}
 
 
-def multilevel_run(offsets, nbrs, xyz, ticks, settings=None, inner="fa2-bh-hip", fa2=None, sgd=None,
+def multilevel_run(offsets, nbrs, xyz, ticks, settings=None, inner="fa2-brute-hip", fa2=None, sgd=None,
                    threads=8):
     """The multilevel wrapper's tick loop (multilevel.rs:333-468) over the
-    oracle FA2 (fa2_run, the Morton tree with exact sums, which the GPU engine
-    matches bit for bit) or the oracle SGD.  Returns (positions of the current
-    level, level, n_levels)."""
+    oracle FA2 (fa2_run with exact sums: the Morton tree for "fa2-bh-hip", no
+    tree for "fa2-brute-hip", the default as in multilevel.rs:176) or the
+    oracle SGD.  Returns (positions of the current level, level, n_levels)."""
     st = dict(ML_DEFAULTS)
     st.update(settings or {})
     n = len(offsets) - 1
@@ -610,7 +610,7 @@ def multilevel_run(offsets, nbrs, xyz, ticks, settings=None, inner="fa2-bh-hip",
 
     def bind(lv, p):
         lo, lnb = level_csr(levels[lv]["n"], levels[lv]["edges"])
-        if inner == "fa2-bh-hip":
+        if inner in ("fa2-bh-hip", "fa2-brute-hip"):
             return {"off": lo, "nb": lnb, "old": None, "se": None}
         k_set = dict(n_pivots=50, eta_max=1.0, eta_min=0.01, n_anneal_steps=30, seed=0x5EEDC0DE12345678)
         k_set.update(sgd or {})
@@ -622,8 +622,9 @@ def multilevel_run(offsets, nbrs, xyz, ticks, settings=None, inner="fa2-bh-hip",
         return {**k_set, "piv": piv, "dist": dist, "eta_max": em, "t": 0}
 
     def run(state, p, k):
-        if inner == "fa2-bh-hip":
-            p, old, se, _ = fa2_run(state["off"], state["nb"], p, k, fa2, tree_kind=1, exact=True,
+        if inner in ("fa2-bh-hip", "fa2-brute-hip"):
+            p, old, se, _ = fa2_run(state["off"], state["nb"], p, k, fa2 or Fa2Settings.make(),
+                                    tree_kind=1 if inner == "fa2-bh-hip" else 2, exact=True,
                                     threads=threads, old_force=state["old"], se=state["se"])
             state["old"], state["se"] = old, se
             return p

@@ -172,6 +172,7 @@ def test_multilevel_settings_decode_without_a_device():
     e = jc.MultilevelEngine()
     assert e.descriptor["id"] == "multilevel-hip"
     e.set_params({"inner": "sgd-stress-hip"})
+    e.set_params({"inner": "fa2-brute-hip"})
     e.set_params({"inner": "fa2-bh-hip", "inner_params": {"theta": 0.9, "nested": [1, {"a": "}"}]},
                   "max_levels": 3, "target_size": 100, "sweeps_per_level": 7, "sweep_schedule": "geometric",
                   "spring_len": 2.5, "seed": 12345})

@@ -72,15 +72,31 @@ def test_sgd_inner_descent_bit_exact():
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
 
 
+def test_brute_inner_descent_bit_exact():
+    """The default inner engine (fa2-brute-hip, multilevel.rs:176) down the cascade."""
+    off, nb, xyz = workloads.sbm_2000()
+    e = jc.MultilevelEngine(**SET)
+    e.init(off, nb, xyz)
+    total = 0
+    for k in (11, 30, 40):
+        got = e.step(k)
+        total += k
+        want, level, n_levels = O.multilevel_run(off, nb, xyz, total, SET)
+        st = e.state()
+        assert (st["level"], st["n_levels"]) == (level, n_levels)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (total, np.abs(got - want).max())
+    assert e.state()["level"] == 0
+
+
 def test_small_graph_is_the_plain_inner_engine():
     """n <= target_size: one level, the wrapper forwards every tick."""
     off, nb = O.ba_graph(300, 3, 4)
     xyz = O.uniform_cube(300, 5.0, 2)
-    e = jc.MultilevelEngine()  # defaults: fa2-bh-hip, target_size 500
+    e = jc.MultilevelEngine()  # defaults: fa2-brute-hip, target_size 500
     e.init(off, nb, xyz)
     assert e.state()["n_levels"] == 1
     got = e.step(25)
-    want, _, _, _ = O.fa2_run(off, nb, xyz, 25, O.Fa2Settings.make(), tree_kind=1, exact=True)
+    want, _, _, _ = O.fa2_run(off, nb, xyz, 25, O.Fa2Settings.make(), tree_kind=2, exact=True)
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
     with pytest.raises(jc.JcError):
         e.get_state()
