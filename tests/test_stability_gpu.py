@@ -40,10 +40,18 @@ def test_fa2_bh_stable_at_vault_scale_synthetic():
           f"final p50 radius {stats[-1]['p50_radius']:.1f}, max |pos| {stats[-1]['max_abs']:.1f}")
 
 
+def test_fa2_brute_stable_on_synthetic_scale_free():
+    """layout_stability.rs:150-165: fa2-brute on synthetic_scale_free(2000, 5,
+    0xC0FFEE), ball seed, 700 steps."""
+    off, nb = jc.ba_graph(2000, 5, 0xC0FFEE)
+    n = len(off) - 1
+    stats = _run_collect(jc.Fa2BruteEngine(), off, nb, 700)
+    assert_stable("fa2-brute-hip/synthetic-2k", stats, np.sqrt(n) * 5.0, 10.0)
+
+
 def test_theta0_stable_on_synthetic_scale_free():
-    """layout_stability.rs:148-161 (fa2-brute: the same force math and
-    integrator, every pair summed): synthetic_scale_free(2000, 5), 700 steps,
-    here as the walk at theta = 0."""
+    """The same graph and bar through the Barnes-Hut walk at theta = 0 (every
+    leaf visited, the brute-force repulsion in tree order)."""
     off, nb = jc.ba_graph(2000, 5, 0xC0FFEE)
     n = len(off) - 1
     stats = _run_collect(jc.Fa2BhEngine(theta=0.0), off, nb, 700)
@@ -76,6 +84,31 @@ def test_fuzz_fa2_bh_hip(case):
         assert np.abs(out).max() <= ceil, (step, float(np.abs(out).max()), ceil)
         if not p["lin_log_mode"]:
             x, old, se, _ = O.fa2_run(off, nb, x, 1, s, tree_kind=1, exact=True, old_force=old, se=se)
+            assert np.array_equal(out.view(np.uint32), x.view(np.uint32)), step
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"case{c['case']}-{c['shape']}-{c['seed_kind']}-n{c['n']}")
+def test_fuzz_fa2_brute_hip(case):
+    """fuzz.rs:400-445 (gpu_fuzz fa2-brute): the same cases and invariants
+    through fa2-brute-hip, and every step equal to the oracle's brute-force
+    iteration bit for bit (lin-log cases excepted)."""
+    off, nb = F.build_graph(case["shape"], case["n"], case["graph_seed"])
+    n = len(off) - 1
+    seed = F.build_seed(case["seed_kind"], n, case["graph_seed"] ^ 0xA5)
+    p = case["fa2"]
+    e = jc.Fa2BruteEngine()
+    e.set_params({k: (bool(v) if isinstance(v, bool) else float(v)) for k, v in p.items()})
+    e.init(off, nb, seed)
+    cap = float(p["max_displacement"]) * max(float(p["time_step"]), 1.0)
+    ceil = F.ceiling(seed, case["steps"], cap)
+    s = O.Fa2Settings.make(**{k: (int(v) if isinstance(v, bool) else float(v)) for k, v in p.items()})
+    x, old, se = seed, None, None
+    for step in range(1, case["steps"] + 1):
+        out = e.step(1)
+        assert len(out) == 3 * n and np.isfinite(out).all(), step
+        assert np.abs(out).max() <= ceil, (step, float(np.abs(out).max()), ceil)
+        if not p["lin_log_mode"]:
+            x, old, se, _ = O.fa2_run(off, nb, x, 1, s, tree_kind=2, exact=True, old_force=old, se=se)
             assert np.array_equal(out.view(np.uint32), x.view(np.uint32)), step
 
 
