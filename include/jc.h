@@ -167,9 +167,9 @@ int jc_fa2bh_last_stats(jc_engine *e, uint64_t *n_tree, uint64_t *interactions, 
 /* Octree walk of the force kernel: 0 = every lane follows its own rope (the
  * top three octree levels read from LDS), 1 = wave-union walk (uniform node
  * loads), 2 / 3 = each lane interleaves the walks of 2 / 4 Morton-adjacent
- * bodies, 4 / 5 = mode 0 in 512- / 1024-thread blocks.  All give bit-identical
+ * bodies, 4 / 5 / 6 = mode 0 in 512- / 1024- / 64-thread blocks.  All give bit-identical
  * forces; by default the engine times them at its first step and keeps the
- * fastest.  mode 0..5 forces one, -1 re-times; *current receives the mode in use.
+ * fastest.  mode 0..6 forces one, -1 re-times; *current receives the mode in use.
  * fa2-brute-hip has three repulsion kernels, timed the same way: 0 = one lane
  * per body, 1 / 2 = 4 / 16 lanes per body (for graphs too small to fill the
  * chip one lane per body). */

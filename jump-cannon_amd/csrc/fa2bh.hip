@@ -44,8 +44,9 @@ namespace {
 constexpr uint32_t kTB = 256;             // threads per block for per-node kernels
 constexpr uint32_t kScanItems = 16;       // count/scan tile = 4096 bodies
 constexpr uint32_t kScanTile = kTB * kScanItems;
-// lane walk, wave-union, 2 and 4 chains per lane, lane walk in 512- and 1024-thread blocks
-constexpr int kWalkModes = 6;
+// lane walk, wave-union, 2 and 4 chains per lane, lane walk in 512-, 1024- and
+// 64-thread blocks (the last spreads a small graph's few waves over as many CUs)
+constexpr int kWalkModes = 7;
 // fa2-brute-hip: one lane per body, 4 lanes per body, 16 lanes per body
 constexpr int kBruteModes = 3;
 constexpr uint32_t kTimedIters = 10;      // timed iterations per host read-back of events
@@ -439,21 +440,27 @@ constexpr uint32_t kFuseChunk = 1024;
 constexpr uint32_t kFuseCap = 512;
 constexpr uint32_t kFuseThreads = 512;
 
+// Graphs of at most kComSmallMax bodies fuse from just below the top tree
+// (Depth = kTopDepth + 1): their cells there hold few nodes, and the two
+// k_com_level launches of the larger graphs go away.
+constexpr uint32_t kComSmallMax = 65536;
+template <uint32_t Depth>
 __global__ __launch_bounds__(kFuseThreads) void k_com_fused(Frame *fr, uint32_t cap,
                                                             Node *__restrict__ nodes,
                                                             const uint8_t *__restrict__ node_depth,
                                                             const uint32_t *__restrict__ cskip,
                                                             uint32_t *__restrict__ big, uint32_t big_max) {
-    com_fused_pass<kFuseDepth, kFuseChunk, kFuseCap, kFuseThreads>(min(fr->n_tree, cap), nodes, node_depth,
-                                                                   cskip, big, big_max, &fr->n_big);
+    com_fused_pass<Depth, kFuseChunk, kFuseCap, kFuseThreads>(min(fr->n_tree, cap), nodes, node_depth,
+                                                              cskip, big, big_max, &fr->n_big);
 }
 
+template <uint32_t Depth>
 __global__ __launch_bounds__(1024) void k_com_big(const Frame *fr, uint32_t cap, Node *__restrict__ nodes,
                                                   const uint8_t *__restrict__ node_depth,
                                                   const uint32_t *__restrict__ big, uint32_t big_max) {
     const uint32_t nbig = min(fr->n_big, big_max);
     if (!nbig) return;
-    com_big_pass<kFuseDepth>(min(fr->n_tree, cap), fr->max_depth, nodes, node_depth, big, nbig);
+    com_big_pass<Depth>(min(fr->n_tree, cap), fr->max_depth, nodes, node_depth, big, nbig);
 }
 
 // The cells of depth d < kFuseDepth, one launch per depth (com_level_pass,
@@ -1480,8 +1487,9 @@ class Fa2BhEngine final : public LayoutEngine {
         JC_TRY(cskip_.alloc(cap_));
         JC_TRY(bbpart_.alloc(6 * kApplyBlocks));
         // Cells that do not fit k_com_fused at one depth are disjoint and hold
-        // more than kFuseCap nodes each.
-        big_max_ = (uint32_t)std::min<uint64_t>((uint64_t)(kLevels + 1 - kFuseDepth) * (cap_ / kFuseCap + 1),
+        // more than kFuseCap nodes each (from depth kTopDepth + 1 for small graphs).
+        const uint32_t fuse_from = n_ <= kComSmallMax ? kTopDepth + 1 : kFuseDepth;
+        big_max_ = (uint32_t)std::min<uint64_t>((uint64_t)(kLevels + 1 - fuse_from) * (cap_ / kFuseCap + 1),
                                                 0xFFFFFFFFull);
         JC_TRY(big_.alloc(big_max_));
         JC_TRY(top_slot_.alloc(kTopSlots));
@@ -1772,8 +1780,11 @@ class Fa2BhEngine final : public LayoutEngine {
   private:
     static uint32_t chains_of(int mode) { return mode == 2 ? 2u : mode == 3 ? 4u : 1u; }
     // modes 4 and 5 are the lane walk in bigger blocks: fewer, larger Morton
-    // ranges share a CU, so its L1 serves more of their common nodes
-    static uint32_t block_of(int mode) { return mode == 4 ? 512u : mode == 5 ? 1024u : kTB; }
+    // ranges share a CU, so its L1 serves more of their common nodes; mode 6
+    // is one wave per block, for graphs with fewer waves than CUs
+    static uint32_t block_of(int mode) {
+        return mode == 4 ? 512u : mode == 5 ? 1024u : mode == 6 ? 64u : kTB;
+    }
     uint32_t force_grid(int mode) const {
         return cdiv(cdiv(n_ ? n_ : 1, chains_of(mode)), block_of(mode));
     }
@@ -1913,14 +1924,21 @@ class Fa2BhEngine final : public LayoutEngine {
                                                   loc_.p, blk_.p, nodes_.p, node_depth_.p, cap_,
                                                   top_slot_.p, cskip_.p);
         if (ev) JC_HIP(hipEventRecord(ev[PH_COM], stream_));
-        k_com_fused<<<com_grid(), kFuseThreads, 0, stream_>>>(frame_.p, (uint32_t)cap_, nodes_.p,
-                                                              node_depth_.p, cskip_.p, big_.p, big_max_);
-        k_com_big<<<1, 1024, 0, stream_>>>(frame_.p, (uint32_t)cap_, nodes_.p, node_depth_.p, big_.p,
-                                           big_max_);
-        const uint32_t gcom = std::min<uint32_t>(cdiv(cap_, kComSpan * (kTB / 64)), 1024);
-        for (int d = (int)kFuseDepth - 1; d > kTopDepth; d--)
-            k_com_level<<<gcom, kTB, 0, stream_>>>(frame_.p, (uint32_t)d, (uint32_t)cap_,
-                                                   nodes_.p, node_depth_.p);
+        if (n <= kComSmallMax) {
+            k_com_fused<kTopDepth + 1><<<com_grid(), kFuseThreads, 0, stream_>>>(
+                frame_.p, (uint32_t)cap_, nodes_.p, node_depth_.p, cskip_.p, big_.p, big_max_);
+            k_com_big<kTopDepth + 1><<<1, 1024, 0, stream_>>>(frame_.p, (uint32_t)cap_, nodes_.p,
+                                                              node_depth_.p, big_.p, big_max_);
+        } else {
+            k_com_fused<kFuseDepth><<<com_grid(), kFuseThreads, 0, stream_>>>(
+                frame_.p, (uint32_t)cap_, nodes_.p, node_depth_.p, cskip_.p, big_.p, big_max_);
+            k_com_big<kFuseDepth><<<1, 1024, 0, stream_>>>(frame_.p, (uint32_t)cap_, nodes_.p,
+                                                           node_depth_.p, big_.p, big_max_);
+            const uint32_t gcom = std::min<uint32_t>(cdiv(cap_, kComSpan * (kTB / 64)), 1024);
+            for (int d = (int)kFuseDepth - 1; d > kTopDepth; d--)
+                k_com_level<<<gcom, kTB, 0, stream_>>>(frame_.p, (uint32_t)d, (uint32_t)cap_,
+                                                       nodes_.p, node_depth_.p);
+        }
         k_top_build<<<1, 1024, 0, stream_>>>(frame_.p, top_slot_.p, nodes_.p, (uint32_t)cap_,
                                              top_a_.p, top_b_.p);
         JC_HIP(hipGetLastError());
@@ -1961,6 +1979,7 @@ class Fa2BhEngine final : public LayoutEngine {
             case 3: JC_FORCE(k_force_multi<4>); break;
             case 4: JC_FORCE((k_force<0, 512>), , top_a_.p, top_b_.p); break;
             case 5: JC_FORCE((k_force<0, 1024>), , top_a_.p, top_b_.p); break;
+            case 6: JC_FORCE((k_force<0, 64>), , top_a_.p, top_b_.p); break;
             default: JC_FORCE(k_force<1>, , top_a_.p, top_b_.p); break;
         }
 #undef JC_FORCE

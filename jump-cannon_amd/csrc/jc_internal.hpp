@@ -116,8 +116,10 @@ Status sort_morton_pairs(uint64_t *keys, uint32_t *vals, RadixScratch &s, hipStr
 // The same result for fewer than kSort24Max keys: three radix passes over bits
 // [40, 64), then each key ranked within its run of equal upper 24 bits.
 // Octree keys leave runs of a few keys (at most 16 in the 1M-node workload).
+// Up to kSmallSort keys are ranked by counting instead (two launches).
 Status sort_morton_pairs24(uint64_t *keys, uint32_t *vals, RadixScratch &s, hipStream_t st);
 constexpr uint32_t kSort24Max = 1u << 24;  // long-run ranks fit the 24 bits above the low 40
+constexpr uint32_t kSmallSort = 8192;      // sort_morton_pairs24: sort by ranking up to here
 constexpr uint32_t kRankRun = 32;          // sort_morton_pairs24: runs ranked in place by each key
 // Keys grouped by segment (many graphs): the caller writes a composite key ck
 // (segment id above the top bits of the raw key) into s.keys_alt and the

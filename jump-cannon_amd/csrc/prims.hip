@@ -538,8 +538,47 @@ Status sort_segmented_pairs(uint64_t *keys, uint32_t *vals, const uint64_t *raw,
     return Status::ok();
 }
 
+// Up to kSmallSort pairs by ranking: the position of pair i in the stable order
+// is the number of pairs j whose composite key (key_j, j) is below (key_i, i).
+// Each block stages all keys in LDS and ranks kRankPerBlock of them, 32 lanes
+// per key counting every 32nd key and summing the counts across the lanes;
+// lane 0 writes the pair to its rank in the other buffer.  A launch and a copy
+// instead of the radix passes' eleven launches, for trees small enough that
+// launches, not bytes, set the sort's time.
+constexpr uint32_t kRankPerBlock = 32;
+__global__ __launch_bounds__(1024) void k_sort_small_rank(const uint64_t *__restrict__ ki,
+                                                          const uint32_t *__restrict__ vi,
+                                                          uint64_t *__restrict__ ko,
+                                                          uint32_t *__restrict__ vo, uint32_t n) {
+    __shared__ uint64_t sk[kSmallSort];
+    for (uint32_t t = threadIdx.x; t < n; t += 1024) sk[t] = ki[t];
+    __syncthreads();
+    const uint32_t part = threadIdx.x & 31u;
+    const uint32_t i = blockIdx.x * kRankPerBlock + (threadIdx.x >> 5);
+    const bool live = i < n;
+    const uint64_t k = live ? sk[i] : 0ull;
+    uint32_t below = 0;
+    for (uint32_t j = part; j < n; j += 32) {
+        const uint64_t kj = sk[j];
+        below += (kj < k || (kj == k && j < i)) ? 1u : 0u;
+    }
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) below += __shfl_xor(below, o, 64);  // within the 32 lanes
+    if (live && part == 0) {
+        ko[below] = k;
+        vo[below] = vi[i];
+    }
+}
+
 Status sort_morton_pairs24(uint64_t *keys, uint32_t *vals, RadixScratch &s, hipStream_t st) {
     if (s.n == 0) return Status::ok();
+    if (s.n <= kSmallSort) {
+        k_sort_small_rank<<<(s.n + kRankPerBlock - 1) / kRankPerBlock, 1024, 0, st>>>(
+            keys, vals, s.keys_alt.p, s.vals_alt.p, s.n);
+        k_copy_pairs<<<(s.n + 255) / 256, 256, 0, st>>>(s.keys_alt.p, s.vals_alt.p, keys, vals, s.n);
+        JC_HIP(hipGetLastError());
+        return Status::ok();
+    }
     if (s.n >= kSort24Max) return Status::err(JC_E_INVALID, "sort_morton_pairs24: 2^24 keys or more");
     uint64_t *ka = keys, *kb = s.keys_alt.p;
     uint32_t *va = vals, *vb = s.vals_alt.p;

@@ -408,8 +408,8 @@ class Fa2BhEngine(LayoutEngine):
 
     def walk_mode(self, mode: int | None = None) -> int:
         """Query (None) or force the walk: 0 per-lane rope, 1 wave-union, 2/3 two/four
-        interleaved bodies per lane, 4/5 per-lane rope in 512/1024-thread blocks,
-        -1 re-time and keep the fastest."""
+        interleaved bodies per lane, 4/5/6 per-lane rope in 512/1024/64-thread
+        blocks, -1 re-time and keep the fastest."""
         cur = np.zeros(1, np.int32)
         _check(lib().jc_fa2bh_walk_mode(self._h, -2 if mode is None else mode, _ptr(cur)))
         return int(cur[0])

@@ -75,8 +75,8 @@ def _tree_equal(g, t):
     assert np.array_equal(g["depth"], want_depth)
 
 
-@pytest.mark.parametrize("walk", [0, 1, 2, 3, 4, 5],
-                         ids=["lane", "union", "chains2", "chains4", "lane512", "lane1024"])
+@pytest.mark.parametrize("walk", [0, 1, 2, 3, 4, 5, 6],
+                         ids=["lane", "union", "chains2", "chains4", "lane512", "lane1024", "lane64"])
 @pytest.mark.parametrize("name,off,nb,xyz", CASES, ids=[c[0] for c in CASES])
 def test_tree_and_forces_bit_exact(name, off, nb, xyz, walk):
     n = len(off) - 1
@@ -180,7 +180,7 @@ def test_forces_within_1e4_of_fp64_ba1m():
     assert rel.max() <= 1e-4 and np.quantile(rel, 0.99) <= 2e-6 and flips <= 100
 
 
-@pytest.mark.parametrize("walk", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("walk", [0, 1, 2, 3, 4, 5, 6])
 def test_trajectory_bit_exact_sbm2000(walk):
     """50 iterations: positions, F_{t-1} and the speed state bit-exact with the
     oracle driving the same (Morton) octree and the exact f64 sums."""
