@@ -4,8 +4,8 @@ workload (m=4, positions uniform in [-10,10]^3, theta 0.8).
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--no-cpu-baseline]
                     [--no-warm-start] [--copy-probe]
-                    [--workload ba_1m|batch_4096|brute_sbm2000|pagerank_ba1m|spmv_ba1m|sgd_rgg200k|
-                                traverse|mpnn_ba1m]
+                    [--workload ba_1m|batch_4096|sbm2000|brute_sbm2000|pagerank_ba1m|spmv_ba1m|
+                                sgd_rgg200k|traverse|mpnn_ba1m]
 
 A step is one full FA2+BH iteration (octree build, walk, attraction, gravity,
 adaptive speed, apply) with inputs already resident in HBM.  The timed region
@@ -25,6 +25,8 @@ calibration run (FETCH_SIZE / WRITE_SIZE of a copy of known size).
 The other workloads print one JSON line each for the sibling paths:
   batch_4096     4,096 graphs (50..20,000 nodes) in one batch, 500 iterations,
                  against the same graphs run one engine at a time (sampled);
+  sbm2000        FA2+BH on the 2,000-node SBM (theta 1.2, 500 iterations), the
+                 reference's CPU-runnable configuration;
   brute_sbm2000  the brute-force engine (fa2-brute-hip) on the 2,000-node SBM, 500
                  iterations;
   pagerank_ba1m  PageRank on the 1M-node BA CSR, d=0.85, L1 tol 1e-6;
@@ -230,6 +232,49 @@ def bench_batch(args) -> int:
                  speedup_vs_sequential=round(seq_ms / b_ms, 2) if len(sample) else None,
                  sequential_sample=(f"{which}, each its own engine, 20 timed iterations after 2, scaled to "
                                     f"{iters} iterations" + ("" if stride == 1 else f" and x{stride}")))
+
+
+def bench_sbm2000(args) -> int:
+    """FA2+BH on the 2,000-node SBM (theta 1.2, 500 iterations): the reference's own
+    CPU-runnable configuration, against the oracle's restatement of its CPU path
+    (insertion octree, serial, f64 node-order sums) on one host thread."""
+    import jumpcannon as jc
+    from jumpcannon import workloads
+
+    iters = 500
+    off, nb, xyz = workloads.sbm_2000()
+    n = len(off) - 1
+    e = jc.Fa2BhEngine(theta=1.2)
+    e.init(off, nb, xyz)
+    e.step(5, readback=False)
+    e.sync()
+    t = time.perf_counter()
+    e.step(iters, readback=False)
+    e.sync()
+    wall = time.perf_counter() - t
+    f_ms, it_ms = e.step_timed(iters)
+    ph = e.profile(args.profile_iters)
+    log(f"sbm2000: {iters} iterations in {wall * 1e3:.1f} ms (graph replay), walk mode {e.walk_mode()}; "
+        f"timed: force {f_ms / iters * 1e3:.1f} us, iteration {it_ms / iters * 1e3:.1f} us")
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        def run(k):
+            O.fa2_run(off, nb, xyz, k, O.Fa2Settings.make(theta=1.2), tree_kind=0, exact=False, threads=1)
+            return k
+        rate, done = _timed_rate(run, args.cpu_seconds / 2)
+        cpu = {"value": rate, "unit": "iterations/s", "cores": 1, "kind": "port", "cpu_model": _cpu_model(),
+               "nproc": os.cpu_count(),
+               "sample": f"{done} iterations of the oracle's restatement of the reference CPU path "
+                         f"(insertion octree fa2_bh.rs:747-1045 built serially, walk, speed, apply) on one thread"}
+        log(f"sbm2000 cpu baseline {rate:.1f} iterations/s on 1 thread")
+    return _line("FA2+BH iterations/s at 2,000 nodes (SBM, theta 1.2)", iters / wall, "iterations/s", iters, 5,
+                 wall * 1e3 / iters, "f32",
+                 {"workload": "sbm2000_fa2bh", "n_nodes": n, "csr_nnz": len(nb), "theta": 1.2,
+                  "parallelism": "single-gpu"},
+                 phase_ms={k: round(v, 4) for k, v in ph.items()}, walk_mode=e.walk_mode(),
+                 cpu_baseline=cpu)
 
 
 def bench_brute(args) -> int:
@@ -557,8 +602,8 @@ def main() -> int:
     ap.add_argument("--no-sequential", action="store_true",
                     help="batch_4096: skip the one-engine-per-graph estimate")
     ap.add_argument("--workload", default="ba_1m",
-                    choices=["ba_1m", "batch_4096", "brute_sbm2000", "pagerank_ba1m", "spmv_ba1m",
-                             "sgd_rgg200k", "traverse", "mpnn_ba1m"])
+                    choices=["ba_1m", "batch_4096", "brute_sbm2000", "sbm2000", "pagerank_ba1m",
+                             "spmv_ba1m", "sgd_rgg200k", "traverse", "mpnn_ba1m"])
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe()
@@ -567,7 +612,8 @@ def main() -> int:
               "(the layout path is single-GPU in this build).", file=sys.stderr)
         return 2
     if args.workload != "ba_1m":
-        return {"batch_4096": bench_batch, "brute_sbm2000": bench_brute, "pagerank_ba1m": bench_pagerank,
+        return {"batch_4096": bench_batch, "brute_sbm2000": bench_brute, "sbm2000": bench_sbm2000,
+                "pagerank_ba1m": bench_pagerank,
                 "spmv_ba1m": bench_spmv,
                 "sgd_rgg200k": bench_sgd, "traverse": bench_traverse,
                 "mpnn_ba1m": bench_mpnn}[args.workload](args)
