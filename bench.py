@@ -289,12 +289,18 @@ def bench_brute(args) -> int:
     e = jc.Fa2BruteEngine()
     e.init(off, nb, xyz)
     e.step(5, readback=False)
+    e.sync()
     mode = e.walk_mode()
-    f_ms, it_ms = e.step_timed(iters)
+    t = time.perf_counter()
+    e.step(iters, readback=False)  # the captured 2-iteration graph, as a caller's step(k) runs
+    e.sync()
+    wall = time.perf_counter() - t
+    f_ms, it_ms = e.step_timed(iters)  # events around every force kernel
     # per pair (force_atlas2.wgsl:82-91): d (3), |d|^2 (5), max (1), coefficient (3), F += d*c (6)
     flops = 18.0 * n * (n - 1)
     achieved = flops / (f_ms / iters / 1e3) / 1e12
-    log(f"brute: {iters} iterations in {it_ms:.1f} ms, force kernels {f_ms:.1f} ms, repulsion kernel "
+    log(f"brute: {iters} iterations in {wall * 1e3:.1f} ms (graph replay); with events around each force "
+        f"kernel {it_ms:.1f} ms, force kernels {f_ms:.1f} ms; repulsion kernel "
         f"{['one lane', '4 lanes', '16 lanes'][mode]} per body")
     cpu = None
     if not args.no_cpu_baseline:
@@ -309,8 +315,8 @@ def bench_brute(args) -> int:
                "sample": f"{done} iterations of the oracle's restatement of the brute-force iteration "
                          f"(force_atlas2.wgsl:69-165, fa2_speed.rs) on one thread"}
         log(f"brute cpu baseline {rate:.1f} iterations/s on 1 thread")
-    return _line("FA2 brute-force iterations/s (2,000-node SBM)", iters / (it_ms / 1e3), "iterations/s",
-                 iters, 5, it_ms / iters, "f32",
+    return _line("FA2 brute-force iterations/s (2,000-node SBM)", iters / wall, "iterations/s",
+                 iters, 5, wall * 1e3 / iters, "f32",
                  {"workload": "brute_sbm2000", "n_nodes": n, "csr_nnz": len(nb), "engine": "fa2-brute-hip",
                   "parallelism": "single-gpu"},
                  roofline={"bound": "valu", "achieved": round(achieved, 3), "peak": VALU_FP32_PEAK_TFLOPS,
