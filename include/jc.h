@@ -273,7 +273,8 @@ void jc_spmv_plan_destroy(jc_spmv_plan *plan);
  * Replaces MultilevelEngine (crates/graph-compute/src/engines/multilevel.rs:196-480),
  * registry id "multilevel-hip": coarsen (greedy matching, coarsen.rs:53-160) ->
  * solve the coarsest level -> prolong (coarsen.rs:166-193) -> refine, with
- * "fa2-bh-hip" (the default) or "sgd-stress-hip" as the inner engine.  Settings
+ * "fa2-brute-hip" (the default, as multilevel.rs:176 has fa2-brute), "fa2-bh-hip"
+ * or "sgd-stress-hip" as the inner engine.  Settings
  * through jc_engine_set_params_json, the MultilevelSettings keys
  * (multilevel.rs:132-165): inner, inner_params (forwarded to the inner engine
  * at init), max_levels, target_size, sweeps_per_level, sweep_schedule
