@@ -383,15 +383,17 @@ def _clustered(n, n_out, spread, seed):
     return xyz.reshape(-1)
 
 
-@pytest.mark.parametrize("n,n_out,spread,clusters", [(3000, 10, 0.01, 1), (10000, 10, 0.01, 1),
+@pytest.mark.parametrize("n,n_out,spread,clusters", [(9000, 10, 0.01, 3), (10000, 10, 0.01, 1),
+                                                      (12000, 8, 0.003, 40), (3000, 10, 0.01, 1),
                                                       (4000, 8, 0.003, 40)],
-                         ids=["run_lds", "run_global", "many_runs"])
+                         ids=["run_lds", "run_global", "many_runs", "small_one_run", "small_many_runs"])
 def test_sort_of_long_equal_prefix_runs(n, n_out, spread, clusters):
-    """The octree sort orders keys by their upper 32 bits with radix passes and
-    each run of equal upper halves by the lower half; runs longer than one
-    thread's share sort in LDS (up to 4096 keys) or in global memory.  Layouts
-    whose bodies share their top ten octree levels exercise both, and the tree
-    must still equal the oracle's stable sort of the full keys."""
+    """Above 8,192 bodies the octree sort orders keys by their upper bits with
+    radix passes and each run of equal upper bits by the rest; runs longer than
+    one thread's share sort in LDS (up to 4096 keys) or in global memory.  Up to
+    8,192 bodies every key is ranked by counting instead.  Layouts whose bodies
+    share their top ten octree levels exercise all three, and the tree must
+    still equal the oracle's stable sort of the full keys."""
     off, nb = O.ba_graph(n, 2, 31)
     xyz = _clustered(n, n_out, spread, 5).reshape(-1, 3)
     if clusters > 1:
