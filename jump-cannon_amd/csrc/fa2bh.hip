@@ -442,8 +442,10 @@ constexpr uint32_t kFuseThreads = 512;
 
 // Graphs of at most kComSmallMax bodies fuse from just below the top tree
 // (Depth = kTopDepth + 1): their cells there hold few nodes, and the two
-// k_com_level launches of the larger graphs go away.
-constexpr uint32_t kComSmallMax = 65536;
+// k_com_level launches of the larger graphs go away.  Larger graphs put too
+// many depth-4 and depth-5 cells past the fused pass's capacity, onto the
+// one-workgroup pass for such cells.
+constexpr uint32_t kComSmallMax = 262144;
 template <uint32_t Depth>
 __global__ __launch_bounds__(kFuseThreads) void k_com_fused(Frame *fr, uint32_t cap,
                                                             Node *__restrict__ nodes,
